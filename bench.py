@@ -1,0 +1,152 @@
+"""Headline benchmark: matrix-free H·v at N=28 on one MI355X.
+
+Workload (synthetic, seeded): chain of 28 atoms at unit spacing, all 378
+pairs V = 10/|i−j|^6, Ω = 1, Δ = 1.2 (reference sign: delta = −1.2), full
+2^28 basis, x ~ N(0,1) from a seeded device generator.  One step = one
+H·v of the whole 2^28 vector, inputs resident in HBM.
+
+Prints one JSON line (value = H·v/s in FP64; the FP32 form is reported in
+``fp32``).  ``roofline`` is for the dominant kernel (k_full_tiled) with
+algorithmic bytes 2·D·sizeof(element) (x read once, y written once) over
+its mean duration measured with HIP events on the launch stream.
+``cpu_baseline`` times the oracle's OpenMP matrix-free H·v on the host
+cores for a bounded sample of the same N=28 rows.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+N_SITES = 28
+
+
+def _model(n):
+    import numpy as np
+    pairs = [{'i': i, 'j': j, 'V': 10.0 / (j - i) ** 6} for i in range(n) for j in range(i + 1, n)]
+    calib = {'atoms': [{'id': i, 'pos': [float(i), 0.0, 0.0]} for i in range(n)],
+             'pairs': pairs, 'terms': ['I1']}
+    return calib, np.ones(n), -1.2 * np.ones(n)
+
+
+def _time_hv(pkg, plan, dtype, steps, warmup):
+    import torch
+    D = plan.dim
+    g = torch.Generator(device=plan.device)
+    g.manual_seed(0)
+    x = torch.randn(D, dtype=torch.float64, device=plan.device, generator=g).to(dtype)
+    y = torch.empty_like(x)
+    for _ in range(warmup):
+        plan.hv(x, y)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        plan.hv(x, y)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = [a.elapsed_time(b) for a, b in ev]
+    del x, y
+    torch.cuda.empty_cache()
+    return wall, sum(kms) / len(kms)
+
+
+def _cpu_baseline(n, rows):
+    """Oracle OpenMP H·v on rows [0, rows) of the N=n workload, scaled to H·v/s."""
+    import numpy as np
+    from oracle import hv_ref
+    calib, Om, delta = _model(n)
+    D = 1 << n
+    x = np.random.default_rng(0).random(D)
+    d = hv_ref.diag(n, delta, calib['pairs'], np.arange(rows, dtype=np.int64))
+    hv_ref.hv_full_rows(n, Om, d, x, 0, rows)      # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        hv_ref.hv_full_rows(n, Om, d, x, 0, rows)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    per_hv = t * D / rows
+    return dict(value=1.0 / per_hv, unit='H·v/s', cores=hv_ref.num_threads(), kind='port',
+                sample='rows [0, %d) of the N=%d FP64 H·v (1/%d of the vector), OpenMP matrix-free '
+                       'oracle, median of 5 after warm-up, diagonal precomputed; scaled by %d'
+                       % (rows, n, D // rows, D // rows),
+                seconds_per_sample=t)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--n', type=int, default=N_SITES)
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: the H·v path is built and measured on a single '
+              'MI355X (multi-GPU sharding is a design note, not a run); use --gpus 1',
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    dev = torch.device('cuda:0')
+    n = args.n
+    calib, Om, delta = _model(n)
+    plan = pkg.Plan(pkg.from_calib(calib, Om, delta), device=dev)
+    D = plan.dim
+    res = {}
+    for name, dt, size in (('f64', torch.float64, 8), ('f32', torch.float32, 4)):
+        torch.cuda.synchronize()
+        wall, kms = _time_hv(pkg, plan, dt, args.steps, args.warmup)
+        alg = 2.0 * D * size
+        res[name] = dict(hv_per_s=args.steps / wall, ms_per_step=1e3 * wall / args.steps,
+                         kernel_ms=kms, alg_bytes=alg, achieved_gbps=alg / (kms * 1e-3) / 1e9)
+        print('[bench] %s: %.3f ms/H·v (kernel %.3f ms), %.1f GB/s algorithmic'
+              % (name, res[name]['ms_per_step'], kms, res[name]['achieved_gbps']),
+              file=sys.stderr, flush=True)
+    f64 = res['f64']
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(n, rows=1 << max(10, n - 6))
+    out = {
+        'metric': 'H·v/s at N=28 (full 2^28 basis, matrix-free, FP64)',
+        'value': f64['hv_per_s'],
+        'unit': 'H·v/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': f64['ms_per_step'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f64',
+        'data': 'synthetic',
+        'config': {'workload': 'hv_full_chain_n%d' % n, 'n_sites': n, 'dim': D, 'basis': 'full',
+                   'model': 'chain a=1, V=10/r^6 all pairs, Omega=1, Delta=1.2',
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': f64['achieved_gbps'], 'peak': HBM_PEAK_GBPS,
+                     'unit': 'GB/s', 'frac': f64['achieved_gbps'] / HBM_PEAK_GBPS,
+                     'traffic': None, 'kernel': 'k_full_tiled<double>',
+                     'alg_bytes_per_launch': f64['alg_bytes'],
+                     'kernel_ms': f64['kernel_ms']},
+        'fp32': {'hv_per_s': res['f32']['hv_per_s'], 'ms_per_step': res['f32']['ms_per_step'],
+                 'kernel_ms': res['f32']['kernel_ms'],
+                 'achieved_gbps': res['f32']['achieved_gbps'],
+                 'frac': res['f32']['achieved_gbps'] / HBM_PEAK_GBPS},
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()

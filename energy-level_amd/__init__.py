@@ -1,0 +1,21 @@
+"""rydhip — MI355X-native matrix-free Rydberg Hamiltonian engine (host side).
+
+H = Σ Ω_i/2 σx_i + Σ delta_i n_i + Σ_{i<j} V_ij n_i n_j on the full 2^N basis,
+applied by hand-written gfx950 HIP kernels behind the C ABI of
+``include/rydhip.h``; this package mirrors the reference's Python entry
+points (tomisnot/energy-level src/core/manybody/) on top of that ABI.
+"""
+from . import _native
+from .model import RydModel, from_calib, active_terms, chain_positions, square_positions, \
+    from_positions, c6_coupling
+from .plan import Plan, cast
+from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, rayleigh_residual, \
+    site_occupations
+from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
+    dos_from_moments, probe_bits
+
+__all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_positions',
+           'from_positions', 'c6_coupling', 'Plan', 'cast', 'hip_lanczos_ground_state',
+           'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
+           'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',
+           'dos_from_moments', 'probe_bits']

@@ -1,0 +1,274 @@
+// Full 2^n basis: fused diagonal + bit-flip H·v with an optional
+// three-term update and FP64 reductions, for gfx950.
+//
+//   y = ch · H(s·x) + cx · (s·x) + cz · z
+//   sums: Re<s·x, y>, |y|^2, |s·x|^2, Σ_e z_e·Re(y_e)  (z_e = ±1 probe bits)
+//
+// Channel semantics follow tomisnot/energy-level
+// src/core/manybody/channels.py:58-77 (diagonal +delta_i n_i + V n_i n_j) and
+// :114-118 (Ω flips: target bit 1 takes Ω/2, target bit 0 takes conj(Ω)/2).
+// Bit position p (LSB = 0) is site n-1-p (site 0 is the MSB).
+//
+// Tiling (per 256-thread block, one tile of 2^TB consecutive codes):
+//   element e = r·1024 + t·4 + c  (t = thread, c = bits 0..1, r = bits 10..TB-1)
+//   bits 0..1 and 10..TB-1 flip inside a thread's registers, bits 2..9 flip
+//   across threads through LDS, bits >= TB read the partner tile through
+//   L2/MALL.  The diagonal is rebuilt on the fly:
+//     d(e) = E_hi[tile] + E_lo[e] + T0[e & 255] + T1[e >> 8]
+//   where T0/T1 hold the cross field Σ_{q>=TB} V_pq b_q of this tile on the
+//   low bits, built per tile in LDS.  Tiles are mapped XCD-aware: block b runs
+//   on XCD b % 8, which sweeps its own contiguous eighth of the vector.
+#pragma once
+#include "common.hpp"
+
+namespace ryd {
+
+constexpr int MAXN = 64;
+
+struct FullModel {
+  int n;              // number of sites (bits)
+  int tb;             // tile bits of the tables below
+  const double* elo;  // [2^tb] diagonal from bits < tb (delta and V inside)
+  const double* ehi;  // [2^(n-tb)] diagonal from bits >= tb
+  const double* vb;   // [n*n] V between bit positions (symmetric)
+  double ar[MAXN];    // Re(Ω)/2 per bit position
+  double ai[MAXN];    // Im(Ω)/2 per bit position
+};
+
+enum ScalarMode : int { SM_CONST = 0, SM_LANCZOS = 1 };
+
+template <class V>
+struct ApplyIO {
+  const V* x;
+  const V* z;               // optional third term
+  V* y;
+  const uint32_t* probe;    // optional probe bits (bit set = +1)
+  int smode;
+  double s, ch, cx, cz;     // SM_CONST
+  const double* nrm;        // SM_LANCZOS: s = 1/nrm[j], cz = -nrm[j]/nrm[j-1]
+  int j;
+  const int* stop;          // optional early exit flag (device)
+  int reduce;
+};
+
+template <class V>
+__device__ __forceinline__ void apply_scalars(const ApplyIO<V>& io, double& s, double& ch,
+                                              double& cx, double& cz) {
+  if (io.smode == SM_LANCZOS) {
+    const double nj = io.nrm[io.j];
+    s = 1.0 / nj;
+    ch = 1.0;
+    cx = 0.0;
+    cz = io.j > 0 ? -nj / io.nrm[io.j - 1] : 0.0;
+  } else {
+    s = io.s; ch = io.ch; cx = io.cx; cz = io.cz;
+  }
+}
+
+template <class V> constexpr int tile_r() { return sizeof(V) >= 16 ? 2 : 4; }
+template <class V> constexpr int tile_bits() { return sizeof(V) >= 16 ? 11 : 12; }
+
+template <class V>
+__global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int R = tile_r<V>();
+  constexpr int TB = tile_bits<V>();
+  constexpr int TILE = 1024 * R;
+  __shared__ V lds[TILE];
+  __shared__ double Fx[16];
+  __shared__ double T0[256];
+  __shared__ double T1[16];
+
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+
+  const int t = threadIdx.x;
+  const int nhi = M.n - TB;
+  const uint64_t ntiles = 1ull << nhi;
+  const uint64_t b = blockIdx.x;
+  const uint64_t tile = ntiles >= 8 ? ((b & 7ull) * (ntiles >> 3) + (b >> 3)) : b;
+  const uint64_t base = tile << TB;
+
+  // stage the tile (scaled) in registers and LDS
+  V xs[R][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + base + r * 1024 + t * 4);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xs[r][c] = vscale(q.v[c], sr);
+    V4<V> w;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) w.v[c] = xs[r][c];
+    *reinterpret_cast<V4<V>*>(lds + r * 1024 + t * 4) = w;
+  }
+
+  // per-tile cross field of the high bits on the low bits
+  if (t < TB) {
+    double f = 0.0;
+    const double* row = M.vb + (size_t)t * M.n;
+    for (int q = TB; q < M.n; ++q)
+      if ((tile >> (q - TB)) & 1ull) f += row[q];
+    Fx[t] = f;
+  }
+  __syncthreads();
+  {
+    double v = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+      if ((t >> p) & 1) v += Fx[p];
+    T0[t] = v;
+    if (t < (1 << (TB - 8))) {
+      double w = 0.0;
+#pragma unroll
+      for (int p = 8; p < TB; ++p)
+        if ((t >> (p - 8)) & 1) w += Fx[p];
+      T1[t] = w;
+    }
+  }
+  const double eh = M.ehi[tile];
+  __syncthreads();
+
+  // diagonal
+  V acc[R][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e0 = r * 1024 + t * 4;
+    V4<double> el = *reinterpret_cast<const V4<double>*>(M.elo + e0);
+    const double t1 = T1[e0 >> 8];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double d = eh + el.v[c] + T0[(e0 + c) & 255] + t1;
+      acc[r][c] = vscale(xs[r][c], (RT)d);
+    }
+  }
+
+  // bits 0..1: inside the thread's 4-run
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int bit = (c >> p) & 1;
+        vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r][c ^ (1 << p)]);
+      }
+  }
+  // bits 10..TB-1: across the thread's r groups
+#pragma unroll
+  for (int k = 0; (1 << k) < R; ++k) {
+    const int p = 10 + k;
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int bit = (r >> k) & 1;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r ^ (1 << k)][c]);
+    }
+  }
+  // bits 2..9: partner thread through LDS
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int p = 2 + k;
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+    const int bit = (t >> k) & 1;
+    const RT aim = bit ? ai : -ai;
+    const int tp = t ^ (1 << k);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V4<V> q = *reinterpret_cast<const V4<V>*>(lds + r * 1024 + tp * 4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
+    }
+  }
+  // bits >= TB: partner tile through the cache hierarchy
+  for (int k = 0; k < nhi; ++k) {
+    const int p = TB + k;
+    const uint64_t pbase = (tile ^ (1ull << k)) << TB;
+    const int bit = (int)((tile >> k) & 1ull);
+    const RT ar = (RT)(M.ar[p] * s);
+    const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + pbase + r * 1024 + t * 4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
+    }
+  }
+
+  // combine, store, reduce
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t g0 = base + r * 1024 + t * 4;
+    V4<V> zq;
+    if (io.z) zq = *reinterpret_cast<const V4<V>*>(io.z + g0);
+    uint32_t pw = 0;
+    if (io.probe) pw = io.probe[g0 >> 5] >> (g0 & 31);
+    V4<V> o;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      V v = vscale(acc[r][c], chr);
+      vfma(v, cxr, xs[r][c]);
+      if (io.z) vfma(v, czr, zq.v[c]);
+      o.v[c] = v;
+      if (io.reduce) {
+        s0 += rdot(xs[r][c], v);
+        s1 += nrm2(v);
+        s2 += nrm2(xs[r][c]);
+        if (io.probe) s3 += zdot(((pw >> c) & 1u) ? 1.0 : -1.0, v);
+      }
+    }
+    *reinterpret_cast<V4<V>*>(io.y + g0) = o;
+  }
+  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+}
+
+// n below the tile size: one element per thread, partners from global memory.
+template <class V>
+__global__ __launch_bounds__(256) void k_full_small(FullModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const uint64_t D = 1ull << M.n;
+  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool ok = g < D;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (ok) {
+    const V xe = vscale(io.x[g], (RT)s);
+    const uint64_t lo = g & ((1ull << M.tb) - 1);
+    const uint64_t hi = g >> M.tb;
+    V acc = vscale(xe, (RT)(M.elo[lo] + M.ehi[hi]));
+    // cross terms between low and high parts (only when tb < n)
+    if (M.tb < M.n) {
+      double dc = 0.0;
+      for (int p = 0; p < M.tb; ++p) {
+        if (!((g >> p) & 1ull)) continue;
+        for (int q = M.tb; q < M.n; ++q)
+          if ((g >> q) & 1ull) dc += M.vb[p * M.n + q];
+      }
+      vfma(acc, (RT)dc, xe);
+    }
+    for (int p = 0; p < M.n; ++p) {
+      const int bit = (int)((g >> p) & 1ull);
+      vcfma(acc, (RT)(M.ar[p] * s), (RT)((bit ? M.ai[p] : -M.ai[p]) * s), io.x[g ^ (1ull << p)]);
+    }
+    V v = vscale(acc, (RT)ch);
+    vfma(v, (RT)cx, xe);
+    if (io.z) vfma(v, (RT)cz, io.z[g]);
+    io.y[g] = v;
+    if (io.reduce) {
+      s0 = rdot(xe, v);
+      s1 = nrm2(v);
+      s2 = nrm2(xe);
+      if (io.probe) s3 = zdot(((io.probe[g >> 5] >> (g & 31)) & 1u) ? 1.0 : -1.0, v);
+    }
+  }
+  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+}
+
+}  // namespace ryd

@@ -1,0 +1,494 @@
+// C ABI of the Rydberg H·v engine: plan construction (host tables), launch
+// logic and the device-resident Krylov drivers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rydhip.h"
+#include "common.hpp"
+#include "hv_full.hpp"
+#include "stream_ops.hpp"
+#include "observables.hpp"
+
+using namespace ryd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                            \
+  do {                                                                           \
+    hipError_t _e = (expr);                                                      \
+    if (_e != hipSuccess)                                                        \
+      return fail(_e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,   \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));            \
+  } while (0)
+
+#define LAUNCH_CHECK()                                                           \
+  do {                                                                           \
+    hipError_t _e = hipGetLastError();                                           \
+    if (_e != hipSuccess)                                                        \
+      return fail(RYD_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(_e)); \
+  } while (0)
+
+}  // namespace
+
+struct ryd_plan {
+  int n = 0;
+  int basis = RYD_BASIS_FULL;
+  int64_t dim = 0;
+  int device = 0;
+  bool cplx = false;                 // Ω has an imaginary part
+  double ar[MAXN] = {0}, ai[MAXN] = {0};   // per bit position, already halved
+  double* d_vb = nullptr;            // [n*n] V by bit position
+  double* d_elo[2] = {nullptr, nullptr};   // tables for tb = tbits[k]
+  double* d_ehi[2] = {nullptr, nullptr};
+  int tbits[2] = {0, 0};
+  size_t ws_bytes = 0;
+};
+
+namespace {
+
+int tb_index(const ryd_plan* P, int dtype) {
+  return (dtype == RYD_C128) ? 1 : 0;   // 16-byte elements use 11-bit tiles
+}
+
+FullModel full_model(const ryd_plan* P, int dtype) {
+  FullModel M;
+  const int k = tb_index(P, dtype);
+  M.n = P->n;
+  M.tb = P->tbits[k];
+  M.elo = P->d_elo[k];
+  M.ehi = P->d_ehi[k];
+  M.vb = P->d_vb;
+  for (int p = 0; p < MAXN; ++p) {
+    M.ar[p] = P->ar[p];
+    M.ai[p] = P->ai[p];
+  }
+  return M;
+}
+
+// Diagonal tables for tile bits tb: elo over bit positions < tb, ehi over >= tb.
+void build_tables(int n, int tb, const std::vector<double>& dl, const std::vector<double>& vb,
+                  std::vector<double>& elo, std::vector<double>& ehi) {
+  const int64_t nl = 1ll << tb, nh = 1ll << (n - tb);
+  elo.assign(nl, 0.0);
+  ehi.assign(nh, 0.0);
+  for (int64_t e = 1; e < nl; ++e) {
+    const int p = __builtin_ctzll(e);
+    const int64_t rest = e & (e - 1);
+    double v = elo[rest] + dl[p];
+    for (int64_t r = rest; r; r &= r - 1) v += vb[p * n + __builtin_ctzll(r)];
+    elo[e] = v;
+  }
+  for (int64_t h = 1; h < nh; ++h) {
+    const int q = __builtin_ctzll(h);
+    const int64_t rest = h & (h - 1);
+    double v = ehi[rest] + dl[tb + q];
+    for (int64_t r = rest; r; r &= r - 1) v += vb[(tb + q) * n + tb + __builtin_ctzll(r)];
+    ehi[h] = v;
+  }
+}
+
+template <class V>
+int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
+  FullModel M = full_model(P, dtype);
+  constexpr int TB = tile_bits<V>();
+  if (P->n >= TB) {
+    const uint64_t ntiles = 1ull << (P->n - TB);
+    k_full_tiled<V><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(M, io, rd);
+  } else {
+    const uint64_t D = 1ull << P->n;
+    const unsigned grid = (unsigned)((D + 255) / 256);
+    k_full_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+  }
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+unsigned apply_grid(const ryd_plan* P, int dtype) {
+  const int TB = (dtype == RYD_C128) ? 11 : 12;
+  if (P->n >= TB) return (unsigned)(1ull << (P->n - TB));
+  return (unsigned)(((1ull << P->n) + 255) / 256);
+}
+
+unsigned stream_grid(int64_t D) {
+  int64_t g = (D + 1023) / 1024;
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return (unsigned)g;
+}
+
+Reduce make_reduce(void* ws, int post, double* out) {
+  Reduce rd;
+  rd.counter = reinterpret_cast<unsigned*>(ws);
+  rd.partials = reinterpret_cast<double*>(reinterpret_cast<char*>(ws) + WS_HEADER);
+  rd.post = post;
+  rd.out = out;
+  rd.alpha = nullptr;
+  rd.nrm = nullptr;
+  rd.st = nullptr;
+  rd.j = 0;
+  return rd;
+}
+
+template <class V>
+int do_apply(const ryd_plan* P, int dtype, const void* x, const void* z, void* y, double s,
+             double ch, double cx, double cz, const uint32_t* probe, double* sums, int accumulate,
+             void* ws, hipStream_t st) {
+  ApplyIO<V> io;
+  io.x = static_cast<const V*>(x);
+  io.z = static_cast<const V*>(z);
+  io.y = static_cast<V*>(y);
+  io.probe = probe;
+  io.smode = SM_CONST;
+  io.s = s; io.ch = ch; io.cx = cx; io.cz = cz;
+  io.nrm = nullptr;
+  io.j = 0;
+  io.stop = nullptr;
+  io.reduce = sums != nullptr;
+  Reduce rd = make_reduce(ws, accumulate ? POST_ACCUM : POST_STORE, sums);
+  return launch_apply<V>(P, dtype, io, rd, st);
+}
+
+__global__ void k_lz_init(LzState* s, double tiny) {
+  s->stop = 0;
+  s->steps = 0;
+  s->tiny = tiny;
+}
+
+template <class V>
+int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64_t j0,
+               int64_t steps, double* alpha, double* nrm, void* state, double tiny, void* ritz0,
+               void* ritz1, const double* y0, const double* y1, int replay, void* ws,
+               hipStream_t st) {
+  V* B[3] = {static_cast<V*>(b0), static_cast<V*>(b1), static_cast<V*>(b2)};
+  LzState* S = static_cast<LzState*>(state);
+  const int64_t D = P->dim;
+  const unsigned sg = stream_grid(D);
+  if (j0 == 0 && !replay) {
+    k_lz_init<<<1, 1, 0, st>>>(S, tiny);
+    LAUNCH_CHECK();
+    Reduce rd = make_reduce(ws, POST_LZ_NORM, nullptr);
+    rd.nrm = nrm;
+    rd.st = S;
+    rd.j = -1;
+    k_dots<V><<<sg, 256, 0, st>>>(nullptr, B[0], D, rd);
+    LAUNCH_CHECK();
+  }
+  const int* stop = replay ? nullptr : &S->stop;
+  for (int64_t j = j0; j < j0 + steps; ++j) {
+    V* uj = B[j % 3];
+    V* ujm = B[(j + 2) % 3];
+    V* w = B[(j + 1) % 3];
+    ApplyIO<V> io;
+    io.x = uj;
+    io.z = j > 0 ? ujm : nullptr;
+    io.y = w;
+    io.probe = nullptr;
+    io.smode = SM_LANCZOS;
+    io.s = io.ch = io.cx = io.cz = 0.0;
+    io.nrm = nrm;
+    io.j = (int)j;
+    io.stop = stop;
+    io.reduce = !replay;
+    Reduce ra = make_reduce(ws, POST_LZ_ALPHA, nullptr);
+    ra.alpha = alpha;
+    ra.j = (int)j;
+    int rc = launch_apply<V>(P, dtype, io, ra, st);
+    if (rc) return rc;
+    Reduce rb = make_reduce(ws, POST_LZ_NORM, nullptr);
+    rb.nrm = nrm;
+    rb.st = S;
+    rb.j = (int)j;
+    k_lz_orth<V><<<sg, 256, 0, st>>>(uj, w, D, alpha, nrm, (int)j, static_cast<V*>(ritz0),
+                                     static_cast<V*>(ritz1), y0, y1, stop, !replay, rb);
+    LAUNCH_CHECK();
+  }
+  return RYD_OK;
+}
+
+template <class V>
+int do_cheb(const ryd_plan* P, int dtype, const uint32_t* probe, void* t0, void* t1, void* t2,
+            int64_t nsteps, double c, double r, double* sums, void* ws, hipStream_t st) {
+  V* B[3] = {static_cast<V*>(t0), static_cast<V*>(t1), static_cast<V*>(t2)};
+  const int64_t D = P->dim;
+  k_probe_expand<V><<<stream_grid(D), 256, 0, st>>>(probe, B[0], D);
+  LAUNCH_CHECK();
+  for (int64_t k = 1; k <= nsteps; ++k) {
+    int rc;
+    if (k == 1)
+      rc = do_apply<V>(P, dtype, B[0], nullptr, B[1], 1.0, 1.0 / r, -c / r, 0.0, probe,
+                       sums + 4 * k, 1, ws, st);
+    else
+      rc = do_apply<V>(P, dtype, B[(k - 1) % 3], B[(k - 2) % 3], B[k % 3], 1.0, 2.0 / r,
+                       -2.0 * c / r, -1.0, probe, sums + 4 * k, 1, ws, st);
+    if (rc) return rc;
+  }
+  return RYD_OK;
+}
+
+int check_dtype(const ryd_plan* P, int dtype) {
+  if (dtype < RYD_F32 || dtype > RYD_C128) return fail(RYD_ERR_INVALID, "unknown dtype");
+  if (P->cplx && (dtype == RYD_F32 || dtype == RYD_F64))
+    return fail(RYD_ERR_INVALID, "complex Rabi phases need a complex dtype (complex64/complex128)");
+  return RYD_OK;
+}
+
+}  // namespace
+
+#define DISPATCH(dtype, FN, ...)                                 \
+  switch (dtype) {                                               \
+    case RYD_F32: return FN<float>(__VA_ARGS__);                 \
+    case RYD_F64: return FN<double>(__VA_ARGS__);                \
+    case RYD_C64: return FN<c32>(__VA_ARGS__);                   \
+    case RYD_C128: return FN<c64>(__VA_ARGS__);                  \
+    default: return fail(RYD_ERR_INVALID, "unknown dtype");     \
+  }
+
+extern "C" {
+
+int ryd_abi_version(void) { return RYD_ABI_VERSION; }
+
+int ryd_last_error(char* buf, size_t size) {
+  if (!buf || !size) return RYD_ERR_INVALID;
+  std::snprintf(buf, size, "%s", g_err.c_str());
+  return RYD_OK;
+}
+
+int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
+  if (!m || !out) return fail(RYD_ERR_INVALID, "null model or output pointer");
+  *out = nullptr;
+  const int n = m->n;
+  if (n < 1 || n > 62) return fail(RYD_ERR_INVALID, "n must be in [1, 62]");
+  if (!m->om_re || !m->delta) return fail(RYD_ERR_INVALID, "om_re and delta are required");
+  if (m->n_pairs < 0 || (m->n_pairs > 0 && (!m->pair_i || !m->pair_j || !m->pair_v)))
+    return fail(RYD_ERR_INVALID, "pair arrays are required when n_pairs > 0");
+  if (m->basis_kind != RYD_BASIS_FULL)
+    return fail(RYD_ERR_UNSUPPORTED, "only the full basis is built into this library version");
+  if (n > 40) return fail(RYD_ERR_INVALID, "full basis needs n <= 40");
+  for (int i = 0; i < n; ++i) {
+    if (!std::isfinite(m->om_re[i]) || !std::isfinite(m->delta[i]) ||
+        (m->om_im && !std::isfinite(m->om_im[i])))
+      return fail(RYD_ERR_INVALID, "Om/delta must be finite");
+  }
+  ryd_plan* P = new ryd_plan();
+  P->n = n;
+  P->basis = m->basis_kind;
+  P->dim = 1ll << n;
+  (void)hipGetDevice(&P->device);
+  std::vector<double> dl(n), vb((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const int p = n - 1 - i;
+    P->ar[p] = 0.5 * m->om_re[i];
+    P->ai[p] = m->om_im ? 0.5 * m->om_im[i] : 0.0;
+    if (P->ai[p] != 0.0) P->cplx = true;
+    dl[p] = m->delta[i];
+  }
+  for (int k = 0; k < m->n_pairs; ++k) {
+    const int i = m->pair_i[k], j = m->pair_j[k];
+    if (i < 0 || j < 0 || i >= n || j >= n || i == j || !std::isfinite(m->pair_v[k])) {
+      delete P;
+      return fail(RYD_ERR_INVALID, "pair indices out of range, self pair or non-finite V");
+    }
+    const int p = n - 1 - i, q = n - 1 - j;
+    vb[(size_t)p * n + q] += m->pair_v[k];
+    vb[(size_t)q * n + p] += m->pair_v[k];
+  }
+  auto cleanup = [&](int code, const std::string& msg) {
+    ryd_plan_destroy(P);
+    return fail(code, msg);
+  };
+  hipError_t e = hipMalloc(&P->d_vb, sizeof(double) * (size_t)n * n);
+  if (e != hipSuccess) return cleanup(RYD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  e = hipMemcpy(P->d_vb, vb.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return cleanup(RYD_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  const int tbs[2] = {12, 11};
+  for (int k = 0; k < 2; ++k) {
+    const int tb = n < tbs[k] ? n : tbs[k];
+    P->tbits[k] = tb;
+    std::vector<double> elo, ehi;
+    build_tables(n, tb, dl, vb, elo, ehi);
+    e = hipMalloc(&P->d_elo[k], sizeof(double) * elo.size());
+    if (e == hipSuccess) e = hipMalloc(&P->d_ehi[k], sizeof(double) * ehi.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(P->d_elo[k], elo.data(), sizeof(double) * elo.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(P->d_ehi[k], ehi.data(), sizeof(double) * ehi.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return cleanup(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
+                     std::string("table upload: ") + hipGetErrorString(e));
+  }
+  // workspace: counter header + NRED partials for the largest grid we launch
+  size_t blocks = 8192;
+  for (int dt = RYD_F32; dt <= RYD_C128; ++dt) {
+    size_t g = apply_grid(P, dt);
+    if (g > blocks) blocks = g;
+  }
+  P->ws_bytes = WS_HEADER + blocks * NRED * sizeof(double);
+  const size_t occ = (size_t)OCC_BLOCKS * (MAXN_OCC + 1) * sizeof(double);
+  if (WS_HEADER + occ > P->ws_bytes) P->ws_bytes = WS_HEADER + occ;
+  *out = P;
+  return RYD_OK;
+}
+
+int ryd_plan_destroy(ryd_plan* P) {
+  if (!P) return RYD_OK;
+  if (P->d_vb) (void)hipFree(P->d_vb);
+  for (int k = 0; k < 2; ++k) {
+    if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
+    if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
+  }
+  delete P;
+  return RYD_OK;
+}
+
+int64_t ryd_plan_dim(const ryd_plan* P) { return P ? P->dim : 0; }
+
+size_t ryd_workspace_bytes(const ryd_plan* P) { return P ? P->ws_bytes : 0; }
+
+int ryd_hv(const ryd_plan* P, int dtype, const void* x, void* y, void* stream) {
+  if (!P || !x || !y) return fail(RYD_ERR_INVALID, "null plan or vector");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_apply, P, dtype, x, nullptr, y, 1.0, 1.0, 0.0, 0.0, nullptr, nullptr, 0,
+           nullptr, st);
+}
+
+int ryd_apply(const ryd_plan* P, int dtype, const void* x, const void* z, void* y, double s,
+              double ch, double cx, double cz, const uint32_t* probe, double* sums,
+              int accumulate, void* ws, void* stream) {
+  if (!P || !x || !y) return fail(RYD_ERR_INVALID, "null plan or vector");
+  if (sums && !ws) return fail(RYD_ERR_INVALID, "reductions need a workspace");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_apply, P, dtype, x, z, y, s, ch, cx, cz, probe, sums, accumulate, ws, st);
+}
+
+size_t ryd_lanczos_state_bytes(void) { return sizeof(LzState); }
+
+int ryd_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64_t j0,
+                int64_t steps, double* alpha, double* nrm, void* state, double tiny, void* ritz0,
+                void* ritz1, const double* y0, const double* y1, int replay, void* ws,
+                void* stream) {
+  if (!P || !b0 || !b1 || !b2 || !alpha || !nrm || !state || !ws)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_lanczos");
+  if (j0 < 0 || steps < 0) return fail(RYD_ERR_INVALID, "negative step range");
+  if ((ritz0 && !y0) || (ritz1 && !y1)) return fail(RYD_ERR_INVALID, "Ritz vector without coefficients");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_lanczos, P, dtype, b0, b1, b2, j0, steps, alpha, nrm, state, tiny, ritz0,
+           ritz1, y0, y1, replay, ws, st);
+}
+
+int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream) {
+  if (!state || !stopped || !steps) return fail(RYD_ERR_INVALID, "null argument");
+  LzState h;
+  HIP_TRY(hipMemcpyAsync(&h, state, sizeof(LzState), hipMemcpyDeviceToHost,
+                         static_cast<hipStream_t>(stream)));
+  HIP_TRY(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  *stopped = h.stop;
+  *steps = h.steps;
+  return RYD_OK;
+}
+
+int ryd_cheb_moments(const ryd_plan* P, int dtype, const uint32_t* probe, void* t0, void* t1,
+                     void* t2, int64_t nsteps, double c, double r, double* sums, void* ws,
+                     void* stream) {
+  if (!P || !probe || !t0 || !t1 || !t2 || !sums || !ws)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_cheb_moments");
+  if (!(r > 0.0) || !std::isfinite(c)) return fail(RYD_ERR_INVALID, "need finite c and r > 0");
+  if (nsteps < 1) return fail(RYD_ERR_INVALID, "nsteps must be >= 1");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_cheb, P, dtype, probe, t0, t1, t2, nsteps, c, r, sums, ws, st);
+}
+
+}  // extern "C"
+
+namespace {
+template <class V>
+int do_dots(const void* a, const void* b, int64_t n, double* sums, void* ws, hipStream_t st) {
+  Reduce rd = make_reduce(ws, POST_STORE, sums);
+  k_dots<V><<<stream_grid(n), 256, 0, st>>>(static_cast<const V*>(a), static_cast<const V*>(b), n, rd);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ryd_dots(int dtype, const void* a, const void* b, int64_t n, double* sums, void* ws,
+             void* stream) {
+  if (!b || !sums || !ws || n < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_dots");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case RYD_F32: return do_dots<float>(a, b, n, sums, ws, st);
+    case RYD_F64: return do_dots<double>(a, b, n, sums, ws, st);
+    case RYD_C64: return do_dots<c32>(a, b, n, sums, ws, st);
+    case RYD_C128: return do_dots<c64>(a, b, n, sums, ws, st);
+    default: return fail(RYD_ERR_INVALID, "unknown dtype");
+  }
+}
+
+int ryd_cast(int din, const void* x, int dout, void* y, int64_t n, void* stream) {
+  if (!x || !y || n < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_cast");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const bool cin = din >= RYD_C64, cout = dout >= RYD_C64;
+  if (cin != cout) return fail(RYD_ERR_INVALID, "cast cannot change real/complex kind");
+  const int64_t m = cin ? 2 * n : n;
+  const bool in32 = (din == RYD_F32 || din == RYD_C64), out32 = (dout == RYD_F32 || dout == RYD_C64);
+  const unsigned g = stream_grid(m);
+  if (in32 && !out32)
+    k_cast<float, double><<<g, 256, 0, st>>>(static_cast<const float*>(x), static_cast<double*>(y), m);
+  else if (!in32 && out32)
+    k_cast<double, float><<<g, 256, 0, st>>>(static_cast<const double*>(x), static_cast<float*>(y), m);
+  else if (in32)
+    k_cast<float, float><<<g, 256, 0, st>>>(static_cast<const float*>(x), static_cast<float*>(y), m);
+  else
+    k_cast<double, double><<<g, 256, 0, st>>>(static_cast<const double*>(x), static_cast<double*>(y), m);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <class V>
+int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t st) {
+  const uint64_t ob = P->n < OB ? P->n : OB;
+  const uint64_t ntiles = (1ull << P->n) >> ob;
+  const unsigned g = (unsigned)(ntiles < OCC_BLOCKS ? ntiles : OCC_BLOCKS);
+  // partials after the counter header, which must stay zero for reducing calls
+  double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
+  k_site_occ<V><<<g, 256, 0, st>>>(static_cast<const V*>(x), P->n, part);
+  LAUNCH_CHECK();
+  k_occ_finish<<<1, 256, 0, st>>>(part, (int)g, P->n, out);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+}  // namespace
+
+extern "C" int ryd_site_occupation(const ryd_plan* P, int dtype, const void* x, double* out,
+                                   void* ws, void* stream) {
+  if (!P || !x || !out || !ws) return fail(RYD_ERR_INVALID, "null argument to ryd_site_occupation");
+  if (P->basis != RYD_BASIS_FULL)
+    return fail(RYD_ERR_UNSUPPORTED, "site occupation kernel covers the full basis");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case RYD_F32: return do_occ<float>(P, x, out, ws, st);
+    case RYD_F64: return do_occ<double>(P, x, out, ws, st);
+    case RYD_C64: return do_occ<c32>(P, x, out, ws, st);
+    case RYD_C128: return do_occ<c64>(P, x, out, ws, st);
+    default: return fail(RYD_ERR_INVALID, "unknown dtype");
+  }
+}

@@ -1,0 +1,87 @@
+// Streaming vector kernels for the Krylov recurrences (any basis).
+//
+// The Lanczos recurrence keeps unnormalised vectors u_j with norms
+// nrm[j] = |u_j| in device memory, so no step waits on the host:
+//   apply step j : w       = H (u_j / nrm[j]) - (nrm[j] / nrm[j-1]) u_{j-1}
+//                  alpha_j = <q_j, w>                       (fused reduction)
+//   orth step j  : u_{j+1} = w - (alpha_j / nrm[j]) u_j,  nrm[j+1] = |u_{j+1}|
+// beta_j = nrm[j+1].  The replay pass re-runs the same kernels with the
+// stored alpha/nrm and accumulates Ritz vectors x_k += (y_k[j]/nrm[j]) u_j.
+#pragma once
+#include "common.hpp"
+
+namespace ryd {
+
+constexpr int STREAM_THREADS = 256;
+
+template <class V>
+__global__ __launch_bounds__(256) void k_lz_orth(const V* __restrict__ uj, V* __restrict__ w,
+                                                 int64_t D, const double* alpha, const double* nrm,
+                                                 int j, V* ritz0, V* ritz1, const double* yc0,
+                                                 const double* yc1, const int* stop, int reduce,
+                                                 Reduce rd) {
+  using RT = typename VT<V>::R;
+  if (stop && load_coherent(stop)) return;
+  const double inv = 1.0 / nrm[j];
+  const RT coef = (RT)(-alpha[j] * inv);
+  const RT r0c = ritz0 ? (RT)(yc0[j] * inv) : (RT)0;
+  const RT r1c = ritz1 ? (RT)(yc1[j] * inv) : (RT)0;
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    const V u = uj[i];
+    V v = w[i];
+    vfma(v, coef, u);
+    w[i] = v;
+    acc += nrm2(v);
+    if (ritz0) { V a = ritz0[i]; vfma(a, r0c, u); ritz0[i] = a; }
+    if (ritz1) { V a = ritz1[i]; vfma(a, r1c, u); ritz1[i] = a; }
+  }
+  if (reduce) block_reduce_finish(0.0, acc, 0.0, 0.0, rd);
+}
+
+// sums: <a, b>, |b|^2, |a|^2 ; used for norms and Rayleigh quotients
+template <class V>
+__global__ __launch_bounds__(256) void k_dots(const V* __restrict__ a, const V* __restrict__ b,
+                                              int64_t D, Reduce rd) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    const V bv = b[i];
+    s1 += nrm2(bv);
+    if (a) { const V av = a[i]; s0 += rdot(av, bv); s2 += nrm2(av); }
+  }
+  block_reduce_finish(s0, s1, s2, 0.0, rd);
+}
+
+// y = a * x + b * y  (real a, b); x may be null (scale only)
+template <class V>
+__global__ __launch_bounds__(256) void k_axpby(const V* __restrict__ x, V* __restrict__ y,
+                                               int64_t D, double a, double b) {
+  using RT = typename VT<V>::R;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    V v = vscale(y[i], (RT)b);
+    if (x) vfma(v, (RT)a, x[i]);
+    y[i] = v;
+  }
+}
+
+// precision casts between state-vector types of the same kind
+template <class A, class B>
+__global__ __launch_bounds__(256) void k_cast(const A* __restrict__ x, B* __restrict__ y, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    y[i] = (B)x[i];
+}
+
+// probe bits -> ±1 vector (real part for complex element types)
+template <class V>
+__global__ __launch_bounds__(256) void k_probe_expand(const uint32_t* __restrict__ bits,
+                                                      V* __restrict__ y, int64_t D) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride)
+    y[i] = vfrom_real<V>(((bits[i >> 5] >> (i & 31)) & 1u) ? 1.0 : -1.0);
+}
+
+}  // namespace ryd

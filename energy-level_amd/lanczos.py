@@ -1,0 +1,305 @@
+"""Matrix-free Lanczos ground state on the HIP engine.
+
+Drop-in for ``torch_lanczos_ground_state``
+(tomisnot/energy-level src/core/manybody/matrix_free.py:118-286): same
+arguments, same result keys, backend ``'hip-matrixfree-lanczos'``.
+
+Algorithm (not the reference's block Krylov; only converged outputs are
+compared):
+
+* the three-term recurrence runs on the device in cycles of ``m`` steps with
+  FP64 alpha/beta and no host synchronisation inside a cycle
+  (``ryd_lanczos``); the recurrence is continued, not restarted, between
+  cycles, up to ``m·(max_restarts+1)`` steps;
+* after each cycle the FP64 tridiagonal matrix is diagonalised on the host;
+  duplicate and Cullum–Willoughby spurious Ritz values are dropped; the two
+  lowest genuine ones are tested with the bound |beta_k · y_k[-1]|;
+* Ritz vectors come from a replay of the recurrence with the stored
+  coefficients (no Krylov basis is kept: memory is 6 vectors);
+* FP32 bulk (``dtype='float32'``/``'complex64'``, ``refine=True``): the FP32
+  Ritz pair seeds a short FP64 recurrence, so E0 and the residuals are
+  certified in FP64 (``precision`` in the result says which parts ran in
+  which precision);
+* each returned energy is the FP64 Rayleigh quotient of its Ritz vector and
+  ``residuals`` are the explicit |H x − E x| of the normalised vectors
+  (certification rule of matrix_free.py:258-266: residual <= tol·max(1,|E|)).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as nat
+from .model import from_calib
+from .plan import Plan, cast
+
+__all__ = ['hip_lanczos_ground_state', 'lanczos_run', 'ritz_vectors', 'rayleigh_residual',
+           'site_occupations']
+
+
+def _tridiag(a, b):
+    from scipy.linalg import eigh_tridiagonal
+    if len(a) == 1:
+        return np.array([a[0]]), np.ones((1, 1))
+    return eigh_tridiagonal(a, b)
+
+
+def _genuine_lowest(a, b, theta, want=2):
+    """Indices (into theta) of the ``want`` lowest distinct, non-spurious Ritz values.
+
+    Duplicates (ghost copies of converged values) merge; a simple value that
+    is also an eigenvalue of T with its first row/column removed is spurious
+    (Cullum–Willoughby)."""
+    from scipy.linalg import eigvalsh_tridiagonal
+    k = len(a)
+    scale = max(1.0, float(np.max(np.abs(theta))))
+    tol = max(1e-10 * scale, 100.0 * k * np.finfo(float).eps * scale)
+    if k > 2:
+        that = eigvalsh_tridiagonal(a[1:], b[1:])
+    elif k == 2:
+        that = a[1:]
+    else:
+        that = np.zeros(0)
+    order = np.argsort(theta)
+    kept = []
+    for i in order:
+        th = theta[i]
+        if kept and abs(th - theta[kept[-1]]) <= tol:
+            continue
+        mult = int(np.sum(np.abs(theta - th) <= tol))
+        if mult == 1 and that.size and np.min(np.abs(that - th)) <= tol:
+            continue
+        kept.append(int(i))
+        if len(kept) == want:
+            break
+    return kept
+
+
+class _Run:
+    pass
+
+
+def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None):
+    """Cycles of ``m`` device-side Lanczos steps from ``v0`` until the ``want``
+    lowest genuine Ritz values pass the bound test or the budget is spent."""
+    import torch
+    L = nat.lib()
+    dim = plan.dim
+    dt = v0.dtype
+    kmax = int(min(max(1, m) * max(1, max_cycles), dim))
+    R = _Run()
+    R.bufs = [torch.empty(dim, dtype=dt, device=plan.device) for _ in range(3)]
+    R.bufs[0].copy_(v0)
+    R.alpha = torch.zeros(kmax + 1, dtype=torch.float64, device=plan.device)
+    R.nrm = torch.zeros(kmax + 2, dtype=torch.float64, device=plan.device)
+    R.state = torch.zeros(int(L.ryd_lanczos_state_bytes()), dtype=torch.uint8, device=plan.device)
+    code = nat.dtype_code(dt)
+    ws = plan.workspace()
+    st = nat.stream_ptr()
+    if tiny is None:
+        tiny = 1e-13 if dt in (torch.float64, torch.complex128) else 1e-6
+    j, cycles = 0, 0
+    stopped = False
+    while True:
+        steps = int(min(m, kmax - j))
+        # the breakdown threshold is relative to the start-vector norm
+        nat.check(L.ryd_lanczos(plan.handle, code, nat.ptr(R.bufs[0]), nat.ptr(R.bufs[1]),
+                                nat.ptr(R.bufs[2]), j, steps, nat.ptr(R.alpha), nat.ptr(R.nrm),
+                                nat.ptr(R.state), float(tiny), None, None, None, None, 0,
+                                nat.ptr(ws), st))
+        cycles += 1
+        import ctypes
+        sflag, nvalid = ctypes.c_int(0), ctypes.c_int64(0)
+        nat.check(L.ryd_lanczos_status(nat.ptr(R.state), ctypes.byref(sflag),
+                                       ctypes.byref(nvalid), st))
+        if sflag.value:
+            stopped = True
+            j = int(nvalid.value)
+        else:
+            j += steps
+        if j == 0:
+            raise ValueError('Lanczos start vector is zero')
+        a = R.alpha[:j].cpu().numpy()
+        nr = R.nrm[:j + 1].cpu().numpy()
+        b = nr[1:j]
+        theta, Y = _tridiag(a, b)
+        picks = _genuine_lowest(a, b, theta, want)
+        blast = 0.0 if stopped else float(nr[j])
+        est = [abs(blast * Y[-1, i]) for i in picks]
+        ok = all(e <= tol * max(1.0, abs(theta[i])) for e, i in zip(est, picks))
+        if ok or stopped or j >= kmax:
+            break
+    R.j, R.theta, R.Y, R.picks, R.est, R.stopped, R.cycles = j, theta, Y, picks, est, stopped, cycles
+    R.a, R.b = a, b
+    R.v0 = v0
+    return R
+
+
+def ritz_vectors(plan, R, which=None):
+    """Replay the recurrence of ``R`` and accumulate up to two Ritz vectors."""
+    import torch
+    L = nat.lib()
+    which = R.picks if which is None else which
+    which = list(which)[:2]
+    dt = R.v0.dtype
+    R.bufs[0].copy_(R.v0)
+    out = [torch.zeros(plan.dim, dtype=dt, device=plan.device) for _ in which]
+    ys = [torch.as_tensor(np.ascontiguousarray(R.Y[:, i]), dtype=torch.float64,
+                          device=plan.device) for i in which]
+    nat.check(L.ryd_lanczos(plan.handle, nat.dtype_code(dt), nat.ptr(R.bufs[0]),
+                            nat.ptr(R.bufs[1]), nat.ptr(R.bufs[2]), 0, R.j, nat.ptr(R.alpha),
+                            nat.ptr(R.nrm), nat.ptr(R.state), 0.0,
+                            nat.ptr(out[0]) if len(out) > 0 else None,
+                            nat.ptr(out[1]) if len(out) > 1 else None,
+                            nat.ptr(ys[0]) if len(ys) > 0 else None,
+                            nat.ptr(ys[1]) if len(ys) > 1 else None, 1,
+                            nat.ptr(plan.workspace()), nat.stream_ptr()))
+    return out
+
+
+def rayleigh_residual(plan, x, theta=0.0):
+    """(E, |H x̂ − E x̂|) for x̂ = x/|x|, E the Rayleigh quotient, in x's precision
+    with FP64 sums."""
+    import torch
+    sums = torch.zeros(4, dtype=torch.float64, device=plan.device)
+    plan.dots(None, x, sums)
+    nx = float(np.sqrt(sums[1].item()))
+    if not nx > 0:
+        raise ValueError('zero vector')
+    y = torch.empty_like(x)
+    plan.apply(x, y, s=1.0 / nx, ch=1.0, cx=-float(theta), sums=sums)
+    r0, r1, r2 = (float(v) for v in sums[:3].cpu().numpy())
+    shift = r0 / r2
+    res = float(np.sqrt(max(r1 / r2 - shift * shift, 0.0)))
+    return float(theta) + shift, res
+
+
+def site_occupations(plan, x):
+    """<n_i> per site for a full-basis state (normalised inside)."""
+    import torch
+    out = torch.zeros(plan.n + 1, dtype=torch.float64, device=plan.device)
+    nat.check(nat.lib().ryd_site_occupation(plan.handle, nat.dtype_code(x.dtype), nat.ptr(x),
+                                            nat.ptr(out), nat.ptr(plan.workspace()),
+                                            nat.stream_ptr()))
+    o = out.cpu().numpy()
+    return o[:plan.n] / o[plan.n]
+
+
+def _combine(vecs, dtype):
+    """x0/|x0| + x1/|x1| in ``dtype`` (the restart / refinement seed)."""
+    import torch
+    start = None
+    for v in vecs:
+        v = v if v.dtype == dtype else cast(v, dtype)
+        nv = torch.linalg.vector_norm(v)
+        start = v / nv if start is None else start + v / nv
+    return start
+
+
+def _certify(P, vecs, R, dtype64):
+    evals, res = [], []
+    for v, i in zip(vecs, R.picks):
+        vv = v if v.dtype == dtype64 else cast(v, dtype64)
+        e, r = rayleigh_residual(P, vv, R.theta[i])
+        evals.append(e)
+        res.append(r)
+    return np.asarray(evals), np.asarray(res)
+
+
+def _solve(P, start, m, cycles, tol, want, stats, key, dtype64):
+    """Lanczos runs with explicit restarts from the Ritz pair until the explicit
+    residuals of the ``want`` lowest Ritz pairs pass ``tol``·max(1,|E|)."""
+    for attempt in range(cycles):
+        R = lanczos_run(P, start, m, cycles, tol, want=want)
+        vecs = ritz_vectors(P, R)
+        stats[key] += R.j
+        stats['restarts'] += R.cycles - 1 + (1 if attempt else 0)
+        evals, res = _certify(P, vecs, R, dtype64)
+        if len(evals) == want and np.all(res <= tol * np.maximum(1.0, np.abs(evals))):
+            break
+        start = _combine(vecs, start.dtype)
+    return vecs, R, evals, res
+
+
+def _start_vector(dim, seed, complex_kind, device):
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    v = torch.randn(dim, dtype=torch.float64, device=device, generator=gen)
+    if complex_kind:
+        v = v.to(torch.complex128) + 1j * torch.randn(dim, dtype=torch.float64, device=device,
+                                                       generator=gen).to(torch.complex128)
+    return v
+
+
+def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
+                             basis_indices=None, m=48, seed=0, dtype='float64', device='cuda',
+                             tol=None, max_restarts=8, return_state=False,
+                             memory_budget_mb=None, refine=True, plan=None):
+    """Ground state and gap with residual certification (see module docstring).
+
+    Keys as ``torch_lanczos_ground_state``: E0, gap, evals, residuals,
+    converged, backend, dim, krylov_dim, restarts, tolerance[, warning][, vec];
+    plus ``precision`` and ``n_atom`` (<n_i> of the ground state).
+    """
+    import torch
+    if str(device).startswith('cpu'):
+        raise RuntimeError('HIP device unavailable')
+    nat.require_gpu()
+    if dtype not in ('float64', 'complex128', 'float32', 'complex64'):
+        raise ValueError('dtype must be float32/float64/complex64/complex128')
+    if int(m) < 2 or int(max_restarts) < 0:
+        raise ValueError('m must be >= 2 and max_restarts >= 0')
+    if basis_indices is not None:
+        raise NotImplementedError('restricted bases go through hip_sector_ground_state')
+    model = from_calib(calib, Om, delta, gamma, terms)
+    P = plan if plan is not None else Plan(model, device=device)
+    dim = P.dim
+    low = dtype in ('float32', 'complex64')
+    complex_kind = (not P.is_real) or dtype.startswith('complex')
+    itemsize = (8 if low else 16) if complex_kind else (4 if low else 8)
+    tolerance = float(tol if tol is not None else (1e-9 if not low or refine else 2e-5))
+    if tolerance <= 0 or not np.isfinite(tolerance):
+        raise ValueError('tol must be positive and finite')
+    required = dim * itemsize * 7 + (dim * 16 * 7 if low and refine else 0)
+    if memory_budget_mb is not None and required > float(memory_budget_mb) * 1024 ** 2:
+        raise MemoryError('HIP Lanczos memory estimate exceeds memory_budget_mb')
+    want = 2 if dim > 1 else 1
+    tdt64 = torch.complex128 if complex_kind else torch.float64
+    tdt32 = torch.complex64 if complex_kind else torch.float32
+    work = tdt32 if low else tdt64
+    v0 = _start_vector(dim, seed, complex_kind, P.device)
+    cycles = int(max_restarts) + 1
+    bulk_tol = 2e-5 if low else tolerance
+    stats = dict(steps_bulk=0, steps_fp64=0, restarts=0)
+    vecs, R, evals, res = _solve(P, v0.to(work), int(m), cycles, bulk_tol, want, stats,
+                                 'steps_bulk', tdt64)
+    if low and refine:
+        # FP64 refinement seeded by the FP32 Ritz pair
+        start = _combine(vecs, tdt64)
+        vecs, R, evals, res = _solve(P, start, int(m), cycles, tolerance, want, stats,
+                                     'steps_fp64', tdt64)
+    steps_bulk, steps64, restarts = stats['steps_bulk'], stats['steps_fp64'], stats['restarts']
+    evals = np.asarray(evals)
+    res = np.asarray(res)
+    count = len(evals)
+    flags = res <= tolerance * np.maximum(1.0, np.abs(evals))
+    converged = bool(count == want and np.all(flags))
+    if R.stopped and count == want:
+        converged = converged or bool(np.all(res <= tolerance * np.maximum(1.0, np.abs(evals))))
+    out = dict(E0=float(evals[0]),
+               gap=float(evals[1] - evals[0]) if converged and count > 1 else float('nan'),
+               evals=evals, residuals=res, converged=converged,
+               backend='hip-matrixfree-lanczos', dim=dim, krylov_dim=int(steps_bulk + steps64),
+               restarts=int(restarts), tolerance=tolerance,
+               precision=dict(bulk='float32' if low else 'float64',
+                              alpha_beta='float64', tridiagonal='float64',
+                              refinement='float64' if (not low or refine) else None,
+                              steps_bulk=int(steps_bulk), steps_fp64=int(steps64)))
+    if not converged:
+        out['warning'] = 'lowest eigenpairs did not all reach the residual tolerance; gap not reported'
+    x0 = vecs[0]
+    out['n_atom'] = site_occupations(P, x0)
+    if return_state:
+        x = x0 / torch.linalg.vector_norm(x0)
+        out['vec'] = x.cpu().numpy().astype(np.complex128)
+    return out

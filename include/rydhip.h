@@ -1,0 +1,149 @@
+/*
+ * rydhip — C ABI of the MI355X (gfx950) matrix-free Rydberg Hamiltonian engine.
+ *
+ *   H = Σ_i Ω_i/2 σx_i + Σ_i delta_i n_i + Σ_{i<j} V_ij n_i n_j
+ *
+ * (reference sign convention: the diagonal carries +delta_i n_i, so the
+ * north-star −Δ_i n_i is delta = −Δ; complex Ω: g→r takes Ω/2, r→g conj(Ω)/2).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * status code (RYD_OK = 0) and never throws.  The caller allocates and owns
+ * every state vector, workspace and result buffer (device memory); a plan
+ * owns only its immutable device tables, so one plan may be used from
+ * several threads.  Calls that take a workspace must not share that
+ * workspace between concurrently running calls.  `stream` is a hipStream_t
+ * (NULL = default stream).  After a failure, ryd_last_error() returns the
+ * message of the calling thread's last error.
+ *
+ * Element types (dtype): RYD_F32 float, RYD_F64 double, RYD_C64 complex64,
+ * RYD_C128 complex128 (interleaved re, im).
+ *
+ * Bit order: site i is bit (n-1-i) of a basis code (site 0 = MSB), bit 1 = |r>.
+ */
+#ifndef RYDHIP_H
+#define RYDHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RYD_ABI_VERSION 1
+
+enum ryd_status {
+  RYD_OK = 0,
+  RYD_ERR_INVALID = 1,     /* bad argument (maps to ValueError)          */
+  RYD_ERR_MEMORY = 2,      /* device allocation failed (MemoryError)     */
+  RYD_ERR_DEVICE = 3,      /* HIP runtime / no device (RuntimeError)     */
+  RYD_ERR_UNSUPPORTED = 4  /* valid request this build does not support  */
+};
+
+enum ryd_dtype { RYD_F32 = 0, RYD_F64 = 1, RYD_C64 = 2, RYD_C128 = 3 };
+
+enum ryd_basis_kind {
+  RYD_BASIS_FULL = 0,       /* all 2^n codes                                */
+  RYD_BASIS_CODES = 1,      /* explicit ascending int64 codes               */
+  RYD_BASIS_CHAIN_FIB = 2   /* chain nearest-neighbour blockade sector      */
+};
+
+/* Model description (host memory, copied at plan creation). */
+typedef struct ryd_model {
+  int32_t n;               /* number of sites, 1..62                        */
+  const double* om_re;     /* [n] Re Ω_i (MHz)                              */
+  const double* om_im;     /* [n] Im Ω_i (MHz); NULL = real drive           */
+  const double* delta;     /* [n] diagonal coefficient of n_i (MHz)         */
+  int32_t n_pairs;
+  const int32_t* pair_i;   /* [n_pairs] site i                               */
+  const int32_t* pair_j;   /* [n_pairs] site j                               */
+  const double* pair_v;    /* [n_pairs] V_ij (MHz), duplicates are summed    */
+  int32_t basis_kind;      /* ryd_basis_kind                                 */
+  const int64_t* codes;    /* RYD_BASIS_CODES: ascending unique codes (host) */
+  int64_t n_codes;
+} ryd_model;
+
+typedef struct ryd_plan ryd_plan;
+
+/* Library ABI version (RYD_ABI_VERSION).  No reference counterpart. */
+int ryd_abi_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated). */
+int ryd_last_error(char* buf, size_t size);
+
+/* Builds the device tables of a model on the current device.
+ * Replaces the per-call channel enumeration and transition tables of the
+ * reference: src/core/manybody/channels.py:91-132 (channel_spec),
+ * :135-175 (ChannelKernel.__init__) and matrix_free.py:65-115
+ * (_channel_arrays / _group_channels, uploaded at matrix_free.py:185-198). */
+int ryd_plan_create(const ryd_model* model, ryd_plan** out);
+int ryd_plan_destroy(ryd_plan* plan);
+
+/* Basis dimension of a plan (0 on a NULL plan). */
+int64_t ryd_plan_dim(const ryd_plan* plan);
+
+/* Bytes of workspace any single reducing call on this plan needs.  The
+ * workspace must be zero-filled once after allocation; calls leave it zeroed. */
+size_t ryd_workspace_bytes(const ryd_plan* plan);
+
+/* y = H x.  Replaces ChannelKernel.matvec (channels.py:210-221) and the GPU
+ * action() of torch_lanczos_ground_state (matrix_free.py:200-204). */
+int ryd_hv(const ryd_plan* plan, int dtype, const void* x, void* y, void* stream);
+
+/* Fused update y = ch·H(s·x) + cx·(s·x) + cz·z (z may be NULL), with FP64
+ * reductions {Re<s·x,y>, |y|^2, |s·x|^2, Σ_e p_e Re(y_e)} (p_e = ±1 from the
+ * optional bit-packed probe, bit set = +1) stored to sums[4] (accumulate = 0)
+ * or added to them (accumulate = 1).  sums may be NULL (no reductions).
+ * Covers the Chebyshev step of kpm.py:146-148 / :69-72. */
+int ryd_apply(const ryd_plan* plan, int dtype, const void* x, const void* z, void* y,
+              double s, double ch, double cx, double cz, const uint32_t* probe,
+              double* sums, int accumulate, void* ws, void* stream);
+
+/* Three-term Lanczos recurrence, steps j0 .. j0+steps-1, entirely on the
+ * device (no host synchronisation per step).  Replaces the Krylov build loop
+ * of torch_lanczos_ground_state (matrix_free.py:231-251) and the recurrence of
+ * spectrum.lanczos (spectrum.py:281-300).
+ *   b0, b1, b2 : three vectors of dim elements; at j0 = 0, b0 holds the start
+ *                vector (any norm).  Vector u_j lives in b[j % 3].
+ *   alpha, nrm : device FP64 arrays (>= j0+steps and >= j0+steps+1 entries);
+ *                alpha_j = alpha[j], beta_j = nrm[j+1], nrm[0] = |start|.
+ *   state      : device buffer of ryd_lanczos_state_bytes(); stop flag and
+ *                step count after a breakdown (|u_{j+1}| <= tiny).
+ *   replay = 1 : recompute the same vectors from stored alpha/nrm and add
+ *                Σ_j yk[j]/nrm[j]·u_j to ritz0/ritz1 (either may be NULL);
+ *                b0 must again hold the start vector at j0 = 0. */
+int ryd_lanczos(const ryd_plan* plan, int dtype, void* b0, void* b1, void* b2, int64_t j0,
+                int64_t steps, double* alpha, double* nrm, void* state, double tiny,
+                void* ritz0, void* ritz1, const double* y0, const double* y1, int replay,
+                void* ws, void* stream);
+size_t ryd_lanczos_state_bytes(void);
+/* Reads the state written by ryd_lanczos (synchronises the stream). */
+int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream);
+
+/* Chebyshev recurrence of one Rademacher probe on H~ = (H - c)/r:
+ * t_0 = p, t_1 = H~ t_0, t_k = 2 H~ t_{k-1} - t_{k-2}, k <= nsteps.
+ * For each k >= 1 adds {<t_{k-1},t_k>, |t_k|^2, |t_{k-1}|^2, <p,t_k>} to
+ * sums[4k .. 4k+3] (device FP64, (nsteps+1)*4 entries).  Replaces the probe
+ * loop of kpm.kpm_moments (kpm.py:139-149) and kpm_moments_gpu (kpm.py:62-73). */
+int ryd_cheb_moments(const ryd_plan* plan, int dtype, const uint32_t* probe, void* t0,
+                     void* t1, void* t2, int64_t nsteps, double c, double r, double* sums,
+                     void* ws, void* stream);
+
+/* out[i] = Σ_s |x_s|^2 b_i(s) for every site i, out[n] = Σ_s |x_s|^2 (device FP64,
+ * n+1 entries; divide by out[n] for <n_i>).  Replaces _gs_observables' n_atom
+ * (eq_route.py:108-122) and the bits @ |psi|^2 reduction of
+ * engine_ed._reduce (engine_ed.py:261-263).  Full basis. */
+int ryd_site_occupation(const ryd_plan* plan, int dtype, const void* x, double* out, void* ws,
+                        void* stream);
+
+/* sums[0..2] = {Re<a,b>, |b|^2, |a|^2} over n elements (a may be NULL). */
+int ryd_dots(int dtype, const void* a, const void* b, int64_t n, double* sums, void* ws,
+             void* stream);
+
+/* Element-wise cast between F32<->F64 or C64<->C128 (n elements). */
+int ryd_cast(int dtype_in, const void* x, int dtype_out, void* y, int64_t n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RYDHIP_H */

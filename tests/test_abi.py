@@ -1,0 +1,45 @@
+"""The C-ABI library loads and exports every symbol include/rydhip.h declares
+(no compute call: runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, 'include', 'rydhip.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:int|int64_t|size_t)\s+(ryd_\w+)\s*\(', text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ('ryd_plan_create', 'ryd_hv', 'ryd_lanczos', 'ryd_cheb_moments'):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    from energy_level_amd import _native as nat
+    assert os.path.exists(nat.LIB_PATH), 'build() first'
+    L = ctypes.CDLL(nat.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+    # the binding table covers the header too
+    assert set(_declared()) <= set(nat.SIGNATURES)
+
+
+def test_abi_version_and_error_path(pkg):
+    from energy_level_amd import _native as nat
+    L = nat.lib()
+    assert L.ryd_abi_version() == 1
+    # invalid model is rejected before any device work
+    handle = ctypes.c_void_p()
+    m = nat.Model()
+    m.n = 0
+    rc = L.ryd_plan_create(ctypes.byref(m), ctypes.byref(handle))
+    assert rc == nat.RYD_ERR_INVALID
+    assert 'n must be' in nat.last_error()
+    with pytest.raises(ValueError):
+        nat.check(rc)

@@ -1,0 +1,130 @@
+"""GPU Lanczos ground state and KPM moments against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import rydberg_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _chain_calib(pkg, n, c6=10.0):
+    return pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6))
+
+
+def test_c1_chain12_ground_state_fp64(pkg, dev):
+    """Plain Lanczos ground state, chain N=12, Ω=1, Δ=1.2 (delta=-1.2), C6=10."""
+    n = 12
+    calib = _chain_calib(pkg, n)
+    Om, delta = np.ones(n), -1.2 * np.ones(n)
+    H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'])
+    ref = orc.ground_state_dense(H)
+    r = pkg.hip_lanczos_ground_state(calib, Om, delta, seed=0, return_state=True)
+    assert r['converged']
+    assert abs(r['E0'] - ref['E0']) <= 1e-10 * abs(ref['E0'])
+    assert abs(r['gap'] - ref['gap']) <= 1e-8 * max(1.0, abs(ref['gap']))
+    assert np.all(r['residuals'] <= 1e-9 * np.maximum(1, np.abs(r['evals'])))
+    n_ref = orc.site_occupations(n, ref['vec'])
+    assert np.max(np.abs(r['n_atom'] - n_ref)) < 1e-8
+    ov = abs(np.vdot(ref['vec'], r['vec']))
+    assert abs(ov - 1.0) < 1e-8
+    assert r['backend'] == 'hip-matrixfree-lanczos' and r['dim'] == 4096
+    for key in ('E0', 'gap', 'evals', 'residuals', 'converged', 'backend', 'dim', 'krylov_dim',
+                'restarts', 'tolerance', 'vec'):
+        assert key in r
+
+
+def test_fp32_bulk_with_fp64_refinement(pkg, dev):
+    n = 14
+    calib = _chain_calib(pkg, n)
+    Om, delta = np.ones(n), -1.2 * np.ones(n)
+    H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'])
+    e0 = orc.ground_state_eigsh(H)['E0']
+    r = pkg.hip_lanczos_ground_state(calib, Om, delta, dtype='float32', seed=1)
+    assert r['converged'] and r['precision']['bulk'] == 'float32'
+    assert r['precision']['steps_fp64'] > 0
+    assert abs(r['E0'] - e0) <= 1e-10 * abs(e0)
+
+
+def test_complex_drive_ground_state(pkg, dev):
+    n = 8
+    rng = np.random.default_rng(4)
+    calib = _chain_calib(pkg, n)
+    Om = rng.uniform(0.5, 1.5, n) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    delta = rng.normal(size=n)
+    H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'])
+    ref = orc.ground_state_dense(H)
+    r = pkg.hip_lanczos_ground_state(calib, Om, delta, dtype='complex128')
+    assert r['converged'] and abs(r['E0'] - ref['E0']) <= 1e-10 * abs(ref['E0'])
+
+
+def test_tiny_dimensions_and_breakdown(pkg, dev):
+    # D = 2 and D = 16: the recurrence breaks down (invariant subspace) early
+    for n in (1, 4):
+        calib = _chain_calib(pkg, n) if n > 1 else {'atoms': [{}], 'pairs': []}
+        Om, delta = 0.8 * np.ones(n), 0.3 * np.ones(n)
+        H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'])
+        ref = orc.ground_state_dense(H)
+        r = pkg.hip_lanczos_ground_state(calib, Om, delta, m=8)
+        assert abs(r['E0'] - ref['E0']) <= 1e-10 * max(1, abs(ref['E0']))
+        assert r['converged']
+
+
+def test_refusals(pkg, dev):
+    calib = _chain_calib(pkg, 4)
+    with pytest.raises(ValueError):
+        pkg.hip_lanczos_ground_state(calib, np.ones(4), np.zeros(4), gamma=[0.1, 0, 0, 0])
+    with pytest.raises(ValueError):
+        pkg.hip_lanczos_ground_state(calib, np.ones(4), np.zeros(4), dtype='float16')
+    with pytest.raises(ValueError):
+        pkg.hip_lanczos_ground_state(calib, np.ones(4), np.zeros(4), m=1)
+    with pytest.raises(MemoryError):
+        pkg.hip_lanczos_ground_state(calib, np.ones(4), np.zeros(4), memory_budget_mb=1e-6)
+    with pytest.raises(RuntimeError):
+        pkg.hip_lanczos_ground_state(calib, np.ones(4), np.zeros(4), device='cpu')
+
+
+@pytest.mark.parametrize('dtype,tol', [('float64', 1e-10), ('float32', 2e-4)])
+def test_kpm_diagonal_known_answer(pkg, dev, dtype, tol):
+    """Ω = 0 makes H diagonal: Hutchinson moments are exact for any probe."""
+    n = 10
+    delta = np.linspace(-1.0, 2.0, n)
+    calib = _chain_calib(pkg, n)
+    P = pkg.Plan(pkg.from_calib(calib, np.zeros(n), delta), device=dev)
+    H = orc.build_H_sparse(n, np.zeros(n), delta, calib['pairs'], ['I1'])
+    d = H.diagonal()
+    c, r = 0.5 * (d.min() + d.max()), 0.5 * (d.max() - d.min()) * 1.001 + 1e-12
+    x = (d - c) / r
+    ref = np.array([np.mean(np.cos(k * np.arccos(x))) for k in range(25)])
+    for doubling in (False, True):
+        mu, _, _ = pkg.kpm_moments_hip(plan=P, num=24, nvec=2, seed=1, dtype=dtype, c=c, r=r,
+                                       doubling=doubling)
+        assert np.max(np.abs(mu - ref)) < tol, (doubling, np.max(np.abs(mu - ref)))
+
+
+@pytest.mark.parametrize('doubling', [False, True])
+def test_kpm_moments_vs_reference_estimator(pkg, dev, doubling):
+    """Same seed ⇒ same Rademacher probes as the reference CPU loop."""
+    n = 10
+    calib = _chain_calib(pkg, n, c6=20.0)
+    Om, delta = np.ones(n), -0.8 * np.ones(n)
+    P = pkg.Plan(pkg.from_calib(calib, Om, delta), device=dev)
+    H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'])
+    emin, emax, c, r = orc.kpm_bounds(H, margin=0.01)
+    mref, _, _ = orc.kpm_moments(H, 40, 3, seed=7, c=c, r=r)
+    mu, _, _ = pkg.kpm_moments_hip(plan=P, num=40, nvec=3, seed=7, c=c, r=r, doubling=doubling)
+    assert np.max(np.abs(mu - mref)) < 1e-10
+    mu32, _, _ = pkg.kpm_moments_hip(plan=P, num=40, nvec=3, seed=7, c=c, r=r,
+                                     doubling=doubling, dtype='float32')
+    assert np.max(np.abs(mu32 - mref)) < 2e-4
+
+
+def test_kpm_bounds_cover_spectrum(pkg, dev):
+    n = 11
+    calib = _chain_calib(pkg, n)
+    Om, delta = np.ones(n), -1.0 * np.ones(n)
+    P = pkg.Plan(pkg.from_calib(calib, Om, delta), device=dev)
+    w = np.linalg.eigvalsh(orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1']).toarray())
+    emin, emax, c, r = pkg.spectral_bounds(P, margin=0.01)
+    assert emin <= w[0] + 1e-9 and emax >= w[-1] - 1e-9
+    assert abs(emin - w[0]) < 1e-6 and abs(emax - w[-1]) < 1e-6
+    assert c - r <= w[0] and c + r >= w[-1]

@@ -1,0 +1,200 @@
+"""Pins the CPU oracle against closed forms, independent constructions and the
+reference's own known answers (CPU only)."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import rydberg_oracle as orc
+from oracle import hv_ref
+
+
+def _random_model(n, seed, complex_terms=True):
+    rng = np.random.default_rng(seed)
+    Om = rng.uniform(0.3, 1.5, n) * (np.exp(1j * rng.uniform(0, 2 * np.pi, n)) if complex_terms else 1)
+    delta = rng.normal(size=n)
+    pairs = []
+    for i in range(n):
+        for j in range(i + 1, n):
+            p = {'i': i, 'j': j, 'V': float(rng.normal())}
+            if complex_terms:
+                p['J'] = complex(rng.normal(), rng.normal())
+                p['A'] = complex(rng.normal(), rng.normal())
+            pairs.append(p)
+    return Om, delta, pairs
+
+
+@pytest.mark.parametrize('n', [1, 2, 3, 4, 5])
+def test_channels_equal_kron_construction(n):
+    # reference checks/equibench_engine_ed_selfcheck.py:128-163 (E1): engine ≡ textbook kron
+    Om, delta, pairs = _random_model(n, 100 + n)
+    terms = ['I1', 'I2', 'I3']
+    Hk = orc.kron_hamiltonian(n, Om, delta, pairs, terms)
+    Hs = orc.build_H_sparse(n, Om, delta, pairs, terms).toarray()
+    assert np.max(np.abs(Hk - Hs)) < 1e-12
+    v = np.random.default_rng(n).normal(size=1 << n) + 1j * np.random.default_rng(n + 9).normal(size=1 << n)
+    assert np.max(np.abs(orc.channel_matvec(n, Om, delta, pairs, terms, v) - Hk @ v)) < 1e-12
+
+
+def test_hand_written_two_site_matrix():
+    # reference checks/equibench_engine_ed_selfcheck.py:245-264 (A3)
+    O0, O1, d0, d1 = 3.0, 5.0, 1.0, -2.0
+    V, J, A = 30.0, -1.2 + 0.7j, 0.9 - 1.3j
+    M = np.zeros((4, 4), complex)
+    M[1, 1], M[2, 2], M[3, 3] = d1, d0, d0 + d1 + V
+    M[0, 1] = M[1, 0] = M[2, 3] = M[3, 2] = O1 / 2
+    M[0, 2] = M[2, 0] = M[1, 3] = M[3, 1] = O0 / 2
+    M[2, 1], M[1, 2] = J, np.conj(J)
+    M[3, 0], M[0, 3] = A, np.conj(A)
+    pairs = [{'i': 0, 'j': 1, 'V': V, 'J': J, 'A': A}]
+    H = orc.build_H_sparse(2, [O0, O1], [d0, d1], pairs, ['I1', 'I2', 'I3']).toarray()
+    assert np.max(np.abs(H - M)) < 1e-12
+
+
+def test_xx_dimer_spectrum():
+    # reference checks/equibench_engine_ed_selfcheck.py:196-208 (A1)
+    J = 1.7
+    H = orc.build_H_sparse(2, [0, 0], [0, 0], [{'i': 0, 'j': 1, 'V': 0.0, 'J': J, 'A': 0.0}],
+                           ['I1', 'I2', 'I3']).toarray()
+    assert np.allclose(np.sort(np.linalg.eigvalsh(H)), [-J, 0, 0, J], atol=1e-10)
+
+
+def test_dtype_derivation():
+    # reference channels.py:41-56
+    assert orc.model_dtype([1.0, 2.0], [{'i': 0, 'j': 1, 'V': 1.0}], ['I1']) == np.float64
+    assert orc.model_dtype([1.0j, 2.0], [], ['I1']) == np.complex128
+    assert orc.model_dtype([1.0, 2.0], [{'i': 0, 'j': 1, 'J': 1j}], ['I2']) == np.complex128
+    assert orc.model_dtype([1.0, 2.0], [], [], gamma=[0.1, 0]) == np.complex128
+
+
+def test_restricted_basis_matvec_unsorted():
+    # reference checks/manybody_channel_selfcheck.py:26-123: restricted and unsorted bases
+    n = 3
+    Om, delta, pairs = _random_model(n, 7)
+    terms = ['I1', 'I2', 'I3']
+    H = orc.build_H_sparse(n, Om, delta, pairs, terms).toarray()
+    for codes in ([1, 2, 4, 7], [7, 1, 4, 2]):
+        codes = np.asarray(codes)
+        v = np.random.default_rng(3).normal(size=len(codes)) + 0j
+        ref = H[np.ix_(codes, codes)] @ v
+        assert np.max(np.abs(orc.channel_matvec(n, Om, delta, pairs, terms, v, codes) - ref)) < 1e-12
+        Hs = orc.build_H_sparse(n, Om, delta, pairs, terms, codes).toarray()
+        assert np.max(np.abs(Hs - H[np.ix_(codes, codes)])) < 1e-12
+
+
+@pytest.mark.parametrize('n', [3, 5, 8, 12])
+def test_chain_blockade_basis(n):
+    edges = [(i, i + 1) for i in range(n - 1)]
+    st = orc.independent_states(n, edges)
+    bf = orc.independent_states_bruteforce(n, edges)
+    assert np.array_equal(np.asarray(st), bf)                   # ascending order
+    assert len(st) == orc.fibonacci(n + 2)                      # F(N+2)
+    assert np.array_equal(orc.chain_fib_rank(n, st), np.arange(len(st)))
+    assert np.array_equal(orc.chain_fib_unrank(n, np.arange(len(st))), np.asarray(st))
+
+
+def test_fibonacci_counts_reference_values():
+    # reference checks/equibench_engine_ed_selfcheck.py:226-238
+    assert [orc.fibonacci(k) for k in (5, 6, 7)] == [5, 8, 13]
+    assert orc.fibonacci(42) == 267914296
+
+
+@pytest.mark.parametrize('n,k', [(6, 0), (6, 3), (10, 4), (12, 12)])
+def test_number_sector_basis(n, k):
+    s = np.arange(1 << n)
+    pc = np.array([bin(x).count('1') for x in s])
+    assert np.array_equal(orc.number_sector_basis(n, k), s[pc == k])
+
+
+def test_kpm_exact_diagonal_known_answer():
+    # reference checks/equibench_gpu_backends_selfcheck.py:27-32
+    lam = np.linspace(-2.0, 3.0, 32)
+    H = np.diag(lam)
+    mu, c, r = orc.kpm_moments(H, 24, 1, exact=True)
+    assert abs(c - 0.5) < 1e-12 and abs(r - (2.5 * 1.001 + 1e-12)) < 1e-12
+    x = (lam - c) / r
+    ref = np.array([np.mean(np.cos(k * np.arccos(x))) for k in range(25)])
+    assert np.max(np.abs(mu - ref)) < 1e-12
+    # Hutchinson is exact on a diagonal matrix (z_s^2 = 1) for any probe
+    mu2, _, _ = orc.kpm_moments(H, 24, 3, seed=5, c=c, r=r)
+    assert np.max(np.abs(mu2 - ref)) < 1e-12
+
+
+def test_jackson_kernel_closed_form():
+    g = orc.jackson_kernel(10)
+    assert abs(g[0] - 1.0) < 1e-14
+    M = 10
+    n = 3
+    a = np.pi / (M + 1)
+    assert abs(g[n] - ((M - n + 1) * np.cos(a * n) + np.sin(a * n) / np.tan(a)) / (M + 1)) < 1e-14
+    assert np.all(np.diff(g) < 0)
+
+
+def test_dos_normalisation():
+    lam = np.linspace(-2.0, 3.0, 64)
+    mu, c, r = orc.kpm_moments(np.diag(lam), 200, 1, exact=True)
+    E = np.linspace(c - 0.99999 * r, c + 0.99999 * r, 20001)
+    rho = orc.dos_from_moments(mu, c, r, E, 200)
+    assert abs(np.trapezoid(rho, E) - 1.0) < 1e-2
+    # uniform spectrum on [-2, 3]: density 1/5 in the bulk
+    assert abs(orc.dos_from_moments(mu, c, r, [0.5], 200)[0] - 0.2) < 5e-3
+
+
+def test_rabi_closed_form():
+    # reference checks/manybody_m1_ed_selfcheck.py:43-54
+    Om = 2 * np.pi * 5.0
+    t = np.linspace(0.0, 200.0, 21)
+    seg = [{'t0': 0.0, 'Om': [Om], 'delta': [0.0]}]
+    psi0 = np.array([1.0, 0.0], complex)
+    psi = orc.propagate_segments(1, [], ['I1'], seg, psi0, t)
+    pred = np.sin(orc.CONV * Om * t / 2) ** 2
+    assert np.max(np.abs(np.abs(psi[:, 1]) ** 2 - pred)) < 1e-10
+
+
+def test_segment_jump_exact():
+    # reference checks/manybody_contract_selfcheck.py:201-206: a drive jump inside
+    # an output interval is integrated exactly at the jump
+    n = 2
+    pairs = [{'i': 0, 'j': 1, 'V': 3.0}]
+    segs = [{'t0': 0.0, 'Om': [1.0, 2.0], 'delta': [0.5, -0.3]},
+            {'t0': 0.7, 'Om': [2.0, 0.5], 'delta': [-1.0, 0.2]}]
+    psi0 = np.zeros(4, complex)
+    psi0[0] = 1.0
+    out = orc.propagate_segments(n, pairs, ['I1'], segs, psi0, np.array([0.0, 1.0]))
+    H1 = orc.kron_hamiltonian(n, [1.0, 2.0], [0.5, -0.3], pairs, ['I1'])
+    H2 = orc.kron_hamiltonian(n, [2.0, 0.5], [-1.0, 0.2], pairs, ['I1'])
+    ref = sla.expm(-1j * orc.CONV * 0.3 * H2) @ sla.expm(-1j * orc.CONV * 0.7 * H1) @ psi0
+    assert np.max(np.abs(out[-1] - ref)) < 1e-12
+
+
+def test_lanczos_reference_form_tridiagonal():
+    # reference spectrum.py:34-103: with full reorthogonalisation the
+    # tridiagonal's extreme eigenvalue is the matrix's
+    n = 6
+    Om, delta, pairs = _random_model(n, 11, complex_terms=False)
+    H = orc.build_H_sparse(n, Om, delta, pairs, ['I1']).toarray()
+    a, b, _ = orc.lanczos(lambda v: H @ v, np.random.default_rng(0).normal(size=1 << n), 40)
+    T = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+    assert abs(np.linalg.eigvalsh(T)[0] - np.linalg.eigvalsh(H)[0]) < 1e-10
+
+
+@pytest.mark.parametrize('n', [4, 9, 13])
+def test_c_oracle_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    Om = rng.uniform(0.5, 1.5, n)
+    delta = rng.normal(size=n)
+    pairs = [{'i': i, 'j': j, 'V': 10.0 / (j - i) ** 6} for i in range(n) for j in range(i + 1, n)]
+    d = hv_ref.diag(n, delta, pairs)
+    x = rng.normal(size=1 << n)
+    ref = orc.channel_matvec(n, Om, delta, pairs, ['I1'], x)
+    assert np.max(np.abs(hv_ref.hv_full(n, Om, d, x) - ref)) < 1e-12 * np.max(np.abs(ref))
+    Omc = Om * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    xc = x + 1j * rng.normal(size=1 << n)
+    refc = orc.channel_matvec(n, Omc, delta, pairs, ['I1'], xc)
+    assert np.max(np.abs(hv_ref.hv_full(n, Omc, d, xc) - refc)) < 1e-12 * np.max(np.abs(refc))
+    # chain sector
+    codes = hv_ref.fib_codes(n)
+    assert np.array_equal(codes, np.asarray(orc.independent_states(n, [(i, i + 1) for i in range(n - 1)])))
+    ds = hv_ref.diag(n, delta, pairs, codes)
+    xs = rng.normal(size=len(codes))
+    refs = orc.channel_matvec(n, Om, delta, pairs, ['I1'], xs, codes)
+    assert np.max(np.abs(hv_ref.hv_fib(n, Om, ds, codes, xs) - refs)) < 1e-12 * np.max(np.abs(refs))
