@@ -90,6 +90,10 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--n', type=int, default=N_SITES)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--only', choices=['f64', 'f32'], default=None,
+                    help='time one element type only (profiling runs)')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='run only a plain device copy of one FP64 N-site vector (counter scale)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the H·v path is built and measured on a single '
@@ -101,11 +105,22 @@ def main():
     pkg = ge.load_package()
     dev = torch.device('cuda:0')
     n = args.n
+    if args.copy_calibration:
+        x = torch.ones(1 << n, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(args.steps):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_calibration_bytes_read': 8 << n, 'bytes_written': 8 << n,
+                          'launches': args.steps}))
+        return
     calib, Om, delta = _model(n)
     plan = pkg.Plan(pkg.from_calib(calib, Om, delta), device=dev)
     D = plan.dim
     res = {}
     for name, dt, size in (('f64', torch.float64, 8), ('f32', torch.float32, 4)):
+        if args.only and name != args.only:
+            continue
         torch.cuda.synchronize()
         wall, kms = _time_hv(pkg, plan, dt, args.steps, args.warmup)
         alg = 2.0 * D * size
@@ -114,6 +129,9 @@ def main():
         print('[bench] %s: %.3f ms/H·v (kernel %.3f ms), %.1f GB/s algorithmic'
               % (name, res[name]['ms_per_step'], kms, res[name]['achieved_gbps']),
               file=sys.stderr, flush=True)
+    if args.only:
+        print(json.dumps({k: v for k, v in res[args.only].items()}))
+        return
     f64 = res['f64']
     cpu = None
     if not args.no_cpu:

@@ -11,6 +11,8 @@ from .model import RydModel, from_calib, active_terms, chain_positions, square_p
 from .plan import Plan, cast
 from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, rayleigh_residual, \
     site_occupations
+from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, sector_codes, \
+    sector_rank, chain_edges
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
@@ -18,4 +20,5 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'from_positions', 'c6_coupling', 'Plan', 'cast', 'hip_lanczos_ground_state',
            'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
            'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',
-           'dos_from_moments', 'probe_bits']
+           'dos_from_moments', 'probe_bits', 'fibonacci', 'chain_blockade_dim',
+           'is_chain_blockade_basis', 'sector_codes', 'sector_rank', 'chain_edges']

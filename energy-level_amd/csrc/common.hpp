@@ -119,10 +119,11 @@ struct Reduce {
 };
 
 // Block-level FP64 reduction of NRED values; the last block to arrive
-// reduces all partials and runs the post-operation.  blockDim.x == 256.
+// reduces all partials and runs the post-operation.  Every thread of the
+// block calls it; blockDim.x is a multiple of 64, at most 1024.
 __device__ inline void block_reduce_finish(double v0, double v1, double v2, double v3,
                                            const Reduce& rd) {
-  __shared__ double red_lds[4 * NRED];
+  __shared__ double red_lds[16 * NRED];
   __shared__ int am_last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2); v3 = wave_sum(v3);

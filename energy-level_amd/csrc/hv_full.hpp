@@ -49,6 +49,8 @@ struct ApplyIO {
   int j;
   const int* stop;          // optional early exit flag (device)
   int reduce;
+  int nfar;                 // far bits the tiled kernel handles (bits TB .. TB+nfar-1)
+  int partial;              // 1: write the raw partial sum (pass A of a split H·v)
 };
 
 template <class V>
@@ -183,8 +185,8 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
       for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
     }
   }
-  // bits >= TB: partner tile through the cache hierarchy
-  for (int k = 0; k < nhi; ++k) {
+  // bits TB .. TB+nfar-1: partner tile through the cache hierarchy
+  for (int k = 0; k < io.nfar; ++k) {
     const int p = TB + k;
     const uint64_t pbase = (tile ^ (1ull << k)) << TB;
     const int bit = (int)((tile >> k) & 1ull);
@@ -198,6 +200,16 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
     }
   }
 
+  if (io.partial) {   // pass A: raw sum for the top-bit pass
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V4<V> o;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o.v[c] = acc[r][c];
+      *reinterpret_cast<V4<V>*>(io.y + base + r * 1024 + t * 4) = o;
+    }
+    return;
+  }
   // combine, store, reduce
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
@@ -223,6 +235,77 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
       }
     }
     *reinterpret_cast<V4<V>*>(io.y + g0) = o;
+  }
+  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+}
+
+// Pass B of the split H·v for large n: the top TOPB bits (n-TOPB .. n-1).
+// A block holds every combination of those bits for 2^CB consecutive codes
+// (one 128-byte run per combination), so each top-bit flip is an LDS or
+// register exchange.  It reads the pass-A partial sum from y and finishes
+//   y = ch·(y_A + Σ_top flips of s·x) + cx·(s·x) + cz·z  with the reductions.
+// Thread u (0..511): c = u & (2^CB-1) selects the code inside the run, u >> CB
+// gives the low (9-CB) top bits, register slot j the remaining top bits.
+constexpr int TOPB = 9;
+template <class V> constexpr int run_bits() {
+  return sizeof(V) == 4 ? 5 : (sizeof(V) == 16 ? 3 : 4);
+}
+
+template <class V>
+__global__ __launch_bounds__(512) void k_full_top(FullModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int CB = run_bits<V>();
+  constexpr int TU = 9 - CB;                 // top bits carried by the thread index
+  constexpr int J = 1 << (TOPB - TU);        // register slots per thread
+  __shared__ V lds[512 * J];
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+  const int u = threadIdx.x;
+  const int S = M.n - TOPB;                  // first top bit
+  const uint64_t nm = 1ull << (S - CB);      // blocks: one per value of bits CB..S-1
+  const uint64_t b = blockIdx.x;
+  const uint64_t m = nm >= 8 ? ((b & 7ull) * (nm >> 3) + (b >> 3)) : b;
+  const uint64_t c = (uint64_t)(u & ((1 << CB) - 1));
+  const uint64_t tlo = (uint64_t)(u >> CB);
+  const uint64_t ebase = (tlo << S) | (m << CB) | c;
+  const uint64_t jstride = 1ull << (S + TU);
+  // stage the scaled x of all top-bit combinations in LDS
+#pragma unroll 8
+  for (int j = 0; j < J; ++j) lds[u + 512 * j] = vscale(io.x[ebase + j * jstride], sr);
+  __syncthreads();
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+#pragma unroll 2
+  for (int j = 0; j < J; ++j) {
+    const uint64_t e = ebase + j * jstride;
+    V a = io.y[e];                           // pass-A partial
+#pragma unroll
+    for (int k = 0; k < TU; ++k) {           // partner in another thread's column
+      const int p = S + k;
+      const RT ai = (RT)M.ai[p];
+      const int bit = (int)((tlo >> k) & 1ull);
+      vcfma(a, (RT)M.ar[p], bit ? ai : -ai, lds[(u ^ (1 << (CB + k))) + 512 * j]);
+    }
+#pragma unroll
+    for (int k = 0; k < TOPB - TU; ++k) {    // partner in another register slot
+      const int p = S + TU + k;
+      const RT ai = (RT)M.ai[p];
+      const int bit = (j >> k) & 1;
+      vcfma(a, (RT)M.ar[p], bit ? ai : -ai, lds[u + 512 * (j ^ (1 << k))]);
+    }
+    const V xj = lds[u + 512 * j];
+    V v = vscale(a, chr);
+    vfma(v, cxr, xj);
+    if (io.z) vfma(v, czr, io.z[e]);
+    io.y[e] = v;
+    if (io.reduce) {
+      s0 += rdot(xj, v);
+      s1 += nrm2(v);
+      s2 += nrm2(xj);
+      if (io.probe) s3 += zdot(((io.probe[e >> 5] >> (e & 31)) & 1u) ? 1.0 : -1.0, v);
+    }
   }
   if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
 }

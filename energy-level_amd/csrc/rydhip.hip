@@ -13,6 +13,7 @@
 #include "hv_full.hpp"
 #include "stream_ops.hpp"
 #include "observables.hpp"
+#include "sector.hpp"
 
 using namespace ryd;
 
@@ -54,6 +55,12 @@ struct ryd_plan {
   double* d_ehi[2] = {nullptr, nullptr};
   int tbits[2] = {0, 0};
   size_t ws_bytes = 0;
+  // chain-blockade sector
+  uint32_t* d_lo_code = nullptr;     // [F(L+2)] low-part codes
+  double* d_lo_diag = nullptr;       // [F(L+2)] low-part diagonal
+  double* d_dl = nullptr;            // [n] delta by bit position
+  int fib_L = 0;
+  int64_t fib_cnt[2] = {0, 0};
 };
 
 namespace {
@@ -99,19 +106,72 @@ void build_tables(int n, int tb, const std::vector<double>& dl, const std::vecto
   }
 }
 
+// Split threshold: from this n on, the top TOPB bits run in a second pass
+// (k_full_top) so that pass A's partner tiles stay within each XCD's L2.
+constexpr int SPLIT_MIN_N = 23;
+
+int64_t fib(int k) {
+  int64_t a = 0, b = 1;
+  for (int i = 0; i < k; ++i) { const int64_t c = a + b; a = b; b = c; }
+  return a;
+}
+
+FibModel fib_model(const ryd_plan* P) {
+  FibModel M;
+  M.n = P->n;
+  M.L = P->fib_L;
+  M.dim = P->dim;
+  M.lo_code = P->d_lo_code;
+  M.lo_diag = P->d_lo_diag;
+  M.vb = P->d_vb;
+  M.dl = P->d_dl;
+  for (int p = 0; p < MAXN; ++p) {
+    M.W[p] = p < P->n ? fib(p + 2) : 0;
+    M.ar[p] = P->ar[p];
+    M.ai[p] = P->ai[p];
+  }
+  M.cnt_lo[0] = P->fib_cnt[0];
+  M.cnt_lo[1] = P->fib_cnt[1];
+  return M;
+}
+
+unsigned fib_grid(const ryd_plan* P) {
+  const int64_t ntiles = (P->dim + FIB_T - 1) / FIB_T;
+  return (unsigned)(ntiles >= 64 ? 8 * ((ntiles + 7) / 8) : ntiles);
+}
+
 template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
+  if (P->basis == RYD_BASIS_CHAIN_FIB) {
+    io.nfar = 0;
+    io.partial = 0;
+    k_fib_apply<V><<<dim3(fib_grid(P)), dim3(256), 0, st>>>(fib_model(P), io, rd);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   FullModel M = full_model(P, dtype);
   constexpr int TB = tile_bits<V>();
   if (P->n >= TB) {
     const uint64_t ntiles = 1ull << (P->n - TB);
+    const bool split = P->n >= SPLIT_MIN_N;
+    io.nfar = split ? P->n - TOPB - TB : P->n - TB;
+    io.partial = split ? 1 : 0;
     k_full_tiled<V><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(M, io, rd);
+    LAUNCH_CHECK();
+    if (split) {
+      io.partial = 0;
+      const uint64_t nb = 1ull << (P->n - TOPB - run_bits<V>());
+      k_full_top<V><<<dim3((unsigned)nb), dim3(512), 0, st>>>(M, io, rd);
+      LAUNCH_CHECK();
+    }
   } else {
     const uint64_t D = 1ull << P->n;
     const unsigned grid = (unsigned)((D + 255) / 256);
+    io.nfar = 0;
+    io.partial = 0;
     k_full_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+    LAUNCH_CHECK();
   }
-  LAUNCH_CHECK();
   return RYD_OK;
 }
 
@@ -273,9 +333,10 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   if (!m->om_re || !m->delta) return fail(RYD_ERR_INVALID, "om_re and delta are required");
   if (m->n_pairs < 0 || (m->n_pairs > 0 && (!m->pair_i || !m->pair_j || !m->pair_v)))
     return fail(RYD_ERR_INVALID, "pair arrays are required when n_pairs > 0");
-  if (m->basis_kind != RYD_BASIS_FULL)
-    return fail(RYD_ERR_UNSUPPORTED, "only the full basis is built into this library version");
-  if (n > 40) return fail(RYD_ERR_INVALID, "full basis needs n <= 40");
+  if (m->basis_kind != RYD_BASIS_FULL && m->basis_kind != RYD_BASIS_CHAIN_FIB)
+    return fail(RYD_ERR_UNSUPPORTED, "basis kind not built into this library version");
+  if (m->basis_kind == RYD_BASIS_FULL && n > 40)
+    return fail(RYD_ERR_INVALID, "full basis needs n <= 40");
   for (int i = 0; i < n; ++i) {
     if (!std::isfinite(m->om_re[i]) || !std::isfinite(m->delta[i]) ||
         (m->om_im && !std::isfinite(m->om_im[i])))
@@ -284,7 +345,7 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   ryd_plan* P = new ryd_plan();
   P->n = n;
   P->basis = m->basis_kind;
-  P->dim = 1ll << n;
+  P->dim = m->basis_kind == RYD_BASIS_CHAIN_FIB ? fib(n + 2) : (1ll << n);
   (void)hipGetDevice(&P->device);
   std::vector<double> dl(n), vb((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) {
@@ -312,6 +373,44 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   if (e != hipSuccess) return cleanup(RYD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
   e = hipMemcpy(P->d_vb, vb.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(RYD_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (P->basis == RYD_BASIS_CHAIN_FIB) {
+    const int L = n < FIB_L ? n : FIB_L;
+    P->fib_L = L;
+    P->fib_cnt[0] = fib(L + 2);
+    P->fib_cnt[1] = fib(L + 1);
+    std::vector<uint32_t> codes;
+    std::vector<double> dlo;
+    for (uint32_t c = 0; c < (1u << L); ++c) {
+      if (c & (c >> 1)) continue;       // adjacent excitations are blockaded
+      codes.push_back(c);
+      double d = 0.0;
+      for (int p = 0; p < L; ++p) {
+        if (!((c >> p) & 1u)) continue;
+        d += dl[p];
+        for (int q = p + 1; q < L; ++q)
+          if ((c >> q) & 1u) d += vb[(size_t)p * n + q];
+      }
+      dlo.push_back(d);
+    }
+    e = hipMalloc(&P->d_lo_code, sizeof(uint32_t) * codes.size());
+    if (e == hipSuccess) e = hipMalloc(&P->d_lo_diag, sizeof(double) * dlo.size());
+    if (e == hipSuccess) e = hipMalloc(&P->d_dl, sizeof(double) * n);
+    if (e == hipSuccess)
+      e = hipMemcpy(P->d_lo_code, codes.data(), sizeof(uint32_t) * codes.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(P->d_lo_diag, dlo.data(), sizeof(double) * dlo.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(P->d_dl, dl.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return cleanup(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
+                     std::string("sector table upload: ") + hipGetErrorString(e));
+    size_t blocks = fib_grid(P);
+    if (blocks < 8192) blocks = 8192;
+    P->ws_bytes = WS_HEADER + blocks * NRED * sizeof(double);
+    const size_t occ = (size_t)OCC_BLOCKS * (MAXN_OCC + 1) * sizeof(double);
+    if (WS_HEADER + occ > P->ws_bytes) P->ws_bytes = WS_HEADER + occ;
+    *out = P;
+    return RYD_OK;
+  }
   const int tbs[2] = {12, 11};
   for (int k = 0; k < 2; ++k) {
     const int tb = n < tbs[k] ? n : tbs[k];
@@ -344,6 +443,9 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
 int ryd_plan_destroy(ryd_plan* P) {
   if (!P) return RYD_OK;
   if (P->d_vb) (void)hipFree(P->d_vb);
+  if (P->d_lo_code) (void)hipFree(P->d_lo_code);
+  if (P->d_lo_diag) (void)hipFree(P->d_lo_diag);
+  if (P->d_dl) (void)hipFree(P->d_dl);
   for (int k = 0; k < 2; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
@@ -465,6 +567,16 @@ int ryd_cast(int din, const void* x, int dout, void* y, int64_t n, void* stream)
 namespace {
 template <class V>
 int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t st) {
+  if (P->basis == RYD_BASIS_CHAIN_FIB) {
+    const int64_t ntiles = (P->dim + FIB_T - 1) / FIB_T;
+    const unsigned g = (unsigned)(ntiles < OCC_BLOCKS ? ntiles : OCC_BLOCKS);
+    double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
+    k_fib_occ<V><<<g, 256, 0, st>>>(fib_model(P), static_cast<const V*>(x), part);
+    LAUNCH_CHECK();
+    k_occ_finish<<<1, 256, 0, st>>>(part, (int)g, P->n, out);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   const uint64_t ob = P->n < OB ? P->n : OB;
   const uint64_t ntiles = (1ull << P->n) >> ob;
   const unsigned g = (unsigned)(ntiles < OCC_BLOCKS ? ntiles : OCC_BLOCKS);
@@ -481,8 +593,6 @@ int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t 
 extern "C" int ryd_site_occupation(const ryd_plan* P, int dtype, const void* x, double* out,
                                    void* ws, void* stream) {
   if (!P || !x || !out || !ws) return fail(RYD_ERR_INVALID, "null argument to ryd_site_occupation");
-  if (P->basis != RYD_BASIS_FULL)
-    return fail(RYD_ERR_UNSUPPORTED, "site occupation kernel covers the full basis");
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (dtype) {
     case RYD_F32: return do_occ<float>(P, x, out, ws, st);
@@ -491,4 +601,28 @@ extern "C" int ryd_site_occupation(const ryd_plan* P, int dtype, const void* x, 
     case RYD_C128: return do_occ<c64>(P, x, out, ws, st);
     default: return fail(RYD_ERR_INVALID, "unknown dtype");
   }
+}
+
+extern "C" int ryd_sector_codes(const ryd_plan* P, const int64_t* ranks, int64_t first,
+                                int64_t count, int64_t* codes, void* stream) {
+  if (!P || !codes || count < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_sector_codes");
+  if (P->basis != RYD_BASIS_CHAIN_FIB)
+    return fail(RYD_ERR_UNSUPPORTED, "ranking is defined for the chain blockade sector");
+  if (count == 0) return RYD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  k_fib_unrank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), first, count, ranks, codes);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+extern "C" int ryd_sector_rank(const ryd_plan* P, const int64_t* codes, int64_t count,
+                               int64_t* ranks, void* stream) {
+  if (!P || !codes || !ranks || count < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_sector_rank");
+  if (P->basis != RYD_BASIS_CHAIN_FIB)
+    return fail(RYD_ERR_UNSUPPORTED, "ranking is defined for the chain blockade sector");
+  if (count == 0) return RYD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  k_fib_rank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), codes, count, ranks);
+  LAUNCH_CHECK();
+  return RYD_OK;
 }

@@ -234,12 +234,18 @@ def _start_vector(dim, seed, complex_kind, device):
 def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
                              basis_indices=None, m=48, seed=0, dtype='float64', device='cuda',
                              tol=None, max_restarts=8, return_state=False,
-                             memory_budget_mb=None, refine=True, plan=None):
+                             memory_budget_mb=None, refine=True, plan=None, basis='full'):
     """Ground state and gap with residual certification (see module docstring).
 
     Keys as ``torch_lanczos_ground_state``: E0, gap, evals, residuals,
     converged, backend, dim, krylov_dim, restarts, tolerance[, warning][, vec];
     plus ``precision`` and ``n_atom`` (<n_i> of the ground state).
+
+    ``basis``: ``'full'`` or ``'chain_blockade'`` (the nearest-neighbour
+    blockade sector of a chain, ranked on the device; no code list is built).
+    ``basis_indices`` equal to that sector (ascending) selects it as well; any
+    other restricted basis raises NotImplementedError.  ``vec`` is in the
+    basis order (ascending codes).
     """
     import torch
     if str(device).startswith('cpu'):
@@ -249,10 +255,16 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
         raise ValueError('dtype must be float32/float64/complex64/complex128')
     if int(m) < 2 or int(max_restarts) < 0:
         raise ValueError('m must be >= 2 and max_restarts >= 0')
-    if basis_indices is not None:
-        raise NotImplementedError('restricted bases go through hip_sector_ground_state')
     model = from_calib(calib, Om, delta, gamma, terms)
-    P = plan if plan is not None else Plan(model, device=device)
+    if basis_indices is not None:
+        from .basis import is_chain_blockade_basis
+        if not is_chain_blockade_basis(model.n, basis_indices):
+            raise NotImplementedError('only the chain blockade sector is a HIP restricted basis')
+        basis = 'chain_blockade'
+    if basis not in ('full', 'chain_blockade'):
+        raise ValueError('basis must be full or chain_blockade')
+    P = plan if plan is not None else Plan(
+        model, basis='chain_fib' if basis == 'chain_blockade' else 'full', device=device)
     dim = P.dim
     low = dtype in ('float32', 'complex64')
     complex_kind = (not P.is_real) or dtype.startswith('complex')

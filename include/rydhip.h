@@ -132,9 +132,23 @@ int ryd_cheb_moments(const ryd_plan* plan, int dtype, const uint32_t* probe, voi
 /* out[i] = Σ_s |x_s|^2 b_i(s) for every site i, out[n] = Σ_s |x_s|^2 (device FP64,
  * n+1 entries; divide by out[n] for <n_i>).  Replaces _gs_observables' n_atom
  * (eq_route.py:108-122) and the bits @ |psi|^2 reduction of
- * engine_ed._reduce (engine_ed.py:261-263).  Full basis. */
+ * engine_ed._reduce (engine_ed.py:261-263).  Full basis and chain sector. */
 int ryd_site_occupation(const ryd_plan* plan, int dtype, const void* x, double* out, void* ws,
                         void* stream);
+
+/* Chain-blockade sector enumeration: codes[k] = code of rank ranks[k] (or of
+ * rank first + k when ranks is NULL); ranks outside [0, dim) give -1.  Codes
+ * come out in the reference's basis order, ascending (independent_states,
+ * src/core/manybody/basis.py:24-47 with edges (i, i+1)); rank(code) =
+ * Σ_{site i excited} F(n-i+1).  Device int64 arrays. */
+int ryd_sector_codes(const ryd_plan* plan, const int64_t* ranks, int64_t first, int64_t count,
+                     int64_t* codes, void* stream);
+
+/* ranks[k] = rank of codes[k] in the chain-blockade sector, -1 if the code is
+ * not in it (replaces the position table / searchsorted lookup of
+ * ChannelKernel._indices, channels.py:177-192).  Device int64 arrays. */
+int ryd_sector_rank(const ryd_plan* plan, const int64_t* codes, int64_t count, int64_t* ranks,
+                    void* stream);
 
 /* sums[0..2] = {Re<a,b>, |b|^2, |a|^2} over n elements (a may be NULL). */
 int ryd_dots(int dtype, const void* a, const void* b, int64_t n, double* sums, void* ws,

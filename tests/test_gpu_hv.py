@@ -60,7 +60,7 @@ def test_hv_complex_vs_kron(pkg, dev, n):
         P.hv(torch.as_tensor(x.real, device=dev))     # complex phases need complex vectors
 
 
-@pytest.mark.parametrize('n', [12, 13, 16, 18, 20])
+@pytest.mark.parametrize('n', [12, 13, 16, 18, 20, 22, 23, 24])
 def test_hv_fp64_fp32_vs_c_oracle(pkg, dev, n):
     import torch
     Om = np.ones(n)
@@ -141,3 +141,36 @@ def test_site_occupations(pkg, dev):
         got = pkg.site_occupations(P, torch.as_tensor(x, device=dev))
         ref = orc.site_occupations(n, x / np.linalg.norm(x))
         assert np.max(np.abs(got - ref)) < 1e-13
+
+
+@pytest.mark.parametrize('n', [23, 24])
+def test_hv_split_complex_and_fused_sums(pkg, dev, n):
+    """Sizes that run the two-pass (contiguous + top-bit) H·v: complex drive,
+    three-term update and reductions against the C oracle."""
+    import torch
+    rng = np.random.default_rng(n)
+    Om = rng.uniform(0.5, 1.5, n) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    delta = rng.normal(size=n)
+    pairs = _chain(n)
+    P = _plan(pkg, dev, n, Om, delta, pairs)
+    d = hv_ref.diag(n, delta, pairs)
+    x = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    ref = hv_ref.hv_full(n, Om, d, x)
+    y = P.hv(torch.as_tensor(x, device=dev)).cpu().numpy()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    y64 = P.hv(torch.as_tensor(x.astype(np.complex64), device=dev)).cpu().numpy()
+    assert np.max(np.abs(y64 - ref)) <= 2e-6 * np.max(np.abs(ref))
+    # real model, fused three-term update with sums
+    Omr = np.abs(Om)
+    Pr = _plan(pkg, dev, n, Omr, delta, pairs)
+    xr, zr = rng.normal(size=1 << n), rng.normal(size=1 << n)
+    s, ch, cx, cz = 0.5, 2.0, -0.3, -1.0
+    ref = ch * hv_ref.hv_full(n, Omr, d, s * xr) + cx * s * xr + cz * zr
+    sums = torch.zeros(4, dtype=torch.float64, device=dev)
+    xt, zt = torch.as_tensor(xr, device=dev), torch.as_tensor(zr, device=dev)
+    yt = torch.empty_like(xt)
+    Pr.apply(xt, yt, s=s, ch=ch, cx=cx, cz=cz, z=zt, sums=sums)
+    yr = yt.cpu().numpy()
+    assert np.max(np.abs(yr - ref)) <= 1e-13 * np.max(np.abs(ref))
+    S = sums.cpu().numpy()
+    assert np.allclose(S[:3], [(s * xr) @ ref, ref @ ref, (s * xr) @ (s * xr)], rtol=1e-11)
