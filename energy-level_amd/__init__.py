@@ -13,6 +13,8 @@ from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, raylei
     site_occupations
 from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, sector_codes, \
     sector_rank, chain_edges
+from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolution, z2_index, \
+    slice_bounds
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
@@ -21,4 +23,6 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
            'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',
            'dos_from_moments', 'probe_bits', 'fibonacci', 'chain_blockade_dim',
-           'is_chain_blockade_basis', 'sector_codes', 'sector_rank', 'chain_edges']
+           'is_chain_blockade_basis', 'sector_codes', 'sector_rank', 'chain_edges', 'CONV',
+           'cheb_coefficients', 'terms_for', 'PulseProblem', 'PulseEvolution', 'z2_index',
+           'slice_bounds']

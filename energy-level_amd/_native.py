@@ -56,6 +56,10 @@ SIGNATURES = {
     'ryd_site_occupation': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     'ryd_sector_codes': (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp]),
     'ryd_sector_rank': (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    'ryd_evolve_forward': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _i64, _vp, _vp, _vp, _vp,
+                                          ctypes.c_int, _vp, _vp, _vp]),
+    'ryd_evolve_adjoint': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
+                                          ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'ryd_dots': (ctypes.c_int, [ctypes.c_int, _vp, _vp, _i64, _vp, _vp, _vp]),
     'ryd_cast': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, _i64, _vp]),
 }

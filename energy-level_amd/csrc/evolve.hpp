@@ -1,0 +1,221 @@
+// Piecewise-constant evolution on the full basis: the fused Chebyshev step of
+// the forward propagator and of its adjoint (Clenshaw) recurrence.
+//
+//   H(Ω, Δ) = om · F + D + dadd · N
+// with F the plan's flip operator (amplitudes Ω_site/2), D the plan's diagonal
+// (delta_site n_i + V n_i n_j), N = Σ_i n_i and per-slice scalars om, dadd.
+// One launch computes, for x~ = s·x,
+//   y    = ch·(om·F x~ + D x~ + dadd·N x~) + cx·x~ + cz·z + cw·w
+//   acc += ca·y                                   (complex cw, ca; optional)
+//   sums = {Re<q, F x~>, Re<q, N x~>, |y|^2, Re<q, y>}   (optional q)
+// Forward slice: t_{j+1} = 2H~t_j − t_{j−1}, ψ_out = Σ a_j t_j (acc).
+// Adjoint slice: b_j = conj(a_j) g + 2H~ b_{j+1} − b_{j+2} with q = t_j gives
+// the gradient dots <F b_{j+1}, t_j> and <N b_{j+1}, t_j>.
+// Same tiling as k_full_tiled (single pass, n <= 22), or one element per
+// thread below the tile size.
+#pragma once
+#include "common.hpp"
+#include "hv_full.hpp"
+
+namespace ryd {
+
+template <class V>
+struct EvoIO {
+  const V* x;
+  const V* z;
+  const V* w;
+  V* y;
+  V* acc;
+  const V* q;
+  double om, dadd, s, ch, cx, cz;
+  double cw_re, cw_im, ca_re, ca_im;
+  int reduce;
+};
+
+template <class V>
+__device__ __forceinline__ void evo_finish(const EvoIO<V>& io, uint64_t g, V fl, V xe, double d,
+                                           double pop, double& s0, double& s1, double& s2,
+                                           double& s3) {
+  using RT = typename VT<V>::R;
+  V hx = vscale(fl, (RT)io.om);
+  vfma(hx, (RT)(d + io.dadd * pop), xe);
+  V v = vscale(hx, (RT)io.ch);
+  vfma(v, (RT)io.cx, xe);
+  if (io.z) vfma(v, (RT)io.cz, io.z[g]);
+  if (io.w) vcfma(v, (RT)io.cw_re, (RT)io.cw_im, io.w[g]);
+  io.y[g] = v;
+  if (io.acc) {
+    V a = io.acc[g];
+    vcfma(a, (RT)io.ca_re, (RT)io.ca_im, v);
+    io.acc[g] = a;
+  }
+  if (io.reduce) {
+    s2 += nrm2(v);
+    if (io.q) {
+      const V qv = io.q[g];
+      s0 += rdot(qv, fl);
+      s1 += pop * rdot(qv, xe);
+      s3 += rdot(qv, v);
+    }
+  }
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int R = tile_r<V>();
+  constexpr int TB = tile_bits<V>();
+  constexpr int TILE = 1024 * R;
+  __shared__ V lds[TILE];
+  __shared__ double Fx[16];
+  __shared__ double T0[256];
+  __shared__ double T1[16];
+  const RT sr = (RT)io.s;
+  const int t = threadIdx.x;
+  const int nhi = M.n - TB;
+  const uint64_t ntiles = 1ull << nhi;
+  const uint64_t b = blockIdx.x;
+  const uint64_t tile = ntiles >= 8 ? ((b & 7ull) * (ntiles >> 3) + (b >> 3)) : b;
+  const uint64_t base = tile << TB;
+  V xs[R][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + base + r * 1024 + t * 4);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xs[r][c] = vscale(qv.v[c], sr);
+    V4<V> wv;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) wv.v[c] = xs[r][c];
+    *reinterpret_cast<V4<V>*>(lds + r * 1024 + t * 4) = wv;
+  }
+  if (t < TB) {
+    double f = 0.0;
+    const double* row = M.vb + (size_t)t * M.n;
+    for (int q = TB; q < M.n; ++q)
+      if ((tile >> (q - TB)) & 1ull) f += row[q];
+    Fx[t] = f;
+  }
+  __syncthreads();
+  {
+    double v = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+      if ((t >> p) & 1) v += Fx[p];
+    T0[t] = v;
+    if (t < (1 << (TB - 8))) {
+      double w = 0.0;
+#pragma unroll
+      for (int p = 8; p < TB; ++p)
+        if ((t >> (p - 8)) & 1) w += Fx[p];
+      T1[t] = w;
+    }
+  }
+  const double eh = M.ehi[tile];
+  const int pop_hi = __popcll(tile);
+  __syncthreads();
+  // flip sum (plan amplitudes, no diagonal)
+  V fl[R][4];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) fl[r][c] = vzero(xs[r][c]);
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int bit = (c >> p) & 1;
+        vcfma(fl[r][c], ar, bit ? ai : -ai, xs[r][c ^ (1 << p)]);
+      }
+  }
+#pragma unroll
+  for (int k = 0; (1 << k) < R; ++k) {
+    const int p = 10 + k;
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int bit = (r >> k) & 1;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(fl[r][c], ar, bit ? ai : -ai, xs[r ^ (1 << k)][c]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int p = 2 + k;
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+    const int bit = (t >> k) & 1;
+    const RT aim = bit ? ai : -ai;
+    const int tp = t ^ (1 << k);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V4<V> qv = *reinterpret_cast<const V4<V>*>(lds + r * 1024 + tp * 4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(fl[r][c], ar, aim, qv.v[c]);
+    }
+  }
+  for (int k = 0; k < nhi; ++k) {
+    const int p = TB + k;
+    const uint64_t pbase = (tile ^ (1ull << k)) << TB;
+    const int bit = (int)((tile >> k) & 1ull);
+    const RT ar = (RT)(M.ar[p] * io.s);
+    const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * io.s);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + pbase + r * 1024 + t * 4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(fl[r][c], ar, aim, qv.v[c]);
+    }
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e0 = r * 1024 + t * 4;
+    V4<double> el = *reinterpret_cast<const V4<double>*>(M.elo + e0);
+    const double t1 = T1[e0 >> 8];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int e = e0 + c;
+      const double d = eh + el.v[c] + T0[e & 255] + t1;
+      const double pop = (double)(pop_hi + __popc((unsigned)e));
+      evo_finish(io, base + e, fl[r][c], xs[r][c], d, pop, s0, s1, s2, s3);
+    }
+  }
+  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_evo_small(FullModel M, EvoIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  const uint64_t D = 1ull << M.n;
+  const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (g < D) {
+    const V xe = vscale(io.x[g], (RT)io.s);
+    V fl = vzero(xe);
+    for (int p = 0; p < M.n; ++p) {
+      const int bit = (int)((g >> p) & 1ull);
+      vcfma(fl, (RT)(M.ar[p] * io.s), (RT)((bit ? M.ai[p] : -M.ai[p]) * io.s),
+            io.x[g ^ (1ull << p)]);
+    }
+    const double d = M.elo[g & ((1ull << M.tb) - 1)] + M.ehi[g >> M.tb];
+    evo_finish(io, g, fl, xe, d, (double)__popcll(g), s0, s1, s2, s3);
+  }
+  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+}
+
+// y = a·x (complex a; y may alias nothing)
+template <class V>
+__global__ __launch_bounds__(256) void k_cscale(const V* __restrict__ x, V* __restrict__ y,
+                                                int64_t D, double are, double aim) {
+  using RT = typename VT<V>::R;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    V v = vzero(x[i]);
+    vcfma(v, (RT)are, (RT)aim, x[i]);
+    y[i] = v;
+  }
+}
+
+}  // namespace ryd

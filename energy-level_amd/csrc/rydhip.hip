@@ -14,6 +14,7 @@
 #include "stream_ops.hpp"
 #include "observables.hpp"
 #include "sector.hpp"
+#include "evolve.hpp"
 
 using namespace ryd;
 
@@ -625,4 +626,155 @@ extern "C" int ryd_sector_rank(const ryd_plan* P, const int64_t* codes, int64_t 
   k_fib_rank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), codes, count, ranks);
   LAUNCH_CHECK();
   return RYD_OK;
+}
+
+namespace {
+
+template <class V>
+int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t st) {
+  FullModel M = full_model(P, dtype);
+  constexpr int TB = tile_bits<V>();
+  if (P->n >= TB) {
+    k_evo_tiled<V><<<dim3((unsigned)(1ull << (P->n - TB))), dim3(256), 0, st>>>(M, io, rd);
+  } else {
+    const unsigned grid = (unsigned)(((1ull << P->n) + 255) / 256);
+    k_evo_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+  }
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+template <class V>
+EvoIO<V> evo_io(double om, double dadd) {
+  EvoIO<V> io;
+  io.x = io.z = io.w = io.q = nullptr;
+  io.y = io.acc = nullptr;
+  io.om = om;
+  io.dadd = dadd;
+  io.s = 1.0;
+  io.ch = 1.0;
+  io.cx = io.cz = 0.0;
+  io.cw_re = io.cw_im = io.ca_re = io.ca_im = 0.0;
+  io.reduce = 0;
+  return io;
+}
+
+template <class V>
+int do_evolve_forward(const ryd_plan* P, int dtype, void* psi_v, int64_t nslices,
+                      const double* om, const double* dadd, const double* c, const double* r,
+                      int K, const double* coef, void* work_v, hipStream_t st) {
+  V* psi = static_cast<V*>(psi_v);
+  const int64_t D = P->dim;
+  V* W = static_cast<V*>(work_v);
+  V* T[3] = {W, W + D, W + 2 * D};
+  V* acc = W + 3 * D;
+  Reduce none = make_reduce(nullptr, POST_NONE, nullptr);
+  for (int64_t k = 0; k < nslices; ++k) {
+    const double* a = coef + (size_t)k * (K + 1) * 2;
+    const double inv = 1.0 / r[k];
+    k_cscale<V><<<stream_grid(D), 256, 0, st>>>(psi, acc, D, a[0], a[1]);
+    LAUNCH_CHECK();
+    for (int j = 0; j < K; ++j) {   // produce t_{j+1}
+      EvoIO<V> io = evo_io<V>(om[k], dadd[k]);
+      io.x = j == 0 ? psi : T[(j - 1) % 3];
+      io.z = j == 0 ? nullptr : (j == 1 ? psi : T[(j - 2) % 3]);
+      io.y = T[j % 3];
+      io.acc = acc;
+      io.ch = (j == 0 ? 1.0 : 2.0) * inv;
+      io.cx = -(j == 0 ? 1.0 : 2.0) * c[k] * inv;
+      io.cz = j == 0 ? 0.0 : -1.0;
+      io.ca_re = a[2 * (j + 1)];
+      io.ca_im = a[2 * (j + 1) + 1];
+      if (int rc = launch_evo<V>(P, dtype, io, none, st)) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(psi, acc, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+  }
+  return RYD_OK;
+}
+
+template <class V>
+int do_evolve_vjp(const ryd_plan* P, int dtype, const void* psi_v, void* g_v, double om,
+                  double dadd, double c, double r, int K, const double* a, void* tstore_v,
+                  void* work_v, double* sums, void* ws, hipStream_t st) {
+  const V* psi = static_cast<const V*>(psi_v);
+  V* g = static_cast<V*>(g_v);
+  const int64_t D = P->dim;
+  V* TS = static_cast<V*>(tstore_v);          // t_0 .. t_{K-1}
+  V* W = static_cast<V*>(work_v);
+  V* B[3] = {W, W + D, W + 2 * D};
+  const double inv = 1.0 / r;
+  Reduce none = make_reduce(nullptr, POST_NONE, nullptr);
+  // recompute the forward recurrence of this slice, keeping t_0 .. t_{K-1}
+  HIP_TRY(hipMemcpyAsync(TS, psi, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+  for (int j = 0; j + 1 < K; ++j) {
+    EvoIO<V> io = evo_io<V>(om, dadd);
+    io.x = TS + (size_t)j * D;
+    io.z = j == 0 ? nullptr : TS + (size_t)(j - 1) * D;
+    io.y = TS + (size_t)(j + 1) * D;
+    io.ch = (j == 0 ? 1.0 : 2.0) * inv;
+    io.cx = -(j == 0 ? 1.0 : 2.0) * c * inv;
+    io.cz = j == 0 ? 0.0 : -1.0;
+    if (int rc = launch_evo<V>(P, dtype, io, none, st)) return rc;
+  }
+  // Clenshaw: b_K = conj(a_K) g ; b_j = conj(a_j) g + 2H~ b_{j+1} - b_{j+2} ; b_0 with H~
+  k_cscale<V><<<stream_grid(D), 256, 0, st>>>(g, B[K % 3], D, a[2 * K], -a[2 * K + 1]);
+  LAUNCH_CHECK();
+  for (int j = K - 1; j >= 0; --j) {
+    EvoIO<V> io = evo_io<V>(om, dadd);
+    io.x = B[(j + 1) % 3];
+    io.z = j + 2 <= K ? B[(j + 2) % 3] : nullptr;
+    io.w = g;
+    io.y = j == 0 ? g : B[j % 3];
+    io.q = TS + (size_t)j * D;
+    io.ch = (j == 0 ? 1.0 : 2.0) * inv;
+    io.cx = -(j == 0 ? 1.0 : 2.0) * c * inv;
+    io.cz = io.z ? -1.0 : 0.0;
+    io.cw_re = a[2 * j];
+    io.cw_im = -a[2 * j + 1];
+    io.reduce = 1;
+    Reduce rd = make_reduce(ws, POST_STORE, sums + 4 * j);
+    if (int rc = launch_evo<V>(P, dtype, io, rd, st)) return rc;
+  }
+  return RYD_OK;
+}
+
+int check_evo(const ryd_plan* P, int dtype, int K) {
+  if (P->basis != RYD_BASIS_FULL)
+    return fail(RYD_ERR_UNSUPPORTED, "evolution kernels cover the full basis");
+  if (P->n >= SPLIT_MIN_N) return fail(RYD_ERR_UNSUPPORTED, "evolution kernels need n <= 22");
+  if (dtype != RYD_C64 && dtype != RYD_C128)
+    return fail(RYD_ERR_INVALID, "evolution needs a complex dtype");
+  if (K < 1 || K > 4096) return fail(RYD_ERR_INVALID, "K must be in [1, 4096]");
+  return RYD_OK;
+}
+
+}  // namespace
+
+extern "C" int ryd_evolve_forward(const ryd_plan* P, int dtype, void* psi, int64_t nslices,
+                                  const double* om, const double* dadd, const double* c,
+                                  const double* r, int K, const double* coef, void* work,
+                                  void* stream) {
+  if (!P || !psi || !om || !dadd || !c || !r || !coef || !work || nslices < 0)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_forward");
+  if (int rc = check_evo(P, dtype, K)) return rc;
+  for (int64_t k = 0; k < nslices; ++k)
+    if (!(r[k] > 0.0)) return fail(RYD_ERR_INVALID, "spectral radius r must be > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == RYD_C64)
+    return do_evolve_forward<c32>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
+  return do_evolve_forward<c64>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
+}
+
+extern "C" int ryd_evolve_adjoint(const ryd_plan* P, int dtype, const void* psi, void* g,
+                                  double om, double dadd, double c, double r, int K,
+                                  const double* coef, void* tstore, void* work, double* sums,
+                                  void* ws, void* stream) {
+  if (!P || !psi || !g || !coef || !tstore || !work || !sums || !ws)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_adjoint");
+  if (int rc = check_evo(P, dtype, K)) return rc;
+  if (!(r > 0.0)) return fail(RYD_ERR_INVALID, "spectral radius r must be > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == RYD_C64)
+    return do_evolve_vjp<c32>(P, dtype, psi, g, om, dadd, c, r, K, coef, tstore, work, sums, ws, st);
+  return do_evolve_vjp<c64>(P, dtype, psi, g, om, dadd, c, r, K, coef, tstore, work, sums, ws, st);
 }

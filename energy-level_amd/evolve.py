@@ -1,0 +1,231 @@
+"""Piecewise-constant time evolution on the HIP engine, forward and adjoint.
+
+Propagator convention of the reference (tomisnot/energy-level
+src/core/manybody/engine_ed.py:145): psi(t + dt) = exp(-i·CONV·H·dt) psi,
+CONV = 2π·1e-3 (H in MHz, t in ns).  Each slice applies a Chebyshev series
+(Jacobi–Anger)
+
+    exp(-iτH) = e^{-iτc} [J_0(τr) + 2 Σ_{k>=1} (-i)^k J_k(τr) T_k((H - c)/r)],
+
+τ = CONV·dt, with [c - r, c + r] a Gershgorin enclosure of the slice's
+spectrum.  The adjoint is the Clenshaw-form reverse recurrence of that same
+series (``ryd_evolve_adjoint``), so gradients are those of the discrete
+propagator; (c, r) are constants of the slice (no gradient flows through
+them).  Checkpoints are kept every ``checkpoint`` slices and the slices
+between them are recomputed during the backward pass.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as nat
+from .model import RydModel, from_calib
+from .plan import Plan
+
+__all__ = ['CONV', 'cheb_coefficients', 'terms_for', 'PulseProblem', 'pulse_loss_and_grad',
+           'PulseEvolution', 'propagate', 'z2_index', 'slice_bounds']
+
+CONV = 2.0 * np.pi * 1e-3
+
+
+def cheb_coefficients(tau, c, r, K):
+    """a_0..a_K of exp(-iτ(c + r x)) = Σ a_k T_k(x) (Jacobi–Anger), as complex128."""
+    from scipy.special import jv
+    k = np.arange(K + 1)
+    a = (2.0 - (k == 0)) * ((-1j) ** k) * jv(k, tau * r)
+    return np.exp(-1j * tau * c) * a
+
+
+def terms_for(tau_r, eps=1e-15):
+    """Smallest K with the series tail below eps (|J_k(z)| ≤ (z/2)^k / k!)."""
+    from math import lgamma, log
+    z = abs(tau_r)
+    k = max(1, int(np.ceil(z)) + 1)
+    while True:
+        if z == 0 or k * log(z / 2.0 if z > 0 else 1.0) - lgamma(k + 1) < log(eps):
+            return k
+        k += 1
+
+
+def _diag_range(model: RydModel, dadd):
+    """Bounds of the diagonal delta·n + dadd·N + V·n n over all codes."""
+    lo = np.minimum(0.0, np.asarray(model.delta) + dadd).sum() + np.minimum(0.0, model.pair_v).sum()
+    hi = np.maximum(0.0, np.asarray(model.delta) + dadd).sum() + np.maximum(0.0, model.pair_v).sum()
+    return lo, hi
+
+
+def slice_bounds(model: RydModel, om, dadd):
+    """(c, r) enclosing the spectrum of om·F + D + dadd·N (Gershgorin)."""
+    lo, hi = _diag_range(model, dadd)
+    rad = abs(om) * float(np.sum(np.abs(model.Om))) / 2.0
+    lo, hi = lo - rad, hi + rad
+    c = 0.5 * (lo + hi)
+    r = max(0.5 * (hi - lo), 1e-12)
+    return c, r
+
+
+class PulseProblem:
+    """Per-slice uniform drive: H_k = Ω_k·F + D + (−Δ_k)·N on the full basis.
+
+    ``model`` supplies the flip pattern F (its Ω, normally 1 on every site),
+    the fixed diagonal D (its delta, normally 0, and V pairs).  Slices have
+    duration ``dt`` ns and ``K+1`` Chebyshev terms.
+    """
+
+    def __init__(self, model: RydModel, dt, K=15, dtype='complex64', device='cuda',
+                 checkpoint=20, plan=None):
+        import torch
+        nat.require_gpu()
+        self.model = model
+        self.plan = plan if plan is not None else Plan(model, device=device)
+        self.dt = float(dt)
+        self.tau = CONV * self.dt
+        self.K = int(K)
+        self.tdt = {'complex64': torch.complex64, 'complex128': torch.complex128}[dtype]
+        self.dtype = dtype
+        self.checkpoint = max(1, int(checkpoint))
+        self.dim = self.plan.dim
+
+    def _slice_params(self, Om, Delta):
+        Om = np.asarray(Om, float)
+        Delta = np.asarray(Delta, float)
+        dadd = -Delta
+        cs, rs, coef = [], [], []
+        for om, da in zip(Om, dadd):
+            c, r = slice_bounds(self.model, om, da)
+            cs.append(c)
+            rs.append(r)
+            a = cheb_coefficients(self.tau, c, r, self.K)
+            coef.append(np.stack([a.real, a.imag], axis=1))
+        coef = np.ascontiguousarray(np.asarray(coef, np.float64))
+        return (np.ascontiguousarray(Om), np.ascontiguousarray(dadd), np.asarray(cs),
+                np.asarray(rs), coef)
+
+    def _forward_slices(self, psi, params, k0, k1, work):
+        Om, dadd, cs, rs, coef = params
+        if k1 <= k0:
+            return
+        sl = slice(k0, k1)
+        om_a = np.ascontiguousarray(Om[sl])
+        da_a = np.ascontiguousarray(dadd[sl])
+        c_a = np.ascontiguousarray(cs[sl])
+        r_a = np.ascontiguousarray(rs[sl])
+        cf = np.ascontiguousarray(coef[sl])
+        nat.check(nat.lib().ryd_evolve_forward(
+            self.plan.handle, nat.dtype_code(psi.dtype), nat.ptr(psi), k1 - k0,
+            om_a.ctypes.data, da_a.ctypes.data, c_a.ctypes.data, r_a.ctypes.data, self.K,
+            cf.ctypes.data, nat.ptr(work), nat.stream_ptr()))
+
+    def propagate(self, psi0, Om, Delta):
+        """ψ(T) from ψ0 (device tensor, copied)."""
+        import torch
+        params = self._slice_params(Om, Delta)
+        psi = psi0.to(self.tdt).clone()
+        work = torch.empty(4 * self.dim, dtype=self.tdt, device=self.plan.device)
+        self._forward_slices(psi, params, 0, len(params[0]), work)
+        return psi
+
+    def loss_and_grad(self, psi0, Om, Delta, target_index):
+        """L = 1 − |<target|ψ(T)>|² and (dL/dΩ_k, dL/dΔ_k) by the adjoint."""
+        import torch
+        P = self.plan
+        params = self._slice_params(Om, Delta)
+        Omv, dadd, cs, rs, coef = params
+        M = len(Omv)
+        psi = psi0.to(self.tdt).clone()
+        work = torch.empty(4 * self.dim, dtype=self.tdt, device=P.device)
+        every = self.checkpoint
+        ckpts = {0: psi.clone()}
+        k = 0
+        while k < M:
+            k1 = min(M, k + every)
+            self._forward_slices(psi, params, k, k1, work)
+            k = k1
+            if k < M:
+                ckpts[k] = psi.clone()
+        o = psi[int(target_index)].item()
+        loss = 1.0 - abs(o) ** 2
+        # cotangent dL/d(conj psi_T) = -o e_target
+        g = torch.zeros(self.dim, dtype=self.tdt, device=P.device)
+        g[int(target_index)] = -o
+        tstore = torch.empty(self.K * self.dim, dtype=self.tdt, device=P.device)
+        vwork = torch.empty(3 * self.dim, dtype=self.tdt, device=P.device)
+        sums = torch.zeros(4 * self.K, dtype=torch.float64, device=P.device)
+        ws = P.workspace()
+        all_sums = np.zeros((M, self.K, 4))
+        L = nat.lib()
+        code = nat.dtype_code(self.tdt)
+        starts = sorted(ckpts)
+        seg_buf = torch.empty((every, self.dim), dtype=self.tdt, device=P.device)
+        for s0 in reversed(starts):
+            s1 = min(M, s0 + every)
+            # recompute slice inputs psi_{s0} .. psi_{s1-1}
+            cur = ckpts[s0].clone()
+            for kk in range(s0, s1):
+                seg_buf[kk - s0].copy_(cur)
+                if kk + 1 < s1:
+                    self._forward_slices(cur, params, kk, kk + 1, work)
+            for kk in range(s1 - 1, s0 - 1, -1):
+                cf = np.ascontiguousarray(coef[kk])
+                nat.check(L.ryd_evolve_adjoint(
+                    P.handle, code, nat.ptr(seg_buf[kk - s0]), nat.ptr(g), float(Omv[kk]),
+                    float(dadd[kk]), float(cs[kk]), float(rs[kk]), self.K, cf.ctypes.data,
+                    nat.ptr(tstore), nat.ptr(vwork), nat.ptr(sums), nat.ptr(ws),
+                    nat.stream_ptr()))
+                all_sums[kk] = sums.view(self.K, 4).cpu().numpy()
+        w = np.full(self.K, 2.0)
+        w[0] = 1.0
+        fac = 2.0 / rs
+        d_om = fac * (all_sums[:, :, 0] @ w)
+        d_dadd = fac * (all_sums[:, :, 1] @ w)
+        return loss, d_om, -d_dadd, psi
+
+
+def pulse_loss_and_grad(problem, psi0, Om, Delta, target_index):
+    return problem.loss_and_grad(psi0, Om, Delta, target_index)
+
+
+def propagate(problem, psi0, Om, Delta):
+    return problem.propagate(psi0, Om, Delta)
+
+
+def _autograd_function():
+    import torch
+
+    class PulseEvolution(torch.autograd.Function):
+        """loss = 1 − |<target|ψ(T)>|² with a hand-written adjoint backward.
+
+        ``PulseEvolution.apply(Om, Delta, problem, psi0, target_index)`` with
+        Om, Delta 1-D float64 tensors (per-slice drive); returns a 0-d tensor."""
+
+        @staticmethod
+        def forward(ctx, Om, Delta, problem, psi0, target_index):
+            loss, gO, gD, _ = problem.loss_and_grad(
+                psi0, Om.detach().cpu().numpy(), Delta.detach().cpu().numpy(), target_index)
+            ctx.save_for_backward(torch.as_tensor(gO, dtype=Om.dtype),
+                                  torch.as_tensor(gD, dtype=Delta.dtype))
+            return torch.tensor(loss, dtype=Om.dtype)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            gO, gD = ctx.saved_tensors
+            return grad_out * gO, grad_out * gD, None, None, None
+
+    return PulseEvolution
+
+
+class _Lazy:
+    _fn = None
+
+    def __getattr__(self, name):
+        if _Lazy._fn is None:
+            _Lazy._fn = _autograd_function()
+        return getattr(_Lazy._fn, name)
+
+
+PulseEvolution = _Lazy()
+
+
+def z2_index(n):
+    """Code of |r g r g ...> (site 0 excited; site 0 is the most significant bit)."""
+    return sum(1 << (n - 1 - i) for i in range(0, int(n), 2))

@@ -150,6 +150,31 @@ int ryd_sector_codes(const ryd_plan* plan, const int64_t* ranks, int64_t first, 
 int ryd_sector_rank(const ryd_plan* plan, const int64_t* codes, int64_t count, int64_t* ranks,
                     void* stream);
 
+/* Piecewise-constant evolution psi <- Π_k exp(-i·CONV·H_k·dt_k) psi on the
+ * full basis (n <= 22, complex dtypes), each slice by a Chebyshev series:
+ *   H_k = om[k]·F + D + dadd[k]·N   (F: the plan's Ω flips, D: its diagonal,
+ *                                    N = Σ_i n_i),
+ *   psi_out = Σ_{j=0..K} a_kj T_j((H_k - c[k])/r[k]) psi_in,
+ * coef = host FP64 [nslices][K+1][re, im] (a_kj, phase and Bessel factors
+ * included).  work: 4 vectors of dim elements.  Replaces the per-interval
+ * expm_multiply of engine_ed._propagate_segments / _propagate_sparse
+ * (engine_ed.py:117-147). */
+int ryd_evolve_forward(const ryd_plan* plan, int dtype, void* psi, int64_t nslices,
+                       const double* om, const double* dadd, const double* c, const double* r,
+                       int K, const double* coef, void* work, void* stream);
+
+/* Vector-Jacobian product of one slice of ryd_evolve_forward (the adjoint;
+ * no reference counterpart — the reference has no gradient).  Given the
+ * slice input psi and g = dL/d(conj psi_out), overwrites g with
+ * dL/d(conj psi_in) and stores for j = 0..K-1 the FP64 sums
+ * {Re<F b_{j+1}, t_j>, Re<N b_{j+1}, t_j>, |b_j|^2, Re<t_j, b_j>} at
+ * sums[4j..4j+3] (t_j: Chebyshev vectors, b_j: Clenshaw adjoints), from which
+ * dL/d om = (2/r)(S_0 + 2 Σ_{j>=1} S_j)[0] and likewise for dadd with [1].
+ * tstore: K vectors; work: 3 vectors; coef: host [K+1][re, im] of this slice. */
+int ryd_evolve_adjoint(const ryd_plan* plan, int dtype, const void* psi, void* g, double om,
+                       double dadd, double c, double r, int K, const double* coef, void* tstore,
+                       void* work, double* sums, void* ws, void* stream);
+
 /* sums[0..2] = {Re<a,b>, |b|^2, |a|^2} over n elements (a may be NULL). */
 int ryd_dots(int dtype, const void* a, const void* b, int64_t n, double* sums, void* ws,
              void* stream);

@@ -1,0 +1,120 @@
+"""Piecewise-constant evolution: forward vs exact exponentials, adjoint gradient
+vs FP64 central differences and a dense-expm autograd reference."""
+import numpy as np
+import pytest
+
+from oracle import rydberg_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(pkg, n, c6=10.0):
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6))
+    return calib, pkg.from_calib(calib, np.ones(n), np.zeros(n))
+
+
+def _drive(M, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.5, 1.5, M), rng.uniform(-1.0, 2.0, M)
+
+
+def test_rabi_closed_form(pkg, dev):
+    import torch
+    n = 1
+    Om = 2 * np.pi * 5.0
+    model = pkg.RydModel(1, np.ones(1, complex), np.zeros(1))
+    prob = pkg.PulseProblem(model, dt=1.0, K=30, dtype='complex128', device=dev)
+    psi0 = torch.zeros(2, dtype=torch.complex128, device=dev)
+    psi0[0] = 1
+    for M in (1, 7, 50):
+        psi = prob.propagate(psi0, np.full(M, Om), np.zeros(M)).cpu().numpy()
+        pred = np.sin(pkg.CONV * Om * M / 2) ** 2
+        assert abs(abs(psi[1]) ** 2 - pred) < 1e-12
+
+
+@pytest.mark.parametrize('n,dtype,tol', [(6, 'complex128', 1e-12), (13, 'complex128', 1e-11),
+                                         (13, 'complex64', 2e-5)])
+def test_forward_vs_exact_segments(pkg, dev, n, dtype, tol):
+    import torch
+    calib, model = _model(pkg, n)
+    M, dt = 12, 0.5
+    Om, De = _drive(M, 1)
+    prob = pkg.PulseProblem(model, dt=dt, K=15, dtype=dtype, device=dev)
+    psi0 = torch.zeros(1 << n, dtype=torch.complex128, device=dev)
+    psi0[0] = 1
+    got = prob.propagate(psi0, Om, De).cpu().numpy()
+    segs = [{'t0': k * dt, 'Om': [Om[k]] * n, 'delta': [-De[k]] * n} for k in range(M)]
+    ref = orc.propagate_segments(n, calib['pairs'], ['I1'], segs, psi0.cpu().numpy(),
+                                 np.array([0.0, M * dt]))[-1]
+    assert np.max(np.abs(got - ref)) < tol
+
+
+def test_adjoint_vs_central_differences(pkg, dev):
+    import torch
+    n, M, dt = 8, 30, 0.05
+    _, model = _model(pkg, n)
+    Om, De = _drive(M, 2)
+    prob = pkg.PulseProblem(model, dt=dt, K=15, dtype='complex128', device=dev, checkpoint=7)
+    psi0 = torch.zeros(1 << n, dtype=torch.complex128, device=dev)
+    psi0[0] = 1
+    tgt = pkg.z2_index(n)
+    L, gO, gD, _ = prob.loss_and_grad(psi0, Om, De, tgt)
+    h = 1e-5
+    for k in (0, 13, M - 1):
+        e = np.zeros(M)
+        e[k] = h
+        fd_o = (prob.loss_and_grad(psi0, Om + e, De, tgt)[0]
+                - prob.loss_and_grad(psi0, Om - e, De, tgt)[0]) / (2 * h)
+        fd_d = (prob.loss_and_grad(psi0, Om, De + e, tgt)[0]
+                - prob.loss_and_grad(psi0, Om, De - e, tgt)[0]) / (2 * h)
+        assert abs(gO[k] - fd_o) < 1e-7 * max(1.0, abs(fd_o)), (k, gO[k], fd_o)
+        assert abs(gD[k] - fd_d) < 1e-7 * max(1.0, abs(fd_d)), (k, gD[k], fd_d)
+
+
+def test_adjoint_vs_dense_expm_autograd(pkg, dev):
+    """Reference gradient of the exact propagator by torch autograd through
+    matrix_exp on the CPU (complex128)."""
+    import torch
+    n, M, dt = 6, 20, 0.05
+    calib, model = _model(pkg, n)
+    Om, De = _drive(M, 3)
+    prob = pkg.PulseProblem(model, dt=dt, K=15, dtype='complex128', device=dev)
+    psi0 = torch.zeros(1 << n, dtype=torch.complex128, device=dev)
+    psi0[0] = 1
+    tgt = pkg.z2_index(n)
+    L, gO, gD, _ = prob.loss_and_grad(psi0, Om, De, tgt)
+    X = torch.as_tensor(orc.kron_hamiltonian(n, np.ones(n), np.zeros(n), [], []))
+    Nop = torch.as_tensor(orc.kron_hamiltonian(n, np.zeros(n), np.ones(n), [], []))
+    V = torch.as_tensor(orc.kron_hamiltonian(n, np.zeros(n), np.zeros(n), calib['pairs'], ['I1']))
+    om_t = torch.tensor(Om, requires_grad=True)
+    de_t = torch.tensor(De, requires_grad=True)
+    psi = torch.zeros(1 << n, dtype=torch.complex128)
+    psi[0] = 1
+    for k in range(M):
+        H = om_t[k] * X - de_t[k] * Nop + V
+        psi = torch.linalg.matrix_exp(-1j * pkg.CONV * dt * H) @ psi
+    loss = 1 - torch.abs(psi[tgt]) ** 2
+    loss.backward()
+    assert abs(float(loss) - L) < 1e-12
+    assert np.max(np.abs(om_t.grad.numpy() - gO)) < 1e-10 * max(1, np.max(np.abs(gO)))
+    assert np.max(np.abs(de_t.grad.numpy() - gD)) < 1e-10 * max(1, np.max(np.abs(gD)))
+
+
+def test_autograd_function_and_complex64(pkg, dev):
+    import torch
+    n, M, dt = 10, 40, 0.05
+    _, model = _model(pkg, n)
+    Om, De = _drive(M, 4)
+    psi0 = torch.zeros(1 << n, dtype=torch.complex128, device=dev)
+    psi0[0] = 1
+    tgt = pkg.z2_index(n)
+    p64 = pkg.PulseProblem(model, dt=dt, dtype='complex128', device=dev)
+    p32 = pkg.PulseProblem(model, dt=dt, dtype='complex64', device=dev)
+    om_t = torch.tensor(Om, requires_grad=True)
+    de_t = torch.tensor(De, requires_grad=True)
+    loss = pkg.PulseEvolution.apply(om_t, de_t, p64, psi0, tgt)
+    loss.backward()
+    L32, g32, d32, _ = p32.loss_and_grad(psi0, Om, De, tgt)
+    assert abs(L32 - float(loss)) < 1e-5
+    assert np.max(np.abs(g32 - om_t.grad.numpy())) < 1e-4 * max(1, np.max(np.abs(g32)))
+    assert np.max(np.abs(d32 - de_t.grad.numpy())) < 1e-4 * max(1, np.max(np.abs(d32)))
