@@ -15,6 +15,7 @@ from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, secto
     sector_rank, chain_edges
 from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolution, z2_index, \
     slice_bounds
+from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
@@ -25,4 +26,4 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'dos_from_moments', 'probe_bits', 'fibonacci', 'chain_blockade_dim',
            'is_chain_blockade_basis', 'sector_codes', 'sector_rank', 'chain_edges', 'CONV',
            'cheb_coefficients', 'terms_for', 'PulseProblem', 'PulseEvolution', 'z2_index',
-           'slice_bounds']
+           'slice_bounds', 'BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip']

@@ -60,6 +60,13 @@ SIGNATURES = {
                                           ctypes.c_int, _vp, _vp, _vp]),
     'ryd_evolve_adjoint': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
                                           ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'ryd_batch_create': (ctypes.c_int, [ctypes.POINTER(Model), ctypes.c_int32, ctypes.c_int,
+                                        ctypes.c_int32, ctypes.POINTER(_vp)]),
+    'ryd_batch_destroy': (ctypes.c_int, [_vp]),
+    'ryd_batch_offset': (_i64, [_vp, ctypes.c_int32]),
+    'ryd_batch_run': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    'ryd_batch_autotune': (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp]),
+    'ryd_batch_results': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     'ryd_dots': (ctypes.c_int, [ctypes.c_int, _vp, _vp, _i64, _vp, _vp, _vp]),
     'ryd_cast': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, _i64, _vp]),
 }

@@ -15,6 +15,7 @@
 #include "observables.hpp"
 #include "sector.hpp"
 #include "evolve.hpp"
+#include "batch.hpp"
 
 using namespace ryd;
 
@@ -777,4 +778,277 @@ extern "C" int ryd_evolve_adjoint(const ryd_plan* P, int dtype, const void* psi,
   if (dtype == RYD_C64)
     return do_evolve_vjp<c32>(P, dtype, psi, g, om, dadd, c, r, K, coef, tstore, work, sums, ws, st);
   return do_evolve_vjp<c64>(P, dtype, psi, g, om, dadd, c, r, K, coef, tstore, work, sums, ws, st);
+}
+
+// ---------------------------------------------------------------------------
+// batched Lanczos
+// ---------------------------------------------------------------------------
+struct ryd_batch {
+  int count = 0;
+  int dtype = RYD_F32;
+  int steps = 0;
+  int device = 0;
+  int64_t total = 0;                 // Σ 2^n
+  std::vector<int> n;
+  std::vector<int64_t> off;
+  // per tile-bit candidate k (tb = 10 + k): instance table and tile map
+  BatchInst* d_inst[3] = {nullptr, nullptr, nullptr};
+  int* d_tile_inst[3] = {nullptr, nullptr, nullptr};
+  int ntiles[3] = {0, 0, 0};
+  double* d_tables = nullptr;        // all elo/ehi/vb tables
+  int choice = 2;                    // chosen candidate (tb = 12 by default)
+  // owned Krylov workspace (fixed addresses for graph capture)
+  void* d_vec = nullptr;             // 3 * total elements
+  double* d_alpha = nullptr;         // [count][steps]
+  double* d_nrm = nullptr;           // [count][steps+1]
+  int* d_stop = nullptr;             // [count]
+  unsigned* d_counter = nullptr;     // [count]
+  double* d_partial = nullptr;       // [max tiles]
+  hipStream_t stream = nullptr;      // capture / replay stream (default priority)
+  hipGraphExec_t exec = nullptr;
+  int exec_choice = -1;
+  hipEvent_t ev = nullptr;
+};
+
+namespace {
+
+BatchLz batch_lz(const ryd_batch* B, int k, int j) {
+  BatchLz L;
+  L.inst = B->d_inst[k];
+  L.tile_inst = B->d_tile_inst[k];
+  L.alpha = B->d_alpha;
+  L.nrm = B->d_nrm;
+  L.stop = B->d_stop;
+  L.counter = B->d_counter;
+  L.partial = B->d_partial;
+  L.ldk = B->steps;
+  L.j = j;
+  L.tiny = B->dtype == RYD_F32 ? 1e-6 : 1e-13;
+  return L;
+}
+
+__global__ void k_batch_reset(int* stop, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) stop[i] = 0;
+}
+
+template <class V>
+int batch_sequence(ryd_batch* B, int k, hipStream_t st) {
+  const size_t vs = sizeof(V);
+  V* base = static_cast<V*>(B->d_vec);
+  V* U[3] = {base, base + B->total, base + 2 * B->total};
+  const unsigned g = (unsigned)B->ntiles[k];
+  k_batch_reset<<<(B->count + 255) / 256, 256, 0, st>>>(B->d_stop, B->count);
+  LAUNCH_CHECK();
+  k_batch_norm0<V><<<g, 256, 0, st>>>(batch_lz(B, k, 0), U[0]);
+  LAUNCH_CHECK();
+  (void)vs;
+  for (int j = 0; j < B->steps; ++j) {
+    BatchLz L = batch_lz(B, k, j);
+    k_batch_apply<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 2) % 3], U[(j + 1) % 3]);
+    LAUNCH_CHECK();
+    k_batch_orth<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 1) % 3]);
+    LAUNCH_CHECK();
+  }
+  return RYD_OK;
+}
+
+int batch_launch(ryd_batch* B, int k, int use_graph) {
+  hipStream_t st = B->stream;
+  if (!use_graph) {
+    return B->dtype == RYD_F32 ? batch_sequence<float>(B, k, st) : batch_sequence<double>(B, k, st);
+  }
+  if (B->exec == nullptr || B->exec_choice != k) {
+    if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = B->dtype == RYD_F32 ? batch_sequence<float>(B, k, st) : batch_sequence<double>(B, k, st);
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(RYD_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(RYD_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+    B->exec_choice = k;
+  }
+  HIP_TRY(hipGraphLaunch(B->exec, st));
+  return RYD_OK;
+}
+
+}  // namespace
+
+extern "C" int ryd_batch_create(const ryd_model* models, int32_t count, int dtype, int32_t steps,
+                                ryd_batch** out) {
+  if (!models || !out || count < 1 || steps < 1)
+    return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_create");
+  if (dtype != RYD_F32 && dtype != RYD_F64)
+    return fail(RYD_ERR_INVALID, "batched Lanczos runs real FP32 or FP64 vectors");
+  *out = nullptr;
+  ryd_batch* B = new ryd_batch();
+  B->count = count;
+  B->dtype = dtype;
+  B->steps = steps;
+  (void)hipGetDevice(&B->device);
+  auto bad = [&](int code, const std::string& msg) {
+    ryd_batch_destroy(B);
+    return fail(code, msg);
+  };
+  // host tables for every candidate tile size
+  std::vector<double> tab;
+  struct Loc { size_t elo, ehi, vb; };
+  std::vector<Loc> loc[3];
+  std::vector<BatchInst> hinst[3];
+  std::vector<int> tmap[3];
+  for (int b = 0; b < count; ++b) {
+    const ryd_model& m = models[b];
+    const int n = m.n;
+    if (n < 1 || n > BMAXN || m.basis_kind != RYD_BASIS_FULL || !m.om_re || !m.delta)
+      return bad(RYD_ERR_INVALID, "batch instances need the full basis with 1 <= n <= 24");
+    for (int i = 0; i < n; ++i)
+      if (m.om_im && m.om_im[i] != 0.0) return bad(RYD_ERR_INVALID, "batch instances need real drives");
+    B->n.push_back(n);
+    B->off.push_back(B->total);
+    std::vector<double> dl(n), vb((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) dl[n - 1 - i] = m.delta[i];
+    for (int q = 0; q < m.n_pairs; ++q) {
+      const int i = m.pair_i[q], j = m.pair_j[q];
+      if (i < 0 || j < 0 || i >= n || j >= n || i == j)
+        return bad(RYD_ERR_INVALID, "pair indices out of range");
+      vb[(size_t)(n - 1 - i) * n + (n - 1 - j)] += m.pair_v[q];
+      vb[(size_t)(n - 1 - j) * n + (n - 1 - i)] += m.pair_v[q];
+    }
+    const size_t vbo = tab.size();
+    tab.insert(tab.end(), vb.begin(), vb.end());
+    for (int k = 0; k < 3; ++k) {
+      const int tb = n < 10 + k ? n : 10 + k;
+      std::vector<double> elo, ehi;
+      build_tables(n, tb, dl, vb, elo, ehi);
+      Loc L;
+      L.vb = vbo;
+      L.elo = tab.size();
+      tab.insert(tab.end(), elo.begin(), elo.end());
+      L.ehi = tab.size();
+      tab.insert(tab.end(), ehi.begin(), ehi.end());
+      loc[k].push_back(L);
+      BatchInst I;
+      I.n = n;
+      I.tb = tb;
+      I.ntiles = 1 << (n - tb);
+      I.pad = 0;
+      I.off = B->total;
+      I.tile0 = (int64_t)tmap[k].size();
+      for (int p = 0; p < BMAXN; ++p) I.ar[p] = 0.0;
+      for (int i = 0; i < n; ++i) I.ar[n - 1 - i] = 0.5 * m.om_re[i];
+      hinst[k].push_back(I);
+      for (int t = 0; t < I.ntiles; ++t) tmap[k].push_back(b);
+    }
+    B->total += 1ll << n;
+  }
+  hipError_t e = hipMalloc(&B->d_tables, sizeof(double) * tab.size());
+  if (e == hipSuccess) e = hipMemcpy(B->d_tables, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice);
+  size_t maxtiles = 0;
+  for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+    for (int b = 0; b < count; ++b) {
+      hinst[k][b].elo = B->d_tables + loc[k][b].elo;
+      hinst[k][b].ehi = B->d_tables + loc[k][b].ehi;
+      hinst[k][b].vb = B->d_tables + loc[k][b].vb;
+    }
+    B->ntiles[k] = (int)tmap[k].size();
+    if (tmap[k].size() > maxtiles) maxtiles = tmap[k].size();
+    e = hipMalloc(&B->d_inst[k], sizeof(BatchInst) * count);
+    if (e == hipSuccess) e = hipMemcpy(B->d_inst[k], hinst[k].data(), sizeof(BatchInst) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&B->d_tile_inst[k], sizeof(int) * tmap[k].size());
+    if (e == hipSuccess) e = hipMemcpy(B->d_tile_inst[k], tmap[k].data(), sizeof(int) * tmap[k].size(), hipMemcpyHostToDevice);
+  }
+  const size_t es = dtype == RYD_F32 ? 4 : 8;
+  if (e == hipSuccess) e = hipMalloc(&B->d_vec, es * 3 * (size_t)B->total);
+  if (e == hipSuccess) e = hipMalloc(&B->d_alpha, sizeof(double) * (size_t)count * steps);
+  if (e == hipSuccess) e = hipMalloc(&B->d_nrm, sizeof(double) * (size_t)count * (steps + 1));
+  if (e == hipSuccess) e = hipMalloc(&B->d_stop, sizeof(int) * count);
+  if (e == hipSuccess) e = hipMalloc(&B->d_counter, sizeof(unsigned) * count);
+  if (e == hipSuccess) e = hipMalloc(&B->d_partial, sizeof(double) * maxtiles);
+  if (e == hipSuccess) {
+    std::vector<unsigned> zero(count, 0u);
+    e = hipMemcpy(B->d_counter, zero.data(), sizeof(unsigned) * count, hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev, hipEventDisableTiming);
+  if (e != hipSuccess)
+    return bad(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
+               std::string("batch allocation: ") + hipGetErrorString(e));
+  *out = B;
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_destroy(ryd_batch* B) {
+  if (!B) return RYD_OK;
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  for (int k = 0; k < 3; ++k) {
+    if (B->d_inst[k]) (void)hipFree(B->d_inst[k]);
+    if (B->d_tile_inst[k]) (void)hipFree(B->d_tile_inst[k]);
+  }
+  void* ptrs[] = {B->d_tables, B->d_vec, B->d_alpha, B->d_nrm, B->d_stop, B->d_counter, B->d_partial};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (B->ev) (void)hipEventDestroy(B->ev);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+  return RYD_OK;
+}
+
+extern "C" int64_t ryd_batch_offset(const ryd_batch* B, int32_t index) {
+  if (!B || index < 0 || index > B->count) return -1;
+  return index == B->count ? B->total : B->off[index];
+}
+
+extern "C" int ryd_batch_run(ryd_batch* B, const void* v0, int use_graph, void* stream) {
+  if (!B || !v0) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_run");
+  hipStream_t caller = static_cast<hipStream_t>(stream);
+  const size_t es = B->dtype == RYD_F32 ? 4 : 8;
+  // order after the caller's stream, copy the start vectors, run, order back
+  HIP_TRY(hipEventRecord(B->ev, caller));
+  HIP_TRY(hipStreamWaitEvent(B->stream, B->ev, 0));
+  HIP_TRY(hipMemcpyAsync(B->d_vec, v0, es * (size_t)B->total, hipMemcpyDeviceToDevice, B->stream));
+  if (int rc = batch_launch(B, B->choice, use_graph)) return rc;
+  HIP_TRY(hipEventRecord(B->ev, B->stream));
+  HIP_TRY(hipStreamWaitEvent(caller, B->ev, 0));
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_autotune(ryd_batch* B, const void* v0, int32_t* chosen_tile_bits,
+                                  double* ms_out /* [3] or NULL */) {
+  if (!B || !v0) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_autotune");
+  const size_t es = B->dtype == RYD_F32 ? 4 : 8;
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  double best = 1e300;
+  int bestk = 2;
+  for (int k = 0; k < 3; ++k) {
+    float ms = 0.f;
+    for (int rep = 0; rep < 2; ++rep) {   // first pass warms the kernels
+      HIP_TRY(hipMemcpyAsync(B->d_vec, v0, es * (size_t)B->total, hipMemcpyDeviceToDevice, B->stream));
+      HIP_TRY(hipEventRecord(a, B->stream));
+      if (int rc = batch_launch(B, k, 0)) return rc;
+      HIP_TRY(hipEventRecord(b, B->stream));
+      HIP_TRY(hipEventSynchronize(b));
+      HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    }
+    if (ms_out) ms_out[k] = ms;
+    if (ms < best) { best = ms; bestk = k; }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  B->choice = bestk;
+  if (chosen_tile_bits) *chosen_tile_bits = 10 + bestk;
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_results(const ryd_batch* B, double* alpha, double* nrm, int32_t* stop) {
+  if (!B) return fail(RYD_ERR_INVALID, "null batch");
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  if (alpha) HIP_TRY(hipMemcpy(alpha, B->d_alpha, sizeof(double) * (size_t)B->count * B->steps, hipMemcpyDeviceToHost));
+  if (nrm) HIP_TRY(hipMemcpy(nrm, B->d_nrm, sizeof(double) * (size_t)B->count * (B->steps + 1), hipMemcpyDeviceToHost));
+  if (stop) HIP_TRY(hipMemcpy(stop, B->d_stop, sizeof(int) * B->count, hipMemcpyDeviceToHost));
+  return RYD_OK;
 }

@@ -175,6 +175,33 @@ int ryd_evolve_adjoint(const ryd_plan* plan, int dtype, const void* psi, void* g
                        double dadd, double c, double r, int K, const double* coef, void* tstore,
                        void* work, double* sums, void* ws, void* stream);
 
+/* Batched Lanczos over many independent full-basis instances of uneven n
+ * (1 <= n <= 24, real drives), FP32 or FP64 vectors with FP64 alpha/beta.
+ * One segmented launch per Lanczos half-step covers every instance; the
+ * whole recurrence is captured once into a hipGraph and replayed
+ * (use_graph = 1).  Replaces the dense batched eigh of batched_ground_state
+ * (src/core/manybody/equilibrium.py:251-311), which needs uniform dim, and
+ * the serial loop of run_ensemble (ensemble.py:51-175).
+ * The batch object owns its Krylov workspace and a default-priority stream
+ * (so that the captured graph has fixed addresses); it is not thread-safe.
+ * Start vectors are concatenated in instance order; ryd_batch_offset(b, i)
+ * gives instance i's offset (i = count gives the total length). */
+typedef struct ryd_batch ryd_batch;
+int ryd_batch_create(const ryd_model* models, int32_t count, int dtype, int32_t steps,
+                     ryd_batch** out);
+int ryd_batch_destroy(ryd_batch* batch);
+int64_t ryd_batch_offset(const ryd_batch* batch, int32_t index);
+/* Runs the recurrence from the device start vectors v0, ordered after and
+ * before `stream`. */
+int ryd_batch_run(ryd_batch* batch, const void* v0, int use_graph, void* stream);
+/* Times the recurrence for tiles of 2^10, 2^11, 2^12 codes and keeps the
+ * fastest; ms_out (host, 3 entries) may be NULL. */
+int ryd_batch_autotune(ryd_batch* batch, const void* v0, int32_t* chosen_tile_bits,
+                       double* ms_out);
+/* Host copies: alpha [count][steps], nrm [count][steps+1] (beta_j = nrm[j+1]),
+ * stop [count] (1 where the recurrence broke down). */
+int ryd_batch_results(const ryd_batch* batch, double* alpha, double* nrm, int32_t* stop);
+
 /* sums[0..2] = {Re<a,b>, |b|^2, |a|^2} over n elements (a may be NULL). */
 int ryd_dots(int dtype, const void* a, const void* b, int64_t n, double* sums, void* ws,
              void* stream);

@@ -546,9 +546,13 @@ def segment_drive(t, segments, n, interp='left'):
     return om, de
 
 
-def propagate_segments(n, pairs, terms, segments, psi0, t, codes=None, interp='left'):
+def propagate_segments(n, pairs, terms, segments, psi0, t, codes=None, interp='left',
+                       method='expm'):
     """ψ(t_k) by exact exponentials per piece, exp(-i·CONV·H·Δt)
-    (reference engine_ed.py:130-147 with expm instead of expm_multiply)."""
+    (reference engine_ed.py:130-147).  ``method='expm_multiply'`` is the
+    reference's own sparse stepping (engine_ed.py:145); ``'expm'`` uses a
+    dense exponential (small N)."""
+    from scipy.sparse.linalg import expm_multiply
     psi = np.asarray(psi0, complex).copy()
     out = [psi.copy()]
     cache = {}
@@ -557,8 +561,12 @@ def propagate_segments(n, pairs, terms, segments, psi0, t, codes=None, interp='l
             om, de = segment_drive(0.5 * (a + b), segments, n, interp)
             key = (tuple(om), tuple(de))
             if key not in cache:
-                cache[key] = build_H_sparse(n, om, de, pairs, terms, codes).toarray()
-            psi = sla.expm(-1j * CONV * (b - a) * cache[key]) @ psi
+                H = build_H_sparse(n, om, de, pairs, terms, codes)
+                cache[key] = H.toarray() if method == 'expm' else H.tocsr()
+            if method == 'expm':
+                psi = sla.expm(-1j * CONV * (b - a) * cache[key]) @ psi
+            else:
+                psi = expm_multiply((-1j * CONV * (b - a)) * cache[key], psi)
         out.append(psi.copy())
     return np.asarray(out)
 

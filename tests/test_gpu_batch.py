@@ -1,0 +1,70 @@
+"""Batched Lanczos over uneven-N instances against per-instance runs and dense
+ground states; hipGraph replays against the eager sequence."""
+import numpy as np
+import pytest
+
+from oracle import rydberg_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _models(pkg, ns, seed=0):
+    rng = np.random.default_rng(seed)
+    out, calibs = [], []
+    for i, n in enumerate(ns):
+        pos = pkg.chain_positions(n) + rng.normal(0, 0.1, size=(n, 3))
+        calib = pkg.from_positions(pos, pkg.c6_coupling(10.0))
+        De = np.linspace(-2, 4, 32)[i % 32]
+        out.append(pkg.from_calib(calib, np.ones(n), -De * np.ones(n)))
+        calibs.append((calib, De))
+    return out, calibs
+
+
+def test_batch_matches_single_instance_recurrence(pkg, dev):
+    import torch
+    ns = [4, 7, 10, 11, 12, 13, 15, 9]
+    models, calibs = _models(pkg, ns)
+    steps = 90
+    B = pkg.BatchLanczos(models, steps=steps, dtype='float64', device=dev)
+    v0 = B.start_vectors(3)
+    B.run(v0, use_graph=False)
+    alpha, nrm, stop = B.coefficients()
+    for b, m in enumerate(models):
+        P = pkg.Plan(m, device=dev)
+        seg = v0[B.offsets[b]:B.offsets[b + 1]].clone()
+        R = pkg.lanczos_run(P, seg, steps, 1, 0.0, want=1)
+        # identical recurrences until loss of orthogonality amplifies rounding
+        k = min(R.j, (1 << m.n) // 2, 20)
+        assert np.allclose(alpha[b, :k], R.a[:k], rtol=1e-10, atol=1e-10)
+        assert np.allclose(nrm[b, 1:k], R.b[:k - 1], rtol=1e-10, atol=1e-10)
+    gs = B.ground_states()
+    for b, (calib, De) in enumerate(calibs):
+        n = ns[b]
+        H = orc.build_H_sparse(n, np.ones(n), -De * np.ones(n), calib['pairs'], ['I1'])
+        e0 = (orc.ground_state_dense(H) if n <= 10 else orc.ground_state_eigsh(H))['E0']
+        assert abs(gs['E0'][b] - e0) < 1e-8 * max(1, abs(e0)), (n, gs['E0'][b], e0)
+
+
+def test_batch_fp32_graph_replays_and_autotune(pkg, dev):
+    import torch
+    models = pkg.disorder_ensemble(count=24, nmin=10, nmax=16, seed=5)
+    B = pkg.BatchLanczos(models, steps=150, dtype='float32', device=dev)
+    v0 = B.start_vectors(1)
+    B.run(v0, use_graph=False)
+    ref_alpha, ref_nrm, _ = B.coefficients()
+    for rep in range(3):
+        B.run(v0, use_graph=True)
+        a, nr, _ = B.coefficients()
+        assert np.array_equal(a, ref_alpha) and np.array_equal(nr, ref_nrm), rep
+        torch.randn(1000, device=dev).sum().item()      # eager work between replays
+    tb = B.autotune(v0)
+    assert tb in (10, 11, 12) and set(B.autotune_ms) == {10, 11, 12}
+    B.run(v0, use_graph=True)
+    gs = B.ground_states()
+    B64 = pkg.BatchLanczos(models, steps=150, dtype='float64', device=dev)
+    B64.run(B64.start_vectors(1), use_graph=True)
+    g64 = B64.ground_states()
+    conv = g64['ritz_bound'] < 1e-6 * np.abs(g64['E0'])
+    assert conv.sum() >= len(models) // 2
+    rel = np.abs(gs['E0'] - g64['E0']) / np.abs(g64['E0'])
+    assert np.max(rel[conv]) < 2e-5
