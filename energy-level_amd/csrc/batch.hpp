@@ -3,8 +3,7 @@
 // All instances' vectors are concatenated; every launch covers the tiles of
 // all instances (a tile is 2^tb consecutive codes of one instance, or the
 // whole instance when n < tb), so one Lanczos half-step of the whole batch is
-// one launch.  Per-instance FP64 sums close through a per-instance arrival
-// counter: the last tile of an instance to finish writes alpha/nrm for it.
+// one launch plus a per-instance finishing launch for the FP64 sums.
 //   apply : w = H (u_j / nrm_j) − (nrm_j / nrm_{j−1}) u_{j−1},   alpha_j = <q_j, w>
 //   orth  : u_{j+1} = w − (alpha_j / nrm_j) u_j,                  nrm_{j+1} = |u_{j+1}|
 // Instances whose recurrence broke down (nrm <= tiny) stop updating.
@@ -35,36 +34,33 @@ struct BatchLz {
   double* alpha;          // [count][ldk]
   double* nrm;            // [count][ldk+1]
   int* stop;              // [count]
-  unsigned* counter;      // [count]
   double* partial;        // [total tiles]
   int ldk;
   int j;
   double tiny;
 };
 
-// Per-instance reduction: each tile stores its partial, the instance's last
-// tile sums them in fixed order and runs the post-operation (0: alpha, 1: norm).
-__device__ inline void batch_finish(const BatchLz& L, const BatchInst& I, int b, int ltile,
-                                    double v, int post) {
+// Per-instance reduction: each tile stores its block sum; k_batch_finish (one
+// block per instance, next launch) sums the instance's tiles in fixed order and
+// stores alpha_j (post 0) or nrm_{j+1} with the breakdown flag (post 1).
+__device__ inline void batch_store(const BatchLz& L, const BatchInst& I, int ltile, double v) {
   __shared__ double red[4];
-  __shared__ int last;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   v = wave_sum(v);
   if (lane == 0) red[wid] = v;
   __syncthreads();
-  if (t == 0) {
-    L.partial[I.tile0 + ltile] = red[0] + red[1] + red[2] + red[3];
-    __threadfence();
-    const unsigned prev = atomicAdd(&L.counter[b], 1u);
-    last = prev == (unsigned)(I.ntiles - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+  if (t == 0) L.partial[I.tile0 + ltile] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_batch_finish(BatchLz L, int post) {
+  __shared__ double red[4];
+  const int b = blockIdx.x;
+  if (L.stop[b]) return;
+  const BatchInst& I = L.inst[b];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   double s = 0.0;
-  for (int k = t; k < I.ntiles; k += blockDim.x) s += load_coherent(L.partial + I.tile0 + k);
+  for (int k = t; k < I.ntiles; k += blockDim.x) s += L.partial[I.tile0 + k];
   s = wave_sum(s);
-  __syncthreads();
   if (lane == 0) red[wid] = s;
   __syncthreads();
   if (t == 0) {
@@ -76,12 +72,14 @@ __device__ inline void batch_finish(const BatchLz& L, const BatchInst& I, int b,
       L.nrm[(size_t)b * (L.ldk + 1) + L.j + 1] = nv;
       if (!(nv > L.tiny)) L.stop[b] = 1;
     }
-    __threadfence();
-    L.counter[b] = 0u;
   }
 }
 
-// apply half-step (real element types)
+// apply half-step (real element types).  Instances with tb >= 10 use the
+// full-basis tile layout (element e = r·1024 + t·4 + c, R = 2^(tb-10) groups):
+// bits 0..1 and 10..tb-1 flip in registers, bits 2..9 through LDS, bits >= tb
+// from the partner tile in global memory with 4-wide loads.  Smaller
+// instances (one tile of 2^n < 1024 codes) take the generic LDS path.
 template <class V>
 __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restrict__ xall,
                                                      const V* __restrict__ zall, V* __restrict__ yall) {
@@ -96,7 +94,6 @@ __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restr
   const int ltile = (int)(gt - I.tile0);
   const int t = threadIdx.x;
   const int n = I.n, tb = I.tb;
-  const int R = tb > 10 ? 1 << (tb - 10) : 1;
   const int64_t base = (int64_t)ltile << tb;
   const V* x = xall + I.off;
   const V* z = zall + I.off;
@@ -105,20 +102,6 @@ __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restr
   const double nj = nrm[L.j];
   const double s = 1.0 / nj;
   const double cz = L.j > 0 ? -nj / nrm[L.j - 1] : 0.0;
-  const int D = 1 << tb;                 // codes in this tile
-  // stage
-  V xs[4][4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (r >= R) break;
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int e = r * 1024 + t * 4 + c;
-      xs[r][c] = e < D ? (V)(x[base + e] * s) : (V)0;
-      if (e < D) lds[e] = xs[r][c];
-    }
-  }
-  // cross field of the high bits on the low bits, per tile
   if (t < tb) {
     double f = 0.0;
     const double* row = I.vb + (size_t)t * n;
@@ -126,34 +109,134 @@ __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restr
       if ((ltile >> (q - tb)) & 1) f += row[q];
     Fx[t] = f;
   }
-  __syncthreads();
-  {
-    double v = 0.0;
-    for (int p = 0; p < 8 && p < tb; ++p)
-      if ((t >> p) & 1) v += Fx[p];
-    T0[t] = v;
-    if (t < 16) {
-      double w = 0.0;
-      for (int p = 8; p < tb; ++p)
-        if ((t >> (p - 8)) & 1) w += Fx[p];
-      T1[t] = w;
-    }
-  }
-  const double eh = I.ehi[ltile];
-  __syncthreads();
   double sa = 0.0;
+  if (tb >= 10) {
+    const int R = 1 << (tb - 10);
+    V xs[4][4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (r >= R) break;
+    for (int r = 0; r < 4; ++r) {
+      if (r < R) {
+        V4<V> q = *reinterpret_cast<const V4<V>*>(x + base + r * 1024 + t * 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) xs[r][c] = (V)(q.v[c] * s);
+        V4<V> w;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) w.v[c] = xs[r][c];
+        *reinterpret_cast<V4<V>*>(lds + r * 1024 + t * 4) = w;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) xs[r][c] = (V)0;
+      }
+    }
+    __syncthreads();
+    {
+      double v = 0.0;
+#pragma unroll
+      for (int p = 0; p < 8; ++p)
+        if ((t >> p) & 1) v += Fx[p];
+      T0[t] = v;
+      if (t < 16) {
+        double w = 0.0;
+        for (int p = 8; p < tb; ++p)
+          if ((t >> (p - 8)) & 1) w += Fx[p];
+        T1[t] = w;
+      }
+    }
+    const double eh = I.ehi[ltile];
+    __syncthreads();
+    V acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int e = r * 1024 + t * 4 + c;
+        acc[r][c] = r < R ? (V)((eh + I.elo[e] + T0[e & 255] + T1[e >> 8]) * xs[r][c]) : (V)0;
+      }
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const V a = (V)I.ar[p];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] += a * xs[r][c ^ (1 << p)];
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (10 + k >= tb) break;
+      const V a = (V)I.ar[10 + k];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] += a * xs[(r ^ (1 << k)) & 3][c];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const V a = (V)I.ar[2 + k];
+      const int tp = t ^ (1 << k);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r >= R) break;
+        V4<V> q = *reinterpret_cast<const V4<V>*>(lds + r * 1024 + tp * 4);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] += a * q.v[c];
+      }
+    }
+    for (int p = tb; p < n; ++p) {
+      const V a = (V)(I.ar[p] * s);
+      const int64_t pbase = (int64_t)(ltile ^ (1 << (p - tb))) << tb;
+      V4<V> q[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < R) q[r] = *reinterpret_cast<const V4<V>*>(x + pbase + r * 1024 + t * 4);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r >= R) break;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] += a * q[r].v[c];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (r >= R) break;
+      const int64_t g0 = base + r * 1024 + t * 4;
+      V4<V> zq;
+      if (L.j > 0) zq = *reinterpret_cast<const V4<V>*>(z + g0);
+      V4<V> o;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        V v = acc[r][c];
+        if (L.j > 0) v += (V)cz * zq.v[c];
+        o.v[c] = v;
+        sa += (double)xs[r][c] * (double)v;
+      }
+      *reinterpret_cast<V4<V>*>(y + g0) = o;
+    }
+  } else {
+    // generic path: one tile of 2^tb < 1024 codes, all flips through LDS
+    const int D = 1 << tb;
+    V xs[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const int e = r * 1024 + t * 4 + c;
+      const int e = t * 4 + c;
+      xs[c] = e < D ? (V)(x[base + e] * s) : (V)0;
+      if (e < D) lds[e] = xs[c];
+    }
+    __syncthreads();
+    {
+      double v = 0.0;
+      for (int p = 0; p < 8 && p < tb; ++p)
+        if ((t >> p) & 1) v += Fx[p];
+      T0[t] = v;
+      if (t < 16) T1[t] = 0.0;
+    }
+    const double eh = I.ehi[ltile];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int e = t * 4 + c;
       if (e >= D) continue;
-      const double d = eh + I.elo[e] + T0[e & 255] + T1[e >> 8];
-      V acc = (V)(d * xs[r][c]);
-      // flips inside the tile (LDS / registers)
+      V acc = (V)((eh + I.elo[e] + T0[e & 255] + T1[e >> 8]) * xs[c]);
       for (int p = 0; p < tb; ++p) acc += (V)I.ar[p] * lds[e ^ (1 << p)];
-      // flips to other tiles of the instance
       for (int p = tb; p < n; ++p) {
         const int64_t pe = ((int64_t)(ltile ^ (1 << (p - tb))) << tb) + e;
         acc += (V)(I.ar[p] * s) * x[pe];
@@ -161,10 +244,10 @@ __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restr
       V v = acc;
       if (L.j > 0) v += (V)cz * z[base + e];
       y[base + e] = v;
-      sa += (double)xs[r][c] * (double)v;
+      sa += (double)xs[c] * (double)v;
     }
   }
-  batch_finish(L, I, b, ltile, sa, 0);
+  batch_store(L, I, ltile, sa);
 }
 
 template <class V>
@@ -185,7 +268,7 @@ __global__ __launch_bounds__(256) void k_batch_orth(BatchLz L, const V* __restri
     wall[base + e] = v;
     acc += (double)v * (double)v;
   }
-  batch_finish(L, I, b, ltile, acc, 1);
+  batch_store(L, I, ltile, acc);
 }
 
 // nrm[b][0] = |start_b|
@@ -199,9 +282,7 @@ __global__ __launch_bounds__(256) void k_batch_norm0(BatchLz L, const V* __restr
   const int64_t base = I.off + ((int64_t)ltile << I.tb);
   double acc = 0.0;
   for (int e = threadIdx.x; e < D; e += blockDim.x) acc += (double)vall[base + e] * (double)vall[base + e];
-  BatchLz L0 = L;
-  L0.j = -1;
-  batch_finish(L0, I, b, ltile, acc, 1);
+  batch_store(L, I, ltile, acc);
 }
 
 }  // namespace ryd

@@ -86,7 +86,7 @@ __device__ __forceinline__ int load_coherent(const int* p) {
 }
 
 // Reduction bookkeeping in the caller's workspace:
-//   ws: [counter : 256 B][partials : nblocks * NRED doubles]
+//   ws: [header : 256 B][partials : nblocks * NRED doubles]
 constexpr int NRED = 4;
 constexpr size_t WS_HEADER = 256;
 
@@ -109,7 +109,6 @@ struct LzState {
 
 struct Reduce {
   double* partials;      // per-block partial sums (NRED each)
-  unsigned* counter;     // arrival counter, zero between launches
   int post;              // PostOp
   double* out;           // POST_STORE / POST_ACCUM target (NRED doubles)
   double* alpha;         // Lanczos alpha array
@@ -118,13 +117,16 @@ struct Reduce {
   int j;                 // Lanczos step
 };
 
-// Block-level FP64 reduction of NRED values; the last block to arrive
-// reduces all partials and runs the post-operation.  Every thread of the
-// block calls it; blockDim.x is a multiple of 64, at most 1024.
-__device__ inline void block_reduce_finish(double v0, double v1, double v2, double v3,
-                                           const Reduce& rd) {
+// Block-level FP64 reduction of NRED values into partials[blockIdx.x][NRED]
+// with plain stores.  Every thread of the block calls it (blockDim.x a
+// multiple of 64, at most 1024).  The sums over blocks are formed by
+// k_reduce_finish in the next launch on the same stream: the kernel boundary
+// makes the partials visible on every XCD, so no device-scope fence (which
+// writes back L2 on this multi-XCD part) is needed, and the fixed-order sum
+// keeps every reduction deterministic.
+__device__ inline void block_reduce_store(double v0, double v1, double v2, double v3,
+                                          double* partials) {
   __shared__ double red_lds[16 * NRED];
-  __shared__ int am_last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   v0 = wave_sum(v0); v1 = wave_sum(v1); v2 = wave_sum(v2); v3 = wave_sum(v3);
   if (lane == 0) {
@@ -137,62 +139,53 @@ __device__ inline void block_reduce_finish(double v0, double v1, double v2, doub
     const int nw = blockDim.x >> 6;
     double s = 0.0;
     for (int w = 0; w < nw; ++w) s += red_lds[w * NRED + k];
-    rd.partials[(size_t)blockIdx.x * NRED + k] = s;
+    partials[(size_t)blockIdx.x * NRED + k] = s;
   }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(rd.counter, 1u);
-    am_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!am_last) return;
-  __threadfence();
-  // last block: fixed-order sum over blocks
+}
+
+// One block: fixed-order sum of nblocks partials and the post-operation.
+__global__ __launch_bounds__(256) void k_reduce_finish(Reduce rd, unsigned nblocks) {
+  __shared__ double red_lds[4 * NRED];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (rd.st && rd.post != POST_LZ_NORM && rd.st->stop) return;
   double s[NRED] = {0.0, 0.0, 0.0, 0.0};
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
-    for (int k = 0; k < NRED; ++k) s[k] += load_coherent(rd.partials + (size_t)b * NRED + k);
+    for (int k = 0; k < NRED; ++k) s[k] += rd.partials[(size_t)b * NRED + k];
   }
 #pragma unroll
   for (int k = 0; k < NRED; ++k) s[k] = wave_sum(s[k]);
-  __syncthreads();
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NRED; ++k) red_lds[wid * NRED + k] = s[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[NRED];
-    const int nw = blockDim.x >> 6;
-    for (int k = 0; k < NRED; ++k) {
-      t[k] = 0.0;
-      for (int w = 0; w < nw; ++w) t[k] += red_lds[w * NRED + k];
-    }
-    switch (rd.post) {
-      case POST_STORE:
-        for (int k = 0; k < NRED; ++k) rd.out[k] = t[k];
-        break;
-      case POST_ACCUM:
-        for (int k = 0; k < NRED; ++k) rd.out[k] += t[k];
-        break;
-      case POST_LZ_ALPHA:
-        rd.alpha[rd.j] = t[0];
-        break;
-      case POST_LZ_NORM: {
-        double nv = sqrt(t[1] > 0.0 ? t[1] : 0.0);
-        rd.nrm[rd.j + 1] = nv;
-        if (!(nv > rd.st->tiny)) {   // also catches NaN
-          rd.st->stop = 1;
-          rd.st->steps = rd.j + 1;
-        }
-        break;
+  if (threadIdx.x != 0) return;
+  double t[NRED];
+  for (int k = 0; k < NRED; ++k) t[k] = red_lds[k] + red_lds[NRED + k] + red_lds[2 * NRED + k] +
+                                        red_lds[3 * NRED + k];
+  switch (rd.post) {
+    case POST_STORE:
+      for (int k = 0; k < NRED; ++k) rd.out[k] = t[k];
+      break;
+    case POST_ACCUM:
+      for (int k = 0; k < NRED; ++k) rd.out[k] += t[k];
+      break;
+    case POST_LZ_ALPHA:
+      rd.alpha[rd.j] = t[0];
+      break;
+    case POST_LZ_NORM: {
+      if (rd.st->stop) break;
+      const double nv = sqrt(t[1] > 0.0 ? t[1] : 0.0);
+      rd.nrm[rd.j + 1] = nv;
+      if (!(nv > rd.st->tiny)) {   // also catches NaN
+        rd.st->stop = 1;
+        rd.st->steps = rd.j + 1;
       }
-      default:
-        break;
+      break;
     }
-    __threadfence();
-    *rd.counter = 0u;   // ready for the next launch on this workspace
+    default:
+      break;
   }
 }
 

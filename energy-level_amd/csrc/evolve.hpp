@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
       evo_finish(io, base + e, fl[r][c], xs[r][c], d, pop, s0, s1, s2, s3);
     }
   }
-  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
 template <class V>
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_evo_small(FullModel M, EvoIO<V> io, Red
     const double d = M.elo[g & ((1ull << M.tb) - 1)] + M.ehi[g >> M.tb];
     evo_finish(io, g, fl, xe, d, (double)__popcll(g), s0, s1, s2, s3);
   }
-  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
 // y = a·x (complex a; y may alias nothing)

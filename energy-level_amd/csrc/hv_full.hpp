@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
     }
     *reinterpret_cast<V4<V>*>(io.y + g0) = o;
   }
-  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
 // Pass B of the split H·v for large n: the top TOPB bits (n-TOPB .. n-1).
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(512) void k_full_top(FullModel M, ApplyIO<V> io, Re
       if (io.probe) s3 += zdot(((io.probe[e >> 5] >> (e & 31)) & 1u) ? 1.0 : -1.0, v);
     }
   }
-  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
 // n below the tile size: one element per thread, partners from global memory.
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void k_full_small(FullModel M, ApplyIO<V> io, 
       if (io.probe) s3 = zdot(((io.probe[g >> 5] >> (g & 31)) & 1u) ? 1.0 : -1.0, v);
     }
   }
-  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
 }  // namespace ryd

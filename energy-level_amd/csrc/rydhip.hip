@@ -149,6 +149,8 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
     io.partial = 0;
     k_fib_apply<V><<<dim3(fib_grid(P)), dim3(256), 0, st>>>(fib_model(P), io, rd);
     LAUNCH_CHECK();
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, fib_grid(P));
+    LAUNCH_CHECK();
     return RYD_OK;
   }
   FullModel M = full_model(P, dtype);
@@ -160,18 +162,24 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
     io.partial = split ? 1 : 0;
     k_full_tiled<V><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(M, io, rd);
     LAUNCH_CHECK();
+    unsigned grid = (unsigned)ntiles;
     if (split) {
       io.partial = 0;
       const uint64_t nb = 1ull << (P->n - TOPB - run_bits<V>());
       k_full_top<V><<<dim3((unsigned)nb), dim3(512), 0, st>>>(M, io, rd);
       LAUNCH_CHECK();
+      grid = (unsigned)nb;
     }
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
+    LAUNCH_CHECK();
   } else {
     const uint64_t D = 1ull << P->n;
     const unsigned grid = (unsigned)((D + 255) / 256);
     io.nfar = 0;
     io.partial = 0;
     k_full_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+    LAUNCH_CHECK();
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
     LAUNCH_CHECK();
   }
   return RYD_OK;
@@ -192,7 +200,6 @@ unsigned stream_grid(int64_t D) {
 
 Reduce make_reduce(void* ws, int post, double* out) {
   Reduce rd;
-  rd.counter = reinterpret_cast<unsigned*>(ws);
   rd.partials = reinterpret_cast<double*>(reinterpret_cast<char*>(ws) + WS_HEADER);
   rd.post = post;
   rd.out = out;
@@ -246,6 +253,8 @@ int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64
     rd.j = -1;
     k_dots<V><<<sg, 256, 0, st>>>(nullptr, B[0], D, rd);
     LAUNCH_CHECK();
+    k_reduce_finish<<<1, 256, 0, st>>>(rd, sg);
+    LAUNCH_CHECK();
   }
   const int* stop = replay ? nullptr : &S->stop;
   for (int64_t j = j0; j < j0 + steps; ++j) {
@@ -265,6 +274,7 @@ int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64
     io.reduce = !replay;
     Reduce ra = make_reduce(ws, POST_LZ_ALPHA, nullptr);
     ra.alpha = alpha;
+    ra.st = S;
     ra.j = (int)j;
     int rc = launch_apply<V>(P, dtype, io, ra, st);
     if (rc) return rc;
@@ -274,6 +284,8 @@ int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64
     rb.j = (int)j;
     k_lz_orth<V><<<sg, 256, 0, st>>>(uj, w, D, alpha, nrm, (int)j, static_cast<V*>(ritz0),
                                      static_cast<V*>(ritz1), y0, y1, stop, !replay, rb);
+    LAUNCH_CHECK();
+    if (!replay) k_reduce_finish<<<1, 256, 0, st>>>(rb, sg);
     LAUNCH_CHECK();
   }
   return RYD_OK;
@@ -429,7 +441,7 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
       return cleanup(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
                      std::string("table upload: ") + hipGetErrorString(e));
   }
-  // workspace: counter header + NRED partials for the largest grid we launch
+  // workspace: header + NRED partials for the largest grid we launch
   size_t blocks = 8192;
   for (int dt = RYD_F32; dt <= RYD_C128; ++dt) {
     size_t g = apply_grid(P, dt);
@@ -525,6 +537,8 @@ int do_dots(const void* a, const void* b, int64_t n, double* sums, void* ws, hip
   Reduce rd = make_reduce(ws, POST_STORE, sums);
   k_dots<V><<<stream_grid(n), 256, 0, st>>>(static_cast<const V*>(a), static_cast<const V*>(b), n, rd);
   LAUNCH_CHECK();
+  k_reduce_finish<<<1, 256, 0, st>>>(rd, stream_grid(n));
+  LAUNCH_CHECK();
   return RYD_OK;
 }
 }  // namespace
@@ -582,7 +596,7 @@ int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t 
   const uint64_t ob = P->n < OB ? P->n : OB;
   const uint64_t ntiles = (1ull << P->n) >> ob;
   const unsigned g = (unsigned)(ntiles < OCC_BLOCKS ? ntiles : OCC_BLOCKS);
-  // partials after the counter header, which must stay zero for reducing calls
+  // partials after the header
   double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
   k_site_occ<V><<<g, 256, 0, st>>>(static_cast<const V*>(x), P->n, part);
   LAUNCH_CHECK();
@@ -635,12 +649,16 @@ template <class V>
 int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t st) {
   FullModel M = full_model(P, dtype);
   constexpr int TB = tile_bits<V>();
+  unsigned grid;
   if (P->n >= TB) {
-    k_evo_tiled<V><<<dim3((unsigned)(1ull << (P->n - TB))), dim3(256), 0, st>>>(M, io, rd);
+    grid = (unsigned)(1ull << (P->n - TB));
+    k_evo_tiled<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
   } else {
-    const unsigned grid = (unsigned)(((1ull << P->n) + 255) / 256);
+    grid = (unsigned)(((1ull << P->n) + 255) / 256);
     k_evo_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
   }
+  LAUNCH_CHECK();
+  if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
   LAUNCH_CHECK();
   return RYD_OK;
 }
@@ -802,7 +820,6 @@ struct ryd_batch {
   double* d_alpha = nullptr;         // [count][steps]
   double* d_nrm = nullptr;           // [count][steps+1]
   int* d_stop = nullptr;             // [count]
-  unsigned* d_counter = nullptr;     // [count]
   double* d_partial = nullptr;       // [max tiles]
   hipStream_t stream = nullptr;      // capture / replay stream (default priority)
   hipGraphExec_t exec = nullptr;
@@ -819,7 +836,6 @@ BatchLz batch_lz(const ryd_batch* B, int k, int j) {
   L.alpha = B->d_alpha;
   L.nrm = B->d_nrm;
   L.stop = B->d_stop;
-  L.counter = B->d_counter;
   L.partial = B->d_partial;
   L.ldk = B->steps;
   L.j = j;
@@ -842,12 +858,18 @@ int batch_sequence(ryd_batch* B, int k, hipStream_t st) {
   LAUNCH_CHECK();
   k_batch_norm0<V><<<g, 256, 0, st>>>(batch_lz(B, k, 0), U[0]);
   LAUNCH_CHECK();
+  k_batch_finish<<<B->count, 256, 0, st>>>(batch_lz(B, k, -1), 1);
+  LAUNCH_CHECK();
   (void)vs;
   for (int j = 0; j < B->steps; ++j) {
     BatchLz L = batch_lz(B, k, j);
     k_batch_apply<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 2) % 3], U[(j + 1) % 3]);
     LAUNCH_CHECK();
+    k_batch_finish<<<B->count, 256, 0, st>>>(L, 0);
+    LAUNCH_CHECK();
     k_batch_orth<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 1) % 3]);
+    LAUNCH_CHECK();
+    k_batch_finish<<<B->count, 256, 0, st>>>(L, 1);
     LAUNCH_CHECK();
   }
   return RYD_OK;
@@ -965,12 +987,7 @@ extern "C" int ryd_batch_create(const ryd_model* models, int32_t count, int dtyp
   if (e == hipSuccess) e = hipMalloc(&B->d_alpha, sizeof(double) * (size_t)count * steps);
   if (e == hipSuccess) e = hipMalloc(&B->d_nrm, sizeof(double) * (size_t)count * (steps + 1));
   if (e == hipSuccess) e = hipMalloc(&B->d_stop, sizeof(int) * count);
-  if (e == hipSuccess) e = hipMalloc(&B->d_counter, sizeof(unsigned) * count);
   if (e == hipSuccess) e = hipMalloc(&B->d_partial, sizeof(double) * maxtiles);
-  if (e == hipSuccess) {
-    std::vector<unsigned> zero(count, 0u);
-    e = hipMemcpy(B->d_counter, zero.data(), sizeof(unsigned) * count, hipMemcpyHostToDevice);
-  }
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev, hipEventDisableTiming);
   if (e != hipSuccess)
@@ -987,7 +1004,7 @@ extern "C" int ryd_batch_destroy(ryd_batch* B) {
     if (B->d_inst[k]) (void)hipFree(B->d_inst[k]);
     if (B->d_tile_inst[k]) (void)hipFree(B->d_tile_inst[k]);
   }
-  void* ptrs[] = {B->d_tables, B->d_vec, B->d_alpha, B->d_nrm, B->d_stop, B->d_counter, B->d_partial};
+  void* ptrs[] = {B->d_tables, B->d_vec, B->d_alpha, B->d_nrm, B->d_stop, B->d_partial};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B->ev) (void)hipEventDestroy(B->ev);

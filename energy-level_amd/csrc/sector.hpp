@@ -113,6 +113,11 @@ __device__ inline void fib_tile_head(const FibModel& M, int64_t r0, uint64_t* hs
   meta[1] = cnt0;
 }
 
+// Flips of bits below FIB_NEAR stay within FIB_HALO = F(FIB_NEAR + 1) ranks,
+// so they read an LDS copy of the tile with that halo on both sides.
+constexpr int FIB_NEAR = 12;
+constexpr int FIB_HALO = 233;      // F(13)
+
 template <class V>
 __global__ __launch_bounds__(256) void k_fib_apply(FibModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
@@ -122,6 +127,7 @@ __global__ __launch_bounds__(256) void k_fib_apply(FibModel M, ApplyIO<V> io, Re
   __shared__ double ehi[2];
   __shared__ uint64_t hs[2];
   __shared__ int64_t meta[2];   // rho0, cnt0
+  __shared__ V xt[FIB_T + 2 * FIB_HALO];
   if (io.stop && load_coherent(io.stop)) return;
   double s, ch, cx, cz;
   apply_scalars(io, s, ch, cx, cz);
@@ -133,43 +139,87 @@ __global__ __launch_bounds__(256) void k_fib_apply(FibModel M, ApplyIO<V> io, Re
   const int64_t per = (ntiles + 7) / 8;
   const int64_t tile = ntiles >= 64 ? ((b & 7) * per + (b >> 3)) : b;
   if (tile >= ntiles) {   // tail of the XCD remap: idle block still joins the reduction
-    if (io.reduce) block_reduce_finish(0.0, 0.0, 0.0, 0.0, rd);
+    if (io.reduce) block_reduce_store(0.0, 0.0, 0.0, 0.0, rd.partials);
     return;
   }
   const int64_t r0 = tile * FIB_T;
+  // stage scaled x over [r0 - HALO, r0 + T + HALO) clipped to the sector
+  for (int k = t; k < FIB_T + 2 * FIB_HALO; k += 256) {
+    const int64_t r = r0 - FIB_HALO + k;
+    xt[k] = (r >= 0 && r < M.dim) ? vscale(io.x[r], sr) : vzero(xt[0]);
+  }
   if (t == 0) fib_tile_head(M, r0, hs, meta);
   __syncthreads();
   fib_hi_tables(M, hs[0], Fx[0], Ec[0], T0[0], T1[0], T2[0], &ehi[0]);
   fib_hi_tables(M, hs[1], Fx[1], Ec[1], T0[1], T1[1], T2[1], &ehi[1]);
   const int64_t rho0 = meta[0], cnt0 = meta[1];
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  uint64_t code[FIB_E];
+  V acc[FIB_E];
+  bool ok[FIB_E];
+#pragma unroll
   for (int e = 0; e < FIB_E; ++e) {
-    const int64_t k = (int64_t)e * 256 + t;
+    const int k = e * 256 + t;
     const int64_t r = r0 + k;
-    if (r >= M.dim) break;
+    ok[e] = r < M.dim;
+    code[e] = 0;
+    acc[e] = vzero(xt[0]);
+    if (!ok[e]) continue;
     const int64_t rl = rho0 + k;
     const int w = rl < cnt0 ? 0 : 1;
     const int64_t lr = w ? rl - cnt0 : rl;
     const uint32_t lo = M.lo_code[lr];
-    const uint64_t code = hs[w] | (uint64_t)lo;
+    code[e] = hs[w] | (uint64_t)lo;
     const double d = ehi[w] + M.lo_diag[lr] + T0[w][lo & 255u] + T1[w][(lo >> 8) & 255u] +
                      T2[w][(lo >> 16) & 3u];
-    const V xe = vscale(io.x[r], sr);
-    V acc = vscale(xe, (RT)d);
-    for (int p = 0; p < M.n; ++p) {
-      const int bit = (int)((code >> p) & 1ull);
-      int64_t src;
-      if (bit) {
-        src = r - M.W[p];
-      } else {
-        const uint64_t nb = (code >> (p + 1)) | (p > 0 ? (code >> (p - 1)) : 0ull);
-        if (nb & 1ull) continue;
-        src = r + M.W[p];
-      }
-      vcfma(acc, (RT)(M.ar[p] * s), (RT)((bit ? M.ai[p] : -M.ai[p]) * s), io.x[src]);
+    acc[e] = vscale(xt[FIB_HALO + k], (RT)d);
+  }
+  // near flips from LDS
+  const int pn = M.n < FIB_NEAR ? M.n : FIB_NEAR;
+  for (int p = 0; p < pn; ++p) {
+    const int W = (int)M.W[p];
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+#pragma unroll
+    for (int e = 0; e < FIB_E; ++e) {
+      if (!ok[e]) continue;
+      const uint64_t c = code[e];
+      const int bit = (int)((c >> p) & 1ull);
+      const bool free_ = !(((c >> (p + 1)) | (p > 0 ? (c >> (p - 1)) : 0ull)) & 1ull);
+      if (!bit && !free_) continue;
+      const int k = e * 256 + t + FIB_HALO + (bit ? -W : W);
+      vcfma(acc[e], ar, bit ? ai : -ai, xt[k]);
     }
-    V v = vscale(acc, chr);
+  }
+  // far flips from global memory: a thread's FIB_E gathers are issued together
+  for (int p = pn; p < M.n; ++p) {
+    const int64_t W = M.W[p];
+    const RT ar = (RT)(M.ar[p] * s), ai = (RT)(M.ai[p] * s);
+    V q[FIB_E];
+    bool use[FIB_E];
+#pragma unroll
+    for (int e = 0; e < FIB_E; ++e) {
+      const uint64_t c = code[e];
+      const int bit = (int)((c >> p) & 1ull);
+      const bool free_ = !(((c >> (p + 1)) | (c >> (p - 1))) & 1ull);
+      use[e] = ok[e] && (bit || free_);
+      const int64_t src = r0 + e * 256 + t + (bit ? -W : W);
+      q[e] = use[e] ? io.x[src] : vzero(xt[0]);
+    }
+#pragma unroll
+    for (int e = 0; e < FIB_E; ++e) {
+      if (!use[e]) continue;
+      const int bit = (int)((code[e] >> p) & 1ull);
+      vcfma(acc[e], ar, bit ? ai : -ai, q[e]);
+    }
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+#pragma unroll
+  for (int e = 0; e < FIB_E; ++e) {
+    if (!ok[e]) continue;
+    const int k = e * 256 + t;
+    const int64_t r = r0 + k;
+    const V xe = xt[FIB_HALO + k];
+    V v = vscale(acc[e], chr);
     vfma(v, cxr, xe);
     if (io.z) vfma(v, czr, io.z[r]);
     io.y[r] = v;
@@ -180,7 +230,7 @@ __global__ __launch_bounds__(256) void k_fib_apply(FibModel M, ApplyIO<V> io, Re
       if (io.probe) s3 += zdot(((io.probe[r >> 5] >> (r & 31)) & 1u) ? 1.0 : -1.0, v);
     }
   }
-  if (io.reduce) block_reduce_finish(s0, s1, s2, s3, rd);
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
 // Site occupations over the sector.  Low bits (< L) accumulate per thread in

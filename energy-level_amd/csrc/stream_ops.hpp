@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_lz_orth(const V* __restrict__ uj, V* __
     if (ritz0) { V a = ritz0[i]; vfma(a, r0c, u); ritz0[i] = a; }
     if (ritz1) { V a = ritz1[i]; vfma(a, r1c, u); ritz1[i] = a; }
   }
-  if (reduce) block_reduce_finish(0.0, acc, 0.0, 0.0, rd);
+  if (reduce) block_reduce_store(0.0, acc, 0.0, 0.0, rd.partials);
 }
 
 // sums: <a, b>, |b|^2, |a|^2 ; used for norms and Rayleigh quotients
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_dots(const V* __restrict__ a, const V* 
     s1 += nrm2(bv);
     if (a) { const V av = a[i]; s0 += rdot(av, bv); s2 += nrm2(av); }
   }
-  block_reduce_finish(s0, s1, s2, 0.0, rd);
+  block_reduce_store(s0, s1, s2, 0.0, rd.partials);
 }
 
 // y = a * x + b * y  (real a, b); x may be null (scale only)

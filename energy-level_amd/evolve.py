@@ -69,37 +69,60 @@ class PulseProblem:
 
     ``model`` supplies the flip pattern F (its Ω, normally 1 on every site),
     the fixed diagonal D (its delta, normally 0, and V pairs).  Slices have
-    duration ``dt`` ns and ``K+1`` Chebyshev terms.
+    duration ``dt`` and ``K+1`` Chebyshev terms.  ``units='reference'`` is the
+    reference propagator exp(−i·CONV·H·dt) (dt in ns, H in MHz);
+    ``'natural'`` is exp(−iH·dt).  A slice whose phase span τ·r would leave
+    a series tail above ``eps`` is integrated as several equal sub-steps of
+    the same drive (gradients are summed back onto the slice).
     """
 
     def __init__(self, model: RydModel, dt, K=15, dtype='complex64', device='cuda',
-                 checkpoint=20, plan=None):
+                 checkpoint=20, plan=None, units='reference', eps=1e-13):
         import torch
         nat.require_gpu()
         self.model = model
         self.plan = plan if plan is not None else Plan(model, device=device)
         self.dt = float(dt)
-        self.tau = CONV * self.dt
+        if units not in ('reference', 'natural'):
+            raise ValueError("units must be 'reference' (exp(-i·CONV·H·dt)) or 'natural' (exp(-iH·dt))")
+        self.units = units
+        self.tau = CONV * self.dt if units == 'reference' else self.dt
+        self.eps = float(eps)
         self.K = int(K)
         self.tdt = {'complex64': torch.complex64, 'complex128': torch.complex128}[dtype]
         self.dtype = dtype
         self.checkpoint = max(1, int(checkpoint))
         self.dim = self.plan.dim
 
+    def _substeps(self, tau_r):
+        """Sub-steps so that a (K+1)-term series of span tau_r/s has tail <= eps."""
+        s = 1
+        while terms_for(tau_r / s, self.eps) > self.K:
+            s += 1
+        return s
+
     def _slice_params(self, Om, Delta):
+        """Per sub-step arrays and the owner slice of each sub-step."""
         Om = np.asarray(Om, float)
         Delta = np.asarray(Delta, float)
         dadd = -Delta
-        cs, rs, coef = [], [], []
-        for om, da in zip(Om, dadd):
+        oms, das, cs, rs, coef, owner = [], [], [], [], [], []
+        for k, (om, da) in enumerate(zip(Om, dadd)):
             c, r = slice_bounds(self.model, om, da)
-            cs.append(c)
-            rs.append(r)
-            a = cheb_coefficients(self.tau, c, r, self.K)
-            coef.append(np.stack([a.real, a.imag], axis=1))
+            sub = self._substeps(self.tau * r)
+            a = cheb_coefficients(self.tau / sub, c, r, self.K)
+            for _ in range(sub):
+                oms.append(om)
+                das.append(da)
+                cs.append(c)
+                rs.append(r)
+                coef.append(np.stack([a.real, a.imag], axis=1))
+                owner.append(k)
         coef = np.ascontiguousarray(np.asarray(coef, np.float64))
-        return (np.ascontiguousarray(Om), np.ascontiguousarray(dadd), np.asarray(cs),
-                np.asarray(rs), coef)
+        self._owner = np.asarray(owner, np.int64)
+        self._nslices = len(Om)
+        return (np.ascontiguousarray(oms, dtype=float), np.ascontiguousarray(das, dtype=float),
+                np.asarray(cs), np.asarray(rs), coef)
 
     def _forward_slices(self, psi, params, k0, k1, work):
         Om, dadd, cs, rs, coef = params
@@ -176,8 +199,10 @@ class PulseProblem:
         w = np.full(self.K, 2.0)
         w[0] = 1.0
         fac = 2.0 / rs
-        d_om = fac * (all_sums[:, :, 0] @ w)
-        d_dadd = fac * (all_sums[:, :, 1] @ w)
+        d_om_sub = fac * (all_sums[:, :, 0] @ w)
+        d_dadd_sub = fac * (all_sums[:, :, 1] @ w)
+        d_om = np.bincount(self._owner, weights=d_om_sub, minlength=self._nslices)
+        d_dadd = np.bincount(self._owner, weights=d_dadd_sub, minlength=self._nslices)
         return loss, d_om, -d_dadd, psi
 
 

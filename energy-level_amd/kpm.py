@@ -29,7 +29,7 @@ from .model import from_calib
 from .plan import Plan
 
 __all__ = ['kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'cheb_coeffs',
-           'thermal_from_moments', 'dos_from_moments', 'probe_bits']
+           'thermal_from_moments', 'dos_from_moments', 'probe_bits', 'device_probes']
 
 
 def probe_bits(D, nvec, seed):
@@ -60,10 +60,18 @@ def spectral_bounds(plan, m=120, seed=0, tol=1e-8, margin=1e-3):
     return emin, emax, c, r
 
 
+def device_probes(plan, nvec, seed):
+    """The reference probe stream, bit-packed and uploaded (list of device tensors)."""
+    import torch
+    return [torch.from_numpy(b).to(plan.device) for b in probe_bits(plan.dim, nvec, seed)]
+
+
 def kpm_moments_hip(calib=None, Om=None, delta=None, terms=None, *, plan=None, num=300, nvec=48,
                     seed=0, dtype='float64', c=None, r=None, margin=1e-3, doubling=True,
-                    device='cuda', return_sums=False):
-    """→ (mu[num+1], c, r) matrix-free on the GPU (no CSR is assembled)."""
+                    device='cuda', return_sums=False, probes=None):
+    """→ (mu[num+1], c, r) matrix-free on the GPU (no CSR is assembled).
+
+    ``probes``: pre-uploaded probes from :func:`device_probes` (else drawn here)."""
     import torch
     nat.require_gpu()
     if dtype not in ('float32', 'float64'):
@@ -87,8 +95,9 @@ def kpm_moments_hip(calib=None, Om=None, delta=None, terms=None, *, plan=None, n
     L = nat.lib()
     code = nat.dtype_code(tdt)
     ws = P.workspace()
-    for bits in probe_bits(D, nvec, seed):
-        pb = torch.from_numpy(bits).to(P.device)
+    if probes is None:
+        probes = (torch.from_numpy(b).to(P.device) for b in probe_bits(D, nvec, seed))
+    for pb in probes:
         nat.check(L.ryd_cheb_moments(P.handle, code, nat.ptr(pb), nat.ptr(bufs[0]),
                                      nat.ptr(bufs[1]), nat.ptr(bufs[2]), K, float(c), float(r),
                                      nat.ptr(sums), nat.ptr(ws), nat.stream_ptr()))

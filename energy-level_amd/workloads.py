@@ -1,0 +1,232 @@
+"""The five benchmark configurations, built from seeded synthetic inputs.
+
+Each ``run_cN`` builds its model (geometry and couplings as the reference's
+model builders: chain ``(i·a, 0, 0)``, row-major square, Gaussian jitter on all
+three coordinates, V = C6/r^6 over all pairs) and returns a dict of measured
+quantities: timings use ``torch.cuda.synchronize`` around the measured work,
+inputs already on the device.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .model import from_positions, chain_positions, square_positions, c6_coupling, from_calib
+
+__all__ = ['chain_calib', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5']
+
+
+def chain_calib(n, c6):
+    return from_positions(chain_positions(n), c6_coupling(c6))
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def run_c1(device='cuda', steps=200, seed=0):
+    """Plain FP64 Lanczos ground state, chain N=12, Ω=1, Δ=1.2, C6=10:
+    ``steps`` device Lanczos steps from a seeded normalised N(0,1) start, E0
+    from the FP64 tridiagonal, <n_i> from the replayed Ritz vector."""
+    import torch
+    from .plan import Plan
+    from .lanczos import lanczos_run, ritz_vectors, rayleigh_residual, site_occupations, \
+        _start_vector
+    n = 12
+    calib = chain_calib(n, 10.0)
+    model = from_calib(calib, np.ones(n), -1.2 * np.ones(n))
+    P = Plan(model, device=device)
+    v0 = _start_vector(P.dim, seed, False, P.device)
+    _sync()
+    t0 = time.perf_counter()
+    R = lanczos_run(P, v0, steps, 1, 0.0, want=2)
+    x = ritz_vectors(P, R)
+    E0, res = rayleigh_residual(P, x[0], R.theta[R.picks[0]])
+    n_atom = site_occupations(P, x[0])
+    _sync()
+    t = time.perf_counter() - t0
+    return dict(config='C1 chain N=12 FP64 Lanczos', steps=int(R.j), E0=E0, residual=res,
+                gap=float(R.theta[R.picks[1]] - R.theta[R.picks[0]]), n_atom=n_atom.tolist(),
+                seconds=t, calib=calib)
+
+
+def run_c2(device='cuda', num=1024, nvec=16, seed=0, margin=0.01, dtypes=('float32', 'float64')):
+    """KPM DOS, 5x5 square N=25, Ω=1, Δ=0.8, C6=20; bounds from FP64 Lanczos
+    with a ``margin`` (1 %); ``num`` moments, ``nvec`` Rademacher probes
+    (the reference stream), Jackson kernel; FP32 bulk against FP64."""
+    import torch
+    from .plan import Plan
+    from .kpm import kpm_moments_hip, spectral_bounds, device_probes
+    n = 25
+    calib = from_positions(square_positions(n), c6_coupling(20.0))
+    model = from_calib(calib, np.ones(n), -0.8 * np.ones(n))
+    P = Plan(model, device=device)
+    _sync()
+    tb = time.perf_counter()
+    emin, emax, c, r = spectral_bounds(P, margin=margin)
+    _sync()
+    t_bounds = time.perf_counter() - tb
+    # probes generated once on the host (inputs), uploaded before timing
+    out = dict(config='C2 square 5x5 N=25 KPM', num=num, nvec=nvec, emin=emin, emax=emax, c=c,
+               r=r, bounds_seconds=t_bounds)
+    mus = {}
+    tp = time.perf_counter()
+    probes = device_probes(P, nvec, seed)
+    _sync()
+    out['probe_generation_seconds_host'] = time.perf_counter() - tp
+    for dt in dtypes:
+        _sync()
+        t0 = time.perf_counter()
+        mu, _, _ = kpm_moments_hip(plan=P, num=num, nvec=nvec, seed=seed, dtype=dt, c=c, r=r,
+                                   probes=probes)
+        _sync()
+        mus[dt] = mu
+        out['seconds_' + dt] = time.perf_counter() - t0
+    if 'float32' in mus and 'float64' in mus:
+        err = np.abs(mus['float32'] - mus['float64'])
+        out['fp32_max_abs_moment_error'] = float(err.max())
+        out['fp32_mean_abs_moment_error'] = float(err.mean())
+    out['mu'] = mus
+    out['steps_per_probe'] = (num + 1) // 2
+    out['cheb_steps_per_s_float32'] = nvec * ((num + 1) // 2) / out['seconds_float32']
+    return out
+
+
+def run_c3(device='cuda', seed=0, m=60, plain=True):
+    """Blockade-sector Lanczos, chain N=40 (D = F(42) = 267,914,296), Ω=1,
+    Δ=1.5, V = 30/r^6 beyond nearest neighbours: FP32 bulk + FP64 refinement,
+    against the plain FP64 path."""
+    from .lanczos import hip_lanczos_ground_state
+    from .plan import Plan
+    n = 40
+    calib = chain_calib(n, 30.0)
+    Om, delta = np.ones(n), -1.5 * np.ones(n)
+    P = Plan(from_calib(calib, Om, delta), basis='chain_fib', device=device)
+    out = dict(config='C3 chain N=40 blockade sector', dim=P.dim)
+    _sync()
+    t0 = time.perf_counter()
+    r32 = hip_lanczos_ground_state(calib, Om, delta, basis='chain_blockade', dtype='float32',
+                                   m=m, seed=seed, plan=P)
+    _sync()
+    out['seconds_fp32_refined'] = time.perf_counter() - t0
+    out['fp32_refined'] = {k: r32[k] for k in ('E0', 'gap', 'residuals', 'converged', 'krylov_dim',
+                                              'precision')}
+    if plain:
+        _sync()
+        t0 = time.perf_counter()
+        r64 = hip_lanczos_ground_state(calib, Om, delta, basis='chain_blockade', dtype='float64',
+                                       m=m, seed=seed, plan=P)
+        _sync()
+        out['seconds_fp64'] = time.perf_counter() - t0
+        out['fp64'] = {k: r64[k] for k in ('E0', 'gap', 'residuals', 'converged', 'krylov_dim')}
+        out['E0_rel_diff'] = abs(r32['E0'] - r64['E0']) / abs(r64['E0'])
+        out['speedup_vs_fp64'] = out['seconds_fp64'] / out['seconds_fp32_refined']
+    return out
+
+
+def run_c4(device='cuda', count=512, steps=150, seed=0, plain=True, plain_count=None):
+    """512 chain instances, N ~ U{10..20}, jitter σ = 0.1a, Δ over 32 values in
+    [−2, 4], Ω=1, C6=10: batched FP32 Lanczos (one graph) against the plain
+    FP64 per-instance device loop."""
+    import torch
+    from .batch import BatchLanczos, disorder_ensemble
+    from .plan import Plan
+    from .lanczos import lanczos_run, _tridiag, _genuine_lowest
+    models = disorder_ensemble(count=count, seed=seed)
+    B = BatchLanczos(models, steps=steps, dtype='float32', device=device)
+    v0 = B.start_vectors(seed)
+    tb = B.autotune(v0)
+    B.run(v0, use_graph=True)          # capture + first replay
+    _sync()
+    t0 = time.perf_counter()
+    B.run(v0, use_graph=True)
+    _sync()
+    t_batch = time.perf_counter() - t0
+    gs = B.ground_states()
+    out = dict(config='C4 512-instance disorder ensemble', count=count, steps=steps,
+               ns=[int(m.n) for m in models], tile_bits=tb, autotune_ms=B.autotune_ms,
+               seconds_batched_fp32=t_batch, solves_per_s_batched=count / t_batch,
+               E0_batched=gs['E0'].tolist(), total_dim=B.total)
+    if plain:
+        idx = list(range(count)) if plain_count is None else list(range(min(count, plain_count)))
+        E64 = []
+        v64 = v0.to(torch.float64)
+        _sync()
+        t0 = time.perf_counter()
+        for b in idx:
+            P = Plan(models[b], device=device)
+            seg = v64[B.offsets[b]:B.offsets[b + 1]]
+            R = lanczos_run(P, seg, steps, 1, 0.0, want=1)
+            E64.append(float(R.theta[R.picks[0]]))
+        _sync()
+        t_plain = time.perf_counter() - t0
+        E64 = np.asarray(E64)
+        rel = np.abs(gs['E0'][idx] - E64) / np.abs(E64)
+        out.update(seconds_plain_fp64=t_plain, plain_count=len(idx),
+                   solves_per_s_plain_fp64=len(idx) / t_plain,
+                   speedup_vs_plain_fp64=(count / t_batch) / (len(idx) / t_plain),
+                   E0_rel_diff_max=float(rel.max()), E0_rel_diff_median=float(np.median(rel)))
+    B.close()
+    return out
+
+
+def run_c5(device='cuda', n=20, slices=200, dt=0.05, K=15, seed=0, fd_components=(0, 99, 199),
+           units='natural'):
+    """Differentiable pulse: chain N=20, 200 slices of dt=0.05 ns, per-slice
+    Ω_k ~ U(0.5,1.5), Δ_k ~ U(−1,2), C6=10, 16-term Chebyshev slices, loss
+    1 − |<Z2|ψ(T)>|², adjoint gradient of all 400 parameters (complex64,
+    checkpoints every 20 slices), against a plain FP64 forward and FP64
+    central differences.  ``units='natural'`` takes dt as the phase
+    exp(−iH·dt) (Ω·T = 10, a non-trivial Z2 overlap); in the reference's
+    units (exp(−i·CONV·H·dt), 10 ns) the state barely leaves |g…g>."""
+    import torch
+    from .evolve import PulseProblem, z2_index
+    from .plan import Plan
+    calib = chain_calib(n, 10.0)
+    model = from_calib(calib, np.ones(n), np.zeros(n))
+    rng = np.random.default_rng(seed)
+    Om = rng.uniform(0.5, 1.5, slices)
+    De = rng.uniform(-1.0, 2.0, slices)
+    P = Plan(model, device=device)
+    p32 = PulseProblem(model, dt=dt, K=K, dtype='complex64', plan=P, units=units)
+    p64 = PulseProblem(model, dt=dt, K=K, dtype='complex128', plan=P, units=units)
+    psi0 = torch.zeros(P.dim, dtype=torch.complex128, device=P.device)
+    psi0[0] = 1.0                         # |g...g>
+    tgt = z2_index(n)
+    p32.loss_and_grad(psi0, Om, De, tgt)  # warm-up
+    _sync()
+    t0 = time.perf_counter()
+    L32, gO32, gD32, _ = p32.loss_and_grad(psi0, Om, De, tgt)
+    _sync()
+    t_grad = time.perf_counter() - t0
+    p64.propagate(psi0, Om, De)
+    _sync()
+    t0 = time.perf_counter()
+    psi = p64.propagate(psi0, Om, De)
+    _sync()
+    t_fwd64 = time.perf_counter() - t0
+    L64, gO64, gD64, _ = p64.loss_and_grad(psi0, Om, De, tgt)
+    fd = []
+    h = 1e-4
+    for k in fd_components:
+        e = np.zeros(slices)
+        e[k] = h
+
+        def loss(o, d):
+            ps = p64.propagate(psi0, o, d)
+            return 1.0 - abs(ps[tgt].item()) ** 2
+        fo = (loss(Om + e, De) - loss(Om - e, De)) / (2 * h)
+        fdd = (loss(Om, De + e) - loss(Om, De - e)) / (2 * h)
+        fd.append(dict(k=int(k), fd_dOm=fo, adj64_dOm=float(gO64[k]), adj32_dOm=float(gO32[k]),
+                       fd_dDelta=fdd, adj64_dDelta=float(gD64[k]), adj32_dDelta=float(gD32[k])))
+    g64 = np.concatenate([gO64, gD64])
+    g32 = np.concatenate([gO32, gD32])
+    p64._slice_params(Om, De)
+    return dict(config='C5 N=20 pulse adjoint', slices=slices, K=K, units=units,
+                substeps_total=int(len(p64._owner)), loss_fp64=L64, loss_c64=L32,
+                seconds_grad_c64=t_grad, grad_per_s=1.0 / t_grad, seconds_forward_fp64=t_fwd64,
+                grad_cost_in_fp64_forwards=t_grad / t_fwd64,
+                grad_c64_vs_fp64_max_rel=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+                fd_checks=fd)

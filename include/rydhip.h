@@ -82,8 +82,8 @@ int ryd_plan_destroy(ryd_plan* plan);
 /* Basis dimension of a plan (0 on a NULL plan). */
 int64_t ryd_plan_dim(const ryd_plan* plan);
 
-/* Bytes of workspace any single reducing call on this plan needs.  The
- * workspace must be zero-filled once after allocation; calls leave it zeroed. */
+/* Bytes of workspace any single reducing call on this plan needs (per-block
+ * partial sums; no initialisation required). */
 size_t ryd_workspace_bytes(const ryd_plan* plan);
 
 /* y = H x.  Replaces ChannelKernel.matvec (channels.py:210-221) and the GPU
