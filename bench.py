@@ -6,7 +6,8 @@ pairs V = 10/|i−j|^6, Ω = 1, Δ = 1.2 (reference sign: delta = −1.2), full
 H·v of the whole 2^28 vector, inputs resident in HBM.
 
 Prints one JSON line (value = H·v/s in FP64; the FP32 form is reported in
-``fp32``).  ``roofline`` is for the dominant kernel (k_full_tiled) with
+``fp32``; the five configurations follow in ``configs``, with the batched
+Lanczos solves/s and the adjoint grad/s of the metric at the top level).  ``roofline`` is for the dominant kernel (k_full_tiled) with
 algorithmic bytes 2·D·sizeof(element) (x read once, y written once) over
 its mean duration measured with HIP events on the launch stream.
 ``cpu_baseline`` times the oracle's OpenMP matrix-free H·v on the host
@@ -83,6 +84,27 @@ def _cpu_baseline(n, rows):
                 seconds_per_sample=t)
 
 
+def _jsonable(o):
+    if hasattr(o, 'tolist'):
+        return o.tolist()
+    return str(o)
+
+
+def _run_configs(dev, which):
+    """The five configurations (summaries only; big arrays dropped)."""
+    from energy_level_amd import workloads as W
+    out = {}
+    for name in which:
+        t0 = time.perf_counter()
+        r = getattr(W, 'run_' + name)(dev)
+        for k in ('calib', 'mu', 'E0_batched', 'ns'):
+            r.pop(k, None)
+        r['wall_seconds'] = time.perf_counter() - t0
+        out[name] = r
+        print('[bench] %s done in %.1f s' % (name, r['wall_seconds']), file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -94,6 +116,9 @@ def main():
                     help='time one element type only (profiling runs)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run only a plain device copy of one FP64 N-site vector (counter scale)')
+    ap.add_argument('--configs', default='c1,c2,c3,c4,c5',
+                    help='comma list of the five configurations to run after the headline '
+                         '("" for none)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the H·v path is built and measured on a single '
@@ -133,6 +158,7 @@ def main():
         print(json.dumps({k: v for k, v in res[args.only].items()}))
         return
     f64 = res['f64']
+    cfg = _run_configs(dev, [c for c in args.configs.split(',') if c])
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(n, rows=1 << max(10, n - 6))
@@ -162,8 +188,16 @@ def main():
                  'achieved_gbps': res['f32']['achieved_gbps'],
                  'frac': res['f32']['achieved_gbps'] / HBM_PEAK_GBPS},
         'cpu_baseline': cpu,
+        'configs': cfg,
     }
-    print(json.dumps(out))
+    c4, c5 = cfg.get('c4') or {}, cfg.get('c5') or {}
+    if c4:
+        out['batched_lanczos_solves_per_s'] = c4.get('solves_per_s_batched')
+        out['batched_vs_plain_fp64'] = c4.get('speedup_vs_plain_fp64')
+    if c5:
+        out['adjoint_grad_per_s'] = c5.get('grad_per_s')
+        out['adjoint_grad_cost_in_fp64_forwards'] = c5.get('grad_cost_in_fp64_forwards')
+    print(json.dumps(out, default=_jsonable))
 
 
 if __name__ == '__main__':
