@@ -60,7 +60,10 @@ struct ryd_plan {
   // chain-blockade sector
   uint32_t* d_lo_code = nullptr;     // [F(L+2)] low-part codes
   double* d_lo_diag = nullptr;       // [F(L+2)] low-part diagonal
-  double* d_dl = nullptr;            // [n] delta by bit position
+  FibHi* d_hi = nullptr;             // [nh] high parts in ascending order
+  int64_t fib_nh = 0;
+  FibTop* d_top = nullptr;           // [FIB_NG] top patterns (split H·v only)
+  bool fib_split = false;
   int fib_L = 0;
   int64_t fib_cnt[2] = {0, 0};
 };
@@ -110,7 +113,16 @@ void build_tables(int n, int tb, const std::vector<double>& dl, const std::vecto
 
 // Split threshold: from this n on, the top TOPB bits run in a second pass
 // (k_full_top) so that pass A's partner tiles stay within each XCD's L2.
-constexpr int SPLIT_MIN_N = 23;
+#ifndef RYD_SPLIT_MIN_N
+#define RYD_SPLIT_MIN_N 23
+#endif
+constexpr int SPLIT_MIN_N = RYD_SPLIT_MIN_N;
+
+// Chain sector: split the top FIB_K bits into a second pass from this n on.
+#ifndef RYD_FIB_SPLIT_MIN_N
+#define RYD_FIB_SPLIT_MIN_N 34
+#endif
+constexpr int FIB_SPLIT_MIN_N = RYD_FIB_SPLIT_MIN_N;
 
 int64_t fib(int k) {
   int64_t a = 0, b = 1;
@@ -122,34 +134,56 @@ FibModel fib_model(const ryd_plan* P) {
   FibModel M;
   M.n = P->n;
   M.L = P->fib_L;
+  M.far_end = P->n;
+  M.pad = 0;
   M.dim = P->dim;
+  M.nh = P->fib_nh;
+  M.hi = P->d_hi;
   M.lo_code = P->d_lo_code;
   M.lo_diag = P->d_lo_diag;
   M.vb = P->d_vb;
-  M.dl = P->d_dl;
   for (int p = 0; p < MAXN; ++p) {
-    M.W[p] = p < P->n ? fib(p + 2) : 0;
+    M.W[p] = fib(p + 2);              // all positions: group sizes read W[IB] with IB = n
     M.ar[p] = P->ar[p];
     M.ai[p] = P->ai[p];
   }
   M.cnt_lo[0] = P->fib_cnt[0];
   M.cnt_lo[1] = P->fib_cnt[1];
+  M.S = P->fib_split ? P->n - FIB_K : P->n;
+  M.ng = FIB_NG;
+  M.top = P->d_top;
+  M.nbody[0] = fib(M.S + 2);
+  M.nbody[1] = fib(M.S + 1);
+  if (P->fib_split) M.far_end = M.S;
   return M;
 }
 
 unsigned fib_grid(const ryd_plan* P) {
-  const int64_t ntiles = (P->dim + FIB_T - 1) / FIB_T;
-  return (unsigned)(ntiles >= 64 ? 8 * ((ntiles + 7) / 8) : ntiles);
+  const int64_t nh = P->fib_nh;
+  return (unsigned)(nh >= 64 ? 8 * ((nh + 7) / 8) : nh);
+}
+
+unsigned fib_top_grid(const ryd_plan* P, int run) {
+  const int64_t nruns = (fib(P->n - FIB_K + 2) + run - 1) / run;
+  return (unsigned)(nruns >= 64 ? 8 * ((nruns + 7) / 8) : nruns);
 }
 
 template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
   if (P->basis == RYD_BASIS_CHAIN_FIB) {
     io.nfar = 0;
-    io.partial = 0;
-    k_fib_apply<V><<<dim3(fib_grid(P)), dim3(256), 0, st>>>(fib_model(P), io, rd);
+    const FibModel M = fib_model(P);
+    io.partial = P->fib_split ? 1 : 0;
+    k_fib_apply<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
     LAUNCH_CHECK();
-    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, fib_grid(P));
+    unsigned grid = fib_grid(P);
+    if (P->fib_split) {
+      io.partial = 0;
+      grid = fib_top_grid(P, fib_run<V>());
+      k_fib_top<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+      LAUNCH_CHECK();
+    }
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
     LAUNCH_CHECK();
     return RYD_OK;
   }
@@ -344,6 +378,8 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   *out = nullptr;
   const int n = m->n;
   if (n < 1 || n > 62) return fail(RYD_ERR_INVALID, "n must be in [1, 62]");
+  if (m->basis_kind == RYD_BASIS_CHAIN_FIB && n > 50)
+    return fail(RYD_ERR_INVALID, "chain blockade sector needs n <= 50");
   if (!m->om_re || !m->delta) return fail(RYD_ERR_INVALID, "om_re and delta are required");
   if (m->n_pairs < 0 || (m->n_pairs > 0 && (!m->pair_i || !m->pair_j || !m->pair_v)))
     return fail(RYD_ERR_INVALID, "pair arrays are required when n_pairs > 0");
@@ -406,18 +442,75 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
       }
       dlo.push_back(d);
     }
+    // high parts in ascending order: Zeckendorf unrank of k in the (n-L)-bit
+    // system, each with its first rank and the diagonal of its own bits
+    const int mh = n - L;
+    std::vector<int64_t> F(mh + 3 > n + 3 ? mh + 3 : n + 3);
+    F[0] = 0;
+    F[1] = 1;
+    for (size_t i = 2; i < F.size(); ++i) F[i] = F[i - 1] + F[i - 2];
+    const int64_t nh = F[mh + 2];
+    std::vector<FibHi> hinfo((size_t)nh);
+    int64_t next_rank = 0;
+    for (int64_t k = 0; k < nh; ++k) {
+      uint64_t h = 0;
+      int64_t r = k;
+      int prev = 0;
+      for (int q = mh - 1; q >= 0; --q) {
+        if (!prev && r >= F[q + 2]) { h |= 1ull << (q + L); r -= F[q + 2]; prev = 1; }
+        else prev = 0;
+      }
+      FibHi& H = hinfo[(size_t)k];
+      H.code = h;
+      H.R = 0;
+      H.ehi = 0.0;
+      H.pad = 0.0;
+      for (uint64_t c = h; c; c &= c - 1) {
+        const int p = __builtin_ctzll(c);
+        H.R += F[p + 2];
+        H.ehi += dl[p];
+        for (uint64_t c2 = c & (c - 1); c2; c2 &= c2 - 1) H.ehi += vb[(size_t)p * n + __builtin_ctzll(c2)];
+      }
+      if (H.R != next_rank) return cleanup(RYD_ERR_DEVICE, "internal: sector high-part ranks out of order");
+      next_rank += ((h >> L) & 1ull) ? F[L + 1] : F[L + 2];
+    }
+    if (next_rank != P->dim) return cleanup(RYD_ERR_DEVICE, "internal: sector dimension mismatch");
+    P->fib_nh = nh;
+    // split H·v: the top FIB_K bits in a second pass once the far partners
+    // of those bits no longer stay in the caches
+    P->fib_split = n >= FIB_SPLIT_MIN_N && n - FIB_K >= L;
+    std::vector<FibTop> top(FIB_NG);
+    if (P->fib_split) {
+      const int S = n - FIB_K;
+      for (int gi = 0; gi < FIB_NG; ++gi) {
+        uint64_t g = 0;
+        int64_t r = gi;
+        int prev = 0;
+        for (int q = FIB_K - 1; q >= 0; --q) {
+          if (!prev && r >= F[q + 2]) { g |= 1ull << (q + S); r -= F[q + 2]; prev = 1; }
+          else prev = 0;
+        }
+        top[gi].g = g;
+        top[gi].R = 0;
+        for (uint64_t c = g; c; c &= c - 1) top[gi].R += F[__builtin_ctzll(c) + 2];
+      }
+    }
     e = hipMalloc(&P->d_lo_code, sizeof(uint32_t) * codes.size());
     if (e == hipSuccess) e = hipMalloc(&P->d_lo_diag, sizeof(double) * dlo.size());
-    if (e == hipSuccess) e = hipMalloc(&P->d_dl, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc(&P->d_hi, sizeof(FibHi) * hinfo.size());
     if (e == hipSuccess)
       e = hipMemcpy(P->d_lo_code, codes.data(), sizeof(uint32_t) * codes.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
       e = hipMemcpy(P->d_lo_diag, dlo.data(), sizeof(double) * dlo.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(P->d_dl, dl.data(), sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(P->d_hi, hinfo.data(), sizeof(FibHi) * hinfo.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && P->fib_split) e = hipMalloc(&P->d_top, sizeof(FibTop) * FIB_NG);
+    if (e == hipSuccess && P->fib_split)
+      e = hipMemcpy(P->d_top, top.data(), sizeof(FibTop) * FIB_NG, hipMemcpyHostToDevice);
     if (e != hipSuccess)
       return cleanup(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
                      std::string("sector table upload: ") + hipGetErrorString(e));
     size_t blocks = fib_grid(P);
+    if (P->fib_split && fib_top_grid(P, 16) > blocks) blocks = fib_top_grid(P, 16);
     if (blocks < 8192) blocks = 8192;
     P->ws_bytes = WS_HEADER + blocks * NRED * sizeof(double);
     const size_t occ = (size_t)OCC_BLOCKS * (MAXN_OCC + 1) * sizeof(double);
@@ -459,7 +552,8 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_vb) (void)hipFree(P->d_vb);
   if (P->d_lo_code) (void)hipFree(P->d_lo_code);
   if (P->d_lo_diag) (void)hipFree(P->d_lo_diag);
-  if (P->d_dl) (void)hipFree(P->d_dl);
+  if (P->d_hi) (void)hipFree(P->d_hi);
+  if (P->d_top) (void)hipFree(P->d_top);
   for (int k = 0; k < 2; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
@@ -584,8 +678,8 @@ namespace {
 template <class V>
 int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t st) {
   if (P->basis == RYD_BASIS_CHAIN_FIB) {
-    const int64_t ntiles = (P->dim + FIB_T - 1) / FIB_T;
-    const unsigned g = (unsigned)(ntiles < OCC_BLOCKS ? ntiles : OCC_BLOCKS);
+    const int64_t nh = P->fib_nh;
+    const unsigned g = (unsigned)(nh < OCC_BLOCKS ? nh : OCC_BLOCKS);
     double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
     k_fib_occ<V><<<g, 256, 0, st>>>(fib_model(P), static_cast<const V*>(x), part);
     LAUNCH_CHECK();

@@ -9,86 +9,89 @@
 //   * setting bit p moves it by +W_p (inside iff bits p-1 and p+1 are clear),
 // and H·v needs no code table and no search.
 //
-// Codes are rebuilt per tile: the bits >= L of a rank come from a greedy
-// Zeckendorf split (once per tile, at most two high parts per tile because a
-// tile is shorter than F(L+1)), the bits < L from a table of the L-bit system.
-// The diagonal is E_hi(h) + E_lo[lo] + cross(h, lo) with the cross field of
-// the high part on the low bits held in per-tile byte tables (LDS).
+// Tiling: a code splits into its high part h (bits >= L) and its low part
+// (bits < L).  All codes sharing h are the contiguous rank range
+// [R(h), R(h) + cnt(h)), R(h) = Σ_{p >= L, b_p = 1} W_p, cnt(h) = F(L+2), or
+// F(L+1) when bit L of h is set (bit L-1 must then stay clear); inside it the
+// offset is the Zeckendorf rank of the low part.  One block owns one high
+// part: every flip of a low bit stays inside the block (LDS), every flip of a
+// high bit reads one contiguous partner range R(h') + offset, and whether that
+// flip is allowed depends on h alone (bit L adds the offset < F(L+1) test).
+// The diagonal is E_hi(h) + E_lo[offset] + cross(h, low bits), with the cross
+// field of h on the low bits held in per-block byte tables (LDS).
 #pragma once
 #include "common.hpp"
 #include "hv_full.hpp"
+#include "observables.hpp"
 
 namespace ryd {
 
-constexpr int FIB_L = 18;          // low bits decoded through the table
-constexpr int FIB_T = 2048;        // ranks per tile (< F(FIB_L + 1) = 4181)
-constexpr int FIB_E = FIB_T / 256; // ranks per thread
+constexpr int FIB_L = 18;          // low bits per block
+constexpr int FIB_TMAX = 6765;     // F(FIB_L + 2): codes of the largest block
+constexpr int FIB_EPT = (FIB_TMAX + 255) / 256;   // offsets per thread (27)
+
+struct FibHi {
+  uint64_t code;    // high part (bits >= L, in place)
+  int64_t R;        // rank of its first code
+  double ehi;       // diagonal of the high bits alone (delta and V among them)
+  double pad;
+};
+
+struct FibTop {
+  uint64_t g;       // top pattern (bits >= S, in place)
+  int64_t R;        // rank offset of the pattern
+};
 
 struct FibModel {
   int n;
   int L;                      // min(n, FIB_L)
+  int far_end;                // high bits L .. far_end-1 are applied (n: all)
+  int pad;
   int64_t dim;                // F(n+2)
+  int64_t nh;                 // high parts, F(n-L+2)
+  const FibHi* hi;            // [nh] in ascending order
   const uint32_t* lo_code;    // [F(L+2)] L-bit sector codes, ascending
   const double* lo_diag;      // [F(L+2)] diagonal from bits < L
   const double* vb;           // [n*n] V by bit position
-  const double* dl;           // [n] delta by bit position
   int64_t W[MAXN];            // W_p = F(p+2)
-  int64_t cnt_lo[2];          // F(L+2), F(L+1): low-part counts after hi bit L clear/set
+  int64_t cnt_lo[2];          // F(L+2), F(L+1)
   double ar[MAXN], ai[MAXN];  // Ω/2 by bit position
+  // top pass (bits S .. n-1, S = n - FIB_K) of a split H·v
+  int S;
+  int ng;                     // valid FIB_K-bit top patterns, F(FIB_K + 2)
+  const struct FibTop* top;   // [ng] ascending
+  int64_t nbody[2];           // F(S+2), F(S+1): body ranks, all / with bit S-1 clear
 };
 
-// rank -> (high part code, low-part rank); bits >= L only
-__device__ inline void fib_split(const FibModel& M, int64_t r, uint64_t& h, int64_t& rho) {
-  uint64_t code = 0;
-  int prev = 0;
-  for (int p = M.n - 1; p >= M.L; --p) {
-    if (!prev && r >= M.W[p]) {
-      code |= 1ull << p;
-      r -= M.W[p];
-      prev = 1;
-    } else {
-      prev = 0;
-    }
-  }
-  h = code;
-  rho = r;
+__device__ __forceinline__ int64_t fib_block_hi(const FibModel& M, int64_t b) {
+  // XCD-aware: XCD b%8 sweeps a contiguous eighth of the high parts
+  const int64_t per = (M.nh + 7) / 8;
+  return M.nh >= 64 ? ((b & 7) * per + (b >> 3)) : b;
 }
 
-// per-tile tables for one high part h (thread-cooperative, 256 threads)
-__device__ inline void fib_hi_tables(const FibModel& M, uint64_t h, double* Fx, double* Ec,
-                                     double* T0, double* T1, double* T2, double* ehi) {
+// per-block cross field of the high part on the low bits, as byte tables
+__device__ inline void fib_cross_tables(const FibModel& M, uint64_t h, double* Fx, double* T0,
+                                        double* T1, double* T2) {
   const int t = threadIdx.x;
-  if (t < M.L) {                       // cross field of the high part on low bit t
+  if (t < M.L) {
     double f = 0.0;
     const double* row = M.vb + (size_t)t * M.n;
-    for (int q = M.L; q < M.n; ++q)
-      if ((h >> q) & 1ull) f += row[q];
+    for (uint64_t c = h; c; c &= c - 1) f += row[__builtin_ctzll(c)];
     Fx[t] = f;
-  } else if (t < M.n) {                // energy of high bit t with the high bits above it
-    double e = 0.0;
-    if ((h >> t) & 1ull) {
-      e = M.dl[t];
-      const double* row = M.vb + (size_t)t * M.n;
-      for (int q = t + 1; q < M.n; ++q)
-        if ((h >> q) & 1ull) e += row[q];
-    }
-    Ec[t] = e;
   }
   __syncthreads();
-  if (t == 0) {
-    double e = 0.0;
-    for (int p = M.L; p < M.n; ++p) e += Ec[p];
-    *ehi = e;
-  }
-  double v0 = 0.0, v1 = 0.0;
-  for (int p = 0; p < 8; ++p) {
-    if ((t >> p) & 1) {
-      if (p < M.L) v0 += Fx[p];
-      if (p + 8 < M.L) v1 += Fx[p + 8];
+  if (t < 256) {
+    double v0 = 0.0, v1 = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      if ((t >> p) & 1) {
+        if (p < M.L) v0 += Fx[p];
+        if (p + 8 < M.L) v1 += Fx[p + 8];
+      }
     }
+    T0[t] = v0;
+    T1[t] = v1;
   }
-  T0[t] = v0;
-  T1[t] = v1;
   if (t < 4) {
     double v2 = 0.0;
     for (int p = 0; p < 2; ++p)
@@ -98,130 +101,186 @@ __device__ inline void fib_hi_tables(const FibModel& M, uint64_t h, double* Fx, 
   __syncthreads();
 }
 
-// Tile header: high parts h0, h1 and the low-part rank/count of the tile start.
-__device__ inline void fib_tile_head(const FibModel& M, int64_t r0, uint64_t* hs, int64_t* meta) {
-  uint64_t h0, h1;
-  int64_t rho0, rho1;
-  fib_split(M, r0, h0, rho0);
-  const int64_t cnt0 = M.cnt_lo[(h0 >> M.L) & 1ull];
-  h1 = h0;
-  const int64_t r1 = r0 + (cnt0 - rho0);
-  if (r1 < M.dim) fib_split(M, r1, h1, rho1);
-  hs[0] = h0;
-  hs[1] = h1;
-  meta[0] = rho0;
-  meta[1] = cnt0;
-}
-
-// Flips of bits below FIB_NEAR stay within FIB_HALO = F(FIB_NEAR + 1) ranks,
-// so they read an LDS copy of the tile with that halo on both sides.
-constexpr int FIB_NEAR = 12;
-constexpr int FIB_HALO = 233;      // F(13)
+// Lane layout inside a block: the low part splits into inner bits (< FIB_IB)
+// and outer bits (FIB_IB .. L-1).  Codes sharing the outer part o form a
+// group of F(IB+2) consecutive offsets starting at B(o) = Σ_{outer b_p} W_p
+// (F(IB+1) when bit IB is set), and lane ℓ of a wavefront owns inner rank ℓ
+// of every group its wave handles.  So an inner flip is a fixed per-lane
+// partner offset and coefficient, an outer or high flip is allowed or not for
+// the whole group (bit IB adds the lane's inner bit IB-1), and all LDS and
+// global accesses of a group are contiguous across lanes.
+constexpr int FIB_IB = 8;          // inner bits: groups of F(10) = 55 or F(9) = 34 offsets
+constexpr int FIB_THREADS = 512;   // 8 wavefronts per block
+constexpr int FIB_WAVES = FIB_THREADS / 64;
+constexpr int FIB_GPW = 18;        // groups per wavefront: F(12) = 144 groups over 8 waves
+constexpr int FIB_SEPT = (FIB_TMAX + FIB_THREADS - 1) / FIB_THREADS;   // staged offsets per thread
+constexpr int FIB_PAD = 128;       // LDS slack read (never used) by lanes past a group's end
 
 template <class V>
-__global__ __launch_bounds__(256) void k_fib_apply(FibModel M, ApplyIO<V> io, Reduce rd) {
+__global__ __launch_bounds__(FIB_THREADS) void k_fib_apply(FibModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
-  __shared__ double Fx[2][FIB_L];
-  __shared__ double Ec[2][MAXN];
-  __shared__ double T0[2][256], T1[2][256], T2[2][4];
-  __shared__ double ehi[2];
-  __shared__ uint64_t hs[2];
-  __shared__ int64_t meta[2];   // rho0, cnt0
-  __shared__ V xt[FIB_T + 2 * FIB_HALO];
+  __shared__ V xt[FIB_TMAX + FIB_PAD];
+  __shared__ double Fx[FIB_L];
+  __shared__ double T0[256], T1[256], T2[4];
+  __shared__ int gbase[FIB_WAVES * FIB_GPW];
+  __shared__ uint32_t gcode[FIB_WAVES * FIB_GPW];
   if (io.stop && load_coherent(io.stop)) return;
   double s, ch, cx, cz;
   apply_scalars(io, s, ch, cx, cz);
   const RT sr = (RT)s;
-  const int t = threadIdx.x;
-  const int64_t ntiles = (M.dim + FIB_T - 1) / FIB_T;
-  const int64_t b = blockIdx.x;
-  // XCD-aware: XCD b%8 sweeps a contiguous eighth of the tiles
-  const int64_t per = (ntiles + 7) / 8;
-  const int64_t tile = ntiles >= 64 ? ((b & 7) * per + (b >> 3)) : b;
-  if (tile >= ntiles) {   // tail of the XCD remap: idle block still joins the reduction
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);   // wave-uniform (SGPR)
+  const int64_t hb = fib_block_hi(M, blockIdx.x);
+  if (hb >= M.nh) {   // tail of the XCD remap: idle block still joins the reduction
     if (io.reduce) block_reduce_store(0.0, 0.0, 0.0, 0.0, rd.partials);
     return;
   }
-  const int64_t r0 = tile * FIB_T;
-  // stage scaled x over [r0 - HALO, r0 + T + HALO) clipped to the sector
-  for (int k = t; k < FIB_T + 2 * FIB_HALO; k += 256) {
-    const int64_t r = r0 - FIB_HALO + k;
-    xt[k] = (r >= 0 && r < M.dim) ? vscale(io.x[r], sr) : vzero(xt[0]);
-  }
-  if (t == 0) fib_tile_head(M, r0, hs, meta);
-  __syncthreads();
-  fib_hi_tables(M, hs[0], Fx[0], Ec[0], T0[0], T1[0], T2[0], &ehi[0]);
-  fib_hi_tables(M, hs[1], Fx[1], Ec[1], T0[1], T1[1], T2[1], &ehi[1]);
-  const int64_t rho0 = meta[0], cnt0 = meta[1];
-  uint64_t code[FIB_E];
-  V acc[FIB_E];
-  bool ok[FIB_E];
+  const FibHi H = M.hi[hb];
+  const uint64_t h = H.code;
+  const int hL = (int)((h >> M.L) & 1ull);
+  const int cnt = (int)M.cnt_lo[hL];
+  const int64_t R = H.R;
+  const int IB = M.L < FIB_IB ? M.L : FIB_IB;
+  const int nout = M.L - IB;
+  const int Fbig = (int)M.W[IB];               // F(IB+2): group size, bit IB clear
+  const int Fsmall = (int)(IB > 0 ? M.W[IB - 1] : 1);   // F(IB+1): bit IB set
+  // groups (outer parts) of this block, ascending
+  const int ng = nout > 0 ? (int)(hL ? M.W[nout - 1] : M.W[nout]) : 1;
+  // stage s·x with every load in flight before the first store
+  {
+    V v[FIB_SEPT];
 #pragma unroll
-  for (int e = 0; e < FIB_E; ++e) {
-    const int k = e * 256 + t;
-    const int64_t r = r0 + k;
-    ok[e] = r < M.dim;
-    code[e] = 0;
-    acc[e] = vzero(xt[0]);
-    if (!ok[e]) continue;
-    const int64_t rl = rho0 + k;
-    const int w = rl < cnt0 ? 0 : 1;
-    const int64_t lr = w ? rl - cnt0 : rl;
-    const uint32_t lo = M.lo_code[lr];
-    code[e] = hs[w] | (uint64_t)lo;
-    const double d = ehi[w] + M.lo_diag[lr] + T0[w][lo & 255u] + T1[w][(lo >> 8) & 255u] +
-                     T2[w][(lo >> 16) & 3u];
-    acc[e] = vscale(xt[FIB_HALO + k], (RT)d);
-  }
-  // near flips from LDS
-  const int pn = M.n < FIB_NEAR ? M.n : FIB_NEAR;
-  for (int p = 0; p < pn; ++p) {
-    const int W = (int)M.W[p];
-    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
-#pragma unroll
-    for (int e = 0; e < FIB_E; ++e) {
-      if (!ok[e]) continue;
-      const uint64_t c = code[e];
-      const int bit = (int)((c >> p) & 1ull);
-      const bool free_ = !(((c >> (p + 1)) | (p > 0 ? (c >> (p - 1)) : 0ull)) & 1ull);
-      if (!bit && !free_) continue;
-      const int k = e * 256 + t + FIB_HALO + (bit ? -W : W);
-      vcfma(acc[e], ar, bit ? ai : -ai, xt[k]);
-    }
-  }
-  // far flips from global memory: a thread's FIB_E gathers are issued together
-  for (int p = pn; p < M.n; ++p) {
-    const int64_t W = M.W[p];
-    const RT ar = (RT)(M.ar[p] * s), ai = (RT)(M.ai[p] * s);
-    V q[FIB_E];
-    bool use[FIB_E];
-#pragma unroll
-    for (int e = 0; e < FIB_E; ++e) {
-      const uint64_t c = code[e];
-      const int bit = (int)((c >> p) & 1ull);
-      const bool free_ = !(((c >> (p + 1)) | (c >> (p - 1))) & 1ull);
-      use[e] = ok[e] && (bit || free_);
-      const int64_t src = r0 + e * 256 + t + (bit ? -W : W);
-      q[e] = use[e] ? io.x[src] : vzero(xt[0]);
+    for (int j = 0; j < FIB_SEPT; ++j) {
+      const int k = j * FIB_THREADS + t;
+      v[j] = k < cnt ? io.x[R + k] : vzero(v[0]);
     }
 #pragma unroll
-    for (int e = 0; e < FIB_E; ++e) {
-      if (!use[e]) continue;
-      const int bit = (int)((code[e] >> p) & 1ull);
-      vcfma(acc[e], ar, bit ? ai : -ai, q[e]);
+    for (int j = 0; j < FIB_SEPT; ++j) {
+      const int k = j * FIB_THREADS + t;
+      if (k < cnt) xt[k] = vscale(v[j], sr);
     }
+  }
+  for (int g = t; g < ng; g += FIB_THREADS) {
+    uint32_t o = 0;
+    int r = g, prev = 0, base = 0;
+    for (int q = nout - 1; q >= 0; --q) {
+      if (!prev && r >= (int)M.W[q]) {
+        o |= 1u << q;
+        r -= (int)M.W[q];
+        base += (int)M.W[IB + q];
+        prev = 1;
+      } else {
+        prev = 0;
+      }
+    }
+    gcode[g] = o;
+    gbase[g] = base;
+  }
+  fib_cross_tables(M, h, Fx, T0, T1, T2);     // its barriers also publish xt and the groups
+  // per-lane inner code, flip offsets and coefficients (upper neighbour of
+  // bit IB-1 taken clear; a set bit IB is handled per group below)
+  uint32_t cin = 0;
+  {
+    int r = lane, prev = 0;
+    for (int p = IB - 1; p >= 0; --p) {
+      if (!prev && r >= (int)M.W[p]) { cin |= 1u << p; r -= (int)M.W[p]; prev = 1; }
+      else prev = 0;
+    }
+  }
+  RT cfr[FIB_IB], cfi[FIB_IB];
+  int off[FIB_IB];
+#pragma unroll
+  for (int p = 0; p < FIB_IB; ++p) {
+    const int bit = (int)((cin >> p) & 1u);
+    const bool fr = !((cin >> (p + 1)) & 1u) && !(p > 0 && ((cin >> (p - 1)) & 1u));
+    const bool ok = p < IB && (bit || fr);
+    off[p] = ok ? (bit ? -(int)M.W[p] : (int)M.W[p]) : 0;   // blocked: own slot, coefficient 0
+    cfr[p] = ok ? (RT)M.ar[p] : (RT)0;
+    cfi[p] = ok ? (RT)(bit ? M.ai[p] : -M.ai[p]) : (RT)0;
+  }
+  const bool top_in = IB > 0 && ((cin >> (IB - 1)) & 1u);   // inner bit IB-1 set
+  const double eh = H.ehi;
+  V acc[FIB_GPW];
+  int gb[FIB_GPW];
+  bool act[FIB_GPW];
+#pragma unroll
+  for (int i = 0; i < FIB_GPW; ++i) {
+    const int g = wv + FIB_WAVES * i;
+    const bool gv = g < ng;
+    // group code and base are the same for the whole wave: keep them scalar so
+    // every group test below is a uniform branch
+    const uint32_t o = (uint32_t)__builtin_amdgcn_readfirstlane(gv ? (int)gcode[g] : 0);
+    const bool small = nout > 0 ? (o & 1u) : (hL != 0);
+    gb[i] = __builtin_amdgcn_readfirstlane(gv ? gbase[g] : 0);
+    act[i] = gv && lane < (small ? Fsmall : Fbig);
+    const int k = gb[i] + lane;
+    const uint32_t lo = (o << IB) | cin;
+    const double d = act[i] ? eh + M.lo_diag[k] + T0[lo & 255u] + T1[(lo >> 8) & 255u] +
+                                  T2[(lo >> 16) & 3u]
+                            : 0.0;
+    const V xv = xt[k];
+    acc[i] = vscale(xv, (RT)d);
+    if (!gv) continue;
+    // inner flips: fixed per lane; in a small group bit IB-1 cannot be set
+#pragma unroll
+    for (int p = 0; p < FIB_IB; ++p) {
+      if (p == IB - 1 && small) continue;
+      vcfma(acc[i], cfr[p], cfi[p], xt[k + off[p]]);
+    }
+    // outer flips: group-uniform, bit IB also needs inner bit IB-1 clear
+    for (int q = 0; q < nout; ++q) {
+      const int p = IB + q;
+      const int bit = (int)((o >> q) & 1u);
+      const bool up = q + 1 < nout ? ((o >> (q + 1)) & 1u) : (hL != 0);
+      const bool dn = q > 0 && ((o >> (q - 1)) & 1u);
+      if (!bit && (up || dn)) continue;
+      const int W = (int)M.W[p];
+      const bool lane_ok = bit || q > 0 || !top_in;
+      const RT ar = lane_ok ? (RT)M.ar[p] : (RT)0;
+      const RT ai = lane_ok ? (RT)(bit ? M.ai[p] : -M.ai[p]) : (RT)0;
+      // a blocked lane reads its own (finite) slot: a zero coefficient times
+      // an unwritten slot past the block's range could be NaN
+      vcfma(acc[i], ar, ai, xt[lane_ok ? k + (bit ? -W : W) : k]);
+    }
+  }
+  // high bits: one contiguous partner range per allowed bit; the loads of
+  // every group of the wave are in flight together
+  for (int p = M.L; p < M.far_end; ++p) {
+    const int bit = (int)((h >> p) & 1ull);
+    const bool set_ok = !((h >> (p + 1)) & 1ull) && (p == M.L || !((h >> (p - 1)) & 1ull));
+    if (!bit && !set_ok) continue;
+    const bool need_lo = !bit && p == M.L;     // setting bit L: low bit L-1 must be clear
+    const int lim = need_lo ? (int)M.cnt_lo[1] : cnt;
+    const int64_t base = R + (bit ? -M.W[p] : M.W[p]);
+    const RT ar = (RT)(M.ar[p] * s), ai = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
+    V q[FIB_GPW];
+#pragma unroll
+    for (int i = 0; i < FIB_GPW; ++i) {
+      const int k = gb[i] + lane;
+      q[i] = act[i] && k < lim ? io.x[base + k] : vzero(acc[0]);
+    }
+#pragma unroll
+    for (int i = 0; i < FIB_GPW; ++i) vcfma(acc[i], ar, ai, q[i]);
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  V zr[FIB_GPW];
+  if (io.z && !io.partial) {
 #pragma unroll
-  for (int e = 0; e < FIB_E; ++e) {
-    if (!ok[e]) continue;
-    const int k = e * 256 + t;
-    const int64_t r = r0 + k;
-    const V xe = xt[FIB_HALO + k];
-    V v = vscale(acc[e], chr);
+    for (int i = 0; i < FIB_GPW; ++i) zr[i] = act[i] ? io.z[R + gb[i] + lane] : vzero(acc[0]);
+  }
+#pragma unroll
+  for (int i = 0; i < FIB_GPW; ++i) {
+    if (!act[i]) continue;
+    const int64_t r = R + gb[i] + lane;
+    if (io.partial) {
+      io.y[r] = acc[i];
+      continue;
+    }
+    const V xe = xt[gb[i] + lane];
+    V v = vscale(acc[i], chr);
     vfma(v, cxr, xe);
-    if (io.z) vfma(v, czr, io.z[r]);
+    if (io.z) vfma(v, czr, zr[i]);
     io.y[r] = v;
     if (io.reduce) {
       s0 += rdot(xe, v);
@@ -233,62 +292,142 @@ __global__ __launch_bounds__(256) void k_fib_apply(FibModel M, ApplyIO<V> io, Re
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
-// Site occupations over the sector.  Low bits (< L) accumulate per thread in
-// registers; the high part is constant per (tile, h) so its weight is reduced
-// per tile and added for the set high bits.  partials: [grid][65], column 64
-// holds the total weight.
+// Pass B of a split sector H·v: the top FIB_K bits.  A code is a top pattern
+// g (bits >= S) over a body of S bits with body rank ρ; its rank is
+// R(g) + ρ, and bit S of g set needs the body's top bit clear (ρ < F(S+1)).
+// A block holds one run of FIB_RUN consecutive body ranks (256 bytes) for
+// every valid pattern, so each top-bit flip (pattern rank ± F(q+2)) is an LDS
+// read.  It reads the pass-A partial from y and finishes
+//   y = ch·(y_A + Σ_top flips of s·x) + cx·(s·x) + cz·z  with the reductions.
+constexpr int FIB_K = 9;
+constexpr int FIB_NG = 89;                      // F(FIB_K + 2)
+template <class V> constexpr int fib_run() { return 256 / (int)sizeof(V); }
+
+template <class V>
+__global__ __launch_bounds__(256) void k_fib_top(FibModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int RUN = fib_run<V>();
+  constexpr int GR = 256 / RUN;                 // patterns per pass of the block
+  constexpr int ROWS = (FIB_NG + GR - 1) / GR;
+  __shared__ V xt[FIB_NG][RUN];
+  __shared__ FibTop tp[FIB_NG];
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+  const int t = threadIdx.x;
+  const int rl = t % RUN, gr = t / RUN;
+  const int64_t nruns = (M.nbody[0] + RUN - 1) / RUN;
+  const int64_t per = (nruns + 7) / 8;
+  const int64_t m = nruns >= 64 ? ((int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3)) : blockIdx.x;
+  if (m >= nruns) {
+    if (io.reduce) block_reduce_store(0.0, 0.0, 0.0, 0.0, rd.partials);
+    return;
+  }
+  if (t < FIB_NG) tp[t] = M.top[t];
+  __syncthreads();
+  const int64_t rho = m * RUN + rl;
+  V acc[ROWS], xs[ROWS], zr[ROWS];
+  bool ok[ROWS];
+#pragma unroll
+  for (int j = 0; j < ROWS; ++j) {
+    const int gi = gr + GR * j;
+    bool v = gi < FIB_NG && rho < M.nbody[0];
+    if (v) v = !((tp[gi].g >> M.S) & 1ull) || rho < M.nbody[1];
+    ok[j] = v;
+    const int64_t r = v ? tp[gi].R + rho : 0;
+    xs[j] = v ? io.x[r] : vzero(acc[0]);
+    acc[j] = v ? io.y[r] : vzero(acc[0]);
+    if (io.z) zr[j] = v ? io.z[r] : vzero(acc[0]);
+  }
+#pragma unroll
+  for (int j = 0; j < ROWS; ++j) {
+    const int gi = gr + GR * j;
+    xs[j] = vscale(xs[j], sr);
+    if (gi < FIB_NG) xt[gi][rl] = xs[j];
+  }
+  __syncthreads();
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+#pragma unroll
+  for (int j = 0; j < ROWS; ++j) {
+    if (!ok[j]) continue;
+    const int gi = gr + GR * j;
+    const uint64_t g = tp[gi].g;
+    V a = acc[j];
+    int w = 1, wprev = 1;                       // F(q+2), F(q+1)
+    for (int q = 0; q < FIB_K; ++q) {
+      const int p = M.S + q;
+      const int bit = (int)((g >> p) & 1ull);
+      const bool up = (g >> (p + 1)) & 1ull;
+      const bool dn = q == 0 ? rho >= M.nbody[1] : (((g >> (p - 1)) & 1ull) != 0);
+      if (bit || !(up || dn)) {
+        const RT ai = (RT)M.ai[p];
+        vcfma(a, (RT)M.ar[p], bit ? ai : -ai, xt[gi + (bit ? -w : w)][rl]);
+      }
+      const int wn = w + wprev;
+      wprev = w;
+      w = wn;
+    }
+    const int64_t r = tp[gi].R + rho;
+    V v = vscale(a, chr);
+    vfma(v, cxr, xs[j]);
+    if (io.z) vfma(v, czr, zr[j]);
+    io.y[r] = v;
+    if (io.reduce) {
+      s0 += rdot(xs[j], v);
+      s1 += nrm2(v);
+      s2 += nrm2(xs[j]);
+      if (io.probe) s3 += zdot(((io.probe[r >> 5] >> (r & 31)) & 1u) ? 1.0 : -1.0, v);
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// Site occupations over the sector.  Low bits accumulate per thread; the high
+// part is constant over a block's range, so its total weight goes to the set
+// high bits.  partials: [grid][MAXN_OCC + 1], column MAXN_OCC holds the total.
 template <class V>
 __global__ __launch_bounds__(256) void k_fib_occ(FibModel M, const V* __restrict__ x,
                                                  double* partials) {
-  __shared__ double acc[64 + 1];
-  __shared__ double red[4][2];
-  __shared__ uint64_t hs[2];
-  __shared__ int64_t meta[2];
+  __shared__ double acc[MAXN_OCC + 1];
+  __shared__ double red[4][FIB_L + 1];
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  if (t <= 64) acc[t] = 0.0;
+  if (t <= MAXN_OCC) acc[t] = 0.0;
   double lo_acc[FIB_L];
 #pragma unroll
   for (int p = 0; p < FIB_L; ++p) lo_acc[p] = 0.0;
-  const int64_t ntiles = (M.dim + FIB_T - 1) / FIB_T;
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t r0 = tile * FIB_T;
-    __syncthreads();
-    if (t == 0) fib_tile_head(M, r0, hs, meta);
-    __syncthreads();
-    double hw0 = 0.0, hw1 = 0.0;
-    for (int e = 0; e < FIB_E; ++e) {
-      const int64_t k = (int64_t)e * 256 + t;
-      const int64_t r = r0 + k;
-      if (r >= M.dim) break;
-      const int64_t rl = meta[0] + k;
-      const int w = rl < meta[1] ? 0 : 1;
-      const uint32_t lo = M.lo_code[w ? rl - meta[1] : rl];
-      const double pw = nrm2(x[r]);
-      if (w) hw1 += pw; else hw0 += pw;
+  for (int64_t hb = blockIdx.x; hb < M.nh; hb += gridDim.x) {
+    const FibHi H = M.hi[hb];
+    const int cnt = (int)M.cnt_lo[(H.code >> M.L) & 1ull];
+    double tot = 0.0;
+    for (int k = t; k < cnt; k += 256) {
+      const double pw = nrm2(x[H.R + k]);
+      const uint32_t c = M.lo_code[k];
+      tot += pw;
 #pragma unroll
       for (int p = 0; p < FIB_L; ++p)
-        if ((lo >> p) & 1u) lo_acc[p] += pw;
+        if ((c >> p) & 1u) lo_acc[p] += pw;
     }
-    hw0 = wave_sum(hw0);
-    hw1 = wave_sum(hw1);
-    if (lane == 0) { red[wid][0] = hw0; red[wid][1] = hw1; }
+    tot = wave_sum(tot);
+    __syncthreads();
+    if (lane == 0) red[wid][FIB_L] = tot;
     __syncthreads();
     if (t == 0) {
-      for (int w = 0; w < 2; ++w) {
-        const double T = red[0][w] + red[1][w] + red[2][w] + red[3][w];
-        acc[64] += T;
-        for (uint64_t c = hs[w]; c; c &= c - 1) acc[__builtin_ctzll(c)] += T;
-      }
+      const double T = red[0][FIB_L] + red[1][FIB_L] + red[2][FIB_L] + red[3][FIB_L];
+      acc[MAXN_OCC] += T;
+      for (uint64_t c = H.code; c; c &= c - 1) acc[__builtin_ctzll(c)] += T;
     }
   }
-  __syncthreads();
 #pragma unroll
   for (int p = 0; p < FIB_L; ++p) {
     const double v = wave_sum(lo_acc[p]);
-    if (lane == 0) atomicAdd(&acc[p], v);
+    if (lane == 0) red[wid][p] = v;
   }
   __syncthreads();
-  if (t <= 64) partials[(size_t)blockIdx.x * (64 + 1) + t] = acc[t];
+  if (t < M.L) acc[t] += red[0][t] + red[1][t] + red[2][t] + red[3][t];
+  __syncthreads();
+  if (t <= MAXN_OCC) partials[(size_t)blockIdx.x * (MAXN_OCC + 1) + t] = acc[t];
 }
 
 // codes[k] = code of rank r0 + k (ranks >= dim give -1)

@@ -102,7 +102,7 @@ int ryd_apply(const ryd_plan* plan, int dtype, const void* x, const void* z, voi
 /* Three-term Lanczos recurrence, steps j0 .. j0+steps-1, entirely on the
  * device (no host synchronisation per step).  Replaces the Krylov build loop
  * of torch_lanczos_ground_state (matrix_free.py:231-251) and the recurrence of
- * spectrum.lanczos (spectrum.py:281-300).
+ * spectrum.lanczos (spectrum.py:76-96).
  *   b0, b1, b2 : three vectors of dim elements; at j0 = 0, b0 holds the start
  *                vector (any norm).  Vector u_j lives in b[j % 3].
  *   alpha, nrm : device FP64 arrays (>= j0+steps and >= j0+steps+1 entries);

@@ -79,7 +79,7 @@ def test_sector_hv_vs_channel_oracle(pkg, dev, n):
     assert np.max(np.abs(y32 - refr)) <= 1e-5 * max(1.0, np.max(np.abs(refr)))
 
 
-@pytest.mark.parametrize('n', [20, 24, 27])
+@pytest.mark.parametrize('n', [20, 24, 27, 30, 34, 36])
 def test_sector_hv_vs_c_oracle(pkg, dev, n):
     import torch
     Om, delta = np.ones(n), -1.5 * np.ones(n)
@@ -126,3 +126,26 @@ def test_sector_ground_state_fp32_refined_n22(pkg, dev):
     r = pkg.hip_lanczos_ground_state(calib, Om, delta, basis='chain_blockade', dtype='float32')
     assert r['converged']
     assert abs(r['E0'] - e0) <= 1e-10 * abs(e0)
+
+
+@pytest.mark.parametrize('n,dtype', [(36, 'complex128'), (36, 'complex64'), (40, 'float64')])
+def test_sector_hv_hermitian_large(pkg, dev, n, dtype):
+    """<z, H x> = conj(<x, H z>) at sizes beyond the oracle (split H·v for n >= 34)."""
+    import torch
+    rng = np.random.default_rng(n)
+    cplx = dtype.startswith('complex')
+    Om = rng.uniform(0.5, 1.5, n) * (np.exp(1j * rng.uniform(0, 2 * np.pi, n)) if cplx else 1.0)
+    delta = rng.uniform(-2, 2, n)
+    P = _plan(pkg, dev, n, Om, delta, _pairs(n))
+    tdt = getattr(torch, dtype)
+    g = torch.Generator(device=dev)
+    g.manual_seed(n)
+    x = torch.randn(P.dim, dtype=tdt, device=dev, generator=g)
+    z = torch.randn(P.dim, dtype=tdt, device=dev, generator=g)
+    hx, hz = P.hv(x), P.hv(z)
+    a = torch.vdot(z.to(torch.complex128 if cplx else torch.float64), hx.to(torch.complex128 if cplx else torch.float64)).item()
+    b = torch.vdot(hz.to(torch.complex128 if cplx else torch.float64), x.to(torch.complex128 if cplx else torch.float64)).item()
+    scale = (torch.linalg.vector_norm(hx.double() if not cplx else hx.to(torch.complex128)) *
+             torch.linalg.vector_norm(z.double() if not cplx else z.to(torch.complex128))).item()
+    tol = 1e-5 if dtype == 'complex64' else 1e-12
+    assert abs(a - b) <= tol * scale
