@@ -120,7 +120,7 @@ constexpr int SPLIT_MIN_N = RYD_SPLIT_MIN_N;
 
 // Chain sector: split the top FIB_K bits into a second pass from this n on.
 #ifndef RYD_FIB_SPLIT_MIN_N
-#define RYD_FIB_SPLIT_MIN_N 34
+#define RYD_FIB_SPLIT_MIN_N 38
 #endif
 constexpr int FIB_SPLIT_MIN_N = RYD_FIB_SPLIT_MIN_N;
 
@@ -172,12 +172,16 @@ template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
   if (P->basis == RYD_BASIS_CHAIN_FIB) {
     io.nfar = 0;
-    const FibModel M = fib_model(P);
-    io.partial = P->fib_split ? 1 : 0;
+    // the top-bit pass pays for 8- and 16-byte elements only (FP32 blocks
+    // read their far partners fast enough in one pass)
+    const bool split = P->fib_split && sizeof(V) >= 8;
+    FibModel M = fib_model(P);
+    if (!split) M.far_end = P->n;
+    io.partial = split ? 1 : 0;
     k_fib_apply<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
     LAUNCH_CHECK();
     unsigned grid = fib_grid(P);
-    if (P->fib_split) {
+    if (split) {
       io.partial = 0;
       grid = fib_top_grid(P, fib_run<V>());
       k_fib_top<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);

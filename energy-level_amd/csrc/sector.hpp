@@ -117,13 +117,14 @@ constexpr int FIB_SEPT = (FIB_TMAX + FIB_THREADS - 1) / FIB_THREADS;   // staged
 constexpr int FIB_PAD = 128;       // LDS slack read (never used) by lanes past a group's end
 
 template <class V>
-__global__ __launch_bounds__(FIB_THREADS) void k_fib_apply(FibModel M, ApplyIO<V> io, Reduce rd) {
+__global__ __launch_bounds__(FIB_THREADS, 4) void k_fib_apply(FibModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
   __shared__ V xt[FIB_TMAX + FIB_PAD];
   __shared__ double Fx[FIB_L];
   __shared__ double T0[256], T1[256], T2[4];
   __shared__ int gbase[FIB_WAVES * FIB_GPW];
   __shared__ uint32_t gcode[FIB_WAVES * FIB_GPW];
+  __shared__ uint32_t gmask[FIB_WAVES * FIB_GPW];   // outer flips allowed for the group
   if (io.stop && load_coherent(io.stop)) return;
   double s, ch, cx, cz;
   apply_scalars(io, s, ch, cx, cz);
@@ -173,8 +174,16 @@ __global__ __launch_bounds__(FIB_THREADS) void k_fib_apply(FibModel M, ApplyIO<V
         prev = 0;
       }
     }
+    uint32_t mk = 0;
+    for (int q = 0; q < nout; ++q) {
+      const bool bit = (o >> q) & 1u;
+      const bool up = q + 1 < nout ? ((o >> (q + 1)) & 1u) != 0 : hL != 0;
+      const bool dn = q > 0 && ((o >> (q - 1)) & 1u);
+      if (bit || !(up || dn)) mk |= 1u << q;
+    }
     gcode[g] = o;
     gbase[g] = base;
+    gmask[g] = mk;
   }
   fib_cross_tables(M, h, Fx, T0, T1, T2);     // its barriers also publish xt and the groups
   // per-lane inner code, flip offsets and coefficients (upper neighbour of
@@ -199,27 +208,43 @@ __global__ __launch_bounds__(FIB_THREADS) void k_fib_apply(FibModel M, ApplyIO<V
     cfi[p] = ok ? (RT)(bit ? M.ai[p] : -M.ai[p]) : (RT)0;
   }
   const bool top_in = IB > 0 && ((cin >> (IB - 1)) & 1u);   // inner bit IB-1 set
+  // outer-bit coefficients: FP32 converts them once into registers, FP64
+  // reads them from the (scalar) kernel arguments
+  constexpr int NOUT = FIB_L - FIB_IB;
+  constexpr bool CONV = sizeof(RT) == 4;
+  RT oar[CONV ? NOUT : 1], oai[CONV ? NOUT : 1];
+  if constexpr (CONV) {
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) {
+      const int p = q + IB < MAXN ? q + IB : 0;
+      oar[q] = (RT)M.ar[p];
+      oai[q] = (RT)M.ai[p];
+    }
+  }
+  auto car = [&](int q) -> RT { if constexpr (CONV) return oar[q]; else return (RT)M.ar[IB + q]; };
+  auto cai = [&](int q) -> RT { if constexpr (CONV) return oai[q]; else return (RT)M.ai[IB + q]; };
   const double eh = H.ehi;
   V acc[FIB_GPW];
-  int gb[FIB_GPW];
+  int koff[FIB_GPW];   // this lane's offset in the block for each group of the wave
   bool act[FIB_GPW];
 #pragma unroll
   for (int i = 0; i < FIB_GPW; ++i) {
     const int g = wv + FIB_WAVES * i;
     const bool gv = g < ng;
-    // group code and base are the same for the whole wave: keep them scalar so
-    // every group test below is a uniform branch
+    // group code, base and mask are the same for the whole wave: keep them
+    // scalar so every group test below is a uniform branch
     const uint32_t o = (uint32_t)__builtin_amdgcn_readfirstlane(gv ? (int)gcode[g] : 0);
+    const uint32_t om = (uint32_t)__builtin_amdgcn_readfirstlane(gv ? (int)gmask[g] : 0);
+    const int gbi = __builtin_amdgcn_readfirstlane(gv ? gbase[g] : 0);
     const bool small = nout > 0 ? (o & 1u) : (hL != 0);
-    gb[i] = __builtin_amdgcn_readfirstlane(gv ? gbase[g] : 0);
+    const int k = gbi + lane;
+    koff[i] = k;
     act[i] = gv && lane < (small ? Fsmall : Fbig);
-    const int k = gb[i] + lane;
     const uint32_t lo = (o << IB) | cin;
     const double d = act[i] ? eh + M.lo_diag[k] + T0[lo & 255u] + T1[(lo >> 8) & 255u] +
                                   T2[(lo >> 16) & 3u]
                             : 0.0;
-    const V xv = xt[k];
-    acc[i] = vscale(xv, (RT)d);
+    acc[i] = vscale(xt[k], (RT)d);
     if (!gv) continue;
     // inner flips: fixed per lane; in a small group bit IB-1 cannot be set
 #pragma unroll
@@ -227,66 +252,67 @@ __global__ __launch_bounds__(FIB_THREADS) void k_fib_apply(FibModel M, ApplyIO<V
       if (p == IB - 1 && small) continue;
       vcfma(acc[i], cfr[p], cfi[p], xt[k + off[p]]);
     }
-    // outer flips: group-uniform, bit IB also needs inner bit IB-1 clear
-    for (int q = 0; q < nout; ++q) {
-      const int p = IB + q;
-      const int bit = (int)((o >> q) & 1u);
-      const bool up = q + 1 < nout ? ((o >> (q + 1)) & 1u) : (hL != 0);
-      const bool dn = q > 0 && ((o >> (q - 1)) & 1u);
-      if (!bit && (up || dn)) continue;
-      const int W = (int)M.W[p];
-      const bool lane_ok = bit || q > 0 || !top_in;
-      const RT ar = lane_ok ? (RT)M.ar[p] : (RT)0;
-      const RT ai = lane_ok ? (RT)(bit ? M.ai[p] : -M.ai[p]) : (RT)0;
-      // a blocked lane reads its own (finite) slot: a zero coefficient times
-      // an unwritten slot past the block's range could be NaN
-      vcfma(acc[i], ar, ai, xt[lane_ok ? k + (bit ? -W : W) : k]);
+    // outer flips: allowed per group (mask); setting bit IB also needs the
+    // lane's inner bit IB-1 clear (a blocked lane reads its own finite slot)
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) {
+      if (q >= nout || !((om >> q) & 1u)) continue;
+      const bool bit = (o >> q) & 1u;
+      const int W = (int)M.W[IB + q];
+      if (q == 0 && !bit) {
+        const RT ar = top_in ? (RT)0 : car(0), ai = top_in ? (RT)0 : -cai(0);
+        vcfma(acc[i], ar, ai, xt[top_in ? k : k + W]);
+      } else {
+        vcfma(acc[i], car(q), bit ? cai(q) : -cai(q), xt[k + (bit ? -W : W)]);
+      }
     }
   }
   // high bits: one contiguous partner range per allowed bit; the loads of
-  // every group of the wave are in flight together
+  // every group of the wave are in flight together (uniform base pointer,
+  // per-lane 32-bit offsets)
   for (int p = M.L; p < M.far_end; ++p) {
     const int bit = (int)((h >> p) & 1ull);
     const bool set_ok = !((h >> (p + 1)) & 1ull) && (p == M.L || !((h >> (p - 1)) & 1ull));
     if (!bit && !set_ok) continue;
     const bool need_lo = !bit && p == M.L;     // setting bit L: low bit L-1 must be clear
     const int lim = need_lo ? (int)M.cnt_lo[1] : cnt;
-    const int64_t base = R + (bit ? -M.W[p] : M.W[p]);
+    const V* px = io.x + (R + (bit ? -M.W[p] : M.W[p]));
     const RT ar = (RT)(M.ar[p] * s), ai = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
     V q[FIB_GPW];
 #pragma unroll
-    for (int i = 0; i < FIB_GPW; ++i) {
-      const int k = gb[i] + lane;
-      q[i] = act[i] && k < lim ? io.x[base + k] : vzero(acc[0]);
-    }
+    for (int i = 0; i < FIB_GPW; ++i) q[i] = act[i] && koff[i] < lim ? px[koff[i]] : vzero(acc[0]);
 #pragma unroll
     for (int i = 0; i < FIB_GPW; ++i) vcfma(acc[i], ar, ai, q[i]);
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
   V zr[FIB_GPW];
+  V* py = io.y + R;
   if (io.z && !io.partial) {
+    const V* pz = io.z + R;
 #pragma unroll
-    for (int i = 0; i < FIB_GPW; ++i) zr[i] = act[i] ? io.z[R + gb[i] + lane] : vzero(acc[0]);
+    for (int i = 0; i < FIB_GPW; ++i) zr[i] = act[i] ? pz[koff[i]] : vzero(acc[0]);
   }
 #pragma unroll
   for (int i = 0; i < FIB_GPW; ++i) {
     if (!act[i]) continue;
-    const int64_t r = R + gb[i] + lane;
     if (io.partial) {
-      io.y[r] = acc[i];
+      py[koff[i]] = acc[i];
       continue;
     }
-    const V xe = xt[gb[i] + lane];
+    const V xe = xt[koff[i]];
     V v = vscale(acc[i], chr);
     vfma(v, cxr, xe);
     if (io.z) vfma(v, czr, zr[i]);
-    io.y[r] = v;
+    py[koff[i]] = v;
     if (io.reduce) {
       s0 += rdot(xe, v);
       s1 += nrm2(v);
       s2 += nrm2(xe);
-      if (io.probe) s3 += zdot(((io.probe[r >> 5] >> (r & 31)) & 1u) ? 1.0 : -1.0, v);
+      if (io.probe) {
+        const int64_t r = R + koff[i];
+        s3 += zdot(((io.probe[r >> 5] >> (r & 31)) & 1u) ? 1.0 : -1.0, v);
+      }
     }
   }
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);

@@ -79,7 +79,7 @@ def test_sector_hv_vs_channel_oracle(pkg, dev, n):
     assert np.max(np.abs(y32 - refr)) <= 1e-5 * max(1.0, np.max(np.abs(refr)))
 
 
-@pytest.mark.parametrize('n', [20, 24, 27, 30, 34, 36])
+@pytest.mark.parametrize('n', [20, 24, 27, 30, 34, 36, 38])
 def test_sector_hv_vs_c_oracle(pkg, dev, n):
     import torch
     Om, delta = np.ones(n), -1.5 * np.ones(n)
@@ -128,9 +128,11 @@ def test_sector_ground_state_fp32_refined_n22(pkg, dev):
     assert abs(r['E0'] - e0) <= 1e-10 * abs(e0)
 
 
-@pytest.mark.parametrize('n,dtype', [(36, 'complex128'), (36, 'complex64'), (40, 'float64')])
+@pytest.mark.parametrize('n,dtype', [(36, 'complex128'), (36, 'complex64'), (38, 'complex128'),
+                                     (40, 'float64'), (40, 'float32')])
 def test_sector_hv_hermitian_large(pkg, dev, n, dtype):
-    """<z, H x> = conj(<x, H z>) at sizes beyond the oracle (split H·v for n >= 34)."""
+    """<z, H x> = conj(<x, H z>) at sizes beyond the oracle (FP64 and complex
+    use the two-pass H·v from n = 38 on, FP32 the single pass)."""
     import torch
     rng = np.random.default_rng(n)
     cplx = dtype.startswith('complex')
@@ -147,5 +149,5 @@ def test_sector_hv_hermitian_large(pkg, dev, n, dtype):
     b = torch.vdot(hz.to(torch.complex128 if cplx else torch.float64), x.to(torch.complex128 if cplx else torch.float64)).item()
     scale = (torch.linalg.vector_norm(hx.double() if not cplx else hx.to(torch.complex128)) *
              torch.linalg.vector_norm(z.double() if not cplx else z.to(torch.complex128))).item()
-    tol = 1e-5 if dtype == 'complex64' else 1e-12
+    tol = 1e-5 if dtype in ('complex64', 'float32') else 1e-12
     assert abs(a - b) <= tol * scale
