@@ -71,6 +71,19 @@ template <> __device__ __forceinline__ c64    vfrom_real<c64>(double v)    { ret
 
 template <class V> struct alignas(4 * sizeof(V)) V4 { V v[4]; };
 
+// Streaming (non-temporal) stores for outputs that are not read again soon:
+// they keep the partner data of a tiled H·v in L2.
+__device__ __forceinline__ void nt_store(float* p, float v)   { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void nt_store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void nt_store(c32* p, c32 v) {
+  __builtin_nontemporal_store(v.re, &p->re);
+  __builtin_nontemporal_store(v.im, &p->im);
+}
+__device__ __forceinline__ void nt_store(c64* p, c64 v) {
+  __builtin_nontemporal_store(v.re, &p->re);
+  __builtin_nontemporal_store(v.im, &p->im);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

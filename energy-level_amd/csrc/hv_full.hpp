@@ -200,14 +200,11 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
     }
   }
 
-  if (io.partial) {   // pass A: raw sum for the top-bit pass
+  if (io.partial) {   // pass A: raw sum for the top-bit pass, streamed past L2
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      V4<V> o;
+    for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) o.v[c] = acc[r][c];
-      *reinterpret_cast<V4<V>*>(io.y + base + r * 1024 + t * 4) = o;
-    }
+      for (int c = 0; c < 4; ++c) nt_store(io.y + base + r * 1024 + t * 4 + c, acc[r][c]);
     return;
   }
   // combine, store, reduce

@@ -112,11 +112,18 @@ void build_tables(int n, int tb, const std::vector<double>& dl, const std::vecto
 }
 
 // Split threshold: from this n on, the top TOPB bits run in a second pass
-// (k_full_top) so that pass A's partner tiles stay within each XCD's L2.
+// (k_full_top) so that pass A's partner tiles stay within the caches.  The
+// split costs a write and a read of the partial: it pays from n = 23 for
+// 8- and 16-byte elements, from n = 25 for an FP32 H·v, and only beyond
+// n = 30 for an FP32 step that also streams z (measured on MI355X).
 #ifndef RYD_SPLIT_MIN_N
 #define RYD_SPLIT_MIN_N 23
 #endif
 constexpr int SPLIT_MIN_N = RYD_SPLIT_MIN_N;
+inline int split_min_n(size_t elem, bool with_z) {
+  if (elem >= 8) return SPLIT_MIN_N;
+  return with_z ? SPLIT_MIN_N + 8 : SPLIT_MIN_N + 2;
+}
 
 // Chain sector: split the top FIB_K bits into a second pass from this n on.
 #ifndef RYD_FIB_SPLIT_MIN_N
@@ -195,7 +202,7 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
   constexpr int TB = tile_bits<V>();
   if (P->n >= TB) {
     const uint64_t ntiles = 1ull << (P->n - TB);
-    const bool split = P->n >= SPLIT_MIN_N;
+    const bool split = P->n >= split_min_n(sizeof(V), io.z != nullptr);
     io.nfar = split ? P->n - TOPB - TB : P->n - TB;
     io.partial = split ? 1 : 0;
     k_full_tiled<V><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(M, io, rd);
