@@ -67,6 +67,7 @@ SIGNATURES = {
     'ryd_batch_run': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     'ryd_batch_autotune': (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp]),
     'ryd_batch_results': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    'ryd_batch_ritz': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'ryd_dots': (ctypes.c_int, [ctypes.c_int, _vp, _vp, _i64, _vp, _vp, _vp]),
     'ryd_cast': (ctypes.c_int, [ctypes.c_int, _vp, ctypes.c_int, _vp, _i64, _vp]),
 }

@@ -20,6 +20,8 @@ from .model import RydModel, from_positions, chain_positions, c6_coupling, from_
 
 __all__ = ['BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip']
 
+BMAXN = 24            # largest instance n (occupation rows hold BMAXN sites + the norm)
+
 
 class BatchLanczos:
     """``ryd_batch`` owner for a list of :class:`RydModel` (full basis, real drive)."""
@@ -116,6 +118,33 @@ class BatchLanczos:
                                               stop.ctypes.data))
         return alpha, nrm, stop
 
+    def _lowest(self, alpha, nrm, stop, b):
+        k = self.steps
+        if stop[b]:
+            # first step whose new norm fell below the breakdown threshold
+            tiny = 1e-6 if self.dtype == 'float32' else 1e-13
+            bad = np.nonzero(~(nrm[b, 1:] > tiny))[0]
+            k = int(bad[0]) + 1 if bad.size else self.steps
+        a, bb = alpha[b, :k], nrm[b, 1:k]
+        theta, Y = _tridiag(a, bb)
+        return k, theta, Y, _genuine_lowest(a, bb, theta, 2)
+
+    def occupations(self, v0):
+        """<n_i> of each instance's lowest Ritz vector (list of arrays, site
+        order), replaying the recurrence from the same start vectors ``v0``."""
+        import torch
+        alpha, nrm, stop = self.coefficients()
+        ycoef = np.zeros((self.count, self.steps))
+        for b in range(self.count):
+            k, theta, Y, picks = self._lowest(alpha, nrm, stop, b)
+            ycoef[b, :k] = Y[:, picks[0]]
+        ritz = torch.empty(self.total, dtype=self.tdt, device=self.device)
+        occ = torch.empty((self.count, BMAXN + 1), dtype=torch.float64, device=self.device)
+        nat.check(nat.lib().ryd_batch_ritz(self._h, nat.ptr(v0.contiguous()), ycoef.ctypes.data,
+                                           nat.ptr(ritz), nat.ptr(occ), nat.stream_ptr()))
+        occ = occ.cpu().numpy()
+        return [occ[b, :int(m.n)] / occ[b, BMAXN] for b, m in enumerate(self.models)]
+
     def ground_states(self):
         """E0, gap (lowest two genuine Ritz values) and Ritz bound per instance."""
         alpha, nrm, stop = self.coefficients()
@@ -124,15 +153,7 @@ class BatchLanczos:
         bound = np.zeros(self.count)
         steps = np.zeros(self.count, np.int64)
         for b in range(self.count):
-            k = self.steps
-            if stop[b]:
-                # first step whose new norm fell below the breakdown threshold
-                tiny = 1e-6 if self.dtype == 'float32' else 1e-13
-                bad = np.nonzero(~(nrm[b, 1:] > tiny))[0]
-                k = int(bad[0]) + 1 if bad.size else self.steps
-            a, bb = alpha[b, :k], nrm[b, 1:k]
-            theta, Y = _tridiag(a, bb)
-            picks = _genuine_lowest(a, bb, theta, 2)
+            k, theta, Y, picks = self._lowest(alpha, nrm, stop, b)
             E0[b] = theta[picks[0]]
             if len(picks) > 1:
                 gap[b] = theta[picks[1]] - theta[picks[0]]
@@ -159,14 +180,21 @@ def disorder_ensemble(count=512, nmin=10, nmax=20, sigma=0.1, deltas=None, c6=10
 
 
 def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device='cuda',
-                             autotune=True, use_graph=True):
-    """Dict like batched_ground_state (E0, gap, backend) for uneven-N instances."""
+                             autotune=True, use_graph=True, observables=True):
+    """Dict like batched_ground_state (tomisnot/energy-level
+    src/core/manybody/equilibrium.py:251-311): E0 (B,), gap (B,), n_atom (a
+    list, one array of <n_i> per instance since N may differ) and backend,
+    for uneven-N instances.  ``observables=False`` skips the Ritz replay."""
     B = BatchLanczos(models, steps=steps, dtype=dtype, device=device)
-    v0 = B.start_vectors(seed)
-    if autotune:
-        B.autotune(v0)
-    B.run(v0, use_graph=use_graph)
-    out = B.ground_states()
-    out.update(backend='hip-batched-lanczos', tile_bits=B.tile_bits, dtype=dtype)
-    B.close()
+    try:
+        v0 = B.start_vectors(seed)
+        if autotune:
+            B.autotune(v0)
+        B.run(v0, use_graph=use_graph)
+        out = B.ground_states()
+        if observables:
+            out['n_atom'] = B.occupations(v0)
+        out.update(backend='hip-batched-lanczos', tile_bits=B.tile_bits, dtype=dtype)
+    finally:
+        B.close()
     return out

@@ -250,9 +250,12 @@ __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restr
   batch_store(L, I, ltile, sa);
 }
 
+// orth half-step; with ritz != nullptr (replay) it also accumulates the Ritz
+// vector x_b += (y_b[j] / nrm_b[j]) u_j with y_b = ycoef[b][0..]
 template <class V>
 __global__ __launch_bounds__(256) void k_batch_orth(BatchLz L, const V* __restrict__ uall,
-                                                    V* __restrict__ wall) {
+                                                    V* __restrict__ wall, V* __restrict__ ritz,
+                                                    const double* __restrict__ ycoef) {
   const int gt = blockIdx.x;
   const int b = L.tile_inst[gt];
   const BatchInst& I = L.inst[b];
@@ -262,13 +265,83 @@ __global__ __launch_bounds__(256) void k_batch_orth(BatchLz L, const V* __restri
   const int64_t base = I.off + ((int64_t)ltile << I.tb);
   const double* nrm = L.nrm + (size_t)b * (L.ldk + 1);
   const V coef = (V)(-L.alpha[(size_t)b * L.ldk + L.j] / nrm[L.j]);
+  const V rc = ritz ? (V)(ycoef[(size_t)b * L.ldk + L.j] / nrm[L.j]) : (V)0;
   double acc = 0.0;
   for (int e = threadIdx.x; e < D; e += blockDim.x) {
-    const V v = wall[base + e] + coef * uall[base + e];
+    const V u = uall[base + e];
+    const V v = wall[base + e] + coef * u;
     wall[base + e] = v;
     acc += (double)v * (double)v;
+    if (ritz) ritz[base + e] += rc * u;
   }
   batch_store(L, I, ltile, acc);
+}
+
+// per-tile site occupations of the Ritz vectors: partial[tile][p] = Σ_e x_e^2
+// over codes with bit p set (p < n), partial[tile][BMAXN] = Σ_e x_e^2
+template <class V>
+__global__ __launch_bounds__(256) void k_batch_occ(BatchLz L, const V* __restrict__ xall,
+                                                   double* __restrict__ partial) {
+  __shared__ double sw[256];
+  __shared__ double sh[4][256];
+  const int gt = blockIdx.x;
+  const int b = L.tile_inst[gt];
+  const BatchInst& I = L.inst[b];
+  const int ltile = (int)(gt - I.tile0);
+  const int D = 1 << I.tb;
+  const int t = threadIdx.x;
+  const V* x = xall + I.off + ((int64_t)ltile << I.tb);
+  double w = 0.0, hi[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int e = t; e < D; e += 256) {       // bits 0..7 of e are those of t
+    const double v = (double)x[e] * (double)x[e];
+    w += v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (((e >> 8) >> q) & 1) hi[q] += v;
+  }
+  sw[t] = w;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) sh[q][t] = hi[q];
+  __syncthreads();
+  double* out = partial + (size_t)gt * (BMAXN + 1);
+  if (t <= BMAXN) {
+    double s = 0.0;
+    const int p = t;
+    if (p == BMAXN) {
+      for (int u = 0; u < 256; ++u) s += sw[u];
+    } else if (p < I.n) {
+      if (p < 8 && p < I.tb) {
+        for (int u = 0; u < 256; ++u) if ((u >> p) & 1) s += sw[u];
+      } else if (p < I.tb) {
+        for (int u = 0; u < 256; ++u) s += sh[p - 8][u];
+      } else if ((ltile >> (p - I.tb)) & 1) {
+        for (int u = 0; u < 256; ++u) s += sw[u];
+      }
+    }
+    out[p] = s;
+  }
+}
+
+// per instance: occ[b][i] = Σ over its tiles (fixed order) for site i = n-1-p,
+// occ[b][BMAXN] = |x_b|^2
+__global__ __launch_bounds__(64) void k_batch_occ_finish(BatchLz L, const double* __restrict__ partial,
+                                                         double* __restrict__ occ) {
+  const int b = blockIdx.x;
+  const BatchInst& I = L.inst[b];
+  const int p = threadIdx.x;
+  if (p > BMAXN) return;
+  double s = 0.0;
+  for (int k = 0; k < I.ntiles; ++k) s += partial[(size_t)(I.tile0 + k) * (BMAXN + 1) + p];
+  double* o = occ + (size_t)b * (BMAXN + 1);
+  if (p == BMAXN) o[BMAXN] = s;
+  else if (p < I.n) o[I.n - 1 - p] = s;
+  else o[p] = 0.0;
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_batch_zero(V* __restrict__ x, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = (V)0;
 }
 
 // nrm[b][0] = |start_b|

@@ -926,6 +926,9 @@ struct ryd_batch {
   double* d_nrm = nullptr;           // [count][steps+1]
   int* d_stop = nullptr;             // [count]
   double* d_partial = nullptr;       // [max tiles]
+  size_t maxtiles = 0;
+  double* d_ycoef = nullptr;         // [count][steps] Ritz coefficients (replay)
+  double* d_occp = nullptr;          // [max tiles][BMAXN+1] occupation partials
   hipStream_t stream = nullptr;      // capture / replay stream (default priority)
   hipGraphExec_t exec = nullptr;
   int exec_choice = -1;
@@ -954,7 +957,8 @@ __global__ void k_batch_reset(int* stop, int count) {
 }
 
 template <class V>
-int batch_sequence(ryd_batch* B, int k, hipStream_t st) {
+int batch_sequence(ryd_batch* B, int k, hipStream_t st, V* ritz = nullptr,
+                   const double* ycoef = nullptr) {
   const size_t vs = sizeof(V);
   V* base = static_cast<V*>(B->d_vec);
   V* U[3] = {base, base + B->total, base + 2 * B->total};
@@ -972,7 +976,7 @@ int batch_sequence(ryd_batch* B, int k, hipStream_t st) {
     LAUNCH_CHECK();
     k_batch_finish<<<B->count, 256, 0, st>>>(L, 0);
     LAUNCH_CHECK();
-    k_batch_orth<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 1) % 3]);
+    k_batch_orth<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 1) % 3], ritz, ycoef);
     LAUNCH_CHECK();
     k_batch_finish<<<B->count, 256, 0, st>>>(L, 1);
     LAUNCH_CHECK();
@@ -1093,6 +1097,7 @@ extern "C" int ryd_batch_create(const ryd_model* models, int32_t count, int dtyp
   if (e == hipSuccess) e = hipMalloc(&B->d_nrm, sizeof(double) * (size_t)count * (steps + 1));
   if (e == hipSuccess) e = hipMalloc(&B->d_stop, sizeof(int) * count);
   if (e == hipSuccess) e = hipMalloc(&B->d_partial, sizeof(double) * maxtiles);
+  B->maxtiles = maxtiles;
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev, hipEventDisableTiming);
   if (e != hipSuccess)
@@ -1105,6 +1110,8 @@ extern "C" int ryd_batch_create(const ryd_model* models, int32_t count, int dtyp
 extern "C" int ryd_batch_destroy(ryd_batch* B) {
   if (!B) return RYD_OK;
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->d_ycoef) (void)hipFree(B->d_ycoef);
+  if (B->d_occp) (void)hipFree(B->d_occp);
   for (int k = 0; k < 3; ++k) {
     if (B->d_inst[k]) (void)hipFree(B->d_inst[k]);
     if (B->d_tile_inst[k]) (void)hipFree(B->d_tile_inst[k]);
@@ -1163,6 +1170,43 @@ extern "C" int ryd_batch_autotune(ryd_batch* B, const void* v0, int32_t* chosen_
   (void)hipEventDestroy(b);
   B->choice = bestk;
   if (chosen_tile_bits) *chosen_tile_bits = 10 + bestk;
+  return RYD_OK;
+}
+
+template <class V>
+int batch_ritz(ryd_batch* B, V* ritz, double* occ) {
+  hipStream_t st = B->stream;
+  const int k = B->choice;
+  k_batch_zero<V><<<stream_grid(B->total), 256, 0, st>>>(ritz, B->total);
+  LAUNCH_CHECK();
+  if (int rc = batch_sequence<V>(B, k, st, ritz, B->d_ycoef)) return rc;
+  const BatchLz L = batch_lz(B, k, 0);
+  k_batch_occ<V><<<(unsigned)B->ntiles[k], 256, 0, st>>>(L, ritz, B->d_occp);
+  LAUNCH_CHECK();
+  k_batch_occ_finish<<<B->count, 64, 0, st>>>(L, B->d_occp, occ);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_ritz(ryd_batch* B, const void* v0, const double* ycoef, void* ritz,
+                              double* occ, void* stream) {
+  if (!B || !v0 || !ycoef || !ritz || !occ) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_ritz");
+  hipStream_t caller = static_cast<hipStream_t>(stream);
+  const size_t es = B->dtype == RYD_F32 ? 4 : 8;
+  if (!B->d_ycoef) HIP_TRY(hipMalloc(&B->d_ycoef, sizeof(double) * (size_t)B->count * B->steps));
+  if (!B->d_occp) HIP_TRY(hipMalloc(&B->d_occp, sizeof(double) * B->maxtiles * (BMAXN + 1)));
+  HIP_TRY(hipEventRecord(B->ev, caller));
+  HIP_TRY(hipStreamWaitEvent(B->stream, B->ev, 0));
+  HIP_TRY(hipMemcpyAsync(B->d_ycoef, ycoef, sizeof(double) * (size_t)B->count * B->steps,
+                         hipMemcpyHostToDevice, B->stream));
+  HIP_TRY(hipMemcpyAsync(B->d_vec, v0, es * (size_t)B->total, hipMemcpyDeviceToDevice, B->stream));
+  const int rc = B->dtype == RYD_F32 ? batch_ritz<float>(B, static_cast<float*>(ritz), occ)
+                                     : batch_ritz<double>(B, static_cast<double*>(ritz), occ);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(B->ev, B->stream));
+  HIP_TRY(hipStreamWaitEvent(caller, B->ev, 0));
+  // the host coefficients must stay valid until the copy has run
+  HIP_TRY(hipStreamSynchronize(B->stream));
   return RYD_OK;
 }
 

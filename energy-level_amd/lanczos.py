@@ -37,10 +37,18 @@ __all__ = ['hip_lanczos_ground_state', 'lanczos_run', 'ritz_vectors', 'rayleigh_
 
 
 def _tridiag(a, b):
+    """Eigenpairs of the FP64 tridiagonal (alpha a, beta b); LAPACK's MRRR
+    driver can fail on off-diagonals near underflow after a breakdown, then
+    the dense symmetric solver takes over."""
     from scipy.linalg import eigh_tridiagonal
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
     if len(a) == 1:
         return np.array([a[0]]), np.ones((1, 1))
-    return eigh_tridiagonal(a, b)
+    try:
+        return eigh_tridiagonal(a, b)
+    except np.linalg.LinAlgError:
+        return np.linalg.eigh(np.diag(a) + np.diag(b, 1) + np.diag(b, -1))
 
 
 def _genuine_lowest(a, b, theta, want=2):

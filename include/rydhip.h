@@ -201,6 +201,16 @@ int ryd_batch_autotune(ryd_batch* batch, const void* v0, int32_t* chosen_tile_bi
 /* Host copies: alpha [count][steps], nrm [count][steps+1] (beta_j = nrm[j+1]),
  * stop [count] (1 where the recurrence broke down). */
 int ryd_batch_results(const ryd_batch* batch, double* alpha, double* nrm, int32_t* stop);
+/* Ritz vectors and site occupations of every instance (the n_atom of
+ * batched_ground_state, equilibrium.py:251-311, there from dense
+ * eigenvectors).  Replays the recurrence from the same device start vectors
+ * v0 with the chosen tile size, accumulating x_b = Σ_j ycoef[b][j] q_j
+ * (ycoef: host [count][steps], the tridiagonal eigenvector, zero past the
+ * instance's steps) into ritz (device, batch element type, concatenated like
+ * v0), then occ (device, [count][25]): occ[b][i] = Σ_s |x_b(s)|^2 n_i(s) for
+ * site i < n_b, occ[b][24] = |x_b|^2.  Ordered after and before `stream`. */
+int ryd_batch_ritz(ryd_batch* batch, const void* v0, const double* ycoef, void* ritz,
+                   double* occ, void* stream);
 
 /* sums[0..2] = {Re<a,b>, |b|^2, |a|^2} over n elements (a may be NULL). */
 int ryd_dots(int dtype, const void* a, const void* b, int64_t n, double* sums, void* ws,

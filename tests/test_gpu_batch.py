@@ -68,3 +68,24 @@ def test_batch_fp32_graph_replays_and_autotune(pkg, dev):
     assert conv.sum() >= len(models) // 2
     rel = np.abs(gs['E0'] - g64['E0']) / np.abs(g64['E0'])
     assert np.max(rel[conv]) < 2e-5
+
+
+@pytest.mark.parametrize('dtype,tol', [('float64', 1e-7), ('float32', 2e-3)])
+def test_batch_occupations_vs_dense(pkg, dev, dtype, tol):
+    """n_atom of every instance (Ritz replay) against dense eigenvectors."""
+    ns = [4, 7, 10, 11, 12, 9, 6, 8]
+    models, calibs = _models(pkg, ns, seed=2)
+    out = pkg.batched_ground_state_hip(models, steps=120, dtype=dtype, device=dev, seed=4)
+    assert len(out['n_atom']) == len(ns)
+    checked = 0
+    for b, (calib, De) in enumerate(calibs):
+        n = ns[b]
+        H = orc.build_H_sparse(n, np.ones(n), -De * np.ones(n), calib['pairs'], ['I1'])
+        gs = orc.ground_state_dense(H)
+        if gs['gap'] < 1e-3:
+            continue                      # near-degenerate: the Ritz vector is not unique
+        ref = orc.site_occupations(n, gs['vec'])
+        assert out['n_atom'][b].shape == (n,)
+        assert np.max(np.abs(out['n_atom'][b] - ref)) < tol, (n, out['n_atom'][b], ref)
+        checked += 1
+    assert checked >= 6
