@@ -16,6 +16,8 @@ from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, secto
 from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolution, z2_index, \
     slice_bounds
 from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip
+from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip, \
+    spectrum_lanczos_hip, quadrature, lorentz_spectrum
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
@@ -26,4 +28,6 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'dos_from_moments', 'probe_bits', 'fibonacci', 'chain_blockade_dim',
            'is_chain_blockade_basis', 'sector_codes', 'sector_rank', 'chain_edges', 'CONV',
            'cheb_coefficients', 'terms_for', 'PulseProblem', 'PulseEvolution', 'z2_index',
-           'slice_bounds', 'BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip']
+           'slice_bounds', 'BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip',
+           'EQ_BACKENDS_HIP', 'normalize_backend', 'solve_equilibrium_hip', 'spectrum_lanczos_hip',
+           'quadrature', 'lorentz_spectrum']
