@@ -14,7 +14,7 @@ from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, raylei
 from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, sector_codes, \
     sector_rank, chain_edges
 from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolution, z2_index, \
-    slice_bounds
+    slice_bounds, segment_intervals, segment_drive, propagate_segments_hip
 from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip
 from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip, \
     spectrum_lanczos_hip, quadrature, lorentz_spectrum
@@ -30,4 +30,5 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'cheb_coefficients', 'terms_for', 'PulseProblem', 'PulseEvolution', 'z2_index',
            'slice_bounds', 'BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip',
            'EQ_BACKENDS_HIP', 'normalize_backend', 'solve_equilibrium_hip', 'spectrum_lanczos_hip',
-           'quadrature', 'lorentz_spectrum']
+           'quadrature', 'lorentz_spectrum', 'segment_intervals', 'segment_drive',
+           'propagate_segments_hip']

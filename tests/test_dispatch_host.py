@@ -53,3 +53,22 @@ def pkg_quadrature():
 def pkg_lorentz():
     import __graft_entry__ as ge
     return ge.load_package().lorentz_spectrum
+
+
+def test_segment_rules_known_cases(pkg):
+    segs = [{'t0': 0.0, 'Om': [1.0, 2.0], 'phase': [0.0, np.pi / 2], 'delta': [0.5, -0.5]},
+            {'t0': 4.0, 'Om': [3.0, 0.0], 'delta': [1.5, 0.5]}]
+    assert pkg.segment_intervals(1.0, 6.0, segs) == [(1.0, 4.0), (4.0, 6.0)]
+    assert pkg.segment_intervals(4.0, 6.0, segs) == [(4.0, 6.0)]      # a start at a is no cut
+    om, de = pkg.segment_drive(2.0, segs, 2)
+    assert np.allclose(om, [1.0, 2.0j]) and np.allclose(de, [0.5, -0.5])
+    om, de = pkg.segment_drive(4.0, segs, 2)                          # latest t0 <= t
+    assert np.allclose(om, [3.0, 0.0]) and np.allclose(de, [1.5, 0.5])
+    om, de = pkg.segment_drive(2.0, segs, 2, interp='midpoint')       # mean with the next
+    assert np.allclose(om, [2.0, 1.0j]) and np.allclose(de, [1.0, 0.0])
+    om, de = pkg.segment_drive(5.0, segs, 2, interp='midpoint')       # last one: its own
+    assert np.allclose(om, [3.0, 0.0])
+    with pytest.raises(ValueError):
+        pkg.segment_drive(1.0, [{'t0': 0.0, 'Om': [1.0]}], 2)
+    with pytest.raises(ValueError):
+        pkg.segment_drive(1.0, segs, 2, interp='cubic')
