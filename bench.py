@@ -25,6 +25,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes of one FP64 H.v at N=28 (both launches), measured with rocprofv3
+# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 6.39 + 2.27 GB,
+# pass B 4.30 + 2.15 GB
+MEASURED_TRAFFIC_F64_N28 = 15.11e9
 N_SITES = 28
 
 
@@ -119,6 +123,9 @@ def main():
     ap.add_argument('--configs', default='c1,c2,c3,c4,c5',
                     help='comma list of the five configurations to run after the headline '
                          '("" for none)')
+    ap.add_argument('--no-headline', action='store_true',
+                    help='skip the N=28 H·v timing and print only the configurations '
+                         '(per-workload profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU: the H·v path is built and measured on a single '
@@ -138,6 +145,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_calibration_bytes_read': 8 << n, 'bytes_written': 8 << n,
                           'launches': args.steps}))
+        return
+    if args.no_headline:
+        cfg = _run_configs(dev, [c for c in args.configs.split(',') if c])
+        print(json.dumps({'configs': cfg}, default=_jsonable))
         return
     calib, Om, delta = _model(n)
     plan = pkg.Plan(pkg.from_calib(calib, Om, delta), device=dev)
@@ -180,7 +191,11 @@ def main():
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': f64['achieved_gbps'], 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': f64['achieved_gbps'] / HBM_PEAK_GBPS,
-                     'traffic': None, 'kernel': 'k_full_tiled<double>',
+                     'traffic': MEASURED_TRAFFIC_F64_N28 if n == 28 else None,
+                     'traffic_source': ('rocprofv3 --pmc FETCH_SIZE x2.00 + WRITE_SIZE x1.00 '
+                                        '(plain-copy scale), both launches of one H.v'
+                                        if n == 28 else None),
+                     'kernel': 'k_full_tiled<double> + k_full_top<double> (one H.v)',
                      'alg_bytes_per_launch': f64['alg_bytes'],
                      'kernel_ms': f64['kernel_ms']},
         'fp32': {'hv_per_s': res['f32']['hv_per_s'], 'ms_per_step': res['f32']['ms_per_step'],

@@ -2,9 +2,10 @@
 //
 // Element types of state vectors: float, double, c32 (complex64), c64
 // (complex128, interleaved re/im).  Reductions are always accumulated in
-// FP64.  Block-level partial sums are combined by the last block to finish
-// (atomic arrival counter in the caller's workspace), so every reduction is
-// deterministic for a fixed launch shape.
+// FP64.  Each block stores its partial sums in the caller's workspace and a
+// one-block finishing launch adds them in block order (no atomics, no
+// device-scope fences), so every reduction is deterministic for a fixed
+// launch shape.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -103,7 +104,7 @@ __device__ __forceinline__ int load_coherent(const int* p) {
 constexpr int NRED = 4;
 constexpr size_t WS_HEADER = 256;
 
-// Post-operations run by the last block on the final sums.
+// Post-operations applied to the final sums by the finishing launch.
 enum PostOp : int {
   POST_NONE = 0,      // sums discarded
   POST_STORE = 1,     // out[k] = sum_k
