@@ -32,7 +32,8 @@ class Model(ctypes.Structure):
                 ('n_pairs', ctypes.c_int32),
                 ('pair_i', _ip), ('pair_j', _ip), ('pair_v', _dp),
                 ('basis_kind', ctypes.c_int32),
-                ('codes', _lp), ('n_codes', ctypes.c_int64)]
+                ('codes', _lp), ('n_codes', ctypes.c_int64),
+                ('pair_jr', _dp), ('pair_ji', _dp), ('pair_ar', _dp), ('pair_ai', _dp)]
 
 
 # name -> (restype, argtypes); kept in the order of include/rydhip.h
@@ -92,7 +93,7 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    if L.ryd_abi_version() != 1:
+    if L.ryd_abi_version() != 2:
         raise RuntimeError('librydhip ABI version mismatch')
     _LIB = L
     return L

@@ -14,7 +14,8 @@ import numpy as np
 from . import _native as nat
 
 __all__ = ['fibonacci', 'chain_blockade_dim', 'is_chain_blockade_basis', 'sector_codes',
-           'sector_rank', 'chain_edges']
+           'sector_rank', 'chain_edges', 'number_sector_basis', 'check_number_conserving',
+           'binomial_table']
 
 
 def fibonacci(k):
@@ -69,3 +70,52 @@ def sector_rank(plan, codes):
     nat.check(nat.lib().ryd_sector_rank(plan.handle, nat.ptr(codes), int(codes.numel()),
                                         nat.ptr(out), nat.stream_ptr()))
     return out
+
+
+# ---------------------------------------------------------------------------
+# number sectors (reference src/core/manybody/matrix_free.py:12-48)
+# ---------------------------------------------------------------------------
+def binomial_table(n):
+    """C[a, b] = binomial(a, b) for 0 <= a, b <= n (int64)."""
+    C = np.zeros((n + 1, n + 1), np.int64)
+    for a in range(n + 1):
+        C[a, 0] = 1
+        for b in range(1, a + 1):
+            C[a, b] = C[a - 1, b - 1] + (C[a - 1, b] if b <= a - 1 else 0)
+    return C
+
+
+def number_sector_basis(n, number):
+    """Ascending int64 codes of n bits with ``number`` bits set.
+
+    Codes come from unranking r = 0 .. C(n,k)-1 in the combinatorial number
+    system (r = Σ_i C(p_i, i) over the set bit positions p_1 < … < p_k,
+    i = 1..k), whose order is ascending code order — the order of the
+    reference's ``number_sector_basis``.  Errors as there: a non-integer
+    number or one outside [0, n], n outside [0, 63], raise ValueError."""
+    if isinstance(number, (bool, np.bool_)) or int(number) != number:
+        raise ValueError('the charge sector must be an integer')
+    n, k = int(n), int(number)
+    if not 0 <= n <= 63 or not 0 <= k <= n:
+        raise ValueError('charge sector out of range or n outside [0, 63]')
+    C = binomial_table(n)
+    total = int(C[n, k])
+    r = np.arange(total, dtype=np.int64)
+    codes = np.zeros(total, np.int64)
+    for i in range(k, 0, -1):
+        col = C[:, i]                          # C(p, i), non-decreasing in p for p >= i-1
+        # largest p with C(p, i) <= r
+        p = np.searchsorted(col, r, side='right') - 1
+        codes |= np.left_shift(np.int64(1), p.astype(np.int64))
+        r -= col[p]
+    return codes
+
+
+def check_number_conserving(calib, Om=None, terms=None):
+    """ValueError when Ω or active pair creation (I3) would leave a
+    fixed-number sector (reference matrix_free.py:41-48)."""
+    use = set((calib or {}).get('terms', ()) if terms is None else terms)
+    if Om is not None and np.any(np.asarray(Om, complex) != 0):
+        raise ValueError('a fixed charge sector needs Omega = 0')
+    if 'I3' in use and any(complex(p.get('A', 0) or 0) != 0 for p in (calib or {}).get('pairs', ())):
+        raise ValueError('a fixed charge sector does not support I3 pair creation')

@@ -1,0 +1,192 @@
+// Generic channel H·v: any basis given as sorted codes, and the full basis
+// when pair channels (J exchange, A pair creation) are present.
+//
+// Channel semantics of tomisnot/energy-level src/core/manybody/channels.py:91-132,
+// in gather form for the row of code c (source s = c ^ mask):
+//   Ω on bit p   : s = c ^ 2^p, amplitude Ω/2 if c has bit p set (g→r),
+//                  conj(Ω)/2 otherwise;
+//   J on (i, j)  : s = c ^ (2^pi | 2^pj), source bits (s_i, s_j) = (1, 0) give
+//                  conj(J), (0, 1) give J;
+//   A on (i, j)  : source (0, 0) gives A, (1, 1) gives conj(A);
+//   diagonal     : Σ delta_p b_p + Σ_{p<q} V_pq b_p b_q, rebuilt per code from
+//                  V held in LDS.
+// A source outside the basis is dropped (the projected operator, as the
+// reference's restricted-basis matvec).  Ranks: identity (full basis), the
+// combinatorial number system (a complete fixed-popcount sector, ascending),
+// or a binary search over the sorted codes.
+#pragma once
+#include "common.hpp"
+#include "hv_full.hpp"
+#include "observables.hpp"
+
+namespace ryd {
+
+enum RankMode : int { RANK_FULL = 0, RANK_NUMBER = 1, RANK_SEARCH = 2 };
+
+struct PairChan {
+  int pi, pj;             // bit positions of sites i, j
+  double jr, ji;          // J
+  double ar, ai;          // A
+};
+
+constexpr int CHAN_MAX_BLOCKS = 16384;
+constexpr int BINOM_LD = 65;          // binom[p * BINOM_LD + i] = C(p, i)
+
+struct ChanModel {
+  int n;
+  int mode;                    // RankMode
+  int m;                       // popcount of a number sector
+  int npairs;
+  int64_t dim;
+  const int64_t* codes;        // [dim] ascending (modes NUMBER, SEARCH)
+  const int64_t* binom;        // [64][BINOM_LD]
+  const PairChan* pairs;       // [npairs]
+  const double* vb;            // [n*n] V by bit position
+  double dl[MAXN];             // delta by bit position
+  double ar[MAXN], ai[MAXN];   // Ω/2 by bit position (0: no Ω channel)
+};
+
+__device__ __forceinline__ uint64_t chan_code(const ChanModel& M, int64_t k) {
+  return M.mode == RANK_FULL ? (uint64_t)k : (uint64_t)M.codes[k];
+}
+
+// rank of code s in the basis, -1 when s is not a basis code
+__device__ inline int64_t chan_rank(const ChanModel& M, uint64_t s) {
+  if (M.n < 64 && (s >> M.n)) return -1;
+  if (M.mode == RANK_FULL) return (int64_t)s;
+  if (M.mode == RANK_NUMBER) {
+    if (__popcll(s) != M.m) return -1;
+    int64_t r = 0;
+    int i = 1;
+    for (uint64_t c = s; c; c &= c - 1, ++i) r += M.binom[__builtin_ctzll(c) * BINOM_LD + i];
+    return r;
+  }
+  int64_t lo = 0, hi = M.dim - 1;
+  while (lo <= hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const uint64_t v = (uint64_t)M.codes[mid];
+    if (v == s) return mid;
+    if (v < s) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  return -1;
+}
+
+__device__ inline double chan_diag(const ChanModel& M, const double* vs, uint64_t c) {
+  double d = 0.0;
+  for (uint64_t a = c; a; a &= a - 1) {
+    const int p = __builtin_ctzll(a);
+    d += M.dl[p];
+    const double* row = vs + (size_t)p * M.n;
+    for (uint64_t b = a & (a - 1); b; b &= b - 1) d += row[__builtin_ctzll(b)];
+  }
+  return d;
+}
+
+// V (n*n doubles) staged in LDS by the whole block
+__device__ inline void chan_stage_v(const ChanModel& M, double* vs) {
+  for (int k = threadIdx.x; k < M.n * M.n; k += blockDim.x) vs[k] = M.vb[k];
+  __syncthreads();
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_chan_apply(ChanModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  __shared__ double vs[MAXN * MAXN];
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+  chan_stage_v(M, vs);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < M.dim; k += stride) {
+    const uint64_t c = chan_code(M, k);
+    const V xe = vscale(io.x[k], sr);
+    V acc = vscale(xe, (RT)chan_diag(M, vs, c));
+    for (int p = 0; p < M.n; ++p) {
+      if (M.ar[p] == 0.0 && M.ai[p] == 0.0) continue;
+      const int64_t r = chan_rank(M, c ^ (1ull << p));
+      if (r < 0) continue;
+      const int bit = (int)((c >> p) & 1ull);
+      vcfma(acc, (RT)(M.ar[p] * s), (RT)((bit ? M.ai[p] : -M.ai[p]) * s), io.x[r]);
+    }
+    for (int q = 0; q < M.npairs; ++q) {
+      const PairChan P = M.pairs[q];
+      const uint64_t src = c ^ ((1ull << P.pi) | (1ull << P.pj));
+      const int bi = (int)((src >> P.pi) & 1ull), bj = (int)((src >> P.pj) & 1ull);
+      double are, aim;
+      if (bi != bj) {                       // exchange: (1,0) conj(J), (0,1) J
+        are = P.jr;
+        aim = bi ? -P.ji : P.ji;
+      } else {                              // pair creation: (0,0) A, (1,1) conj(A)
+        are = P.ar;
+        aim = bi ? -P.ai : P.ai;
+      }
+      if (are == 0.0 && aim == 0.0) continue;
+      const int64_t r = chan_rank(M, src);
+      if (r < 0) continue;
+      vcfma(acc, (RT)(are * s), (RT)(aim * s), io.x[r]);
+    }
+    V v = vscale(acc, chr);
+    vfma(v, cxr, xe);
+    if (io.z) vfma(v, czr, io.z[k]);
+    io.y[k] = v;
+    if (io.reduce) {
+      s0 += rdot(xe, v);
+      s1 += nrm2(v);
+      s2 += nrm2(xe);
+      if (io.probe) s3 += zdot(((io.probe[k >> 5] >> (k & 31)) & 1u) ? 1.0 : -1.0, v);
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// site occupations over a code basis: partials[block][p] = Σ |x_k|^2 b_p(code_k),
+// partials[block][MAXN_OCC] = Σ |x_k|^2 (deterministic: fixed chunk per block)
+template <class V>
+__global__ __launch_bounds__(256) void k_chan_occ(ChanModel M, const V* __restrict__ x,
+                                                  double* partials) {
+  __shared__ double red[4][MAXN_OCC + 1];
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  const int64_t per = (M.dim + gridDim.x - 1) / gridDim.x;
+  const int64_t k0 = (int64_t)blockIdx.x * per;
+  const int64_t k1 = k0 + per < M.dim ? k0 + per : M.dim;
+  double* out = partials + (size_t)blockIdx.x * (MAXN_OCC + 1);
+  for (int p = -1; p < M.n; ++p) {
+    double acc = 0.0;
+    for (int64_t k = k0 + t; k < k1; k += 256) {
+      const double w = nrm2(x[k]);
+      if (p < 0) acc += w;
+      else if ((chan_code(M, k) >> p) & 1ull) acc += w;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) red[wid][p < 0 ? MAXN_OCC : p] = acc;
+  }
+  __syncthreads();
+  if (t <= MAXN_OCC) {
+    const bool used = t == MAXN_OCC || t < M.n;
+    out[t] = used ? red[0][t] + red[1][t] + red[2][t] + red[3][t] : 0.0;
+  }
+}
+
+// codes[k] = basis code of rank ranks[k] (or of r0 + k), -1 outside the basis
+__global__ __launch_bounds__(256) void k_chan_codes(ChanModel M, int64_t r0, int64_t count,
+                                                    const int64_t* __restrict__ ranks,
+                                                    int64_t* __restrict__ codes) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+    const int64_t r = ranks ? ranks[k] : r0 + k;
+    codes[k] = (r < 0 || r >= M.dim) ? -1 : (int64_t)chan_code(M, r);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chan_rank(ChanModel M, const int64_t* __restrict__ codes,
+                                                   int64_t count, int64_t* __restrict__ ranks) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride)
+    ranks[k] = codes[k] < 0 ? -1 : chan_rank(M, (uint64_t)codes[k]);
+}
+
+}  // namespace ryd

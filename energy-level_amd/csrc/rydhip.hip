@@ -12,6 +12,7 @@
 #include "common.hpp"
 #include "hv_full.hpp"
 #include "stream_ops.hpp"
+#include "channels.hpp"
 #include "observables.hpp"
 #include "sector.hpp"
 #include "evolve.hpp"
@@ -64,6 +65,15 @@ struct ryd_plan {
   int64_t fib_nh = 0;
   FibTop* d_top = nullptr;           // [FIB_NG] top patterns (split H·v only)
   bool fib_split = false;
+  // generic channel path: code bases and pair flips (J, A)
+  bool generic = false;
+  int rank_mode = 0;                 // RankMode
+  int popcount = 0;                  // number sector
+  int64_t* d_codes = nullptr;        // [dim] ascending
+  int64_t* d_binom = nullptr;        // [64][BINOM_LD]
+  PairChan* d_pairs = nullptr;       // pair flip channels
+  int npairs = 0;
+  double dl[MAXN] = {0};             // delta by bit position
   int fib_L = 0;
   int64_t fib_cnt[2] = {0, 0};
 };
@@ -165,6 +175,30 @@ FibModel fib_model(const ryd_plan* P) {
   return M;
 }
 
+ChanModel chan_model(const ryd_plan* P) {
+  ChanModel M;
+  M.n = P->n;
+  M.mode = P->rank_mode;
+  M.m = P->popcount;
+  M.npairs = P->npairs;
+  M.dim = P->dim;
+  M.codes = P->d_codes;
+  M.binom = P->d_binom;
+  M.pairs = P->d_pairs;
+  M.vb = P->d_vb;
+  for (int p = 0; p < MAXN; ++p) {
+    M.dl[p] = P->dl[p];
+    M.ar[p] = P->ar[p];
+    M.ai[p] = P->ai[p];
+  }
+  return M;
+}
+
+unsigned chan_grid(const ryd_plan* P) {
+  const int64_t g = (P->dim + 255) / 256;
+  return (unsigned)(g < CHAN_MAX_BLOCKS ? (g < 1 ? 1 : g) : CHAN_MAX_BLOCKS);
+}
+
 unsigned fib_grid(const ryd_plan* P) {
   const int64_t nh = P->fib_nh;
   return (unsigned)(nh >= 64 ? 8 * ((nh + 7) / 8) : nh);
@@ -177,6 +211,16 @@ unsigned fib_top_grid(const ryd_plan* P, int run) {
 
 template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
+  if (P->generic) {
+    io.nfar = 0;
+    io.partial = 0;
+    const unsigned grid = chan_grid(P);
+    k_chan_apply<V><<<dim3(grid), dim3(256), 0, st>>>(chan_model(P), io, rd);
+    LAUNCH_CHECK();
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   if (P->basis == RYD_BASIS_CHAIN_FIB) {
     io.nfar = 0;
     // the top-bit pass pays for 8- and 16-byte elements only (FP32 blocks
@@ -394,8 +438,29 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   if (!m->om_re || !m->delta) return fail(RYD_ERR_INVALID, "om_re and delta are required");
   if (m->n_pairs < 0 || (m->n_pairs > 0 && (!m->pair_i || !m->pair_j || !m->pair_v)))
     return fail(RYD_ERR_INVALID, "pair arrays are required when n_pairs > 0");
-  if (m->basis_kind != RYD_BASIS_FULL && m->basis_kind != RYD_BASIS_CHAIN_FIB)
-    return fail(RYD_ERR_UNSUPPORTED, "basis kind not built into this library version");
+  if (m->basis_kind != RYD_BASIS_FULL && m->basis_kind != RYD_BASIS_CHAIN_FIB &&
+      m->basis_kind != RYD_BASIS_CODES)
+    return fail(RYD_ERR_INVALID, "unknown basis kind");
+  bool pairflip = false;
+  for (int k = 0; k < m->n_pairs; ++k) {
+    const double jr = m->pair_jr ? m->pair_jr[k] : 0.0, ji = m->pair_ji ? m->pair_ji[k] : 0.0;
+    const double ar = m->pair_ar ? m->pair_ar[k] : 0.0, ai = m->pair_ai ? m->pair_ai[k] : 0.0;
+    if (!std::isfinite(jr) || !std::isfinite(ji) || !std::isfinite(ar) || !std::isfinite(ai))
+      return fail(RYD_ERR_INVALID, "J/A must be finite");
+    if (jr != 0.0 || ji != 0.0 || ar != 0.0 || ai != 0.0) pairflip = true;
+  }
+  if (pairflip && m->basis_kind == RYD_BASIS_CHAIN_FIB)
+    return fail(RYD_ERR_UNSUPPORTED, "the chain sector kernels take Omega/delta/V only; give the "
+                                     "sector as codes for J/A models");
+  if (m->basis_kind == RYD_BASIS_CODES) {
+    if (!m->codes || m->n_codes < 1) return fail(RYD_ERR_INVALID, "codes basis needs codes");
+    for (int64_t k = 0; k < m->n_codes; ++k) {
+      if (m->codes[k] < 0 || (n < 63 && (m->codes[k] >> n)))
+        return fail(RYD_ERR_INVALID, "codes must lie in [0, 2^n)");
+      if (k && m->codes[k] <= m->codes[k - 1])
+        return fail(RYD_ERR_INVALID, "codes must be ascending and unique");
+    }
+  }
   if (m->basis_kind == RYD_BASIS_FULL && n > 40)
     return fail(RYD_ERR_INVALID, "full basis needs n <= 40");
   for (int i = 0; i < n; ++i) {
@@ -406,7 +471,10 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   ryd_plan* P = new ryd_plan();
   P->n = n;
   P->basis = m->basis_kind;
-  P->dim = m->basis_kind == RYD_BASIS_CHAIN_FIB ? fib(n + 2) : (1ll << n);
+  P->dim = m->basis_kind == RYD_BASIS_CHAIN_FIB ? fib(n + 2)
+           : m->basis_kind == RYD_BASIS_CODES   ? m->n_codes
+                                                : (1ll << n);
+  P->generic = pairflip || m->basis_kind == RYD_BASIS_CODES;
   (void)hipGetDevice(&P->device);
   std::vector<double> dl(n), vb((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) {
@@ -415,6 +483,7 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
     P->ai[p] = m->om_im ? 0.5 * m->om_im[i] : 0.0;
     if (P->ai[p] != 0.0) P->cplx = true;
     dl[p] = m->delta[i];
+    P->dl[p] = m->delta[i];
   }
   for (int k = 0; k < m->n_pairs; ++k) {
     const int i = m->pair_i[k], j = m->pair_j[k];
@@ -434,6 +503,62 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   if (e != hipSuccess) return cleanup(RYD_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
   e = hipMemcpy(P->d_vb, vb.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice);
   if (e != hipSuccess) return cleanup(RYD_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (P->generic) {
+    // pair flip channels by bit position, the sorted codes and the rank rule
+    std::vector<PairChan> pc;
+    for (int k = 0; k < m->n_pairs; ++k) {
+      PairChan c;
+      c.pi = n - 1 - m->pair_i[k];
+      c.pj = n - 1 - m->pair_j[k];
+      c.jr = m->pair_jr ? m->pair_jr[k] : 0.0;
+      c.ji = m->pair_ji ? m->pair_ji[k] : 0.0;
+      c.ar = m->pair_ar ? m->pair_ar[k] : 0.0;
+      c.ai = m->pair_ai ? m->pair_ai[k] : 0.0;
+      if (c.jr != 0.0 || c.ji != 0.0 || c.ar != 0.0 || c.ai != 0.0) pc.push_back(c);
+      if (c.ji != 0.0 || c.ai != 0.0) P->cplx = true;     // complex H: complex vectors only
+    }
+    P->npairs = (int)pc.size();
+    std::vector<int64_t> binom(64 * BINOM_LD, 0);
+    for (int a = 0; a < 64; ++a) {
+      binom[a * BINOM_LD] = 1;
+      for (int b = 1; b <= a; ++b)
+        binom[a * BINOM_LD + b] = binom[(a - 1) * BINOM_LD + b - 1] +
+                                  (b <= a - 1 ? binom[(a - 1) * BINOM_LD + b] : 0);
+    }
+    P->rank_mode = RANK_FULL;
+    if (m->basis_kind == RYD_BASIS_CODES) {
+      // a complete fixed-popcount sector ranks in closed form
+      const int pc0 = __builtin_popcountll((uint64_t)m->codes[0]);
+      bool same = true;
+      for (int64_t k = 1; k < m->n_codes && same; ++k)
+        same = __builtin_popcountll((uint64_t)m->codes[k]) == pc0;
+      P->popcount = pc0;
+      P->rank_mode = (same && m->n_codes == binom[n * BINOM_LD + pc0]) ? RANK_NUMBER : RANK_SEARCH;
+    }
+    e = hipMalloc(&P->d_binom, sizeof(int64_t) * binom.size());
+    if (e == hipSuccess)
+      e = hipMemcpy(P->d_binom, binom.data(), sizeof(int64_t) * binom.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !pc.empty()) e = hipMalloc(&P->d_pairs, sizeof(PairChan) * pc.size());
+    if (e == hipSuccess && !pc.empty())
+      e = hipMemcpy(P->d_pairs, pc.data(), sizeof(PairChan) * pc.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && m->basis_kind == RYD_BASIS_CODES) {
+      e = hipMalloc(&P->d_codes, sizeof(int64_t) * (size_t)m->n_codes);
+      if (e == hipSuccess)
+        e = hipMemcpy(P->d_codes, m->codes, sizeof(int64_t) * (size_t)m->n_codes,
+                      hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess)
+      return cleanup(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
+                     std::string("channel table upload: ") + hipGetErrorString(e));
+    size_t blocks = CHAN_MAX_BLOCKS;
+    P->ws_bytes = WS_HEADER + blocks * NRED * sizeof(double);
+    const size_t occ = (size_t)OCC_BLOCKS * (MAXN_OCC + 1) * sizeof(double);
+    if (WS_HEADER + occ > P->ws_bytes) P->ws_bytes = WS_HEADER + occ;
+    if (m->basis_kind == RYD_BASIS_CODES) {
+      *out = P;
+      return RYD_OK;
+    }
+  }
   if (P->basis == RYD_BASIS_CHAIN_FIB) {
     const int L = n < FIB_L ? n : FIB_L;
     P->fib_L = L;
@@ -546,7 +671,7 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
                      std::string("table upload: ") + hipGetErrorString(e));
   }
   // workspace: header + NRED partials for the largest grid we launch
-  size_t blocks = 8192;
+  size_t blocks = P->generic ? (size_t)CHAN_MAX_BLOCKS : 8192;
   for (int dt = RYD_F32; dt <= RYD_C128; ++dt) {
     size_t g = apply_grid(P, dt);
     if (g > blocks) blocks = g;
@@ -565,6 +690,9 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_lo_diag) (void)hipFree(P->d_lo_diag);
   if (P->d_hi) (void)hipFree(P->d_hi);
   if (P->d_top) (void)hipFree(P->d_top);
+  if (P->d_codes) (void)hipFree(P->d_codes);
+  if (P->d_binom) (void)hipFree(P->d_binom);
+  if (P->d_pairs) (void)hipFree(P->d_pairs);
   for (int k = 0; k < 2; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
@@ -688,6 +816,16 @@ int ryd_cast(int din, const void* x, int dout, void* y, int64_t n, void* stream)
 namespace {
 template <class V>
 int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t st) {
+  if (P->basis == RYD_BASIS_CODES) {
+    const int64_t g0 = (P->dim + 4095) / 4096;
+    const unsigned g = (unsigned)(g0 < OCC_BLOCKS ? g0 : OCC_BLOCKS);
+    double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
+    k_chan_occ<V><<<g, 256, 0, st>>>(chan_model(P), static_cast<const V*>(x), part);
+    LAUNCH_CHECK();
+    k_occ_finish<<<1, 256, 0, st>>>(part, (int)g, P->n, out);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   if (P->basis == RYD_BASIS_CHAIN_FIB) {
     const int64_t nh = P->fib_nh;
     const unsigned g = (unsigned)(nh < OCC_BLOCKS ? nh : OCC_BLOCKS);
@@ -727,11 +865,14 @@ extern "C" int ryd_site_occupation(const ryd_plan* P, int dtype, const void* x, 
 extern "C" int ryd_sector_codes(const ryd_plan* P, const int64_t* ranks, int64_t first,
                                 int64_t count, int64_t* codes, void* stream) {
   if (!P || !codes || count < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_sector_codes");
-  if (P->basis != RYD_BASIS_CHAIN_FIB)
-    return fail(RYD_ERR_UNSUPPORTED, "ranking is defined for the chain blockade sector");
+  if (P->basis == RYD_BASIS_FULL)
+    return fail(RYD_ERR_UNSUPPORTED, "codes and ranks are defined for restricted bases");
   if (count == 0) return RYD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  k_fib_unrank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), first, count, ranks, codes);
+  if (P->basis == RYD_BASIS_CODES)
+    k_chan_codes<<<stream_grid(count), 256, 0, st>>>(chan_model(P), first, count, ranks, codes);
+  else
+    k_fib_unrank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), first, count, ranks, codes);
   LAUNCH_CHECK();
   return RYD_OK;
 }
@@ -739,11 +880,14 @@ extern "C" int ryd_sector_codes(const ryd_plan* P, const int64_t* ranks, int64_t
 extern "C" int ryd_sector_rank(const ryd_plan* P, const int64_t* codes, int64_t count,
                                int64_t* ranks, void* stream) {
   if (!P || !codes || !ranks || count < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_sector_rank");
-  if (P->basis != RYD_BASIS_CHAIN_FIB)
-    return fail(RYD_ERR_UNSUPPORTED, "ranking is defined for the chain blockade sector");
+  if (P->basis == RYD_BASIS_FULL)
+    return fail(RYD_ERR_UNSUPPORTED, "codes and ranks are defined for restricted bases");
   if (count == 0) return RYD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  k_fib_rank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), codes, count, ranks);
+  if (P->basis == RYD_BASIS_CODES)
+    k_chan_rank<<<stream_grid(count), 256, 0, st>>>(chan_model(P), codes, count, ranks);
+  else
+    k_fib_rank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), codes, count, ranks);
   LAUNCH_CHECK();
   return RYD_OK;
 }
@@ -863,8 +1007,8 @@ int do_evolve_vjp(const ryd_plan* P, int dtype, const void* psi_v, void* g_v, do
 }
 
 int check_evo(const ryd_plan* P, int dtype, int K) {
-  if (P->basis != RYD_BASIS_FULL)
-    return fail(RYD_ERR_UNSUPPORTED, "evolution kernels cover the full basis");
+  if (P->basis != RYD_BASIS_FULL || P->generic)
+    return fail(RYD_ERR_UNSUPPORTED, "evolution kernels cover the full basis with Omega/delta/V");
   if (P->n >= SPLIT_MIN_N) return fail(RYD_ERR_UNSUPPORTED, "evolution kernels need n <= 22");
   if (dtype != RYD_C64 && dtype != RYD_C128)
     return fail(RYD_ERR_INVALID, "evolution needs a complex dtype");
@@ -1035,6 +1179,10 @@ extern "C" int ryd_batch_create(const ryd_model* models, int32_t count, int dtyp
     const int n = m.n;
     if (n < 1 || n > BMAXN || m.basis_kind != RYD_BASIS_FULL || !m.om_re || !m.delta)
       return bad(RYD_ERR_INVALID, "batch instances need the full basis with 1 <= n <= 24");
+    for (int q = 0; q < m.n_pairs; ++q)
+      if ((m.pair_jr && m.pair_jr[q] != 0.0) || (m.pair_ji && m.pair_ji[q] != 0.0) ||
+          (m.pair_ar && m.pair_ar[q] != 0.0) || (m.pair_ai && m.pair_ai[q] != 0.0))
+        return bad(RYD_ERR_UNSUPPORTED, "batch instances take Omega/delta/V only (no J/A)");
     for (int i = 0; i < n; ++i)
       if (m.om_im && m.om_im[i] != 0.0) return bad(RYD_ERR_INVALID, "batch instances need real drives");
     B->n.push_back(n);

@@ -251,9 +251,11 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
 
     ``basis``: ``'full'`` or ``'chain_blockade'`` (the nearest-neighbour
     blockade sector of a chain, ranked on the device; no code list is built).
-    ``basis_indices`` equal to that sector (ascending) selects it as well; any
-    other restricted basis raises NotImplementedError.  ``vec`` is in the
-    basis order (ascending codes).
+    ``basis_indices`` (any order, unique codes) restricts to those codes: the
+    chain blockade sector takes the sector kernels, any other list (number
+    sectors, blockade graphs, J/A models) the generic channel kernel with
+    closed-form or binary-search ranks; ``vec`` comes back in the order of
+    ``basis_indices`` (as matrix_free.py:118-286 with basis_indices).
     """
     import torch
     if str(device).startswith('cpu'):
@@ -264,15 +266,30 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     if int(m) < 2 or int(max_restarts) < 0:
         raise ValueError('m must be >= 2 and max_restarts >= 0')
     model = from_calib(calib, Om, delta, gamma, terms)
+    order = None
+    codes = None
     if basis_indices is not None:
         from .basis import is_chain_blockade_basis
-        if not is_chain_blockade_basis(model.n, basis_indices):
-            raise NotImplementedError('only the chain blockade sector is a HIP restricted basis')
-        basis = 'chain_blockade'
-    if basis not in ('full', 'chain_blockade'):
-        raise ValueError('basis must be full or chain_blockade')
-    P = plan if plan is not None else Plan(
-        model, basis='chain_fib' if basis == 'chain_blockade' else 'full', device=device)
+        idx = np.asarray(basis_indices, np.int64).reshape(-1)
+        if idx.size == 0:
+            raise ValueError('basis_indices is empty')
+        if np.any(idx < 0) or np.any(idx >= (1 << model.n)):
+            raise ValueError('basis_indices must lie in [0, 2^n)')
+        if not model.has_pair_flips and is_chain_blockade_basis(model.n, idx):
+            basis = 'chain_blockade'
+        else:
+            # any basis: ascending on the device, results back in the caller's order
+            order = np.argsort(idx, kind='stable')
+            codes = idx[order]
+            if np.any(np.diff(codes) == 0):
+                raise ValueError('basis_indices contains duplicate codes')
+            basis = 'codes'
+    if basis not in ('full', 'chain_blockade', 'codes'):
+        raise ValueError('basis must be full or chain_blockade (or given by basis_indices)')
+    if basis == 'codes' and codes is None:
+        raise ValueError("basis='codes' needs basis_indices")
+    kind = {'full': 'full', 'chain_blockade': 'chain_fib', 'codes': 'codes'}[basis]
+    P = plan if plan is not None else Plan(model, basis=kind, codes=codes, device=device)
     dim = P.dim
     low = dtype in ('float32', 'complex64')
     complex_kind = (not P.is_real) or dtype.startswith('complex')
@@ -321,5 +338,10 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     out['n_atom'] = site_occupations(P, x0)
     if return_state:
         x = x0 / torch.linalg.vector_norm(x0)
-        out['vec'] = x.cpu().numpy().astype(np.complex128)
+        v = x.cpu().numpy().astype(np.complex128)
+        if order is not None:
+            back = np.empty_like(v)
+            back[order] = v
+            v = back
+        out['vec'] = v
     return out

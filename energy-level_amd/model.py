@@ -32,29 +32,46 @@ def active_terms(calib):
 
 @dataclass
 class RydModel:
-    """North-star model on n sites: Ω_i (complex), delta_i, V pairs."""
+    """Model on n sites: Ω_i (complex), delta_i, and per pair (i, j) the
+    density interaction V, exchange J and pair creation A (complex; zero
+    when the term is not active)."""
     n: int
     Om: np.ndarray
     delta: np.ndarray
     pair_i: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     pair_j: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
     pair_v: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float64))
+    pair_J: np.ndarray = field(default_factory=lambda: np.zeros(0, np.complex128))
+    pair_A: np.ndarray = field(default_factory=lambda: np.zeros(0, np.complex128))
+
+    def __post_init__(self):
+        k = len(self.pair_i)
+        if len(self.pair_J) != k:
+            self.pair_J = np.zeros(k, np.complex128)
+        if len(self.pair_A) != k:
+            self.pair_A = np.zeros(k, np.complex128)
+
+    @property
+    def has_pair_flips(self):
+        return bool(np.any(self.pair_J != 0) or np.any(self.pair_A != 0))
 
     @property
     def is_real(self):
-        return not np.any(np.asarray(self.Om).imag != 0)
+        return not (np.any(np.asarray(self.Om).imag != 0) or np.any(self.pair_J.imag != 0)
+                    or np.any(self.pair_A.imag != 0))
 
     def pairs(self):
-        return [{'i': int(i), 'j': int(j), 'V': float(v)}
-                for i, j, v in zip(self.pair_i, self.pair_j, self.pair_v)]
+        return [{'i': int(i), 'j': int(j), 'V': float(v), 'J': complex(J), 'A': complex(A)}
+                for i, j, v, J, A in zip(self.pair_i, self.pair_j, self.pair_v, self.pair_J,
+                                         self.pair_A)]
 
 
 def from_calib(calib, Om=None, delta=None, gamma=None, terms=None):
     """``(calib, Om, delta, gamma, terms)`` → :class:`RydModel`.
 
-    Raises like the reference GPU path: Γ≠0 is refused
-    (matrix_free.py:142-143); J (I2) and A (I3) channels are refused with
-    NotImplementedError because the HIP kernels implement the Ω/δ/V model.
+    Γ≠0 is refused like the reference GPU path (matrix_free.py:142-143).
+    Active terms (``terms`` or ``active_terms(calib)``) select V (I1),
+    J (I2) and A (I3) per pair, as channel_spec (channels.py:91-132).
     """
     atoms = (calib or {}).get('atoms') or []
     n = len(atoms)
@@ -69,24 +86,26 @@ def from_calib(calib, Om=None, delta=None, gamma=None, terms=None):
         raise ValueError('the HIP engine needs a Hermitian Hamiltonian (gamma=0)')
     use = set(active_terms(calib) if terms is None else terms)
     pairs = list(calib.get('pairs') or [])
+    pi, pj, pv, pJ, pA = [], [], [], [], []
     for p in pairs:
-        if 'I2' in use and complex(p.get('J', 0.0) or 0.0) != 0:
-            raise NotImplementedError('exchange (I2/J) channels are not implemented in the HIP kernels')
-        if 'I3' in use and complex(p.get('A', 0.0) or 0.0) != 0:
-            raise NotImplementedError('pair-creation (I3/A) channels are not implemented in the HIP kernels')
-    pi, pj, pv = [], [], []
-    if 'I1' in use:
-        for p in pairs:
-            v = float(np.real(p.get('V', 0.0) or 0.0))
-            if v != 0.0:
-                i, j = int(p['i']), int(p['j'])
-                if not (0 <= i < n and 0 <= j < n) or i == j:
-                    raise ValueError('pair (%d, %d) out of range' % (i, j))
-                pi.append(i)
-                pj.append(j)
-                pv.append(v)
+        v = float(np.real(p.get('V', 0.0) or 0.0)) if 'I1' in use else 0.0
+        J = complex(p.get('J', 0.0) or 0.0) if 'I2' in use else 0j
+        A = complex(p.get('A', 0.0) or 0.0) if 'I3' in use else 0j
+        if v == 0.0 and J == 0 and A == 0:
+            continue
+        i, j = int(p['i']), int(p['j'])
+        if not (0 <= i < n and 0 <= j < n) or i == j:
+            raise ValueError('pair (%d, %d) out of range' % (i, j))
+        if not (np.isfinite(v) and np.isfinite(J) and np.isfinite(A)):
+            raise ValueError('pair couplings must be finite')
+        pi.append(i)
+        pj.append(j)
+        pv.append(v)
+        pJ.append(J)
+        pA.append(A)
     return RydModel(n, Om, delta, np.asarray(pi, np.int32), np.asarray(pj, np.int32),
-                    np.asarray(pv, np.float64))
+                    np.asarray(pv, np.float64), np.asarray(pJ, np.complex128),
+                    np.asarray(pA, np.complex128))
 
 
 # ---------------------------------------------------------------------------

@@ -48,7 +48,14 @@ class Plan:
         dl = arr(model.delta, np.float64)
         pi, pj = arr(model.pair_i, np.int32), arr(model.pair_j, np.int32)
         pv = arr(model.pair_v, np.float64)
+        pJ = np.asarray(model.pair_J, np.complex128)
+        pA = np.asarray(model.pair_A, np.complex128)
         m = nat.Model()
+        if model.has_pair_flips:
+            m.pair_jr = arr(pJ.real, np.float64).ctypes.data_as(nat._dp)
+            m.pair_ji = arr(pJ.imag, np.float64).ctypes.data_as(nat._dp)
+            m.pair_ar = arr(pA.real, np.float64).ctypes.data_as(nat._dp)
+            m.pair_ai = arr(pA.imag, np.float64).ctypes.data_as(nat._dp)
         m.n = self.n
         m.om_re = ore.ctypes.data_as(nat._dp)
         m.om_im = oim.ctypes.data_as(nat._dp)

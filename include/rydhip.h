@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 1
+#define RYD_ABI_VERSION 2
 
 enum ryd_status {
   RYD_OK = 0,
@@ -44,7 +44,8 @@ enum ryd_dtype { RYD_F32 = 0, RYD_F64 = 1, RYD_C64 = 2, RYD_C128 = 3 };
 
 enum ryd_basis_kind {
   RYD_BASIS_FULL = 0,       /* all 2^n codes                                */
-  RYD_BASIS_CODES = 1,      /* explicit ascending int64 codes               */
+  RYD_BASIS_CODES = 1,      /* explicit ascending int64 codes (number
+                               sectors, general blockade graphs)           */
   RYD_BASIS_CHAIN_FIB = 2   /* chain nearest-neighbour blockade sector      */
 };
 
@@ -61,6 +62,14 @@ typedef struct ryd_model {
   int32_t basis_kind;      /* ryd_basis_kind                                 */
   const int64_t* codes;    /* RYD_BASIS_CODES: ascending unique codes (host) */
   int64_t n_codes;
+  /* Pair flip channels (channels.py:119-131), NULL = none: J exchange
+   * |r_i g_j> -> |g_i r_j> takes conj(J), the reverse J; A pair creation
+   * |g_i g_j> -> |r_i r_j> takes A, the reverse conj(A).  A model with pair
+   * flips runs the generic channel kernel (full basis or codes). */
+  const double* pair_jr;   /* [n_pairs] Re J_ij                              */
+  const double* pair_ji;   /* [n_pairs] Im J_ij                              */
+  const double* pair_ar;   /* [n_pairs] Re A_ij                              */
+  const double* pair_ai;   /* [n_pairs] Im A_ij                              */
 } ryd_model;
 
 typedef struct ryd_plan ryd_plan;

@@ -108,8 +108,11 @@ def test_sector_ground_state_vs_projected_dense(pkg, dev):
     assert abs(abs(np.vdot(ref['vec'], r['vec'])) - 1) < 1e-8
     nref = orc.site_occupations(n, ref['vec'], codes)
     assert np.max(np.abs(r['n_atom'] - nref)) < 1e-8
-    with pytest.raises(NotImplementedError):
-        pkg.hip_lanczos_ground_state(calib, Om, delta, basis_indices=codes[:-1])
+    # any other code list runs the generic kernel (binary-search ranks)
+    sub = codes[:-1]
+    r2 = pkg.hip_lanczos_ground_state(calib, Om, delta, basis_indices=sub)
+    H2 = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'], sub)
+    assert abs(r2['E0'] - orc.ground_state_dense(H2)['E0']) <= 1e-10 * abs(ref['E0'])
 
 
 def test_sector_ground_state_fp32_refined_n22(pkg, dev):
