@@ -60,33 +60,40 @@ __device__ __forceinline__ void evo_finish(const EvoIO<V>& io, uint64_t g, V fl,
   }
 }
 
-template <class V>
+// RR r-groups per block: RR = tile_r (one tile per block, bits 10.. of the
+// tile in registers) or RR = 1 (a tile spread over tile_r blocks, so a small
+// vector still fills the chip; the tile's bits 10.. then come from the other
+// groups through L2 like the far bits).
+template <class V, int RR>
 __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
   constexpr int R = tile_r<V>();
   constexpr int TB = tile_bits<V>();
-  constexpr int TILE = 1024 * R;
-  __shared__ V lds[TILE];
+  constexpr int SUB = R / RR;          // blocks per tile
+  static_assert(RR == R || RR == 1, "a block holds one group or the whole tile");
+  __shared__ V lds[RR * 1024];
   __shared__ double Fx[16];
   __shared__ double T0[256];
   __shared__ double T1[16];
   const RT sr = (RT)io.s;
   const int t = threadIdx.x;
   const int nhi = M.n - TB;
-  const uint64_t ntiles = 1ull << nhi;
+  const uint64_t nblk = (1ull << nhi) * SUB;
   const uint64_t b = blockIdx.x;
-  const uint64_t tile = ntiles >= 8 ? ((b & 7ull) * (ntiles >> 3) + (b >> 3)) : b;
+  const uint64_t blk = nblk >= 8 ? ((b & 7ull) * (nblk >> 3) + (b >> 3)) : b;
+  const uint64_t tile = blk / SUB;
+  const int r0 = (int)(blk % SUB) * RR;
   const uint64_t base = tile << TB;
-  V xs[R][4];
+  V xs[RR][4];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + base + r * 1024 + t * 4);
+  for (int rr = 0; rr < RR; ++rr) {
+    V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + base + (r0 + rr) * 1024 + t * 4);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) xs[r][c] = vscale(qv.v[c], sr);
+    for (int c = 0; c < 4; ++c) xs[rr][c] = vscale(qv.v[c], sr);
     V4<V> wv;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) wv.v[c] = xs[r][c];
-    *reinterpret_cast<V4<V>*>(lds + r * 1024 + t * 4) = wv;
+    for (int c = 0; c < 4; ++c) wv.v[c] = xs[rr][c];
+    *reinterpret_cast<V4<V>*>(lds + rr * 1024 + t * 4) = wv;
   }
   if (t < TB) {
     double f = 0.0;
@@ -114,31 +121,41 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
   const int pop_hi = __popcll(tile);
   __syncthreads();
   // flip sum (plan amplitudes, no diagonal)
-  V fl[R][4];
+  V fl[RR][4];
 #pragma unroll
-  for (int r = 0; r < R; ++r)
+  for (int rr = 0; rr < RR; ++rr)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) fl[r][c] = vzero(xs[r][c]);
+    for (int c = 0; c < 4; ++c) fl[rr][c] = vzero(xs[rr][c]);
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int rr = 0; rr < RR; ++rr)
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int bit = (c >> p) & 1;
-        vcfma(fl[r][c], ar, bit ? ai : -ai, xs[r][c ^ (1 << p)]);
+        vcfma(fl[rr][c], ar, bit ? ai : -ai, xs[rr][c ^ (1 << p)]);
       }
   }
+  // bits 10..TB-1: the tile's other groups (registers, or global for RR = 1)
 #pragma unroll
   for (int k = 0; (1 << k) < R; ++k) {
     const int p = 10 + k;
-    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+    if constexpr (RR == R) {
+      const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const int bit = (r >> k) & 1;
+      for (int rr = 0; rr < RR; ++rr) {
+        const int bit = (rr >> k) & 1;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vcfma(fl[r][c], ar, bit ? ai : -ai, xs[r ^ (1 << k)][c]);
+        for (int c = 0; c < 4; ++c) vcfma(fl[rr][c], ar, bit ? ai : -ai, xs[rr ^ (1 << k)][c]);
+      }
+    } else {
+      const int bit = (r0 >> k) & 1;
+      const RT ar = (RT)(M.ar[p] * io.s);
+      const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * io.s);
+      V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + base + (r0 ^ (1 << k)) * 1024 + t * 4);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vcfma(fl[0][c], ar, aim, qv.v[c]);
     }
   }
 #pragma unroll
@@ -149,10 +166,10 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
     const RT aim = bit ? ai : -ai;
     const int tp = t ^ (1 << k);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      V4<V> qv = *reinterpret_cast<const V4<V>*>(lds + r * 1024 + tp * 4);
+    for (int rr = 0; rr < RR; ++rr) {
+      V4<V> qv = *reinterpret_cast<const V4<V>*>(lds + rr * 1024 + tp * 4);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vcfma(fl[r][c], ar, aim, qv.v[c]);
+      for (int c = 0; c < 4; ++c) vcfma(fl[rr][c], ar, aim, qv.v[c]);
     }
   }
   for (int k = 0; k < nhi; ++k) {
@@ -162,16 +179,16 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
     const RT ar = (RT)(M.ar[p] * io.s);
     const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * io.s);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + pbase + r * 1024 + t * 4);
+    for (int rr = 0; rr < RR; ++rr) {
+      V4<V> qv = *reinterpret_cast<const V4<V>*>(io.x + pbase + (r0 + rr) * 1024 + t * 4);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vcfma(fl[r][c], ar, aim, qv.v[c]);
+      for (int c = 0; c < 4; ++c) vcfma(fl[rr][c], ar, aim, qv.v[c]);
     }
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int e0 = r * 1024 + t * 4;
+  for (int rr = 0; rr < RR; ++rr) {
+    const int e0 = (r0 + rr) * 1024 + t * 4;
     V4<double> el = *reinterpret_cast<const V4<double>*>(M.elo + e0);
     const double t1 = T1[e0 >> 8];
 #pragma unroll
@@ -179,7 +196,7 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
       const int e = e0 + c;
       const double d = eh + el.v[c] + T0[e & 255] + t1;
       const double pop = (double)(pop_hi + __popc((unsigned)e));
-      evo_finish(io, base + e, fl[r][c], xs[r][c], d, pop, s0, s1, s2, s3);
+      evo_finish(io, base + e, fl[rr][c], xs[rr][c], d, pop, s0, s1, s2, s3);
     }
   }
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);

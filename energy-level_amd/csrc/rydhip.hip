@@ -900,8 +900,16 @@ int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t
   constexpr int TB = tile_bits<V>();
   unsigned grid;
   if (P->n >= TB) {
+    // below ~1000 tiles one tile per block leaves most CUs idle: spread each
+    // tile over tile_r blocks
+    constexpr int R = tile_r<V>();
     grid = (unsigned)(1ull << (P->n - TB));
-    k_evo_tiled<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+    if (grid < 1024) {
+      grid *= R;
+      k_evo_tiled<V, 1><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+    } else {
+      k_evo_tiled<V, R><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+    }
   } else {
     grid = (unsigned)(((1ull << P->n) + 255) / 256);
     k_evo_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
