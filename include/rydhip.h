@@ -4,7 +4,7 @@
  *   H = Σ_i Ω_i/2 σx_i + Σ_i delta_i n_i + Σ_{i<j} V_ij n_i n_j
  *
  * (reference sign convention: the diagonal carries +delta_i n_i, so the
- * north-star −Δ_i n_i is delta = −Δ; complex Ω: g→r takes Ω/2, r→g conj(Ω)/2).
+ * Hamiltonian term −Δ_i n_i is delta = −Δ; complex Ω: g→r takes Ω/2, r→g conj(Ω)/2).
  *
  * Every entry point is extern "C", takes plain pointers and sizes, returns a
  * status code (RYD_OK = 0) and never throws.  The caller allocates and owns

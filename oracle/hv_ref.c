@@ -3,7 +3,7 @@
  *
  * Restates the reference's channel semantics (tomisnot/energy-level
  * src/core/manybody/channels.py:58-77 diagonal, :91-132 flip channels,
- * :210-221 matvec) for the north-star model: Ω flips, +delta·n_i, V·n_i n_j.
+ * :210-221 matvec) for the Rydberg model: Ω flips, +delta·n_i, V·n_i n_j.
  * Site i is bit (n-1-i) of a code (reference basis.py:6-9).  Only tests/,
  * the smoke check and bench.py's cpu_baseline leg load this library.
  *
