@@ -52,6 +52,10 @@ SIGNATURES = {
     'ryd_lanczos_state_bytes': (ctypes.c_size_t, []),
     'ryd_lanczos_status': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(_i64), _vp]),
+    'ryd_lanczos_stored': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _i64, _i64, _i64, _i64, _vp, _vp,
+                                          _vp, _dbl, _vp, _vp]),
+    'ryd_ritz_stored': (ctypes.c_int, [ctypes.c_int, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
+                                       ctypes.c_int, _vp, _vp, _vp, _vp]),
     'ryd_cheb_moments': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _dbl,
                                         _dbl, _vp, _vp, _vp]),
     'ryd_site_occupation': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),

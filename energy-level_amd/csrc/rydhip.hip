@@ -380,6 +380,59 @@ int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64
   return RYD_OK;
 }
 
+// Recurrence on a stored basis chunk: u_j at basis + (j - jbase)·ld for
+// j in [jbase, jbase + rows); steps j0 .. j0+steps-1 write u_{j+1} in place.
+template <class V>
+int do_lanczos_stored(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, int64_t jbase,
+                      int64_t j0, int64_t steps, double* alpha, double* nrm, void* state,
+                      double tiny, void* ws, hipStream_t st) {
+  V* basis = static_cast<V*>(basis_v);
+  auto row = [&](int64_t j) { return basis + (j - jbase) * ld; };
+  LzState* S = static_cast<LzState*>(state);
+  const int64_t D = P->dim;
+  const unsigned sg = stream_grid(D);
+  if (j0 == 0) {
+    k_lz_init<<<1, 1, 0, st>>>(S, tiny);
+    LAUNCH_CHECK();
+    Reduce rd = make_reduce(ws, POST_LZ_NORM, nullptr);
+    rd.nrm = nrm;
+    rd.st = S;
+    rd.j = -1;
+    k_dots<V><<<sg, 256, 0, st>>>(nullptr, row(0), D, rd);
+    LAUNCH_CHECK();
+    k_reduce_finish<<<1, 256, 0, st>>>(rd, sg);
+    LAUNCH_CHECK();
+  }
+  for (int64_t j = j0; j < j0 + steps; ++j) {
+    ApplyIO<V> io;
+    io.x = row(j);
+    io.z = j > 0 ? row(j - 1) : nullptr;
+    io.y = row(j + 1);
+    io.probe = nullptr;
+    io.smode = SM_LANCZOS;
+    io.s = io.ch = io.cx = io.cz = 0.0;
+    io.nrm = nrm;
+    io.j = (int)j;
+    io.stop = &S->stop;
+    io.reduce = 1;
+    Reduce ra = make_reduce(ws, POST_LZ_ALPHA, nullptr);
+    ra.alpha = alpha;
+    ra.st = S;
+    ra.j = (int)j;
+    if (int rc = launch_apply<V>(P, dtype, io, ra, st)) return rc;
+    Reduce rb = make_reduce(ws, POST_LZ_NORM, nullptr);
+    rb.nrm = nrm;
+    rb.st = S;
+    rb.j = (int)j;
+    k_lz_orth<V><<<sg, 256, 0, st>>>(row(j), row(j + 1), D, alpha, nrm, (int)j, nullptr, nullptr,
+                                     nullptr, nullptr, &S->stop, 1, rb);
+    LAUNCH_CHECK();
+    k_reduce_finish<<<1, 256, 0, st>>>(rb, sg);
+    LAUNCH_CHECK();
+  }
+  return RYD_OK;
+}
+
 template <class V>
 int do_cheb(const ryd_plan* P, int dtype, const uint32_t* probe, void* t0, void* t1, void* t2,
             int64_t nsteps, double c, double r, double* sums, void* ws, hipStream_t st) {
@@ -724,6 +777,57 @@ int ryd_apply(const ryd_plan* P, int dtype, const void* x, const void* z, void* 
 }
 
 size_t ryd_lanczos_state_bytes(void) { return sizeof(LzState); }
+
+int ryd_lanczos_stored(const ryd_plan* P, int dtype, void* basis, int64_t ld, int64_t jbase,
+                       int64_t j0, int64_t steps, double* alpha, double* nrm, void* state,
+                       double tiny, void* ws, void* stream) {
+  if (!P || !basis || !alpha || !nrm || !state || !ws || ld < P->dim || j0 < jbase ||
+      (j0 > 0 && j0 - 1 < jbase) || steps < 0)
+    return fail(RYD_ERR_INVALID, "bad argument to ryd_lanczos_stored");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_lanczos_stored, P, dtype, basis, ld, jbase, j0, steps, alpha, nrm, state,
+           tiny, ws, st);
+}
+
+int ryd_ritz_stored(int dtype, const void* basis, int64_t ld, int64_t dim, int64_t count,
+                    const double* y0, const double* y1, const double* nrm, int64_t j0,
+                    int accumulate, void* out0, void* out1, void* ws, void* stream) {
+  if (!basis || !y0 || !nrm || !out0 || !ws || dim < 1 || ld < dim || count < 0 ||
+      count > RITZ_MAX || (y1 && !out1))
+    return fail(RYD_ERR_INVALID, "bad argument to ryd_ritz_stored");
+  if (count == 0 && accumulate) return RYD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double* yd = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
+  HIP_TRY(hipMemcpyAsync(yd, y0, sizeof(double) * count, hipMemcpyHostToDevice, st));
+  if (y1) HIP_TRY(hipMemcpyAsync(yd + RITZ_MAX, y1, sizeof(double) * count, hipMemcpyHostToDevice, st));
+  const int two = y1 ? 1 : 0;
+  const unsigned g = stream_grid(dim);
+  switch (dtype) {
+    case RYD_F32:
+      k_ritz_stored<float><<<g, 256, 0, st>>>(static_cast<const float*>(basis), ld, dim, (int)count,
+          yd, two, nrm, j0, accumulate, static_cast<float*>(out0), static_cast<float*>(out1));
+      break;
+    case RYD_F64:
+      k_ritz_stored<double><<<g, 256, 0, st>>>(static_cast<const double*>(basis), ld, dim, (int)count,
+          yd, two, nrm, j0, accumulate, static_cast<double*>(out0), static_cast<double*>(out1));
+      break;
+    case RYD_C64:
+      k_ritz_stored<c32><<<g, 256, 0, st>>>(static_cast<const c32*>(basis), ld, dim, (int)count,
+          yd, two, nrm, j0, accumulate, static_cast<c32*>(out0), static_cast<c32*>(out1));
+      break;
+    case RYD_C128:
+      k_ritz_stored<c64><<<g, 256, 0, st>>>(static_cast<const c64*>(basis), ld, dim, (int)count,
+          yd, two, nrm, j0, accumulate, static_cast<c64*>(out0), static_cast<c64*>(out1));
+      break;
+    default:
+      return fail(RYD_ERR_INVALID, "unknown dtype");
+  }
+  LAUNCH_CHECK();
+  // the host coefficient copies must finish before the caller's arrays go away
+  HIP_TRY(hipStreamSynchronize(st));
+  return RYD_OK;
+}
 
 int ryd_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64_t j0,
                 int64_t steps, double* alpha, double* nrm, void* state, double tiny, void* ritz0,

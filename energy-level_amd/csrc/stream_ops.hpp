@@ -40,6 +40,40 @@ __global__ __launch_bounds__(256) void k_lz_orth(const V* __restrict__ uj, V* __
   if (reduce) block_reduce_store(0.0, acc, 0.0, 0.0, rd.partials);
 }
 
+// Ritz vectors from a stored Krylov basis: out_k (+)= Σ_j (y_k[j] / nrm[j0+j]) u_{j0+j}
+// over `count` rows of `basis` (row stride ld); coefficients precomputed per
+// block in LDS (count <= RITZ_MAX).
+constexpr int RITZ_MAX = 2048;
+template <class V>
+__global__ __launch_bounds__(256) void k_ritz_stored(const V* __restrict__ basis, int64_t ld,
+                                                     int64_t dim, int count, const double* y,
+                                                     int two, const double* nrm, int64_t j0,
+                                                     int accumulate, V* __restrict__ out0,
+                                                     V* __restrict__ out1) {
+  using RT = typename VT<V>::R;
+  __shared__ RT c0[RITZ_MAX], c1[RITZ_MAX];
+  for (int j = threadIdx.x; j < count; j += blockDim.x) {
+    const double inv = 1.0 / nrm[j0 + j];
+    c0[j] = (RT)(y[j] * inv);
+    c1[j] = two ? (RT)(y[RITZ_MAX + j] * inv) : (RT)0;
+  }
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    V a0 = accumulate ? out0[i] : vzero(basis[i]);
+    V a1 = vzero(basis[i]);
+    if (two) a1 = accumulate ? out1[i] : a1;
+#pragma unroll 4
+    for (int j = 0; j < count; ++j) {
+      const V u = basis[(int64_t)j * ld + i];
+      vfma(a0, c0[j], u);
+      if (two) vfma(a1, c1[j], u);
+    }
+    out0[i] = a0;
+    if (two) out1[i] = a1;
+  }
+}
+
 // sums: <a, b>, |b|^2, |a|^2 ; used for norms and Rayleigh quotients
 template <class V>
 __global__ __launch_bounds__(256) void k_dots(const V* __restrict__ a, const V* __restrict__ b,

@@ -6,9 +6,9 @@ names and keep every result key.
 Backends (declared like ``EQ_BACKENDS``, backends.py:112-118):
 
 * ``hip-lanczos`` (gs): matrix-free device Lanczos on the full basis, or on
-  the chain blockade sector when ``system['plan']`` is a projected chain
-  plan; keys of ``torch_lanczos_ground_state`` plus ``n_atom``
-  (eq_route.py:689-699).
+  the states of a projected ``system['plan']`` (the reference's gpu-lanczos
+  refuses projected plans, eq_route.py:689-691); keys of
+  ``torch_lanczos_ground_state`` plus ``n_atom`` (eq_route.py:689-699).
 * ``hip-kpm`` (thermal, dos): matrix-free KPM; no host CSR is assembled
   (the reference builds one through ``_system_H``, eq_route.py:743, :774).
 * ``hip-batch`` (gs): the batched solver on this one system
@@ -53,16 +53,16 @@ def normalize_backend(name, want):
     return name
 
 
-def _chain_projected(system, n):
-    """True for a projected plan whose states are the chain blockade sector."""
+def _projected_states(system):
+    """The states of a projected plan (any independent-set or sector basis),
+    or None for the full basis."""
     plan = system.get('plan') or {}
     if not plan.get('projected'):
-        return False
-    from .basis import is_chain_blockade_basis
+        return None
     states = plan.get('states')
-    if states is None or not is_chain_blockade_basis(n, states):
-        raise NotImplementedError('HIP backends take the full basis or the chain blockade sector')
-    return True
+    if states is None:
+        raise ValueError('a projected plan needs its states')
+    return np.asarray(states, dtype=np.int64)
 
 
 def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=None, num=400,
@@ -85,15 +85,18 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
         raise ValueError('equilibrium needs a Hermitian Hamiltonian (gamma = 0)')
     device = system.get('device', 'cuda')
     base = dict(want=want, backend=backend, n_sites=n)
-    projected = _chain_projected(system, n)
+    states = _projected_states(system)
+    projected = states is not None
     if want == 'gs':
         if backend == 'hip-lanczos':
             from .lanczos import hip_lanczos_ground_state
+            # a projected plan's states become the basis (the chain blockade
+            # sector takes the sector kernels, any other list the generic one)
             out = hip_lanczos_ground_state(
                 calib, system.get('Om'), system.get('delta'), gamma, system.get('terms'),
-                m=int(krylov_dim or system.get('krylov_dim', 48)), seed=seed, dtype=dtype,
-                device=device, return_state=return_state, memory_budget_mb=memory_budget_mb,
-                basis='chain_blockade' if projected else 'full')
+                basis_indices=states, m=int(krylov_dim or system.get('krylov_dim', 48)),
+                seed=seed, dtype=dtype, device=device, return_state=return_state,
+                memory_budget_mb=memory_budget_mb)
             result = dict(base, **out)
         else:
             if projected:
@@ -139,9 +142,7 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
     result.setdefault('requested_backend', requested)
     result.setdefault('resolved_backend', result.get('backend'))
     if result.get('vec') is not None:
-        indices = None
-        if projected:
-            indices = np.asarray(system['plan']['states'], dtype=np.int64)
+        indices = states
         result['basis'] = dict(bit_order='site0-msb', indices=indices, dim=len(result['vec']),
                                n_sites=n)
     return result
@@ -157,7 +158,7 @@ def spectrum_lanczos_hip(plan, phi, n_krylov=200, tiny=1e-12):
     ``ortho_loss`` = |<q_0, q_n>| reports how far orthogonality was lost, as
     the reference does after its run."""
     import torch
-    from .lanczos import lanczos_run
+    from .lanczos import lanczos_run, lanczos_vector
     t0 = time.perf_counter()
     v = phi.reshape(-1)
     if v.numel() == 0:
@@ -175,11 +176,12 @@ def spectrum_lanczos_hip(plan, phi, n_krylov=200, tiny=1e-12):
     alphas = np.asarray(R.a[:j], float)
     betas = np.asarray(R.b[:max(j - 1, 0)], float)
     converged = bool(R.stopped or j >= plan.dim)
-    # q_n = u_n / |u_n| sits in buffer n % 3 after n steps
+    # q_n = u_n / |u_n|, the last vector the run produced
     ortho = 0.0
     last = R.nrm[j].item() if j < R.nrm.numel() else 0.0
-    if j >= 1 and last > 0 and not R.stopped:
-        qn = R.bufs[j % 3] / last
+    un = lanczos_vector(R, j)
+    if j >= 1 and last > 0 and not R.stopped and un is not None:
+        qn = un / last
         ortho = float(abs(torch.vdot(v.to(qn.dtype), qn).item()) / norm)
     return {'alphas': alphas, 'betas': betas, 'norm2': norm ** 2, 'n': j,
             'converged': converged, 'ortho_loss': ortho, 'reortho': False,

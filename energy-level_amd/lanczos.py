@@ -14,8 +14,10 @@ compared):
 * after each cycle the FP64 tridiagonal matrix is diagonalised on the host;
   duplicate and Cullum–Willoughby spurious Ritz values are dropped; the two
   lowest genuine ones are tested with the bound |beta_k · y_k[-1]|;
-* Ritz vectors come from a replay of the recurrence with the stored
-  coefficients (no Krylov basis is kept: memory is 6 vectors);
+* for dim >= 2^22 the Krylov basis stays in HBM (one chunk per cycle) and a
+  Ritz vector is one streaming combination of it; below that, or when free
+  memory runs short, the recurrence keeps three vectors and Ritz vectors
+  come from a replay with the stored coefficients;
 * FP32 bulk (``dtype='float32'``/``'complex64'``, ``refine=True``): the FP32
   Ritz pair seeds a short FP64 recurrence, so E0 and the residuals are
   certified in FP64 (``precision`` in the result says which parts ran in
@@ -86,36 +88,91 @@ class _Run:
     pass
 
 
-def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None):
+STORE_MIN_DIM = 1 << 22     # below this the replay of a cycle costs less than a stored basis
+RITZ_ROWS = 2048            # coefficients per ryd_ritz_stored call
+
+
+def _fits(plan, nbytes, dt):
+    """True when ``nbytes`` more device memory leave the working vectors room."""
+    import torch
+    free, _ = torch.cuda.mem_get_info(plan.device)
+    itemsize = torch.tensor([], dtype=dt).element_size()
+    reserve = 8 * plan.dim * itemsize + (4 << 30)
+    return nbytes + reserve <= free
+
+
+def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
     """Cycles of ``m`` device-side Lanczos steps from ``v0`` until the ``want``
-    lowest genuine Ritz values pass the bound test or the budget is spent."""
+    lowest genuine Ritz values pass the bound test or the budget is spent.
+
+    ``store``: keep every Lanczos vector in HBM (one chunk per cycle, while
+    free memory allows) so that Ritz vectors are one streaming combination
+    instead of a replay of the recurrence; 'auto' stores for dim >= 2^22."""
+    import ctypes
     import torch
     L = nat.lib()
     dim = plan.dim
     dt = v0.dtype
+    itemsize = torch.tensor([], dtype=dt).element_size()
     kmax = int(min(max(1, m) * max(1, max_cycles), dim))
     R = _Run()
-    R.bufs = [torch.empty(dim, dtype=dt, device=plan.device) for _ in range(3)]
-    R.bufs[0].copy_(v0)
+    R.v0 = v0
     R.alpha = torch.zeros(kmax + 1, dtype=torch.float64, device=plan.device)
     R.nrm = torch.zeros(kmax + 2, dtype=torch.float64, device=plan.device)
     R.state = torch.zeros(int(L.ryd_lanczos_state_bytes()), dtype=torch.uint8, device=plan.device)
+    R.chunks = []                 # (jbase, rows tensor) of a stored basis
+    R.bufs = None
+    storing = (dim >= STORE_MIN_DIM) if store == 'auto' else bool(store)
     code = nat.dtype_code(dt)
     ws = plan.workspace()
     st = nat.stream_ptr()
     if tiny is None:
         tiny = 1e-13 if dt in (torch.float64, torch.complex128) else 1e-6
+
+    def three_buffers(j):
+        """Switch to (or start) the rotating three-vector recurrence at step j."""
+        if R.bufs is None:
+            R.bufs = [torch.empty(dim, dtype=dt, device=plan.device) for _ in range(3)]
+        if R.chunks:
+            jb, rows = R.chunks[-1]
+            R.bufs[j % 3].copy_(rows[j - jb])
+            if j > 0:
+                R.bufs[(j - 1) % 3].copy_(rows[j - 1 - jb])
+            R.chunks = []
+        elif j == 0:
+            R.bufs[0].copy_(v0)
+
     j, cycles = 0, 0
     stopped = False
+    R.stored = storing
     while True:
         steps = int(min(m, kmax - j))
-        # the breakdown threshold is relative to the start-vector norm
-        nat.check(L.ryd_lanczos(plan.handle, code, nat.ptr(R.bufs[0]), nat.ptr(R.bufs[1]),
-                                nat.ptr(R.bufs[2]), j, steps, nat.ptr(R.alpha), nat.ptr(R.nrm),
-                                nat.ptr(R.state), float(tiny), None, None, None, None, 0,
-                                nat.ptr(ws), st))
+        if storing:
+            jb = j - 1 if j > 0 else 0
+            nrows = j + steps + 1 - jb
+            if _fits(plan, nrows * dim * itemsize, dt):
+                rows = torch.empty((nrows, dim), dtype=dt, device=plan.device)
+                if j == 0:
+                    rows[0].copy_(v0)
+                else:
+                    pjb, prev = R.chunks[-1]
+                    rows[0].copy_(prev[j - 1 - pjb])
+                    rows[1].copy_(prev[j - pjb])
+                R.chunks.append((jb, rows))
+                nat.check(L.ryd_lanczos_stored(plan.handle, code, nat.ptr(rows), dim, jb, j, steps,
+                                               nat.ptr(R.alpha), nat.ptr(R.nrm), nat.ptr(R.state),
+                                               float(tiny), nat.ptr(ws), st))
+            else:
+                storing = R.stored = False
+                three_buffers(j)
+        if not storing:
+            if R.bufs is None:
+                three_buffers(j)
+            nat.check(L.ryd_lanczos(plan.handle, code, nat.ptr(R.bufs[0]), nat.ptr(R.bufs[1]),
+                                    nat.ptr(R.bufs[2]), j, steps, nat.ptr(R.alpha), nat.ptr(R.nrm),
+                                    nat.ptr(R.state), float(tiny), None, None, None, None, 0,
+                                    nat.ptr(ws), st))
         cycles += 1
-        import ctypes
         sflag, nvalid = ctypes.c_int(0), ctypes.c_int64(0)
         nat.check(L.ryd_lanczos_status(nat.ptr(R.state), ctypes.byref(sflag),
                                        ctypes.byref(nvalid), st))
@@ -138,19 +195,52 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None):
             break
     R.j, R.theta, R.Y, R.picks, R.est, R.stopped, R.cycles = j, theta, Y, picks, est, stopped, cycles
     R.a, R.b = a, b
-    R.v0 = v0
     return R
 
 
+def lanczos_vector(R, j):
+    """u_j (unnormalised, |u_j| = R.nrm[j]) when the run still holds it, else None."""
+    if R.chunks:
+        for jb, rows in reversed(R.chunks):
+            if jb <= j < jb + rows.shape[0]:
+                return rows[j - jb]
+        return None
+    if R.bufs is not None:
+        return R.bufs[j % 3]
+    return None
+
+
 def ritz_vectors(plan, R, which=None):
-    """Replay the recurrence of ``R`` and accumulate up to two Ritz vectors."""
+    """Up to two Ritz vectors: one streaming combination of the stored
+    basis, or a replay of the recurrence of ``R`` that accumulates them."""
     import torch
     L = nat.lib()
     which = R.picks if which is None else which
     which = list(which)[:2]
     dt = R.v0.dtype
-    R.bufs[0].copy_(R.v0)
     out = [torch.zeros(plan.dim, dtype=dt, device=plan.device) for _ in which]
+    if R.stored and R.chunks:
+        ys = [np.ascontiguousarray(R.Y[:, i], dtype=np.float64) for i in which]
+        first = True
+        for jb, rows in R.chunks:
+            j0 = jb if jb == 0 else jb + 1          # rows 0/1 repeat the previous chunk
+            j1 = min(jb + rows.shape[0] - 1, R.j)   # u_j used for j < R.j
+            for a in range(j0, j1, RITZ_ROWS):      # the kernel keeps <= RITZ_ROWS coefficients
+                b = min(a + RITZ_ROWS, j1)
+                base = rows[a - jb]
+                y0 = np.ascontiguousarray(ys[0][a:b])
+                y1 = np.ascontiguousarray(ys[1][a:b]) if len(ys) > 1 else None
+                nat.check(L.ryd_ritz_stored(nat.dtype_code(dt), nat.ptr(base), plan.dim, plan.dim,
+                                            b - a, y0.ctypes.data,
+                                            y1.ctypes.data if y1 is not None else None,
+                                            nat.ptr(R.nrm), a, 0 if first else 1, nat.ptr(out[0]),
+                                            nat.ptr(out[1]) if len(out) > 1 else None,
+                                            nat.ptr(plan.workspace()), nat.stream_ptr()))
+                first = False
+        return out
+    if R.bufs is None:
+        R.bufs = [torch.empty(plan.dim, dtype=dt, device=plan.device) for _ in range(3)]
+    R.bufs[0].copy_(R.v0)
     ys = [torch.as_tensor(np.ascontiguousarray(R.Y[:, i]), dtype=torch.float64,
                           device=plan.device) for i in which]
     nat.check(L.ryd_lanczos(plan.handle, nat.dtype_code(dt), nat.ptr(R.bufs[0]),

@@ -129,6 +129,23 @@ size_t ryd_lanczos_state_bytes(void);
 /* Reads the state written by ryd_lanczos (synchronises the stream). */
 int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream);
 
+/* The same recurrence keeping every Lanczos vector (no replay for Ritz
+ * vectors; the reference keeps its block Krylov basis too,
+ * matrix_free.py:219-251).  basis: device chunk of rows of ld >= dim
+ * elements, u_j at basis + (j - jbase)·ld for the rows this call touches,
+ * j in [max(j0-1, jbase), j0 + steps]; at j0 = 0 row 0 holds the start
+ * vector.  A chunk that starts a later cycle repeats u_{j0-1}, u_{j0}. */
+int ryd_lanczos_stored(const ryd_plan* plan, int dtype, void* basis, int64_t ld, int64_t jbase,
+                       int64_t j0, int64_t steps, double* alpha, double* nrm, void* state,
+                       double tiny, void* ws, void* stream);
+/* out0 (+)= Σ_{j<count} (y0[j] / nrm[j0+j]) u_{j0+j}, likewise out1 with y1
+ * (may be NULL), over rows basis + j·ld (count <= 2048; y0/y1 host arrays;
+ * accumulate = 0 overwrites).  ws: >= 256 + 32768 bytes.  Returns after the
+ * launch has been ordered on `stream` and the coefficients copied. */
+int ryd_ritz_stored(int dtype, const void* basis, int64_t ld, int64_t dim, int64_t count,
+                    const double* y0, const double* y1, const double* nrm, int64_t j0,
+                    int accumulate, void* out0, void* out1, void* ws, void* stream);
+
 /* Chebyshev recurrence of one Rademacher probe on H~ = (H - c)/r:
  * t_0 = p, t_1 = H~ t_0, t_k = 2 H~ t_{k-1} - t_{k-2}, k <= nsteps.
  * For each k >= 1 adds {<t_{k-1},t_k>, |t_k|^2, |t_{k-1}|^2, <p,t_k>} to

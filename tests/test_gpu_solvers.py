@@ -128,3 +128,37 @@ def test_kpm_bounds_cover_spectrum(pkg, dev):
     assert emin <= w[0] + 1e-9 and emax >= w[-1] - 1e-9
     assert abs(emin - w[0]) < 1e-6 and abs(emax - w[-1]) < 1e-6
     assert c - r <= w[0] and c + r >= w[-1]
+
+
+@pytest.mark.parametrize('dtype', ['float64', 'complex64'])
+def test_stored_basis_ritz_equals_replay(pkg, dev, dtype, monkeypatch):
+    """Ritz vectors from the HBM-resident Krylov basis equal the replayed ones,
+    also when memory runs out after the first cycle (fallback to the replay)."""
+    import torch
+    from energy_level_amd import lanczos as lz
+    n = 14
+    calib = _chain_calib(pkg, n)
+    Om = np.ones(n) * (np.exp(0.3j) if dtype.startswith('complex') else 1.0)
+    P = pkg.Plan(pkg.from_calib(calib, Om, -1.2 * np.ones(n)), device=dev)
+    tdt = getattr(torch, dtype)
+    v0 = lz._start_vector(P.dim, 3, dtype.startswith('complex'), P.device).to(tdt)
+    tol = 1e-10 if dtype == 'float64' else 1e-5
+    Rr = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=False)
+    xr = lz.ritz_vectors(P, Rr)
+    Rs = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=True)
+    assert Rs.stored and len(Rs.chunks) == Rs.cycles and Rs.j == Rr.j
+    xs = lz.ritz_vectors(P, Rs)
+    for a, b in zip(xr, xs):
+        assert (a - b).abs().max().item() <= tol * a.abs().max().item()
+    calls = {'n': 0}
+
+    def one_chunk(plan, nbytes, dt):
+        calls['n'] += 1
+        return calls['n'] == 1
+    monkeypatch.setattr(lz, '_fits', one_chunk)
+    Rf = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=True)
+    assert not Rf.stored and Rf.j == Rr.j
+    assert np.allclose(Rf.a, Rr.a, rtol=0, atol=0) and np.allclose(Rf.b, Rr.b, rtol=0, atol=0)
+    xf = lz.ritz_vectors(P, Rf)
+    for a, b in zip(xr, xf):
+        assert (a - b).abs().max().item() == 0.0
