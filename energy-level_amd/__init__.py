@@ -12,7 +12,8 @@ from .plan import Plan, cast
 from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, rayleigh_residual, \
     site_occupations
 from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, sector_codes, \
-    sector_rank, chain_edges, number_sector_basis, check_number_conserving
+    sector_rank, chain_edges, number_sector_basis, check_number_conserving, validate_edges, \
+    graph_plan, independent_states_hip, independent_set_count_hip, blockade_edges, build_plan_hip
 from .sector_scan import scan_ground_sectors_hip
 from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolution, z2_index, \
     slice_bounds, segment_intervals, segment_drive, propagate_segments_hip
@@ -33,4 +34,5 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'EQ_BACKENDS_HIP', 'normalize_backend', 'solve_equilibrium_hip', 'spectrum_lanczos_hip',
            'quadrature', 'lorentz_spectrum', 'segment_intervals', 'segment_drive',
            'propagate_segments_hip', 'number_sector_basis', 'check_number_conserving',
-           'scan_ground_sectors_hip']
+           'scan_ground_sectors_hip', 'validate_edges', 'graph_plan', 'independent_states_hip',
+           'independent_set_count_hip', 'blockade_edges', 'build_plan_hip']

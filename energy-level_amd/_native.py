@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_HERE, 'librydhip.so')
 
 RYD_OK, RYD_ERR_INVALID, RYD_ERR_MEMORY, RYD_ERR_DEVICE, RYD_ERR_UNSUPPORTED = range(5)
 RYD_F32, RYD_F64, RYD_C64, RYD_C128 = range(4)
-RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB = range(3)
+ABI_VERSION = 3
+RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH = range(4)
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -33,7 +34,8 @@ class Model(ctypes.Structure):
                 ('pair_i', _ip), ('pair_j', _ip), ('pair_v', _dp),
                 ('basis_kind', ctypes.c_int32),
                 ('codes', _lp), ('n_codes', ctypes.c_int64),
-                ('pair_jr', _dp), ('pair_ji', _dp), ('pair_ar', _dp), ('pair_ai', _dp)]
+                ('pair_jr', _dp), ('pair_ji', _dp), ('pair_ar', _dp), ('pair_ai', _dp),
+                ('n_edges', ctypes.c_int32), ('edge_i', _ip), ('edge_j', _ip)]
 
 
 # name -> (restype, argtypes); kept in the order of include/rydhip.h
@@ -43,6 +45,7 @@ SIGNATURES = {
     'ryd_plan_create': (ctypes.c_int, [ctypes.POINTER(Model), ctypes.POINTER(_vp)]),
     'ryd_plan_destroy': (ctypes.c_int, [_vp]),
     'ryd_plan_dim': (_i64, [_vp]),
+    'ryd_plan_rank_mode': (ctypes.c_int, [_vp]),
     'ryd_workspace_bytes': (ctypes.c_size_t, [_vp]),
     'ryd_hv': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
     'ryd_apply': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
@@ -97,7 +100,7 @@ def lib():
         fn = getattr(L, name)
         fn.restype = res
         fn.argtypes = args
-    if L.ryd_abi_version() != 2:
+    if L.ryd_abi_version() != ABI_VERSION:
         raise RuntimeError('librydhip ABI version mismatch')
     _LIB = L
     return L

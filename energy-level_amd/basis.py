@@ -15,7 +15,9 @@ from . import _native as nat
 
 __all__ = ['fibonacci', 'chain_blockade_dim', 'is_chain_blockade_basis', 'sector_codes',
            'sector_rank', 'chain_edges', 'number_sector_basis', 'check_number_conserving',
-           'binomial_table']
+           'binomial_table', 'validate_edges', 'is_chain_edges', 'graph_plan',
+           'independent_states_hip', 'independent_set_count_hip', 'blockade_edges',
+           'build_plan_hip']
 
 
 def fibonacci(k):
@@ -119,3 +121,140 @@ def check_number_conserving(calib, Om=None, terms=None):
         raise ValueError('a fixed charge sector needs Omega = 0')
     if 'I3' in use and any(complex(p.get('A', 0) or 0) != 0 for p in (calib or {}).get('pairs', ())):
         raise ValueError('a fixed charge sector does not support I3 pair creation')
+
+
+# ---------------------------------------------------------------------------
+# blockade graphs (reference src/core/manybody/basis.py:12-76 and
+# hamiltonian_plan.py:15-72); enumeration and ranks on the device
+# ---------------------------------------------------------------------------
+def validate_edges(n, edges):
+    """Sorted unique (i, j) with i < j; ValueError for a non-integer index, an
+    index outside [0, n) or a self loop (as basis.py:12-21)."""
+    out = set()
+    for edge in edges or []:
+        if len(edge) != 2 or any(isinstance(i, (bool, np.bool_)) or int(i) != i for i in edge):
+            raise ValueError('a blockade edge must be two integer site indices')
+        i, j = int(edge[0]), int(edge[1])
+        if not 0 <= i < n or not 0 <= j < n or i == j:
+            raise ValueError('blockade edge out of range or a self loop: %r' % (tuple(edge),))
+        out.add((min(i, j), max(i, j)))
+    return sorted(out)
+
+
+def is_chain_edges(n, edges):
+    """True when the validated edges are exactly the chain (i, i+1)."""
+    return list(edges) == chain_edges(n)
+
+
+def graph_plan(n, edges, model=None, device='cuda'):
+    """A ``Plan`` on the independent sets of the blockade graph (the device
+    enumerates them; ``plan.dim`` is their number)."""
+    from .model import RydModel
+    from .plan import Plan
+    n = int(n)
+    e = validate_edges(n, edges)
+    if model is None:
+        model = RydModel(n=n, Om=np.zeros(n, complex), delta=np.zeros(n))
+    return Plan(model, basis='graph', edges=e, device=device)
+
+
+def independent_states_hip(n, edges, max_states=None, device='cuda'):
+    """Ascending int64 codes (a device tensor) of the independent sets of the
+    graph: the list of ``independent_states`` (basis.py:24-47) without its
+    2^20 default cap.  ``max_states``: ValueError when the basis is larger
+    (the reference's explicit output-size limit).  n in [0, 60]."""
+    import torch
+    n = int(n)
+    if not 0 <= n <= 60:
+        raise ValueError('blockade-graph bases need 0 <= N <= 60')
+    if n == 0:
+        validate_edges(0, edges)
+        return torch.zeros(1, dtype=torch.int64, device=device)
+    P = graph_plan(n, edges, device=device)
+    try:
+        if max_states is not None and P.dim > int(max_states):
+            raise ValueError('restricted basis has %d states, more than max_states=%d'
+                             % (P.dim, int(max_states)))
+        return sector_codes(P)
+    finally:
+        P.close()
+
+
+def independent_set_count_hip(n, edges, device='cuda'):
+    """Number of independent sets (``independent_set_count``, basis.py:50-76),
+    counted by the device enumeration (no node budget)."""
+    n = int(n)
+    if n == 0:
+        validate_edges(0, edges)
+        return 1
+    P = graph_plan(n, edges, device=device)
+    try:
+        return int(P.dim)
+    finally:
+        P.close()
+
+
+def _constraint_dict(constraint):
+    """Normalised constraint (hamiltonian_plan.py:15-33)."""
+    if constraint is None:
+        return dict(kind='none', mode='full', threshold_mhz=0.0, edges=None)
+    if isinstance(constraint, str):
+        constraint = dict(kind=constraint)
+    if not isinstance(constraint, dict):
+        raise ValueError('manybody.constraint must be a dict or a string')
+    kind = str(constraint.get('kind', 'none')).lower()
+    mode = str(constraint.get('mode', 'project' if kind in ('pxp', 'blockade') else 'full')).lower()
+    if kind not in ('none', 'blockade', 'pxp'):
+        raise ValueError('constraint.kind must be none/blockade/pxp')
+    if mode not in ('full', 'project'):
+        raise ValueError('constraint.mode must be full/project')
+    threshold = float(constraint.get('threshold_mhz', 0.0) or 0.0)
+    if not np.isfinite(threshold) or threshold < 0:
+        raise ValueError('constraint.threshold_mhz must be finite and non-negative')
+    return dict(kind=kind, mode=mode, threshold_mhz=threshold, edges=constraint.get('edges'))
+
+
+def blockade_edges(calib, constraint):
+    """Blockade edges of a projected constraint (hamiltonian_plan.py:36-50):
+    explicit edges, or pairs with |V| >= threshold_mhz."""
+    c = _constraint_dict(constraint)
+    n = len(calib.get('atoms') or [])
+    explicit = c.get('edges')
+    if explicit is not None and not isinstance(explicit, str):
+        return validate_edges(n, explicit)
+    if explicit not in (None, 'auto'):
+        raise ValueError('constraint.edges must be auto or a list of integer edges')
+    threshold = float(c.get('threshold_mhz', 0.0))
+    if threshold <= 0:
+        raise ValueError('blockade/PXP projection needs threshold_mhz > 0 or explicit edges')
+    out = set()
+    for p in calib.get('pairs') or []:
+        if abs(complex(p.get('V', 0.0))) >= threshold:
+            i, j = int(p['i']), int(p['j'])
+            out.add((min(i, j), max(i, j)))
+    return sorted(out)
+
+
+def build_plan_hip(calib, Om=None, delta=None, gamma=None, terms=None, constraint=None,
+                   max_states=None, device='cuda'):
+    """``build_plan`` (hamiltonian_plan.py:53-72) with the projected basis
+    enumerated on the device: the same keys, ``states`` a device int64 tensor
+    (ascending, the reference's order), no 2^20 default cap
+    (``max_states`` still limits when given)."""
+    from .model import active_terms
+    atoms = calib.get('atoms') or []
+    n = len(atoms)
+    Om = np.zeros(n, complex) if Om is None else np.asarray(Om, complex)
+    delta = np.zeros(n) if delta is None else np.asarray(delta, float)
+    gamma = np.zeros(n) if gamma is None else np.asarray(gamma, float)
+    if any(x.shape != (n,) or not np.isfinite(x).all() for x in (Om, delta, gamma)):
+        raise ValueError('Om/delta/gamma must be finite arrays with one entry per atom')
+    c = _constraint_dict(constraint)
+    projected = c['kind'] in ('blockade', 'pxp') and c['mode'] == 'project'
+    edges = blockade_edges(calib, c) if projected else []
+    states = independent_states_hip(n, edges, max_states, device=device) if projected else None
+    return dict(version=1, kind='HamiltonianPlan', constraint=c, n_atoms=n, full_dim=1 << n,
+                dim=(int(states.numel()) if states is not None else 1 << n),
+                projected=projected, blockade_edges=[list(e) for e in edges], states=states,
+                Om=Om.tolist(), delta=delta.tolist(), gamma=gamma.tolist(),
+                terms=list(active_terms(calib) if terms is None else terms), calib=calib)

@@ -13,15 +13,17 @@
 // A source outside the basis is dropped (the projected operator, as the
 // reference's restricted-basis matvec).  Ranks: identity (full basis), the
 // combinatorial number system (a complete fixed-popcount sector, ascending),
-// or a binary search over the sorted codes.
+// a binary search over the sorted codes, or the split tables of an
+// independent-set basis (graph.hpp).
 #pragma once
 #include "common.hpp"
+#include "graph.hpp"
 #include "hv_full.hpp"
 #include "observables.hpp"
 
 namespace ryd {
 
-enum RankMode : int { RANK_FULL = 0, RANK_NUMBER = 1, RANK_SEARCH = 2 };
+enum RankMode : int { RANK_FULL = 0, RANK_NUMBER = 1, RANK_SEARCH = 2, RANK_GRAPH = 3 };
 
 struct PairChan {
   int pi, pj;             // bit positions of sites i, j
@@ -42,6 +44,13 @@ struct ChanModel {
   const int64_t* binom;        // [64][BINOM_LD]
   const PairChan* pairs;       // [npairs]
   const double* vb;            // [n*n] V by bit position
+  // RANK_GRAPH tables (graph.hpp)
+  int gl;                      // low bits L
+  const uint64_t* hpk;         // [2^H + 1] first rank << 24 | class
+  const uint32_t* lo_idx;      // [2^L] index of a low set or GR_NONE
+  const uint32_t* fc;          // [classes] low bits blocked by the class
+  const uint32_t* pre;         // [classes][tl + 1] compatible low sets before t
+  int64_t tl;                  // number of low sets
   double dl[MAXN];             // delta by bit position
   double ar[MAXN], ai[MAXN];   // Ω/2 by bit position (0: no Ω channel)
 };
@@ -54,6 +63,15 @@ __device__ __forceinline__ uint64_t chan_code(const ChanModel& M, int64_t k) {
 __device__ inline int64_t chan_rank(const ChanModel& M, uint64_t s) {
   if (M.n < 64 && (s >> M.n)) return -1;
   if (M.mode == RANK_FULL) return (int64_t)s;
+  if (M.mode == RANK_GRAPH) {
+    const uint64_t e = M.hpk[s >> M.gl];
+    const uint64_t c = e & GR_CLS;
+    if (c == GR_CLS) return -1;
+    const uint32_t l = (uint32_t)(s & ((1ull << M.gl) - 1ull));
+    const uint32_t t = M.lo_idx[l];
+    if (t == GR_NONE || (l & M.fc[c])) return -1;
+    return (int64_t)(e >> 24) + (int64_t)M.pre[(size_t)c * (size_t)(M.tl + 1) + t];
+  }
   if (M.mode == RANK_NUMBER) {
     if (__popcll(s) != M.m) return -1;
     int64_t r = 0;

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -74,6 +75,15 @@ struct ryd_plan {
   PairChan* d_pairs = nullptr;       // pair flip channels
   int npairs = 0;
   double dl[MAXN] = {0};             // delta by bit position
+  // independent-set basis of a blockade graph (graph.hpp)
+  int gr_L = 0;                      // low bits
+  int64_t gr_tl = 0;                 // low sets
+  int64_t gr_nc = 0;                 // boundary classes
+  uint64_t* d_hpk = nullptr;         // [2^H + 1]
+  uint32_t* d_lo_idx = nullptr;      // [2^L]
+  uint32_t* d_lowtab = nullptr;      // [tl]
+  uint32_t* d_fc = nullptr;          // [nc]
+  uint32_t* d_pre = nullptr;         // [nc][tl + 1] (RANK_GRAPH only)
   int fib_L = 0;
   int64_t fib_cnt[2] = {0, 0};
 };
@@ -186,6 +196,12 @@ ChanModel chan_model(const ryd_plan* P) {
   M.binom = P->d_binom;
   M.pairs = P->d_pairs;
   M.vb = P->d_vb;
+  M.gl = P->gr_L;
+  M.hpk = P->d_hpk;
+  M.lo_idx = P->d_lo_idx;
+  M.fc = P->d_fc;
+  M.pre = P->d_pre;
+  M.tl = P->gr_tl;
   for (int p = 0; p < MAXN; ++p) {
     M.dl[p] = P->dl[p];
     M.ar[p] = P->ar[p];
@@ -462,6 +478,166 @@ int check_dtype(const ryd_plan* P, int dtype) {
 
 }  // namespace
 
+namespace {
+
+// Phases 1-2 of the graph scans: block sums, their prefix, and the total.
+template <class F>
+int gr_total(F f, int64_t count, int64_t* bsum, int64_t& total) {
+  const int64_t nb = (count + GR_TILE - 1) / GR_TILE;
+  k_gr_scan_sum<F><<<dim3((unsigned)nb), dim3(256)>>>(f, count, bsum);
+  LAUNCH_CHECK();
+  k_gr_scan_blocks<<<1, 256>>>(bsum, nb);
+  LAUNCH_CHECK();
+  HIP_TRY(hipMemcpy(&total, bsum + nb, sizeof(int64_t), hipMemcpyDeviceToHost));
+  return RYD_OK;
+}
+
+template <class F, class W>
+int gr_write(F f, W w, int64_t count, const int64_t* bsum) {
+  const int64_t nb = (count + GR_TILE - 1) / GR_TILE;
+  k_gr_scan_write<F, W><<<dim3((unsigned)nb), dim3(256)>>>(f, w, count, bsum);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+// Bytes allowed for the per-class prefix table: beyond this the ranks fall
+// back to a binary search over the code list.  Default: twice the code list,
+// at least 256 MiB; RYD_GRAPH_PREFIX_MAX_BYTES overrides it (0 = always search).
+size_t gr_prefix_budget(int64_t dim) {
+  if (const char* env = std::getenv("RYD_GRAPH_PREFIX_MAX_BYTES")) {
+    char* end = nullptr;
+    const unsigned long long v = std::strtoull(env, &end, 10);
+    if (end != env) return (size_t)v;
+  }
+  const size_t codes = (size_t)dim * sizeof(int64_t);
+  return codes * 2 > ((size_t)256 << 20) ? codes * 2 : ((size_t)256 << 20);
+}
+
+// Independent-set basis of the blockade graph in m (graph.hpp): device tables,
+// the dimension and the ascending code list.
+int build_graph(ryd_plan* P, const ryd_model* m) {
+  const int n = P->n;
+  std::vector<uint64_t> nbb(n, 0);          // neighbours by bit position
+  for (int k = 0; k < m->n_edges; ++k) {
+    const int p = n - 1 - m->edge_i[k], q = n - 1 - m->edge_j[k];
+    nbb[p] |= 1ull << q;
+    nbb[q] |= 1ull << p;
+  }
+  auto boundary = [&](int L) {
+    const uint64_t low = (1ull << L) - 1ull;
+    int c = 0;
+    for (int p = L; p < n; ++p) c += (nbb[p] & low) ? 1 : 0;
+    return c;
+  };
+  // low part: all bits up to 16 sites; else the cut near n/2 with the fewest
+  // boundary sites (fewest classes), both parts at most GR_MAXB bits
+  int L = n;
+  if (n > 16) {
+    int best = -1, bscore = 1 << 30;
+    for (int c = n / 2 - 4; c <= n / 2 + 4; ++c) {
+      if (c < 1 || c >= n || c > GR_MAXB || n - c > GR_MAXB) continue;
+      const int score = 4 * boundary(c) + (c > n / 2 ? c - n / 2 : n / 2 - c);
+      if (score < bscore) { bscore = score; best = c; }
+    }
+    if (best < 0) return fail(RYD_ERR_INVALID, "blockade-graph basis needs n <= 60");
+    L = best;
+  }
+  const int H = n - L;
+  const uint64_t low = L == 64 ? ~0ull : (1ull << L) - 1ull;
+  GraphParts G;
+  std::memset(&G, 0, sizeof(G));
+  G.L = L;
+  G.H = H;
+  for (int q = 0; q < L; ++q) G.nb_lo[q] = (uint32_t)(nbb[q] & low);
+  for (int p = 0; p < H; ++p) G.nb_hi[p] = (uint32_t)(nbb[p + L] >> L);
+  int k = 0;
+  for (int p = 0; p < H; ++p)
+    if (nbb[p + L] & low) {
+      G.hb_pos[k] = (uint32_t)p;
+      G.hb_low[k] = (uint32_t)(nbb[p + L] & low);
+      ++k;
+    }
+  G.k = k;
+  for (int j = 0; j < k; ++j)
+    for (int j2 = 0; j2 < k; ++j2)
+      if ((G.nb_hi[G.hb_pos[j]] >> G.hb_pos[j2]) & 1u) G.nb_cls[j] |= 1u << j2;
+  P->gr_L = L;
+
+  const int64_t nlo = 1ll << L, ncls = 1ll << k, nhi = 1ll << H;
+  int64_t most = nlo > ncls ? nlo : ncls;
+  if (nhi > most) most = nhi;
+  int64_t* bsum = nullptr;
+  uint32_t* cls_idx = nullptr;
+  uint32_t* clslist = nullptr;
+  int64_t* ccnt = nullptr;
+  auto release = [&]() {
+    if (bsum) (void)hipFree(bsum);
+    if (cls_idx) (void)hipFree(cls_idx);
+    if (clslist) (void)hipFree(clslist);
+    if (ccnt) (void)hipFree(ccnt);
+  };
+  auto run = [&]() -> int {
+    HIP_TRY(hipMalloc(&bsum, sizeof(int64_t) * (size_t)((most + GR_TILE - 1) / GR_TILE + 1)));
+    // low sets
+    int64_t tl = 0;
+    if (int rc = gr_total(GrLowFlag{G}, nlo, bsum, tl)) return rc;
+    HIP_TRY(hipMalloc(&P->d_lo_idx, sizeof(uint32_t) * (size_t)nlo));
+    HIP_TRY(hipMalloc(&P->d_lowtab, sizeof(uint32_t) * (size_t)tl));
+    if (int rc = gr_write(GrLowFlag{G}, GrIndexWrite{P->d_lo_idx, P->d_lowtab}, nlo, bsum)) return rc;
+    // boundary classes and the low bits each blocks
+    int64_t nc = 0;
+    if (int rc = gr_total(GrClassFlag{G}, ncls, bsum, nc)) return rc;
+    HIP_TRY(hipMalloc(&cls_idx, sizeof(uint32_t) * (size_t)ncls));
+    HIP_TRY(hipMalloc(&clslist, sizeof(uint32_t) * (size_t)nc));
+    if (int rc = gr_write(GrClassFlag{G}, GrIndexWrite{cls_idx, clslist}, ncls, bsum)) return rc;
+    if (nc >= (int64_t)GR_CLS)
+      return fail(RYD_ERR_UNSUPPORTED, "blockade graph: 2^24 or more boundary classes");
+    if ((double)nc * (double)tl > (double)(1ll << 38))
+      return fail(RYD_ERR_UNSUPPORTED, "blockade graph: boundary classes x low sets exceeds 2^38; "
+                                       "order the sites so that the cut crosses fewer edges");
+    HIP_TRY(hipMalloc(&P->d_fc, sizeof(uint32_t) * (size_t)nc));
+    const unsigned cg = (unsigned)(nc < 65536 ? nc : 65536);
+    k_gr_class_mask<<<dim3((unsigned)((nc + 255) / 256)), dim3(256)>>>(G, clslist, nc, P->d_fc);
+    LAUNCH_CHECK();
+    HIP_TRY(hipMalloc(&ccnt, sizeof(int64_t) * (size_t)nc));
+    k_gr_class_count<<<dim3(cg), dim3(256)>>>(P->d_lowtab, tl, P->d_fc, nc, ccnt);
+    LAUNCH_CHECK();
+    // high parts: first ranks and classes
+    int64_t dim = 0;
+    if (int rc = gr_total(GrHighCount{G, cls_idx, ccnt}, nhi, bsum, dim)) return rc;
+    if (dim >= (1ll << 40)) return fail(RYD_ERR_MEMORY, "blockade-graph basis has 2^40 codes or more");
+    HIP_TRY(hipMalloc(&P->d_hpk, sizeof(uint64_t) * (size_t)(nhi + 1)));
+    if (int rc = gr_write(GrHighCount{G, cls_idx, ccnt}, GrHighWrite{G, cls_idx, P->d_hpk}, nhi, bsum))
+      return rc;
+    k_gr_seal<<<1, 1>>>(P->d_hpk, nhi, bsum + (nhi + GR_TILE - 1) / GR_TILE);
+    LAUNCH_CHECK();
+    P->dim = dim;
+    P->gr_tl = tl;
+    P->gr_nc = nc;
+    // constant-time ranks when the prefix table is affordable
+    const size_t pre_bytes = sizeof(uint32_t) * (size_t)nc * (size_t)(tl + 1);
+    if (pre_bytes <= gr_prefix_budget(dim)) {
+      HIP_TRY(hipMalloc(&P->d_pre, pre_bytes));
+      k_gr_prefix<<<dim3(cg), dim3(256)>>>(P->d_lowtab, tl, P->d_fc, nc, P->d_pre);
+      LAUNCH_CHECK();
+      P->rank_mode = RANK_GRAPH;
+    } else {
+      P->rank_mode = RANK_SEARCH;
+    }
+    HIP_TRY(hipMalloc(&P->d_codes, sizeof(int64_t) * (size_t)dim));
+    const unsigned eg = (unsigned)(nhi < 65536 ? nhi : 65536);
+    k_gr_emit<<<dim3(eg), dim3(256)>>>(L, P->d_hpk, nhi, P->d_lowtab, tl, P->d_fc, P->d_codes);
+    LAUNCH_CHECK();
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    return RYD_OK;
+  };
+  const int rc = run();
+  release();
+  return rc;
+}
+
+}  // namespace
+
 #define DISPATCH(dtype, FN, ...)                                 \
   switch (dtype) {                                               \
     case RYD_F32: return FN<float>(__VA_ARGS__);                 \
@@ -492,8 +668,18 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   if (m->n_pairs < 0 || (m->n_pairs > 0 && (!m->pair_i || !m->pair_j || !m->pair_v)))
     return fail(RYD_ERR_INVALID, "pair arrays are required when n_pairs > 0");
   if (m->basis_kind != RYD_BASIS_FULL && m->basis_kind != RYD_BASIS_CHAIN_FIB &&
-      m->basis_kind != RYD_BASIS_CODES)
+      m->basis_kind != RYD_BASIS_CODES && m->basis_kind != RYD_BASIS_GRAPH)
     return fail(RYD_ERR_INVALID, "unknown basis kind");
+  if (m->basis_kind == RYD_BASIS_GRAPH) {
+    if (n > 2 * GR_MAXB) return fail(RYD_ERR_INVALID, "blockade-graph basis needs n <= 60");
+    if (m->n_edges < 0 || (m->n_edges > 0 && (!m->edge_i || !m->edge_j)))
+      return fail(RYD_ERR_INVALID, "edge arrays are required when n_edges > 0");
+    for (int k = 0; k < m->n_edges; ++k) {
+      const int i = m->edge_i[k], j = m->edge_j[k];
+      if (i < 0 || j < 0 || i >= n || j >= n || i == j)
+        return fail(RYD_ERR_INVALID, "blockade edge out of range or a self loop");
+    }
+  }
   bool pairflip = false;
   for (int k = 0; k < m->n_pairs; ++k) {
     const double jr = m->pair_jr ? m->pair_jr[k] : 0.0, ji = m->pair_ji ? m->pair_ji[k] : 0.0;
@@ -526,8 +712,9 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   P->basis = m->basis_kind;
   P->dim = m->basis_kind == RYD_BASIS_CHAIN_FIB ? fib(n + 2)
            : m->basis_kind == RYD_BASIS_CODES   ? m->n_codes
+           : m->basis_kind == RYD_BASIS_GRAPH   ? 0
                                                 : (1ll << n);
-  P->generic = pairflip || m->basis_kind == RYD_BASIS_CODES;
+  P->generic = pairflip || m->basis_kind == RYD_BASIS_CODES || m->basis_kind == RYD_BASIS_GRAPH;
   (void)hipGetDevice(&P->device);
   std::vector<double> dl(n), vb((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) {
@@ -603,11 +790,18 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
     if (e != hipSuccess)
       return cleanup(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
                      std::string("channel table upload: ") + hipGetErrorString(e));
+    if (m->basis_kind == RYD_BASIS_GRAPH) {
+      if (int rc = build_graph(P, m)) {
+        const std::string msg = g_err;
+        ryd_plan_destroy(P);
+        return fail(rc, msg);
+      }
+    }
     size_t blocks = CHAN_MAX_BLOCKS;
     P->ws_bytes = WS_HEADER + blocks * NRED * sizeof(double);
     const size_t occ = (size_t)OCC_BLOCKS * (MAXN_OCC + 1) * sizeof(double);
     if (WS_HEADER + occ > P->ws_bytes) P->ws_bytes = WS_HEADER + occ;
-    if (m->basis_kind == RYD_BASIS_CODES) {
+    if (m->basis_kind == RYD_BASIS_CODES || m->basis_kind == RYD_BASIS_GRAPH) {
       *out = P;
       return RYD_OK;
     }
@@ -746,6 +940,11 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_codes) (void)hipFree(P->d_codes);
   if (P->d_binom) (void)hipFree(P->d_binom);
   if (P->d_pairs) (void)hipFree(P->d_pairs);
+  if (P->d_hpk) (void)hipFree(P->d_hpk);
+  if (P->d_lo_idx) (void)hipFree(P->d_lo_idx);
+  if (P->d_lowtab) (void)hipFree(P->d_lowtab);
+  if (P->d_fc) (void)hipFree(P->d_fc);
+  if (P->d_pre) (void)hipFree(P->d_pre);
   for (int k = 0; k < 2; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
@@ -755,6 +954,12 @@ int ryd_plan_destroy(ryd_plan* P) {
 }
 
 int64_t ryd_plan_dim(const ryd_plan* P) { return P ? P->dim : 0; }
+
+int ryd_plan_rank_mode(const ryd_plan* P) {
+  if (!P) return -1;
+  if (P->basis == RYD_BASIS_CHAIN_FIB) return 4;
+  return P->generic ? P->rank_mode : RANK_FULL;
+}
 
 size_t ryd_workspace_bytes(const ryd_plan* P) { return P ? P->ws_bytes : 0; }
 
@@ -920,7 +1125,7 @@ int ryd_cast(int din, const void* x, int dout, void* y, int64_t n, void* stream)
 namespace {
 template <class V>
 int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t st) {
-  if (P->basis == RYD_BASIS_CODES) {
+  if (P->basis == RYD_BASIS_CODES || P->basis == RYD_BASIS_GRAPH) {
     const int64_t g0 = (P->dim + 4095) / 4096;
     const unsigned g = (unsigned)(g0 < OCC_BLOCKS ? g0 : OCC_BLOCKS);
     double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
@@ -973,7 +1178,7 @@ extern "C" int ryd_sector_codes(const ryd_plan* P, const int64_t* ranks, int64_t
     return fail(RYD_ERR_UNSUPPORTED, "codes and ranks are defined for restricted bases");
   if (count == 0) return RYD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (P->basis == RYD_BASIS_CODES)
+  if (P->basis == RYD_BASIS_CODES || P->basis == RYD_BASIS_GRAPH)
     k_chan_codes<<<stream_grid(count), 256, 0, st>>>(chan_model(P), first, count, ranks, codes);
   else
     k_fib_unrank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), first, count, ranks, codes);
@@ -988,7 +1193,7 @@ extern "C" int ryd_sector_rank(const ryd_plan* P, const int64_t* codes, int64_t 
     return fail(RYD_ERR_UNSUPPORTED, "codes and ranks are defined for restricted bases");
   if (count == 0) return RYD_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (P->basis == RYD_BASIS_CODES)
+  if (P->basis == RYD_BASIS_CODES || P->basis == RYD_BASIS_GRAPH)
     k_chan_rank<<<stream_grid(count), 256, 0, st>>>(chan_model(P), codes, count, ranks);
   else
     k_fib_rank<<<stream_grid(count), 256, 0, st>>>(fib_model(P), codes, count, ranks);

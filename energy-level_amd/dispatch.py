@@ -53,15 +53,26 @@ def normalize_backend(name, want):
     return name
 
 
-def _projected_states(system):
-    """The states of a projected plan (any independent-set or sector basis),
-    or None for the full basis."""
+def _projected(system):
+    """(states, blockade_edges) of a projected plan, (None, None) for the full
+    basis.  ``states`` may be a host list/array or a device tensor
+    (``build_plan_hip``); the edges let the device enumerate and rank the
+    basis instead of searching the list."""
     plan = system.get('plan') or {}
     if not plan.get('projected'):
-        return None
+        return None, None
     states = plan.get('states')
+    edges = plan.get('blockade_edges')
+    if states is None and edges is None:
+        raise ValueError('a projected plan needs its states or blockade_edges')
+    return states, edges
+
+
+def _host_states(states):
     if states is None:
-        raise ValueError('a projected plan needs its states')
+        return None
+    if hasattr(states, 'cpu'):
+        return states.cpu().numpy().astype(np.int64)
     return np.asarray(states, dtype=np.int64)
 
 
@@ -85,18 +96,21 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
         raise ValueError('equilibrium needs a Hermitian Hamiltonian (gamma = 0)')
     device = system.get('device', 'cuda')
     base = dict(want=want, backend=backend, n_sites=n)
-    states = _projected_states(system)
-    projected = states is not None
+    states, edges = _projected(system)
+    projected = states is not None or edges is not None
     if want == 'gs':
         if backend == 'hip-lanczos':
             from .lanczos import hip_lanczos_ground_state
-            # a projected plan's states become the basis (the chain blockade
-            # sector takes the sector kernels, any other list the generic one)
+            # a projected plan's basis: the blockade graph enumerated on the
+            # device when its edges are given (and agree with the states),
+            # else the states as a code list
+            dev_states = hasattr(states, 'cpu')
             out = hip_lanczos_ground_state(
                 calib, system.get('Om'), system.get('delta'), gamma, system.get('terms'),
-                basis_indices=states, m=int(krylov_dim or system.get('krylov_dim', 48)),
-                seed=seed, dtype=dtype, device=device, return_state=return_state,
-                memory_budget_mb=memory_budget_mb)
+                basis_indices=None if (dev_states and edges is not None) else states,
+                blockade_edges=edges,
+                m=int(krylov_dim or system.get('krylov_dim', 48)), seed=seed, dtype=dtype,
+                device=device, return_state=return_state, memory_budget_mb=memory_budget_mb)
             result = dict(base, **out)
         else:
             if projected:
@@ -142,7 +156,10 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
     result.setdefault('requested_backend', requested)
     result.setdefault('resolved_backend', result.get('backend'))
     if result.get('vec') is not None:
-        indices = states
+        indices = _host_states(states)
+        if indices is None and edges is not None:
+            from .basis import independent_states_hip
+            indices = independent_states_hip(n, edges, device=device).cpu().numpy()
         result['basis'] = dict(bit_order='site0-msb', indices=indices, dim=len(result['vec']),
                                n_sites=n)
     return result

@@ -332,7 +332,8 @@ def _start_vector(dim, seed, complex_kind, device):
 def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
                              basis_indices=None, m=48, seed=0, dtype='float64', device='cuda',
                              tol=None, max_restarts=8, return_state=False,
-                             memory_budget_mb=None, refine=True, plan=None, basis='full'):
+                             memory_budget_mb=None, refine=True, plan=None, basis='full',
+                             blockade_edges=None):
     """Ground state and gap with residual certification (see module docstring).
 
     Keys as ``torch_lanczos_ground_state``: E0, gap, evals, residuals,
@@ -346,6 +347,12 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     sectors, blockade graphs, J/A models) the generic channel kernel with
     closed-form or binary-search ranks; ``vec`` comes back in the order of
     ``basis_indices`` (as matrix_free.py:118-286 with basis_indices).
+
+    ``blockade_edges``: the basis is the independent sets of this graph,
+    enumerated and ranked on the device (``RYD_BASIS_GRAPH``; chain edges
+    take the chain sector kernels).  Given together with ``basis_indices``,
+    the graph basis is used when it holds exactly those codes (a plan from
+    ``build_plan``), else the codes themselves.
     """
     import torch
     if str(device).startswith('cpu'):
@@ -358,8 +365,38 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     model = from_calib(calib, Om, delta, gamma, terms)
     order = None
     codes = None
+    graph = None
+    if blockade_edges is not None and plan is None:
+        from .basis import validate_edges, is_chain_edges
+        edges = validate_edges(model.n, blockade_edges)
+        if is_chain_edges(model.n, edges) and not model.has_pair_flips:
+            cand = 'chain_blockade'
+        else:
+            cand = Plan(model, basis='graph', edges=edges, device=device)
+        if basis_indices is None:
+            basis, graph = ('chain_blockade', None) if cand == 'chain_blockade' else ('graph', cand)
+        else:
+            import torch
+            idx = np.asarray(basis_indices.cpu() if isinstance(basis_indices, torch.Tensor)
+                             else basis_indices, np.int64).reshape(-1)
+            if cand == 'chain_blockade':
+                same = None     # the chain check below takes it
+            else:
+                same = idx.size == cand.dim
+                if same:
+                    from .basis import sector_codes
+                    srt = np.sort(idx)
+                    same = bool(np.array_equal(srt, sector_codes(cand).cpu().numpy()))
+            if same:
+                order = np.argsort(idx, kind='stable')
+                basis, graph = 'graph', cand
+                basis_indices = None
+            elif cand != 'chain_blockade':
+                cand.close()
     if basis_indices is not None:
         from .basis import is_chain_blockade_basis
+        if hasattr(basis_indices, 'cpu'):
+            basis_indices = basis_indices.cpu().numpy()
         idx = np.asarray(basis_indices, np.int64).reshape(-1)
         if idx.size == 0:
             raise ValueError('basis_indices is empty')
@@ -374,12 +411,20 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
             if np.any(np.diff(codes) == 0):
                 raise ValueError('basis_indices contains duplicate codes')
             basis = 'codes'
-    if basis not in ('full', 'chain_blockade', 'codes'):
-        raise ValueError('basis must be full or chain_blockade (or given by basis_indices)')
+    if basis not in ('full', 'chain_blockade', 'codes', 'graph'):
+        raise ValueError('basis must be full or chain_blockade (or given by basis_indices '
+                         'or blockade_edges)')
     if basis == 'codes' and codes is None:
         raise ValueError("basis='codes' needs basis_indices")
-    kind = {'full': 'full', 'chain_blockade': 'chain_fib', 'codes': 'codes'}[basis]
-    P = plan if plan is not None else Plan(model, basis=kind, codes=codes, device=device)
+    if basis == 'graph' and graph is None and plan is None:
+        raise ValueError("basis='graph' needs blockade_edges")
+    kind = {'full': 'full', 'chain_blockade': 'chain_fib', 'codes': 'codes', 'graph': 'graph'}[basis]
+    if plan is not None:
+        P = plan
+    elif graph is not None:
+        P = graph
+    else:
+        P = Plan(model, basis=kind, codes=codes, device=device)
     dim = P.dim
     low = dtype in ('float32', 'complex64')
     complex_kind = (not P.is_real) or dtype.startswith('complex')

@@ -19,7 +19,7 @@ __all__ = ['Plan']
 class Plan:
     """``ryd_plan`` owner.  Vectors are torch tensors on ``self.device``."""
 
-    def __init__(self, model: RydModel, basis='full', codes=None, device='cuda'):
+    def __init__(self, model: RydModel, basis='full', codes=None, device='cuda', edges=None):
         import torch
         nat.require_gpu()
         L = nat.lib()
@@ -32,7 +32,7 @@ class Plan:
         self.device = torch.device('cuda', torch.cuda.current_device() if dv.index is None
                                    else dv.index)
         kinds = {'full': nat.RYD_BASIS_FULL, 'codes': nat.RYD_BASIS_CODES,
-                 'chain_fib': nat.RYD_BASIS_CHAIN_FIB}
+                 'chain_fib': nat.RYD_BASIS_CHAIN_FIB, 'graph': nat.RYD_BASIS_GRAPH}
         if basis not in kinds:
             raise ValueError('unknown basis %r' % (basis,))
         self.basis = basis
@@ -72,6 +72,11 @@ class Plan:
         else:
             m.codes = None
             m.n_codes = 0
+        if basis == 'graph':
+            e = np.asarray(edges if edges is not None else [], np.int64).reshape(-1, 2)
+            m.n_edges = len(e)
+            m.edge_i = arr(e[:, 0], np.int32).ctypes.data_as(nat._ip)
+            m.edge_j = arr(e[:, 1], np.int32).ctypes.data_as(nat._ip)
         handle = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             nat.check(L.ryd_plan_create(ctypes.byref(m), ctypes.byref(handle)))
@@ -97,6 +102,12 @@ class Plan:
     @property
     def handle(self):
         return self._h
+
+    @property
+    def rank_mode(self):
+        """'full', 'number', 'search', 'graph' or 'chain' (how codes are ranked)."""
+        m = int(nat.lib().ryd_plan_rank_mode(self._h))
+        return {0: 'full', 1: 'number', 2: 'search', 3: 'graph', 4: 'chain'}.get(m, 'unknown')
 
     def workspace(self):
         """Zero-filled workspace of ``ryd_workspace_bytes`` (reused by this plan)."""

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 2
+#define RYD_ABI_VERSION 3
 
 enum ryd_status {
   RYD_OK = 0,
@@ -45,13 +45,17 @@ enum ryd_dtype { RYD_F32 = 0, RYD_F64 = 1, RYD_C64 = 2, RYD_C128 = 3 };
 enum ryd_basis_kind {
   RYD_BASIS_FULL = 0,       /* all 2^n codes                                */
   RYD_BASIS_CODES = 1,      /* explicit ascending int64 codes (number
-                               sectors, general blockade graphs)           */
-  RYD_BASIS_CHAIN_FIB = 2   /* chain nearest-neighbour blockade sector      */
+                               sectors, any subset of codes)               */
+  RYD_BASIS_CHAIN_FIB = 2,  /* chain nearest-neighbour blockade sector      */
+  RYD_BASIS_GRAPH = 3       /* independent sets of a blockade graph (edges
+                               below), enumerated on the device in
+                               ascending order as independent_states,
+                               basis.py:24-47; n <= 60, no 2^20 cap        */
 };
 
 /* Model description (host memory, copied at plan creation). */
 typedef struct ryd_model {
-  int32_t n;               /* number of sites, 1..62                        */
+  int32_t n;               /* number of sites, 1..62 (full basis 1..40)     */
   const double* om_re;     /* [n] Re Ω_i (MHz)                              */
   const double* om_im;     /* [n] Im Ω_i (MHz); NULL = real drive           */
   const double* delta;     /* [n] diagonal coefficient of n_i (MHz)         */
@@ -70,6 +74,11 @@ typedef struct ryd_model {
   const double* pair_ji;   /* [n_pairs] Im J_ij                              */
   const double* pair_ar;   /* [n_pairs] Re A_ij                              */
   const double* pair_ai;   /* [n_pairs] Im A_ij                              */
+  /* RYD_BASIS_GRAPH: blockade edges (site pairs that may not both be
+   * excited), as blockade_edges of build_plan, hamiltonian_plan.py:36-72 */
+  int32_t n_edges;
+  const int32_t* edge_i;   /* [n_edges] */
+  const int32_t* edge_j;   /* [n_edges] */
 } ryd_model;
 
 typedef struct ryd_plan ryd_plan;
@@ -90,6 +99,10 @@ int ryd_plan_destroy(ryd_plan* plan);
 
 /* Basis dimension of a plan (0 on a NULL plan). */
 int64_t ryd_plan_dim(const ryd_plan* plan);
+/* How the plan ranks codes: 0 identity (full basis), 1 combinatorial number
+ * system, 2 binary search over the code list, 3 blockade-graph tables,
+ * 4 chain-sector Zeckendorf rank.  No reference counterpart (diagnostic). */
+int ryd_plan_rank_mode(const ryd_plan* plan);
 
 /* Bytes of workspace any single reducing call on this plan needs (per-block
  * partial sums; no initialisation required). */
@@ -162,16 +175,17 @@ int ryd_cheb_moments(const ryd_plan* plan, int dtype, const uint32_t* probe, voi
 int ryd_site_occupation(const ryd_plan* plan, int dtype, const void* x, double* out, void* ws,
                         void* stream);
 
-/* Chain-blockade sector enumeration: codes[k] = code of rank ranks[k] (or of
- * rank first + k when ranks is NULL); ranks outside [0, dim) give -1.  Codes
- * come out in the reference's basis order, ascending (independent_states,
- * src/core/manybody/basis.py:24-47 with edges (i, i+1)); rank(code) =
- * Σ_{site i excited} F(n-i+1).  Device int64 arrays. */
+/* Restricted-basis enumeration (chain sector, codes and blockade-graph
+ * plans): codes[k] = code of rank ranks[k] (or of rank first + k when ranks
+ * is NULL); ranks outside [0, dim) give -1.  Codes come out in the
+ * reference's basis order, ascending (independent_states,
+ * src/core/manybody/basis.py:24-47; chain: edges (i, i+1), rank(code) =
+ * Σ_{site i excited} F(n-i+1)).  Device int64 arrays. */
 int ryd_sector_codes(const ryd_plan* plan, const int64_t* ranks, int64_t first, int64_t count,
                      int64_t* codes, void* stream);
 
-/* ranks[k] = rank of codes[k] in the chain-blockade sector, -1 if the code is
- * not in it (replaces the position table / searchsorted lookup of
+/* ranks[k] = rank of codes[k] in the plan's restricted basis, -1 if the code
+ * is not in it (replaces the position table / searchsorted lookup of
  * ChannelKernel._indices, channels.py:177-192).  Device int64 arrays. */
 int ryd_sector_rank(const ryd_plan* plan, const int64_t* codes, int64_t count, int64_t* ranks,
                     void* stream);

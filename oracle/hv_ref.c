@@ -195,3 +195,75 @@ void ref_hv_full_f64_rows(int n, const double* om_re, const double* diag_rows,
         y[t - t0] = acc;
     }
 }
+
+/* ---- blockade graphs (reference basis.py:24-47) ----------------------------
+ * Independent sets of a graph given by per-site neighbour masks (nb[i] has
+ * bit j set when sites i and j are joined), in the reference's depth-first
+ * order: site 0 first, "empty" before "excited", so codes come out
+ * ascending.  The recursion is unrolled into an explicit stack.
+ * Returns the number of sets; writes at most cap codes (codes may be NULL). */
+int64_t ref_indep_codes(int n, const uint64_t* nb, int64_t* codes, int64_t cap) {
+    int64_t count = 0;
+    if (n <= 0) {
+        if (codes && cap > 0) codes[0] = 0;
+        return 1;
+    }
+    /* choice[s] = 0: site s empty taken, 1: excited taken */
+    int choice[64];
+    uint64_t occ[65];   /* occupied sites (by site index) before site s */
+    int64_t state[65];
+    int s = 0;
+    occ[0] = 0;
+    state[0] = 0;
+    choice[0] = -1;
+    while (s >= 0) {
+        if (s == n) {
+            if (codes && count < cap) codes[count] = state[n];
+            ++count;
+            --s;
+            continue;
+        }
+        if (choice[s] == -1) {             /* empty first */
+            choice[s] = 0;
+            occ[s + 1] = occ[s];
+            state[s + 1] = state[s];
+            if (s + 1 < n) choice[s + 1] = -1;
+            ++s;
+            continue;
+        }
+        if (choice[s] == 0 && !(nb[s] & occ[s])) {   /* then excited, if allowed */
+            choice[s] = 1;
+            occ[s + 1] = occ[s] | ((uint64_t)1 << s);
+            state[s + 1] = state[s] | ((int64_t)1 << (n - 1 - s));
+            if (s + 1 < n) choice[s + 1] = -1;
+            ++s;
+            continue;
+        }
+        --s;                                /* both branches done */
+    }
+    return count;
+}
+
+/* y = H x on a sorted code list (FP64 real Ω): every flip of site i whose
+ * target code is in the list (binary search) contributes Ω_i/2 x[target];
+ * sources outside the list are dropped (the projected operator, as the
+ * reference's restricted-basis ChannelKernel, channels.py:177-221). */
+void ref_hv_codes_f64(int n, const double* om_re, const double* diag,
+                      const int64_t* codes, int64_t dim, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < dim; ++r) {
+        const int64_t c = codes[r];
+        double acc = diag[r] * x[r];
+        for (int i = 0; i < n; ++i) {
+            if (om_re[i] == 0.0) continue;
+            const int64_t s = c ^ ((int64_t)1 << (n - 1 - i));
+            int64_t lo = 0, hi = dim - 1;
+            while (lo <= hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (codes[mid] == s) { acc += 0.5 * om_re[i] * x[mid]; break; }
+                if (codes[mid] < s) lo = mid + 1; else hi = mid - 1;
+            }
+        }
+        y[r] = acc;
+    }
+}

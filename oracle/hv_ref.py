@@ -40,6 +40,10 @@ def lib():
         L.ref_fib_unrank.restype = ctypes.c_int64
         L.ref_hv_full_f64_rows.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int64,
                                            ctypes.c_int64]
+        L.ref_indep_codes.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), _lp,
+                                      ctypes.c_int64]
+        L.ref_indep_codes.restype = ctypes.c_int64
+        L.ref_hv_codes_f64.argtypes = [ctypes.c_int, _dp, _dp, _lp, ctypes.c_int64, _dp, _dp]
         L.ref_num_threads.restype = ctypes.c_int
         L.ref_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
@@ -128,4 +132,42 @@ def hv_full_rows(n, Om, d_rows, x, t0, t1):
     y = np.empty(int(t1 - t0), np.float64)
     lib().ref_hv_full_f64_rows(int(n), _p(ore, _dp), _p(d_rows, _dp), _p(x, _dp), _p(y, _dp),
                                int(t0), int(t1))
+    return y
+
+
+def _neighbour_masks(n, edges):
+    nb = np.zeros(max(int(n), 1), np.uint64)
+    for i, j in edges:
+        nb[int(i)] |= np.uint64(1) << np.uint64(int(j))
+        nb[int(j)] |= np.uint64(1) << np.uint64(int(i))
+    return nb
+
+
+def indep_count(n, edges):
+    """Number of independent sets (depth-first count, no list)."""
+    nb = _neighbour_masks(n, edges)
+    return int(lib().ref_indep_codes(int(n), nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                     None, 0))
+
+
+def indep_codes(n, edges):
+    """Ascending codes of the independent sets (reference basis.py:24-47 order)."""
+    cnt = indep_count(n, edges)
+    nb = _neighbour_masks(n, edges)
+    out = np.empty(cnt, np.int64)
+    got = lib().ref_indep_codes(int(n), nb.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                _p(out, _lp), cnt)
+    assert got == cnt
+    return out
+
+
+def hv_codes(n, Om, d, codes, x):
+    """Projected FP64 H·v on an ascending code list (real Ω)."""
+    ore = np.ascontiguousarray(np.asarray(Om, complex).real, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    codes = np.ascontiguousarray(codes, np.int64)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    lib().ref_hv_codes_f64(int(n), _p(ore, _dp), _p(d, _dp), _p(codes, _lp), len(codes),
+                           _p(x, _dp), _p(y, _dp))
     return y

@@ -42,3 +42,94 @@ def test_from_calib_pair_channels(pkg):
     m1 = pkg.from_calib({'atoms': [{}] * 2, 'pairs': [{'i': 0, 'j': 1, 'V': 2.0, 'J': 1.0}],
                          'terms': ['I1']})
     assert not m1.has_pair_flips and m1.pair_v[0] == 2.0
+
+
+# ---- blockade graphs -------------------------------------------------------
+def _grid_edges(w, h, diag=False):
+    """Row-major w x h lattice: nearest neighbours (and diagonals)."""
+    idx = lambda r, c: r * w + c
+    out = []
+    for r in range(h):
+        for c in range(w):
+            if c + 1 < w:
+                out.append((idx(r, c), idx(r, c + 1)))
+            if r + 1 < h:
+                out.append((idx(r, c), idx(r + 1, c)))
+                if diag and c + 1 < w:
+                    out.append((idx(r, c), idx(r + 1, c + 1)))
+                if diag and c > 0:
+                    out.append((idx(r, c), idx(r + 1, c - 1)))
+    return out
+
+
+def _transfer_count(w, h, diag=False):
+    """Independent sets of a w x h lattice by the row transfer matrix (an
+    independent method: rows are independent sets of a path, consecutive
+    rows may not share a column, nor a diagonal when diag)."""
+    rows = [r for r in range(1 << w) if not (r & (r >> 1))]
+    cnt = {r: 1 for r in rows}
+    for _ in range(h - 1):
+        new = dict.fromkeys(rows, 0)
+        for r in rows:
+            for q, c in cnt.items():
+                if r & q or (diag and (r & (q << 1) or r & (q >> 1))):
+                    continue
+                new[r] += c
+        cnt = new
+    return sum(cnt.values())
+
+
+def test_c_oracle_independent_sets_vs_brute_force():
+    from oracle import hv_ref, rydberg_oracle as orc
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 3, 6, 9, 12):
+        for _ in range(4):
+            edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.3]
+            a = hv_ref.indep_codes(n, edges)
+            assert np.array_equal(a, np.asarray(orc.independent_states_bruteforce(n, edges)))
+            assert np.array_equal(a, np.asarray(orc.independent_states(n, edges)))
+
+
+@pytest.mark.parametrize('w,h,diag', [(4, 4, False), (5, 5, False), (5, 5, True), (6, 6, True)])
+def test_c_oracle_lattice_counts_vs_transfer_matrix(w, h, diag):
+    from oracle import hv_ref
+    assert hv_ref.indep_count(w * h, _grid_edges(w, h, diag)) == _transfer_count(w, h, diag)
+
+
+def test_c_oracle_chain_count_is_fibonacci(pkg):
+    from oracle import hv_ref
+    for n in (1, 2, 10, 24):
+        assert hv_ref.indep_count(n, pkg.chain_edges(n)) == pkg.fibonacci(n + 2)
+
+
+def test_c_oracle_projected_hv_vs_channel_oracle():
+    from oracle import hv_ref, rydberg_oracle as orc
+    rng = np.random.default_rng(3)
+    n = 10
+    edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.25]
+    codes = hv_ref.indep_codes(n, edges)
+    pairs = [{'i': i, 'j': j, 'V': float(rng.uniform(0, 3))} for i in range(n) for j in range(i + 1, n)]
+    Om = rng.uniform(0.2, 1.5, n)
+    delta = rng.uniform(-2, 2, n)
+    x = rng.normal(size=len(codes))
+    ref = orc.channel_matvec(n, Om, delta, pairs, ['I1'], x, codes=codes)
+    d = hv_ref.diag(n, delta, pairs, codes)
+    y = hv_ref.hv_codes(n, Om, d, codes, x)
+    assert np.max(np.abs(y - ref.real)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_validate_edges_and_plan_edges_mirror_reference(pkg):
+    assert pkg.validate_edges(4, [(2, 1), (1, 2), [0, 3]]) == [(0, 3), (1, 2)]
+    for bad in ([(0, 0)], [(0, 4)], [(0, 1.5)], [(True, 1)], [(0, 1, 2)]):
+        with pytest.raises(ValueError):
+            pkg.validate_edges(4, bad)
+    calib = {'atoms': [{}] * 4, 'pairs': [{'i': 0, 'j': 1, 'V': 50.0}, {'i': 1, 'j': 3, 'V': -60.0},
+                                          {'i': 0, 'j': 2, 'V': 5.0}]}
+    assert pkg.blockade_edges(calib, {'kind': 'blockade', 'threshold_mhz': 10}) == [(0, 1), (1, 3)]
+    assert pkg.blockade_edges(calib, {'kind': 'pxp', 'edges': [(2, 3)]}) == [(2, 3)]
+    with pytest.raises(ValueError):
+        pkg.blockade_edges(calib, {'kind': 'blockade'})             # no threshold, no edges
+    with pytest.raises(ValueError):
+        pkg.blockade_edges(calib, {'kind': 'blockade', 'edges': 'all'})
+    with pytest.raises(ValueError):
+        pkg.build_plan_hip(calib, constraint={'kind': 'ring'})
