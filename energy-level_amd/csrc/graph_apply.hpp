@@ -1,0 +1,205 @@
+// H·v on an independent-set basis of a blockade graph (graph.hpp tables),
+// one block per valid high part h.
+//
+// The rows of h form one contiguous segment [off, off + cnt); row off + j
+// holds the low set lowtab[post_c[j]] (c = class of h), so x and y move in
+// coalesced runs.  A flip of low bit q keeps h: its source is
+// off + pre_c[lonb[q][t]] (lonb = index of the flipped low set, GR_NONE when
+// it is not independent; an excitation blocked by h is dropped by F_c).  A
+// flip of high bit p moves to the segment of h ^ 2^p with the same low set:
+// off2 + pre_c2[t] — block-uniform offsets held in LDS.  The diagonal is
+// rebuilt per row: dhi(h) (block-uniform) + dlo[t] + Σ_{q in l} w_q(h), with
+// w_q(h) = Σ_{p in h} V_{p,q} the high-low cross terms in LDS.
+#pragma once
+#include "common.hpp"
+#include "graph.hpp"
+#include "hv_full.hpp"
+
+namespace ryd {
+
+struct GraphModel {
+  int n, L, H;
+  int64_t tl;                   // low sets
+  int64_t nv;                   // valid high parts
+  const uint32_t* hlist;        // [nv] valid high parts, ascending
+  const uint64_t* hpk;          // [2^H + 1]
+  const uint32_t* lowtab;       // [tl]
+  const uint32_t* lonb;         // [L][tl]
+  const double* dlo;            // [tl]
+  const uint32_t* fc;           // [nc]
+  const uint32_t* pre;          // [nc][tl + 1]
+  const uint32_t* post;         // rows of every class, concatenated
+  const int64_t* post_off;      // [nc] start of class c in post
+  const double* vb;             // [n*n] V by bit position
+  double dl[MAXN];              // delta by bit position
+  double ar[MAXN], ai[MAXN];    // Ω/2 by bit position
+};
+
+template <class V>
+__global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  __shared__ double wq[GR_MAXB];
+  __shared__ double dhi_s;
+  __shared__ int64_t off2[GR_MAXB];     // first rank of h ^ 2^p, -1 when not independent
+  __shared__ int64_t prow2[GR_MAXB];    // its class row in pre
+  __shared__ uint32_t f2[GR_MAXB];      // the low bits its class blocks
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s, chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  const int tid = threadIdx.x;
+  const int L = M.L, H = M.H;
+  const int64_t tl = M.tl;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  // contiguous runs of high parts per XCD (blocks go round-robin to the 8 XCDs)
+  int64_t v0, v1, vstep;
+  if (gridDim.x >= 8 && (gridDim.x & 7) == 0) {
+    const int64_t chunk = (M.nv + 7) / 8;
+    v0 = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+    v1 = ((int64_t)(blockIdx.x & 7) + 1) * chunk;
+    if (v1 > M.nv) v1 = M.nv;
+    vstep = gridDim.x >> 3;
+  } else {
+    v0 = blockIdx.x;
+    v1 = M.nv;
+    vstep = gridDim.x;
+  }
+  for (int64_t vi = v0; vi < v1; vi += vstep) {
+    const uint32_t h = M.hlist[vi];
+    const uint64_t e = M.hpk[h];
+    const uint64_t c = e & GR_CLS;
+    const int64_t off = (int64_t)(e >> 24);
+    const int64_t cnt = (int64_t)(M.hpk[h + 1] >> 24) - off;
+    __syncthreads();                            // LDS of the previous high part is free
+    if (tid < L) {
+      double w = 0.0;
+      for (uint32_t y = h; y; y &= y - 1) w += M.vb[(size_t)(__builtin_ctz(y) + L) * M.n + tid];
+      wq[tid] = w;
+    } else if (tid >= 64 && tid < 64 + H) {
+      const int p = tid - 64;
+      const uint64_t e2 = M.hpk[h ^ (1u << p)];
+      const uint64_t c2 = e2 & GR_CLS;
+      if (c2 == GR_CLS) {
+        off2[p] = -1;
+      } else {
+        off2[p] = (int64_t)(e2 >> 24);
+        prow2[p] = (int64_t)c2 * (tl + 1);
+        f2[p] = M.fc[c2];
+      }
+    } else if (tid == 128) {
+      double d = 0.0;
+      for (uint32_t y = h; y; y &= y - 1) {
+        const int p = __builtin_ctz(y) + L;
+        d += M.dl[p];
+        for (uint32_t z = y & (y - 1); z; z &= z - 1) d += M.vb[(size_t)p * M.n + __builtin_ctz(z) + L];
+      }
+      dhi_s = d;
+    }
+    __syncthreads();
+    const uint32_t F = M.fc[c];
+    const uint32_t* prec = M.pre + (size_t)c * (size_t)(tl + 1);
+    const uint32_t* postc = M.post + M.post_off[c];
+    const double dh = dhi_s;
+    for (int64_t j = tid; j < cnt; j += 256) {
+      const int64_t row = off + j;
+      const uint32_t t = postc[j];
+      const uint32_t l = M.lowtab[t];
+      double dg = dh + M.dlo[t];
+      for (uint32_t y = l; y; y &= y - 1) dg += wq[__builtin_ctz(y)];
+      const V xe = vscale(io.x[row], sr);
+      V acc = vscale(xe, (RT)dg);
+      for (int q = 0; q < L; ++q) {
+        const double are = M.ar[q], aim = M.ai[q];
+        if (are == 0.0 && aim == 0.0) continue;
+        const uint32_t t2 = M.lonb[(size_t)q * (size_t)tl + t];
+        const int bit = (int)((l >> q) & 1u);
+        if (t2 == GR_NONE || (!bit && ((F >> q) & 1u))) continue;
+        vcfma(acc, (RT)(are * s), (RT)((bit ? aim : -aim) * s), io.x[off + prec[t2]]);
+      }
+      for (int p = 0; p < H; ++p) {
+        const double are = M.ar[p + L], aim = M.ai[p + L];
+        if (are == 0.0 && aim == 0.0) continue;
+        const int64_t o2 = off2[p];
+        const int bit = (int)((h >> p) & 1u);
+        if (o2 < 0 || (!bit && (l & f2[p]))) continue;
+        vcfma(acc, (RT)(are * s), (RT)((bit ? aim : -aim) * s), io.x[o2 + M.pre[prow2[p] + t]]);
+      }
+      V v = vscale(acc, chr);
+      vfma(v, cxr, xe);
+      if (io.z) vfma(v, czr, io.z[row]);
+      io.y[row] = v;
+      if (io.reduce) {
+        s0 += rdot(xe, v);
+        s1 += nrm2(v);
+        s2 += nrm2(xe);
+        if (io.probe) s3 += zdot(((io.probe[row >> 5] >> (row & 31)) & 1u) ? 1.0 : -1.0, v);
+      }
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// ---- tables for k_graph_apply -------------------------------------------------
+// lonb[q][t] = index of lowtab[t] ^ 2^q among the low sets, or GR_NONE
+__global__ __launch_bounds__(256) void k_gr_lonb(int L, const uint32_t* __restrict__ lowtab, int64_t tl,
+                                                 const uint32_t* __restrict__ lo_idx,
+                                                 uint32_t* __restrict__ lonb) {
+  const int64_t total = (int64_t)L * tl;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+    const int q = (int)(k / tl);
+    const int64_t t = k - (int64_t)q * tl;
+    lonb[k] = lo_idx[lowtab[t] ^ (1u << q)];
+  }
+}
+
+// dlo[t] = diagonal of the low bits of lowtab[t] alone
+__global__ __launch_bounds__(256) void k_gr_dlo(int n, const uint32_t* __restrict__ lowtab, int64_t tl,
+                                                const double* __restrict__ dl,
+                                                const double* __restrict__ vb, double* __restrict__ dlo) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tl; t += stride) {
+    const uint32_t l = lowtab[t];
+    double d = 0.0;
+    for (uint32_t y = l; y; y &= y - 1) {
+      const int q = __builtin_ctz(y);
+      d += dl[q];
+      for (uint32_t z = y & (y - 1); z; z &= z - 1) d += vb[(size_t)q * n + __builtin_ctz(z)];
+    }
+    dlo[t] = d;
+  }
+}
+
+// post[post_off[c] + j] = t of the j-th low set compatible with class c
+__global__ __launch_bounds__(256) void k_gr_post(const uint32_t* __restrict__ lowtab, int64_t tl,
+                                                 const uint32_t* __restrict__ fc, int64_t nc,
+                                                 const int64_t* __restrict__ post_off,
+                                                 uint32_t* __restrict__ post) {
+  __shared__ uint32_t lds[4];
+  for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    const uint32_t F = fc[c];
+    uint32_t* out = post + post_off[c];
+    uint32_t run = 0;
+    for (int64_t t0 = 0; t0 < tl; t0 += 256) {
+      const int64_t t = t0 + threadIdx.x;
+      const bool ok = t < tl && !(lowtab[t] & F);
+      uint32_t tot;
+      const uint32_t pos = gr_flag_scan(ok, lds, tot);
+      if (ok) out[run + pos] = (uint32_t)t;
+      run += tot;
+    }
+  }
+}
+
+struct GrValidFlag {           // h is a valid high part
+  const uint64_t* hpk;
+  __device__ int64_t operator()(int64_t h) const { return (hpk[h] & GR_CLS) != GR_CLS ? 1 : 0; }
+};
+struct GrListWrite {           // list[pos] = i for set flags
+  uint32_t* list;
+  __device__ void operator()(int64_t i, int64_t pre, int64_t v) const {
+    if (v) list[pre] = (uint32_t)i;
+  }
+};
+
+}  // namespace ryd

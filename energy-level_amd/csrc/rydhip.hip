@@ -14,6 +14,7 @@
 #include "hv_full.hpp"
 #include "stream_ops.hpp"
 #include "channels.hpp"
+#include "graph_apply.hpp"
 #include "observables.hpp"
 #include "sector.hpp"
 #include "evolve.hpp"
@@ -84,6 +85,13 @@ struct ryd_plan {
   uint32_t* d_lowtab = nullptr;      // [tl]
   uint32_t* d_fc = nullptr;          // [nc]
   uint32_t* d_pre = nullptr;         // [nc][tl + 1] (RANK_GRAPH only)
+  // k_graph_apply tables (RANK_GRAPH only)
+  int64_t gr_nv = 0;                 // valid high parts
+  uint32_t* d_hlist = nullptr;       // [nv]
+  uint32_t* d_lonb = nullptr;        // [L][tl]
+  double* d_dlo = nullptr;           // [tl]
+  uint32_t* d_post = nullptr;        // [rows of all classes]
+  int64_t* d_post_off = nullptr;     // [nc]
   int fib_L = 0;
   int64_t fib_cnt[2] = {0, 0};
 };
@@ -210,6 +218,43 @@ ChanModel chan_model(const ryd_plan* P) {
   return M;
 }
 
+GraphModel graph_model(const ryd_plan* P) {
+  GraphModel M;
+  M.n = P->n;
+  M.L = P->gr_L;
+  M.H = P->n - P->gr_L;
+  M.tl = P->gr_tl;
+  M.nv = P->gr_nv;
+  M.hlist = P->d_hlist;
+  M.hpk = P->d_hpk;
+  M.lowtab = P->d_lowtab;
+  M.lonb = P->d_lonb;
+  M.dlo = P->d_dlo;
+  M.fc = P->d_fc;
+  M.pre = P->d_pre;
+  M.post = P->d_post;
+  M.post_off = P->d_post_off;
+  M.vb = P->d_vb;
+  for (int p = 0; p < MAXN; ++p) {
+    M.dl[p] = P->dl[p];
+    M.ar[p] = P->ar[p];
+    M.ai[p] = P->ai[p];
+  }
+  return M;
+}
+
+// k_graph_apply: whole XCD groups of blocks, at most CHAN_MAX_BLOCKS
+unsigned graph_grid(const ryd_plan* P) {
+  const int64_t nv = P->gr_nv;
+  if (nv < 64) return (unsigned)(nv < 1 ? 1 : nv);
+  const int64_t g = 8 * ((nv + 7) / 8);
+  return (unsigned)(g < CHAN_MAX_BLOCKS ? g : CHAN_MAX_BLOCKS);
+}
+
+bool graph_kernel(const ryd_plan* P) {
+  return P->basis == RYD_BASIS_GRAPH && P->rank_mode == RANK_GRAPH && P->npairs == 0 && P->d_hlist;
+}
+
 unsigned chan_grid(const ryd_plan* P) {
   const int64_t g = (P->dim + 255) / 256;
   return (unsigned)(g < CHAN_MAX_BLOCKS ? (g < 1 ? 1 : g) : CHAN_MAX_BLOCKS);
@@ -227,6 +272,16 @@ unsigned fib_top_grid(const ryd_plan* P, int run) {
 
 template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
+  if (graph_kernel(P)) {
+    io.nfar = 0;
+    io.partial = 0;
+    const unsigned grid = graph_grid(P);
+    k_graph_apply<V><<<dim3(grid), dim3(256), 0, st>>>(graph_model(P), io, rd);
+    LAUNCH_CHECK();
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   if (P->generic) {
     io.nfar = 0;
     io.partial = 0;
@@ -570,7 +625,9 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
   uint32_t* cls_idx = nullptr;
   uint32_t* clslist = nullptr;
   int64_t* ccnt = nullptr;
+  double* dl_dev = nullptr;
   auto release = [&]() {
+    if (dl_dev) (void)hipFree(dl_dev);
     if (bsum) (void)hipFree(bsum);
     if (cls_idx) (void)hipFree(cls_idx);
     if (clslist) (void)hipFree(clslist);
@@ -614,15 +671,51 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
     P->dim = dim;
     P->gr_tl = tl;
     P->gr_nc = nc;
-    // constant-time ranks when the prefix table is affordable
+    // constant-time ranks (and the segment kernel) when the prefix and row
+    // tables are affordable
+    std::vector<int64_t> cc((size_t)nc), post_off((size_t)nc);
+    HIP_TRY(hipMemcpy(cc.data(), ccnt, sizeof(int64_t) * (size_t)nc, hipMemcpyDeviceToHost));
+    int64_t rows_all = 0;
+    for (int64_t c = 0; c < nc; ++c) {
+      post_off[(size_t)c] = rows_all;
+      rows_all += cc[(size_t)c];
+    }
     const size_t pre_bytes = sizeof(uint32_t) * (size_t)nc * (size_t)(tl + 1);
-    if (pre_bytes <= gr_prefix_budget(dim)) {
+    const size_t post_bytes = sizeof(uint32_t) * (size_t)rows_all;
+    if (pre_bytes + post_bytes <= gr_prefix_budget(dim)) {
       HIP_TRY(hipMalloc(&P->d_pre, pre_bytes));
       k_gr_prefix<<<dim3(cg), dim3(256)>>>(P->d_lowtab, tl, P->d_fc, nc, P->d_pre);
       LAUNCH_CHECK();
       P->rank_mode = RANK_GRAPH;
     } else {
       P->rank_mode = RANK_SEARCH;
+    }
+    if (P->rank_mode == RANK_GRAPH && P->npairs == 0) {
+      // tables of the segment kernel (Omega/delta/V models)
+      HIP_TRY(hipMalloc(&P->d_post_off, sizeof(int64_t) * (size_t)nc));
+      HIP_TRY(hipMemcpy(P->d_post_off, post_off.data(), sizeof(int64_t) * (size_t)nc,
+                        hipMemcpyHostToDevice));
+      HIP_TRY(hipMalloc(&P->d_post, post_bytes > 0 ? post_bytes : sizeof(uint32_t)));
+      k_gr_post<<<dim3(cg), dim3(256)>>>(P->d_lowtab, tl, P->d_fc, nc, P->d_post_off, P->d_post);
+      LAUNCH_CHECK();
+      HIP_TRY(hipMalloc(&P->d_lonb, sizeof(uint32_t) * (size_t)L * (size_t)tl + sizeof(uint32_t)));
+      if (L > 0) {
+        const int64_t tot = (int64_t)L * tl;
+        k_gr_lonb<<<dim3((unsigned)((tot + 255) / 256 < 65536 ? (tot + 255) / 256 : 65536)),
+                    dim3(256)>>>(L, P->d_lowtab, tl, P->d_lo_idx, P->d_lonb);
+        LAUNCH_CHECK();
+      }
+      HIP_TRY(hipMalloc(&dl_dev, sizeof(double) * (size_t)n));
+      HIP_TRY(hipMemcpy(dl_dev, P->dl, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+      HIP_TRY(hipMalloc(&P->d_dlo, sizeof(double) * (size_t)tl));
+      k_gr_dlo<<<dim3((unsigned)((tl + 255) / 256 < 65536 ? (tl + 255) / 256 : 65536)), dim3(256)>>>(
+          n, P->d_lowtab, tl, dl_dev, P->d_vb, P->d_dlo);
+      LAUNCH_CHECK();
+      int64_t nv = 0;
+      if (int rc = gr_total(GrValidFlag{P->d_hpk}, nhi, bsum, nv)) return rc;
+      HIP_TRY(hipMalloc(&P->d_hlist, sizeof(uint32_t) * (size_t)nv));
+      if (int rc = gr_write(GrValidFlag{P->d_hpk}, GrListWrite{P->d_hlist}, nhi, bsum)) return rc;
+      P->gr_nv = nv;
     }
     HIP_TRY(hipMalloc(&P->d_codes, sizeof(int64_t) * (size_t)dim));
     const unsigned eg = (unsigned)(nhi < 65536 ? nhi : 65536);
@@ -945,6 +1038,11 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_lowtab) (void)hipFree(P->d_lowtab);
   if (P->d_fc) (void)hipFree(P->d_fc);
   if (P->d_pre) (void)hipFree(P->d_pre);
+  if (P->d_hlist) (void)hipFree(P->d_hlist);
+  if (P->d_lonb) (void)hipFree(P->d_lonb);
+  if (P->d_dlo) (void)hipFree(P->d_dlo);
+  if (P->d_post) (void)hipFree(P->d_post);
+  if (P->d_post_off) (void)hipFree(P->d_post_off);
   for (int k = 0; k < 2; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
