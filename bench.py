@@ -6,8 +6,10 @@ pairs V = 10/|i−j|^6, Ω = 1, Δ = 1.2 (reference sign: delta = −1.2), full
 H·v of the whole 2^28 vector, inputs resident in HBM.
 
 Prints one JSON line (value = H·v/s in FP64; the FP32 form is reported in
-``fp32``; the five configurations follow in ``configs``, with the batched
-Lanczos solves/s and the adjoint grad/s of the metric at the top level).  ``roofline`` is for the dominant kernel (k_full_tiled) with
+``fp32``; the five configurations follow in ``configs`` with ``g1``, the
+blockade-graph basis of a 6x7 lattice enumerated on the device, and the
+batched Lanczos solves/s and the adjoint grad/s of the metric at the top
+level).  ``roofline`` is for the dominant kernel (k_full_tiled) with
 algorithmic bytes 2·D·sizeof(element) (x read once, y written once) over
 its mean duration measured with HIP events on the launch stream.
 ``cpu_baseline`` times the oracle's OpenMP matrix-free H·v on the host
@@ -120,9 +122,12 @@ def main():
                     help='time one element type only (profiling runs)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run only a plain device copy of one FP64 N-site vector (counter scale)')
-    ap.add_argument('--configs', default='c1,c2,c3,c4,c5',
-                    help='comma list of the five configurations to run after the headline '
-                         '("" for none)')
+    ap.add_argument('--configs', default='c1,c2,c3,c4,c5,g1',
+                    help='comma list of the configurations to run after the headline: the '
+                         'five workloads and g1 (blockade-graph lattice) ("" for none)')
+    ap.add_argument('--graph-lattice', default=None, metavar='WxH[d]',
+                    help='time only the blockade-graph H·v on a w x h square lattice '
+                         '("6x7"; suffix d adds diagonal edges) and print its summary')
     ap.add_argument('--no-headline', action='store_true',
                     help='skip the N=28 H·v timing and print only the configurations '
                          '(per-workload profiling runs)')
@@ -145,6 +150,14 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_calibration_bytes_read': 8 << n, 'bytes_written': 8 << n,
                           'launches': args.steps}))
+        return
+    if args.graph_lattice:
+        from energy_level_amd import workloads as W
+        spec = args.graph_lattice.lower()
+        diag = spec.endswith('d')
+        w, h = (int(v) for v in spec.rstrip('d').split('x'))
+        r = W.run_g1(dev, w=w, h=h, diag=diag, steps=args.steps, ground_state=False)
+        print(json.dumps({'graph': r}, default=_jsonable))
         return
     if args.no_headline:
         cfg = _run_configs(dev, [c for c in args.configs.split(',') if c])

@@ -2,13 +2,15 @@
 // one block per valid high part h.
 //
 // The rows of h form one contiguous segment [off, off + cnt); row off + j
-// holds the low set lowtab[post_c[j]] (c = class of h), so x and y move in
-// coalesced runs.  A flip of low bit q keeps h: its source is
-// off + pre_c[lonb[q][t]] (lonb = index of the flipped low set, GR_NONE when
-// it is not independent; an excitation blocked by h is dropped by F_c).  A
-// flip of high bit p moves to the segment of h ^ 2^p with the same low set:
-// off2 + pre_c2[t] — block-uniform offsets held in LDS.  The diagonal is
-// rebuilt per row: dhi(h) (block-uniform) + dlo[t] + Σ_{q in l} w_q(h), with
+// holds low set t = post_c[j] (c = class of h), so x and y move in
+// coalesced runs.  Every high part of class c has the same row layout, so
+// the per-class row tables (indexed by base_c + j, coalesced across lanes)
+// hold what depends on the low set only: its bits rcode, its diagonal rdiag
+// and, for each low bit q, the segment position lrow[q] of the row with q
+// flipped (GR_NONE when that set is not independent or h blocks it).  A flip
+// of high bit p moves to the segment of h ^ 2^p with the same low set:
+// off2 + pre_c2[t], block-uniform offsets held in LDS.  The diagonal is
+// dhi(h) (block-uniform) + rdiag + Σ_{q in l} w_q(h), with
 // w_q(h) = Σ_{p in h} V_{p,q} the high-low cross terms in LDS.
 #pragma once
 #include "common.hpp"
@@ -17,19 +19,22 @@
 
 namespace ryd {
 
+constexpr int GR_FLIPS = 8;      // flips whose loads are issued together
+
 struct GraphModel {
   int n, L, H;
   int64_t tl;                   // low sets
   int64_t nv;                   // valid high parts
   const uint32_t* hlist;        // [nv] valid high parts, ascending
   const uint64_t* hpk;          // [2^H + 1]
-  const uint32_t* lowtab;       // [tl]
-  const uint32_t* lonb;         // [L][tl]
-  const double* dlo;            // [tl]
+  int64_t rows;                 // rows of all classes (class c from post_off[c])
   const uint32_t* fc;           // [nc]
   const uint32_t* pre;          // [nc][tl + 1]
-  const uint32_t* post;         // rows of every class, concatenated
-  const int64_t* post_off;      // [nc] start of class c in post
+  const uint32_t* post;         // [rows] low-set index of each class row
+  const int64_t* post_off;      // [nc] start of class c in the row tables
+  const uint32_t* rcode;        // [rows] low bits of the row
+  const double* rdiag;          // [rows] diagonal of those bits alone
+  const uint32_t* lrow;         // [L][rows] segment position of the row with bit q flipped
   const double* vb;             // [n*n] V by bit position
   double dl[MAXN];              // delta by bit position
   double ar[MAXN], ai[MAXN];    // Ω/2 by bit position
@@ -96,33 +101,62 @@ __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io
       dhi_s = d;
     }
     __syncthreads();
-    const uint32_t F = M.fc[c];
-    const uint32_t* prec = M.pre + (size_t)c * (size_t)(tl + 1);
-    const uint32_t* postc = M.post + M.post_off[c];
+    const int64_t base = M.post_off[c];
     const double dh = dhi_s;
     for (int64_t j = tid; j < cnt; j += 256) {
       const int64_t row = off + j;
-      const uint32_t t = postc[j];
-      const uint32_t l = M.lowtab[t];
-      double dg = dh + M.dlo[t];
+      const int64_t cr = base + j;
+      const uint32_t t = M.post[cr];
+      const uint32_t l = M.rcode[cr];
+      double dg = dh + M.rdiag[cr];
       for (uint32_t y = l; y; y &= y - 1) dg += wq[__builtin_ctz(y)];
       const V xe = vscale(io.x[row], sr);
       V acc = vscale(xe, (RT)dg);
-      for (int q = 0; q < L; ++q) {
-        const double are = M.ar[q], aim = M.ai[q];
-        if (are == 0.0 && aim == 0.0) continue;
-        const uint32_t t2 = M.lonb[(size_t)q * (size_t)tl + t];
-        const int bit = (int)((l >> q) & 1u);
-        if (t2 == GR_NONE || (!bit && ((F >> q) & 1u))) continue;
-        vcfma(acc, (RT)(are * s), (RT)((bit ? aim : -aim) * s), io.x[off + prec[t2]]);
+      // flips in groups of GR_FLIPS: every load of a stage is issued before
+      // the next stage needs it (position, then x)
+      for (int q0 = 0; q0 < L; q0 += GR_FLIPS) {
+        int64_t r2[GR_FLIPS];
+#pragma unroll
+        for (int u = 0; u < GR_FLIPS; ++u) {
+          const int q = q0 + u;
+          const uint32_t pos = q < L ? M.lrow[(size_t)q * (size_t)M.rows + cr] : GR_NONE;
+          r2[u] = (pos != GR_NONE && (M.ar[q] != 0.0 || M.ai[q] != 0.0)) ? off + pos : -1;
+        }
+        V xv[GR_FLIPS];
+#pragma unroll
+        for (int u = 0; u < GR_FLIPS; ++u) xv[u] = r2[u] >= 0 ? io.x[r2[u]] : vzero(V{});
+#pragma unroll
+        for (int u = 0; u < GR_FLIPS; ++u) {
+          if (r2[u] < 0) continue;
+          const int q = q0 + u;
+          const double aim = M.ai[q];
+          vcfma(acc, (RT)(M.ar[q] * s), (RT)((((l >> q) & 1u) ? aim : -aim) * s), xv[u]);
+        }
       }
-      for (int p = 0; p < H; ++p) {
-        const double are = M.ar[p + L], aim = M.ai[p + L];
-        if (are == 0.0 && aim == 0.0) continue;
-        const int64_t o2 = off2[p];
-        const int bit = (int)((h >> p) & 1u);
-        if (o2 < 0 || (!bit && (l & f2[p]))) continue;
-        vcfma(acc, (RT)(are * s), (RT)((bit ? aim : -aim) * s), io.x[o2 + M.pre[prow2[p] + t]]);
+      for (int p0 = 0; p0 < H; p0 += GR_FLIPS) {
+        int64_t r2[GR_FLIPS];
+#pragma unroll
+        for (int u = 0; u < GR_FLIPS; ++u) {
+          const int p = p0 + u;
+          bool ok = p < H;
+          if (ok) {
+            const int64_t o2 = off2[p];
+            ok = o2 >= 0 && (((h >> p) & 1u) || !(l & f2[p])) &&
+                 (M.ar[p + L] != 0.0 || M.ai[p + L] != 0.0);
+          }
+          r2[u] = ok ? (int64_t)M.pre[prow2[p] + t] : -1;
+        }
+        V xv[GR_FLIPS];
+#pragma unroll
+        for (int u = 0; u < GR_FLIPS; ++u)
+          xv[u] = r2[u] >= 0 ? io.x[off2[p0 + u] + r2[u]] : vzero(V{});
+#pragma unroll
+        for (int u = 0; u < GR_FLIPS; ++u) {
+          if (r2[u] < 0) continue;
+          const int p = p0 + u;
+          const double aim = M.ai[p + L];
+          vcfma(acc, (RT)(M.ar[p + L] * s), (RT)((((h >> p) & 1u) ? aim : -aim) * s), xv[u]);
+        }
       }
       V v = vscale(acc, chr);
       vfma(v, cxr, xe);
@@ -140,33 +174,42 @@ __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io
 }
 
 // ---- tables for k_graph_apply -------------------------------------------------
-// lonb[q][t] = index of lowtab[t] ^ 2^q among the low sets, or GR_NONE
-__global__ __launch_bounds__(256) void k_gr_lonb(int L, const uint32_t* __restrict__ lowtab, int64_t tl,
-                                                 const uint32_t* __restrict__ lo_idx,
-                                                 uint32_t* __restrict__ lonb) {
-  const int64_t total = (int64_t)L * tl;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
-    const int q = (int)(k / tl);
-    const int64_t t = k - (int64_t)q * tl;
-    lonb[k] = lo_idx[lowtab[t] ^ (1u << q)];
-  }
-}
-
-// dlo[t] = diagonal of the low bits of lowtab[t] alone
-__global__ __launch_bounds__(256) void k_gr_dlo(int n, const uint32_t* __restrict__ lowtab, int64_t tl,
-                                                const double* __restrict__ dl,
-                                                const double* __restrict__ vb, double* __restrict__ dlo) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tl; t += stride) {
-    const uint32_t l = lowtab[t];
-    double d = 0.0;
-    for (uint32_t y = l; y; y &= y - 1) {
-      const int q = __builtin_ctz(y);
-      d += dl[q];
-      for (uint32_t z = y & (y - 1); z; z &= z - 1) d += vb[(size_t)q * n + __builtin_ctz(z)];
+// Per class c and segment position j (row cr = post_off[c] + j, t = post[cr]):
+// rcode = lowtab[t], rdiag = diagonal of those low bits, lrow[q][cr] =
+// pre_c[index of lowtab[t] ^ 2^q] or GR_NONE.  One block per class.
+__global__ __launch_bounds__(256) void k_gr_rows(int n, int L, const uint32_t* __restrict__ lowtab,
+                                                 int64_t tl, const uint32_t* __restrict__ lo_idx,
+                                                 const uint32_t* __restrict__ fc, int64_t nc,
+                                                 const uint32_t* __restrict__ pre,
+                                                 const uint32_t* __restrict__ post,
+                                                 const int64_t* __restrict__ post_off, int64_t rows,
+                                                 const double* __restrict__ dl,
+                                                 const double* __restrict__ vb,
+                                                 uint32_t* __restrict__ rcode,
+                                                 double* __restrict__ rdiag,
+                                                 uint32_t* __restrict__ lrow) {
+  for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    const uint32_t F = fc[c];
+    const uint32_t* prec = pre + (size_t)c * (size_t)(tl + 1);
+    const int64_t b0 = post_off[c];
+    const int64_t b1 = c + 1 < nc ? post_off[c + 1] : rows;
+    for (int64_t cr = b0 + threadIdx.x; cr < b1; cr += blockDim.x) {
+      const uint32_t l = lowtab[post[cr]];
+      rcode[cr] = l;
+      double d = 0.0;
+      for (uint32_t y = l; y; y &= y - 1) {
+        const int q = __builtin_ctz(y);
+        d += dl[q];
+        for (uint32_t z = y & (y - 1); z; z &= z - 1) d += vb[(size_t)q * n + __builtin_ctz(z)];
+      }
+      rdiag[cr] = d;
+      for (int q = 0; q < L; ++q) {
+        const uint32_t l2 = l ^ (1u << q);
+        const uint32_t t2 = lo_idx[l2];
+        const bool ok = t2 != GR_NONE && !(l2 & F);
+        lrow[(size_t)q * (size_t)rows + cr] = ok ? prec[t2] : GR_NONE;
+      }
     }
-    dlo[t] = d;
   }
 }
 

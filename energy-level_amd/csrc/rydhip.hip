@@ -88,10 +88,12 @@ struct ryd_plan {
   // k_graph_apply tables (RANK_GRAPH only)
   int64_t gr_nv = 0;                 // valid high parts
   uint32_t* d_hlist = nullptr;       // [nv]
-  uint32_t* d_lonb = nullptr;        // [L][tl]
-  double* d_dlo = nullptr;           // [tl]
-  uint32_t* d_post = nullptr;        // [rows of all classes]
+  int64_t gr_rows = 0;               // rows of all classes
+  uint32_t* d_post = nullptr;        // [rows]
   int64_t* d_post_off = nullptr;     // [nc]
+  uint32_t* d_rcode = nullptr;       // [rows]
+  double* d_rdiag = nullptr;         // [rows]
+  uint32_t* d_lrow = nullptr;        // [L][rows]
   int fib_L = 0;
   int64_t fib_cnt[2] = {0, 0};
 };
@@ -227,13 +229,14 @@ GraphModel graph_model(const ryd_plan* P) {
   M.nv = P->gr_nv;
   M.hlist = P->d_hlist;
   M.hpk = P->d_hpk;
-  M.lowtab = P->d_lowtab;
-  M.lonb = P->d_lonb;
-  M.dlo = P->d_dlo;
+  M.rows = P->gr_rows;
   M.fc = P->d_fc;
   M.pre = P->d_pre;
   M.post = P->d_post;
   M.post_off = P->d_post_off;
+  M.rcode = P->d_rcode;
+  M.rdiag = P->d_rdiag;
+  M.lrow = P->d_lrow;
   M.vb = P->d_vb;
   for (int p = 0; p < MAXN; ++p) {
     M.dl[p] = P->dl[p];
@@ -682,7 +685,9 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
     }
     const size_t pre_bytes = sizeof(uint32_t) * (size_t)nc * (size_t)(tl + 1);
     const size_t post_bytes = sizeof(uint32_t) * (size_t)rows_all;
-    if (pre_bytes + post_bytes <= gr_prefix_budget(dim)) {
+    const size_t rows_bytes = (2 * sizeof(uint32_t) + sizeof(double) +
+                               sizeof(uint32_t) * (size_t)L) * (size_t)rows_all;
+    if (pre_bytes <= gr_prefix_budget(dim)) {
       HIP_TRY(hipMalloc(&P->d_pre, pre_bytes));
       k_gr_prefix<<<dim3(cg), dim3(256)>>>(P->d_lowtab, tl, P->d_fc, nc, P->d_pre);
       LAUNCH_CHECK();
@@ -690,7 +695,8 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
     } else {
       P->rank_mode = RANK_SEARCH;
     }
-    if (P->rank_mode == RANK_GRAPH && P->npairs == 0) {
+    if (P->rank_mode == RANK_GRAPH && P->npairs == 0 &&
+        pre_bytes + rows_bytes <= gr_prefix_budget(dim)) {
       // tables of the segment kernel (Omega/delta/V models)
       HIP_TRY(hipMalloc(&P->d_post_off, sizeof(int64_t) * (size_t)nc));
       HIP_TRY(hipMemcpy(P->d_post_off, post_off.data(), sizeof(int64_t) * (size_t)nc,
@@ -698,19 +704,17 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
       HIP_TRY(hipMalloc(&P->d_post, post_bytes > 0 ? post_bytes : sizeof(uint32_t)));
       k_gr_post<<<dim3(cg), dim3(256)>>>(P->d_lowtab, tl, P->d_fc, nc, P->d_post_off, P->d_post);
       LAUNCH_CHECK();
-      HIP_TRY(hipMalloc(&P->d_lonb, sizeof(uint32_t) * (size_t)L * (size_t)tl + sizeof(uint32_t)));
-      if (L > 0) {
-        const int64_t tot = (int64_t)L * tl;
-        k_gr_lonb<<<dim3((unsigned)((tot + 255) / 256 < 65536 ? (tot + 255) / 256 : 65536)),
-                    dim3(256)>>>(L, P->d_lowtab, tl, P->d_lo_idx, P->d_lonb);
-        LAUNCH_CHECK();
-      }
       HIP_TRY(hipMalloc(&dl_dev, sizeof(double) * (size_t)n));
       HIP_TRY(hipMemcpy(dl_dev, P->dl, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
-      HIP_TRY(hipMalloc(&P->d_dlo, sizeof(double) * (size_t)tl));
-      k_gr_dlo<<<dim3((unsigned)((tl + 255) / 256 < 65536 ? (tl + 255) / 256 : 65536)), dim3(256)>>>(
-          n, P->d_lowtab, tl, dl_dev, P->d_vb, P->d_dlo);
+      const size_t rows1 = (size_t)(rows_all > 0 ? rows_all : 1);
+      HIP_TRY(hipMalloc(&P->d_rcode, sizeof(uint32_t) * rows1));
+      HIP_TRY(hipMalloc(&P->d_rdiag, sizeof(double) * rows1));
+      HIP_TRY(hipMalloc(&P->d_lrow, sizeof(uint32_t) * rows1 * (size_t)(L > 0 ? L : 1)));
+      k_gr_rows<<<dim3(cg), dim3(256)>>>(n, L, P->d_lowtab, tl, P->d_lo_idx, P->d_fc, nc, P->d_pre,
+                                          P->d_post, P->d_post_off, rows_all, dl_dev, P->d_vb,
+                                          P->d_rcode, P->d_rdiag, P->d_lrow);
       LAUNCH_CHECK();
+      P->gr_rows = rows_all;
       int64_t nv = 0;
       if (int rc = gr_total(GrValidFlag{P->d_hpk}, nhi, bsum, nv)) return rc;
       HIP_TRY(hipMalloc(&P->d_hlist, sizeof(uint32_t) * (size_t)nv));
@@ -1039,10 +1043,11 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_fc) (void)hipFree(P->d_fc);
   if (P->d_pre) (void)hipFree(P->d_pre);
   if (P->d_hlist) (void)hipFree(P->d_hlist);
-  if (P->d_lonb) (void)hipFree(P->d_lonb);
-  if (P->d_dlo) (void)hipFree(P->d_dlo);
   if (P->d_post) (void)hipFree(P->d_post);
   if (P->d_post_off) (void)hipFree(P->d_post_off);
+  if (P->d_rcode) (void)hipFree(P->d_rcode);
+  if (P->d_rdiag) (void)hipFree(P->d_rdiag);
+  if (P->d_lrow) (void)hipFree(P->d_lrow);
   for (int k = 0; k < 2; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);

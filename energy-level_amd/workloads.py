@@ -14,7 +14,8 @@ import numpy as np
 
 from .model import from_positions, chain_positions, square_positions, c6_coupling, from_calib
 
-__all__ = ['chain_calib', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5']
+__all__ = ['chain_calib', 'lattice_calib', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5',
+           'run_g1', 'time_hv']
 
 
 def chain_calib(n, c6):
@@ -24,6 +25,39 @@ def chain_calib(n, c6):
 def _sync():
     import torch
     torch.cuda.synchronize()
+
+
+def lattice_calib(w, h, c6=20.0, cutoff=2.9):
+    """Row-major w x h square lattice at unit spacing (site r*w + c at (c, r)),
+    V = c6/r^6 for pairs within ``cutoff``."""
+    pos = [(float(c), float(r), 0.0) for r in range(h) for c in range(w)]
+    pairs = []
+    for i in range(len(pos)):
+        for j in range(i + 1, len(pos)):
+            d = float(np.hypot(pos[i][0] - pos[j][0], pos[i][1] - pos[j][1]))
+            if d <= cutoff:
+                pairs.append({'i': i, 'j': j, 'V': c6 / d ** 6})
+    return {'atoms': [{'id': i, 'pos': list(p)} for i, p in enumerate(pos)], 'pairs': pairs,
+            'terms': ['I1']}
+
+
+def time_hv(plan, dtype, steps=10, warmup=2, seed=0):
+    """Mean kernel time (ms) of ``plan.hv`` over ``steps`` launches, HIP events
+    on the current stream, x ~ N(0,1) resident on the device."""
+    import torch
+    g = torch.Generator(device=plan.device)
+    g.manual_seed(seed)
+    x = torch.randn(plan.dim, dtype=torch.float64, device=plan.device, generator=g).to(dtype)
+    y = torch.empty_like(x)
+    for _ in range(warmup):
+        plan.hv(x, y)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        plan.hv(x, y)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
 
 
 def run_c1(device='cuda', steps=200, seed=0):
@@ -230,3 +264,41 @@ def run_c5(device='cuda', n=20, slices=200, dt=0.05, K=15, seed=0, fd_components
                 grad_cost_in_fp64_forwards=t_grad / t_fwd64,
                 grad_c64_vs_fp64_max_rel=float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
                 fd_checks=fd)
+
+
+def run_g1(device='cuda', w=6, h=7, diag=False, steps=10, seed=0, ground_state=True):
+    """Blockade-graph basis: w x h square lattice (6 x 7: N = 42), V = 20/r^6
+    within 2.9a, blockade edges |V| >= 10 MHz (nearest neighbours; with
+    ``diag`` also the diagonals, threshold 2 MHz), Ω = 1, Δ = 1.5.  The
+    basis (D = 69,050,253 for 6 x 7) is enumerated on the device; reports the
+    plan build time, the H·v time in FP64 and FP32 (2·D·b algorithmic bytes)
+    and the FP64 ground state."""
+    from .basis import blockade_edges
+    from .lanczos import hip_lanczos_ground_state
+    from .plan import Plan
+    n = w * h
+    calib = lattice_calib(w, h)
+    edges = blockade_edges(calib, {'kind': 'blockade', 'threshold_mhz': 2.0 if diag else 10.0})
+    Om, delta = np.ones(n), -1.5 * np.ones(n)
+    _sync()
+    t0 = time.perf_counter()
+    P = Plan(from_calib(calib, Om, delta), basis='graph', edges=edges, device=device)
+    _sync()
+    out = dict(config='G1 %dx%d lattice blockade graph%s' % (w, h, ' (diagonals)' if diag else ''),
+               n_sites=n, edges=len(edges), dim=P.dim, rank_mode=P.rank_mode,
+               build_seconds=time.perf_counter() - t0)
+    import torch
+    for name, dt, b in (('f64', torch.float64, 8), ('f32', torch.float32, 4)):
+        ms = time_hv(P, dt, steps=steps, seed=seed)
+        out['hv_ms_' + name] = ms
+        out['hv_gbps_' + name] = 2.0 * P.dim * b / (ms * 1e-3) / 1e9
+    if ground_state:
+        _sync()
+        t0 = time.perf_counter()
+        r = hip_lanczos_ground_state(calib, Om, delta, blockade_edges=edges, dtype='float64',
+                                     seed=seed, plan=P, basis='graph')
+        _sync()
+        out['seconds_fp64_gs'] = time.perf_counter() - t0
+        out['fp64'] = {k: r[k] for k in ('E0', 'gap', 'residuals', 'converged', 'krylov_dim')}
+    P.close()
+    return out
