@@ -235,4 +235,17 @@ __global__ __launch_bounds__(256) void k_cscale(const V* __restrict__ x, V* __re
   }
 }
 
+// y += a·x (complex a)
+template <class V>
+__global__ __launch_bounds__(256) void k_caxpy(const V* __restrict__ x, V* __restrict__ y,
+                                               int64_t D, double are, double aim) {
+  using RT = typename VT<V>::R;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    V v = y[i];
+    vcfma(v, (RT)are, (RT)aim, x[i]);
+    y[i] = v;
+  }
+}
+
 }  // namespace ryd

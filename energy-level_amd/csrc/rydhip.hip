@@ -1380,6 +1380,55 @@ int do_evolve_forward(const ryd_plan* P, int dtype, void* psi_v, int64_t nslices
   return RYD_OK;
 }
 
+// Forward slices on any plan with the plan's own H (om = 1, dadd = 0): the
+// Chebyshev recurrence through the plan's H·v launches (restricted bases,
+// J/A models, full bases beyond the fused kernel), ψ accumulated by a
+// streaming complex axpy per term.
+template <class V>
+int do_evolve_generic(const ryd_plan* P, int dtype, void* psi_v, int64_t nslices,
+                      const double* c, const double* r, int K, const double* coef,
+                      void* work_v, hipStream_t st) {
+  V* psi = static_cast<V*>(psi_v);
+  const int64_t D = P->dim;
+  V* W = static_cast<V*>(work_v);
+  V* T[3] = {W, W + D, W + 2 * D};
+  V* acc = W + 3 * D;
+  Reduce none = make_reduce(nullptr, POST_NONE, nullptr);
+  const unsigned sg = stream_grid(D);
+  for (int64_t k = 0; k < nslices; ++k) {
+    const double* a = coef + (size_t)k * (K + 1) * 2;
+    const double inv = 1.0 / r[k];
+    k_cscale<V><<<sg, 256, 0, st>>>(psi, acc, D, a[0], a[1]);
+    LAUNCH_CHECK();
+    for (int j = 0; j < K; ++j) {   // t_{j+1} = (1 or 2)·H~ t_j − t_{j−1}
+      ApplyIO<V> io;
+      io.x = j == 0 ? psi : T[(j - 1) % 3];
+      io.z = j == 0 ? nullptr : (j == 1 ? psi : T[(j - 2) % 3]);
+      io.y = T[j % 3];
+      io.probe = nullptr;
+      io.smode = SM_CONST;
+      io.s = 1.0;
+      io.ch = (j == 0 ? 1.0 : 2.0) * inv;
+      io.cx = -(j == 0 ? 1.0 : 2.0) * c[k] * inv;
+      io.cz = j == 0 ? 0.0 : -1.0;
+      io.nrm = nullptr;
+      io.j = 0;
+      io.stop = nullptr;
+      io.reduce = 0;
+      if (int rc = launch_apply<V>(P, dtype, io, none, st)) return rc;
+      k_caxpy<V><<<sg, 256, 0, st>>>(T[j % 3], acc, D, a[2 * (j + 1)], a[2 * (j + 1) + 1]);
+      LAUNCH_CHECK();
+    }
+    HIP_TRY(hipMemcpyAsync(psi, acc, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+  }
+  return RYD_OK;
+}
+
+// the fused slice kernels cover the full basis with Omega/delta/V up to n = 22
+bool evo_fused(const ryd_plan* P) {
+  return P->basis == RYD_BASIS_FULL && !P->generic && P->n < SPLIT_MIN_N;
+}
+
 template <class V>
 int do_evolve_vjp(const ryd_plan* P, int dtype, const void* psi_v, void* g_v, double om,
                   double dadd, double c, double r, int K, const double* a, void* tstore_v,
@@ -1444,10 +1493,23 @@ extern "C" int ryd_evolve_forward(const ryd_plan* P, int dtype, void* psi, int64
                                   void* stream) {
   if (!P || !psi || !om || !dadd || !c || !r || !coef || !work || nslices < 0)
     return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_forward");
-  if (int rc = check_evo(P, dtype, K)) return rc;
   for (int64_t k = 0; k < nslices; ++k)
     if (!(r[k] > 0.0)) return fail(RYD_ERR_INVALID, "spectral radius r must be > 0");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!evo_fused(P)) {
+    // any other plan evolves under its own H
+    if (dtype != RYD_C64 && dtype != RYD_C128)
+      return fail(RYD_ERR_INVALID, "evolution needs a complex dtype");
+    if (K < 1 || K > 4096) return fail(RYD_ERR_INVALID, "K must be in [1, 4096]");
+    for (int64_t k = 0; k < nslices; ++k)
+      if (om[k] != 1.0 || dadd[k] != 0.0)
+        return fail(RYD_ERR_UNSUPPORTED, "restricted bases, pair flips and n > 22 evolve under "
+                                         "the plan's own H (om = 1, dadd = 0 per slice)");
+    if (dtype == RYD_C64)
+      return do_evolve_generic<c32>(P, dtype, psi, nslices, c, r, K, coef, work, st);
+    return do_evolve_generic<c64>(P, dtype, psi, nslices, c, r, K, coef, work, st);
+  }
+  if (int rc = check_evo(P, dtype, K)) return rc;
   if (dtype == RYD_C64)
     return do_evolve_forward<c32>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
   return do_evolve_forward<c64>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);

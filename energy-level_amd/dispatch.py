@@ -9,8 +9,9 @@ Backends (declared like ``EQ_BACKENDS``, backends.py:112-118):
   the states of a projected ``system['plan']`` (the reference's gpu-lanczos
   refuses projected plans, eq_route.py:689-691); keys of
   ``torch_lanczos_ground_state`` plus ``n_atom`` (eq_route.py:689-699).
-* ``hip-kpm`` (thermal, dos): matrix-free KPM; no host CSR is assembled
-  (the reference builds one through ``_system_H``, eq_route.py:743, :774).
+* ``hip-kpm`` (thermal, dos): matrix-free KPM on the full basis or a
+  projected plan's basis; no host CSR is assembled (the reference builds one
+  through ``_system_H``, eq_route.py:73-74, :743, :774).
 * ``hip-batch`` (gs): the batched solver on this one system
   (eq_route.py:702-709 returns E0, gap, n_atom).
 
@@ -126,15 +127,20 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
             result = dict(base, E0=float(rb['E0'][0]), gap=float(rb['gap'][0]),
                           n_atom=rb['n_atom'][0])
     else:
-        if projected:
-            raise NotImplementedError('hip-kpm takes the full basis')
         from .kpm import kpm_moments_hip, spectral_bounds, thermal_from_moments, dos_from_moments
         from .model import from_calib
         from .plan import Plan
         if dtype not in ('float32', 'float64'):
             raise ValueError('hip-kpm dtype must be float32/float64')
-        P = Plan(from_calib(calib, system.get('Om'), system.get('delta'), gamma,
-                            system.get('terms')), device=device)
+        model = from_calib(calib, system.get('Om'), system.get('delta'), gamma, system.get('terms'))
+        if projected:
+            # the projected H of compile_sparse(plan) (eq_route.py:73-74), matrix-free
+            from .lanczos import restricted_plan
+            dev_states = hasattr(states, 'cpu')
+            P, _ = restricted_plan(model, None if (dev_states and edges is not None) else states,
+                                   edges, device=device)
+        else:
+            P = Plan(model, device=device)
         try:
             emin, emax, c, r = spectral_bounds(P, margin=margin)
             mu, c, r = kpm_moments_hip(plan=P, num=num, nvec=nvec, seed=seed, dtype=dtype,

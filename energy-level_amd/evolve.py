@@ -299,19 +299,31 @@ def segment_drive(t, segments, n, interp='left'):
 
 def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='left',
                            dtype='complex128', device='cuda', return_states=False,
-                           basis_probabilities_max_dim=4096):
-    """Exact piecewise-constant evolution on the full basis, output at the
-    times ``t`` (ns): every output interval is split at the segment starts
-    and each piece [a, b] is propagated by exp(−i·CONV·H·(b−a)) with the drive
-    of ``segment_drive((a+b)/2)`` (engine_ed.py:130-147), here by the device
+                           basis_probabilities_max_dim=4096, basis_indices=None,
+                           blockade_edges=None, plan=None):
+    """Exact piecewise-constant evolution, output at the times ``t`` (ns):
+    every output interval is split at the segment starts and each piece
+    [a, b] is propagated by exp(−i·CONV·H·(b−a)) with the drive of
+    ``segment_drive((a+b)/2)`` (engine_ed.py:130-147), here by the device
     Chebyshev series (accurate to 1e-13 per piece; one plan per distinct
     drive, as the reference caches one CSR per drive).
 
+    The basis is the full 2^n one, or the projected basis of a
+    ``HamiltonianPlan`` (``plan`` with ``projected``; or ``basis_indices`` /
+    ``blockade_edges`` directly), as engine_ed.py:191-193 restricts to
+    ``plan['states']``: graph bases are enumerated on the device.
+
+    ``psi0``: None (|g…g>, engine_ed.py:165), a vector of the basis
+    dimension (in the order of ``basis_indices``), or a full 2^n vector,
+    restricted to the basis and normalised by its full norm (weight outside
+    the basis is an error, as state.py:112-140).
+
     Returns {t, pops_atom (len(t), n), norms (len(t),), psi (final state,
-    host complex128), method='manybody', backend='hip-chebyshev'[, states]
-    [, basis_probabilities]}; ``psi0`` defaults to |g…g> (engine_ed.py:165)."""
+    host complex128, basis order), method='manybody', backend='hip-chebyshev',
+    basis_states (codes, projected runs only)[, states][, basis_probabilities]}."""
     import torch
-    from .lanczos import site_occupations
+    from .lanczos import site_occupations, restricted_plan
+    from .basis import sector_codes
     nat.require_gpu()
     n = len(calib['atoms'])
     t = np.asarray(t, float)
@@ -319,41 +331,73 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
         raise ValueError('t must be a non-decreasing 1-D time grid')
     if dtype not in ('complex64', 'complex128'):
         raise ValueError('dtype must be complex64 or complex128')
+    if plan is not None and plan.get('projected'):
+        if basis_indices is None and blockade_edges is None:
+            st = plan.get('states')
+            blockade_edges = plan.get('blockade_edges')
+            if st is not None and not (hasattr(st, 'cpu') and blockade_edges is not None):
+                basis_indices = st
+    projected = basis_indices is not None or blockade_edges is not None
     tdt = torch.complex64 if dtype == 'complex64' else torch.complex128
-    dim = 1 << n
     dv = torch.device(device)
-    if psi0 is None:
-        psi = torch.zeros(dim, dtype=tdt, device=dv)
-        psi[0] = 1.0
-    else:
-        psi = torch.as_tensor(np.asarray(psi0) if not torch.is_tensor(psi0) else psi0,
-                              device=dv).to(tdt).clone()
-        if psi.numel() != dim:
-            raise ValueError('psi0 must have 2^n entries')
     plans = {}
     pops, norms, states, probs = [], [], [], []
-
-    def record(p, x):
-        norms.append(float(torch.linalg.vector_norm(x.to(torch.complex128)).item()))
-        occ = site_occupations(p, x) if p is not None else None
-        pops.append(np.asarray(occ) if occ is not None else None)
-        if return_states:
-            states.append(x.to(torch.complex128).cpu().numpy())
-        if dim <= basis_probabilities_max_dim:
-            probs.append((x.abs() ** 2).to(torch.float64).cpu().numpy())
+    order = None            # plan order -> caller order of basis_indices
 
     def plan_for(om, de):
+        nonlocal order
         key = (tuple(np.round(om.real, 15)), tuple(np.round(om.imag, 15)), tuple(de))
         if key not in plans:
             model = from_calib(calib, om, de, None, terms)
-            plans[key] = (model, Plan(model, device=dv))
+            if projected:
+                P, order = restricted_plan(model, basis_indices, blockade_edges, device=dv)
+            else:
+                P = Plan(model, device=dv)
+            plans[key] = (model, P)
         return plans[key]
+
+    def record(p, x):
+        norms.append(float(torch.linalg.vector_norm(x.to(torch.complex128)).item()))
+        pops.append(np.asarray(site_occupations(p, x)))
+        if return_states or (x.numel() <= basis_probabilities_max_dim):
+            v = x.to(torch.complex128).cpu().numpy()
+            if order is not None:
+                back = np.empty_like(v)
+                back[order] = v
+                v = back
+            if return_states:
+                states.append(v)
+            if x.numel() <= basis_probabilities_max_dim:
+                probs.append(np.abs(v) ** 2)
 
     try:
         first = segment_drive(float(t[0]), segments, n, interp) if segments else None
         if first is None:
             raise ValueError('segments are required')
         _, p0 = plan_for(*first)
+        dim = p0.dim
+        codes = sector_codes(p0).cpu().numpy() if projected else None
+        if psi0 is None:
+            psi = torch.zeros(dim, dtype=tdt, device=dv)
+            if projected and (codes.size == 0 or codes[0] != 0):
+                raise ValueError('|g...g> is not in the projected basis; give psi0')
+            psi[0] = 1.0
+        else:
+            v = psi0.detach().cpu().numpy() if torch.is_tensor(psi0) else np.asarray(psi0)
+            v = np.asarray(v, complex).reshape(-1)
+            if v.size == dim:
+                if order is not None:
+                    v = v[order]
+            elif projected and v.size == (1 << n):
+                norm = np.linalg.norm(v)
+                inside = v[codes]
+                if norm > 0 and abs(np.linalg.norm(inside) - norm) > 1e-12 * norm:
+                    raise ValueError('initial state has weight outside the projected basis')
+                v = inside / norm if norm > 0 else inside
+            else:
+                raise ValueError('psi0 must have %d entries%s'
+                                 % (dim, ' (or 2^n)' if projected else ''))
+            psi = torch.as_tensor(v, device=dv).to(tdt)
         record(p0, psi)
         for k in range(1, len(t)):
             for a, b in segment_intervals(t[k - 1], t[k], segments):
@@ -365,9 +409,17 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
                                     units='reference')
                 psi = prob.propagate(psi, [1.0], [0.0])
             record(p0, psi)
-        out = dict(t=t, pops_atom=np.asarray(pops), norms=np.asarray(norms),
-                   psi=psi.to(torch.complex128).cpu().numpy(), method='manybody',
-                   backend='hip-chebyshev')
+        fin = psi.to(torch.complex128).cpu().numpy()
+        if order is not None:
+            back = np.empty_like(fin)
+            back[order] = fin
+            fin = back
+        out = dict(t=t, pops_atom=np.asarray(pops), norms=np.asarray(norms), psi=fin,
+                   method='manybody', backend='hip-chebyshev')
+        if projected:
+            out['basis_states'] = (np.asarray(basis_indices.cpu() if hasattr(basis_indices, 'cpu')
+                                              else basis_indices, np.int64)
+                                   if basis_indices is not None else codes)
         if return_states:
             out['states'] = np.asarray(states)
         if probs:

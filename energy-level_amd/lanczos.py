@@ -35,7 +35,7 @@ from .model import from_calib
 from .plan import Plan, cast
 
 __all__ = ['hip_lanczos_ground_state', 'lanczos_run', 'ritz_vectors', 'rayleigh_residual',
-           'site_occupations']
+           'site_occupations', 'restricted_plan', 'lanczos_vector']
 
 
 def _tridiag(a, b):
@@ -329,6 +329,52 @@ def _start_vector(dim, seed, complex_kind, device):
     return v
 
 
+
+def restricted_plan(model, basis_indices=None, blockade_edges=None, device='cuda'):
+    """(Plan, order) for a restricted basis.
+
+    ``blockade_edges``: the independent sets of that graph, enumerated on the
+    device (chain edges without pair flips take the chain-sector kernels);
+    with ``basis_indices`` as well, the graph basis is used only when it holds
+    exactly those codes.  Otherwise ``basis_indices`` (unique codes in any
+    order): the chain sector when it is exactly that, else a code-list plan
+    (closed-form ranks for a complete number sector, binary search
+    otherwise).  ``order`` maps the plan's ascending order back to the order
+    of ``basis_indices`` (None when the caller gave no list)."""
+    import torch
+    from .basis import is_chain_blockade_basis, validate_edges, is_chain_edges, sector_codes
+    idx = None
+    if basis_indices is not None:
+        if isinstance(basis_indices, torch.Tensor):
+            basis_indices = basis_indices.cpu().numpy()
+        idx = np.asarray(basis_indices, np.int64).reshape(-1)
+        if idx.size == 0:
+            raise ValueError('basis_indices is empty')
+        if np.any(idx < 0) or np.any(idx >= (1 << model.n)):
+            raise ValueError('basis_indices must lie in [0, 2^n)')
+    if blockade_edges is not None:
+        edges = validate_edges(model.n, blockade_edges)
+        if is_chain_edges(model.n, edges) and not model.has_pair_flips:
+            if idx is None:
+                return Plan(model, basis='chain_fib', device=device), None
+        else:
+            cand = Plan(model, basis='graph', edges=edges, device=device)
+            if idx is None:
+                return cand, None
+            if idx.size == cand.dim and np.array_equal(np.sort(idx),
+                                                       sector_codes(cand).cpu().numpy()):
+                return cand, np.argsort(idx, kind='stable')
+            cand.close()
+    if idx is None:
+        raise ValueError('a restricted basis needs basis_indices or blockade_edges')
+    order = np.argsort(idx, kind='stable')
+    codes = idx[order]
+    if np.any(np.diff(codes) == 0):
+        raise ValueError('basis_indices contains duplicate codes')
+    if not model.has_pair_flips and is_chain_blockade_basis(model.n, codes):
+        return Plan(model, basis='chain_fib', device=device), order
+    return Plan(model, basis='codes', codes=codes, device=device), order
+
 def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
                              basis_indices=None, m=48, seed=0, dtype='float64', device='cuda',
                              tol=None, max_restarts=8, return_state=False,
@@ -364,67 +410,15 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
         raise ValueError('m must be >= 2 and max_restarts >= 0')
     model = from_calib(calib, Om, delta, gamma, terms)
     order = None
-    codes = None
-    graph = None
-    if blockade_edges is not None and plan is None:
-        from .basis import validate_edges, is_chain_edges
-        edges = validate_edges(model.n, blockade_edges)
-        if is_chain_edges(model.n, edges) and not model.has_pair_flips:
-            cand = 'chain_blockade'
-        else:
-            cand = Plan(model, basis='graph', edges=edges, device=device)
-        if basis_indices is None:
-            basis, graph = ('chain_blockade', None) if cand == 'chain_blockade' else ('graph', cand)
-        else:
-            import torch
-            idx = np.asarray(basis_indices.cpu() if isinstance(basis_indices, torch.Tensor)
-                             else basis_indices, np.int64).reshape(-1)
-            if cand == 'chain_blockade':
-                same = None     # the chain check below takes it
-            else:
-                same = idx.size == cand.dim
-                if same:
-                    from .basis import sector_codes
-                    srt = np.sort(idx)
-                    same = bool(np.array_equal(srt, sector_codes(cand).cpu().numpy()))
-            if same:
-                order = np.argsort(idx, kind='stable')
-                basis, graph = 'graph', cand
-                basis_indices = None
-            elif cand != 'chain_blockade':
-                cand.close()
-    if basis_indices is not None:
-        from .basis import is_chain_blockade_basis
-        if hasattr(basis_indices, 'cpu'):
-            basis_indices = basis_indices.cpu().numpy()
-        idx = np.asarray(basis_indices, np.int64).reshape(-1)
-        if idx.size == 0:
-            raise ValueError('basis_indices is empty')
-        if np.any(idx < 0) or np.any(idx >= (1 << model.n)):
-            raise ValueError('basis_indices must lie in [0, 2^n)')
-        if not model.has_pair_flips and is_chain_blockade_basis(model.n, idx):
-            basis = 'chain_blockade'
-        else:
-            # any basis: ascending on the device, results back in the caller's order
-            order = np.argsort(idx, kind='stable')
-            codes = idx[order]
-            if np.any(np.diff(codes) == 0):
-                raise ValueError('basis_indices contains duplicate codes')
-            basis = 'codes'
-    if basis not in ('full', 'chain_blockade', 'codes', 'graph'):
-        raise ValueError('basis must be full or chain_blockade (or given by basis_indices '
-                         'or blockade_edges)')
-    if basis == 'codes' and codes is None:
-        raise ValueError("basis='codes' needs basis_indices")
-    if basis == 'graph' and graph is None and plan is None:
-        raise ValueError("basis='graph' needs blockade_edges")
-    kind = {'full': 'full', 'chain_blockade': 'chain_fib', 'codes': 'codes', 'graph': 'graph'}[basis]
     if plan is not None:
         P = plan
-    elif graph is not None:
-        P = graph
+    elif basis_indices is not None or blockade_edges is not None:
+        P, order = restricted_plan(model, basis_indices, blockade_edges, device=device)
     else:
-        P = Plan(model, basis=kind, codes=codes, device=device)
+        if basis not in ('full', 'chain_blockade'):
+            raise ValueError('basis must be full or chain_blockade (or given by basis_indices '
+                             'or blockade_edges)')
+        P = Plan(model, basis='full' if basis == 'full' else 'chain_fib', device=device)
     dim = P.dim
     low = dtype in ('float32', 'complex64')
     complex_kind = (not P.is_real) or dtype.startswith('complex')

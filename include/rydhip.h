@@ -190,15 +190,19 @@ int ryd_sector_codes(const ryd_plan* plan, const int64_t* ranks, int64_t first, 
 int ryd_sector_rank(const ryd_plan* plan, const int64_t* codes, int64_t count, int64_t* ranks,
                     void* stream);
 
-/* Piecewise-constant evolution psi <- Π_k exp(-i·CONV·H_k·dt_k) psi on the
- * full basis (n <= 22, complex dtypes), each slice by a Chebyshev series:
+/* Piecewise-constant evolution psi <- Π_k exp(-i·CONV·H_k·dt_k) psi
+ * (complex dtypes), each slice by a Chebyshev series:
  *   H_k = om[k]·F + D + dadd[k]·N   (F: the plan's Ω flips, D: its diagonal,
  *                                    N = Σ_i n_i),
  *   psi_out = Σ_{j=0..K} a_kj T_j((H_k - c[k])/r[k]) psi_in,
  * coef = host FP64 [nslices][K+1][re, im] (a_kj, phase and Bessel factors
- * included).  work: 4 vectors of dim elements.  Replaces the per-interval
- * expm_multiply of engine_ed._propagate_segments / _propagate_sparse
- * (engine_ed.py:117-147). */
+ * included).  work: 4 vectors of dim elements.  Full basis with Ω/Δ/V and
+ * n <= 22: fused slice kernels, any om/dadd.  Every other plan (restricted
+ * bases: chain sector, code lists, blockade graphs; J/A models; full basis
+ * n > 22) evolves under its own H: om[k] = 1 and dadd[k] = 0, else
+ * RYD_ERR_UNSUPPORTED.  Replaces the per-interval expm_multiply of
+ * engine_ed._propagate_segments / _propagate_sparse (engine_ed.py:117-147),
+ * also on a projected plan's basis_indices (engine_ed.py:191-192, :288). */
 int ryd_evolve_forward(const ryd_plan* plan, int dtype, void* psi, int64_t nslices,
                        const double* om, const double* dadd, const double* c, const double* r,
                        int K, const double* coef, void* work, void* stream);

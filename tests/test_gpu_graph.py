@@ -206,3 +206,86 @@ def test_lattice_6x7_properties(pkg, dev):
     a = torch.dot(yv, P.hv(x)).item()
     b = torch.dot(P.hv(yv), x).item()
     assert abs(a - b) <= 1e-10 * abs(a)
+
+
+@pytest.mark.parametrize('doubling', [False, True])
+def test_kpm_on_graph_basis_vs_reference_estimator(pkg, dev, doubling):
+    """KPM on a projected plan (the reference compiles plan['states'] into the
+    CSR, eq_route.py:73-74): same probes, same moments as the oracle's
+    estimator on the projected H."""
+    n = 12
+    calib = _lattice_calib(4, 3, c6=30.0)
+    edges = pkg.blockade_edges(calib, {'kind': 'blockade', 'threshold_mhz': 10.0})
+    Om, delta = np.ones(n), -0.8 * np.ones(n)
+    P = pkg.Plan(pkg.from_calib(calib, Om, delta), basis='graph', edges=edges, device=dev)
+    codes = np.asarray(orc.independent_states(n, edges))
+    H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'], codes=codes)
+    emin, emax, c, r = orc.kpm_bounds(H, margin=0.01)
+    mref, _, _ = orc.kpm_moments(H, 40, 3, seed=5, c=c, r=r)
+    mu, _, _ = pkg.kpm_moments_hip(plan=P, num=40, nvec=3, seed=5, c=c, r=r, doubling=doubling)
+    assert np.max(np.abs(mu - mref)) < 1e-10
+    sysd = {'calib': calib, 'Om': Om, 'delta': delta, 'device': dev,
+            'plan': pkg.build_plan_hip(calib, Om, delta, constraint={'kind': 'blockade',
+                                                                      'threshold_mhz': 10.0},
+                                       device=dev)}
+    th = pkg.solve_equilibrium_hip(sysd, 'thermal', backend='hip-kpm', betas=[0.5], num=200,
+                                   nvec=40)
+    ev = np.linalg.eigvalsh(H.toarray())
+    w = np.exp(-0.5 * (ev - ev.min()))
+    e_exact = float(np.sum(ev * w) / np.sum(w))
+    assert abs(th['E'][0] - e_exact) < 0.05 * max(1.0, abs(e_exact))
+
+
+@pytest.mark.parametrize('how', ['edges', 'states', 'plan'])
+def test_segments_on_graph_basis_vs_exact(pkg, dev, how):
+    """Piecewise-constant evolution restricted to the independent sets of a
+    4x3 lattice (engine_ed.py:191-193, :288) against the oracle's exact
+    exponentials on the same projected H; psi0 = |g...g>, then a full 2^n
+    vector restricted to the basis."""
+    n = 12
+    calib = _lattice_calib(4, 3, c6=30.0)
+    cons = {'kind': 'blockade', 'threshold_mhz': 10.0}
+    edges = pkg.blockade_edges(calib, cons)
+    codes = np.asarray(orc.independent_states(n, edges))
+    rng = np.random.default_rng(2)
+    segs = [dict(t0=t0, Om=rng.uniform(0.5, 3.0, n), phase=rng.uniform(0, 2 * np.pi, n),
+                 delta=rng.uniform(-2, 2, n)) for t0 in (0.0, 60.0)]
+    t = np.array([0.0, 45.0, 120.0])
+    kw = {'edges': dict(blockade_edges=edges), 'states': dict(basis_indices=codes[::-1].copy()),
+          'plan': dict(plan=pkg.build_plan_hip(calib, constraint=cons, device=dev))}[how]
+    out = pkg.propagate_segments_hip(calib, segs, t, device=dev, return_states=True, **kw)
+    psi0 = np.zeros(len(codes), complex)
+    psi0[0] = 1.0
+    ref = orc.propagate_segments(n, calib['pairs'], ['I1'], segs, psi0, t, codes=codes)
+    got = out['states'][:, ::-1] if how == 'states' else out['states']
+    assert np.max(np.abs(got - ref)) < 1e-10
+    assert np.array_equal(np.sort(out['basis_states']), codes)
+    pops = np.array([orc.site_occupations(n, v, codes=codes) for v in ref])
+    assert np.allclose(out['pops_atom'], pops, atol=1e-10)
+    full = np.zeros(1 << n, complex)
+    full[codes[3]] = 2.0j
+    o2 = pkg.propagate_segments_hip(calib, segs, t[:2], psi0=full, blockade_edges=edges,
+                                    device=dev)
+    p2 = np.zeros(len(codes), complex)
+    p2[3] = 1.0j
+    r2 = orc.propagate_segments(n, calib['pairs'], ['I1'], segs, p2, t[:2], codes=codes)[-1]
+    assert np.max(np.abs(o2['psi'] - r2)) < 1e-10
+    bad = np.zeros(1 << n, complex)
+    bad[(1 << (n - 1)) | (1 << (n - 2))] = 1.0       # sites 0 and 1 both excited: blockaded
+    with pytest.raises(ValueError):
+        pkg.propagate_segments_hip(calib, segs, t[:2], psi0=bad, blockade_edges=edges, device=dev)
+
+
+def test_generic_evolution_matches_fused_full_basis(pkg, dev):
+    """The generic slice path (here a code-list plan holding all 2^n codes)
+    reproduces the fused full-basis slice kernels."""
+    n = 10
+    calib = _lattice_calib(5, 2, c6=15.0)
+    rng = np.random.default_rng(8)
+    segs = [dict(t0=t0, Om=rng.uniform(0.5, 2.0, n), phase=rng.uniform(0, 6.3, n),
+                 delta=rng.uniform(-1, 1, n)) for t0 in (0.0, 30.0)]
+    t = np.array([0.0, 50.0])
+    full = pkg.propagate_segments_hip(calib, segs, t, device=dev)
+    codes = np.arange(1 << n, dtype=np.int64)
+    gen = pkg.propagate_segments_hip(calib, segs, t, device=dev, basis_indices=codes)
+    assert np.max(np.abs(full['psi'] - gen['psi'])) < 1e-12
