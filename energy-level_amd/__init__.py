@@ -20,6 +20,8 @@ from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolu
 from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip
 from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip, \
     spectrum_lanczos_hip, quadrature, lorentz_spectrum
+from .observables import normalize_pauli_strings, PauliPlanHip, measure_pauli_strings_hip, \
+    corr_matrix_hip, site_correlation_sums
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
@@ -35,4 +37,6 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'quadrature', 'lorentz_spectrum', 'segment_intervals', 'segment_drive',
            'propagate_segments_hip', 'number_sector_basis', 'check_number_conserving',
            'scan_ground_sectors_hip', 'validate_edges', 'graph_plan', 'independent_states_hip',
-           'independent_set_count_hip', 'blockade_edges', 'build_plan_hip']
+           'independent_set_count_hip', 'blockade_edges', 'build_plan_hip',
+           'normalize_pauli_strings', 'PauliPlanHip', 'measure_pauli_strings_hip',
+           'corr_matrix_hip', 'site_correlation_sums']

@@ -62,6 +62,10 @@ SIGNATURES = {
     'ryd_cheb_moments': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _dbl,
                                         _dbl, _vp, _vp, _vp]),
     'ryd_site_occupation': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    'ryd_pauli_expect': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                        _vp, _vp]),
+    'ryd_site_correlations': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_size_t,
+                                             _vp]),
     'ryd_sector_codes': (ctypes.c_int, [_vp, _vp, _i64, _i64, _vp, _vp]),
     'ryd_sector_rank': (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),
     'ryd_evolve_forward': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _i64, _vp, _vp, _vp, _vp,

@@ -55,12 +55,15 @@ struct ChanModel {
   double ar[MAXN], ai[MAXN];   // Ω/2 by bit position (0: no Ω channel)
 };
 
-__device__ __forceinline__ uint64_t chan_code(const ChanModel& M, int64_t k) {
+// code and rank helpers take any struct with the rank-table fields of ChanModel
+template <class M_>
+__device__ __forceinline__ uint64_t chan_code(const M_& M, int64_t k) {
   return M.mode == RANK_FULL ? (uint64_t)k : (uint64_t)M.codes[k];
 }
 
 // rank of code s in the basis, -1 when s is not a basis code
-__device__ inline int64_t chan_rank(const ChanModel& M, uint64_t s) {
+template <class M_>
+__device__ inline int64_t chan_rank(const M_& M, uint64_t s) {
   if (M.n < 64 && (s >> M.n)) return -1;
   if (M.mode == RANK_FULL) return (int64_t)s;
   if (M.mode == RANK_GRAPH) {

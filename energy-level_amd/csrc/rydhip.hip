@@ -15,6 +15,7 @@
 #include "stream_ops.hpp"
 #include "channels.hpp"
 #include "graph_apply.hpp"
+#include "measure.hpp"
 #include "observables.hpp"
 #include "sector.hpp"
 #include "evolve.hpp"
@@ -1270,6 +1271,134 @@ extern "C" int ryd_site_occupation(const ryd_plan* P, int dtype, const void* x, 
     case RYD_F64: return do_occ<double>(P, x, out, ws, st);
     case RYD_C64: return do_occ<c32>(P, x, out, ws, st);
     case RYD_C128: return do_occ<c64>(P, x, out, ws, st);
+    default: return fail(RYD_ERR_INVALID, "unknown dtype");
+  }
+}
+
+namespace {
+
+ObsBasis obs_basis(const ryd_plan* P) {
+  ObsBasis B;
+  std::memset(&B, 0, sizeof(B));
+  B.n = P->n;
+  B.dim = P->dim;
+  if (P->basis == RYD_BASIS_CHAIN_FIB) {
+    B.kind = 2;
+    for (int p = 0; p < MAXN; ++p) B.W[p] = fib(p + 2);
+  } else if (P->generic) {
+    B.kind = 1;
+    B.mode = P->rank_mode;
+    B.m = P->popcount;
+    B.gl = P->gr_L;
+    B.tl = P->gr_tl;
+    B.codes = P->d_codes;
+    B.binom = P->d_binom;
+    B.hpk = P->d_hpk;
+    B.lo_idx = P->d_lo_idx;
+    B.fc = P->d_fc;
+    B.pre = P->d_pre;
+  } else {
+    B.kind = 0;
+  }
+  return B;
+}
+
+template <class V>
+int do_pauli(const ryd_plan* P, const void* x, int count, const uint64_t* flip,
+             const uint64_t* sign, const int32_t* ny, double* out, void* ws, hipStream_t st) {
+  const ObsBasis B = obs_basis(P);
+  int64_t g0 = (P->dim + 4095) / 4096;
+  const int64_t cap = (int64_t)((P->ws_bytes - WS_HEADER) / (2 * PAULI_BATCH * sizeof(double))) - 1;
+  if (g0 > cap) g0 = cap;
+  if (g0 > 2048) g0 = 2048;
+  if (g0 < 1) return fail(RYD_ERR_INVALID, "workspace too small");
+  const unsigned g = (unsigned)g0;
+  double* part = reinterpret_cast<double*>(static_cast<char*>(ws) + WS_HEADER);
+  for (int q0 = 0; q0 < count; q0 += PAULI_BATCH) {
+    PauliBatch S;
+    S.count = count - q0 < PAULI_BATCH ? count - q0 : PAULI_BATCH;
+    for (int q = 0; q < PAULI_BATCH; ++q) {
+      S.flip[q] = q < S.count ? flip[q0 + q] : 0;
+      S.sign[q] = q < S.count ? sign[q0 + q] : 0;
+    }
+    k_pauli<V><<<g, 256, 0, st>>>(B, S, static_cast<const V*>(x), part);
+    LAUNCH_CHECK();
+    // (re, im) sums, then the i^{ny} factor on the host side of the copy
+    k_sum_columns<<<1, 256, 0, st>>>(part, (int)g, 2 * PAULI_BATCH, part + (size_t)g * 2 * PAULI_BATCH);
+    LAUNCH_CHECK();
+    double h[2 * PAULI_BATCH];
+    HIP_TRY(hipMemcpyAsync(h, part + (size_t)g * 2 * PAULI_BATCH, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double o[2 * PAULI_BATCH];
+    for (int q = 0; q < S.count; ++q) {
+      double re = h[2 * q], im = h[2 * q + 1];
+      for (int k = 0; k < (ny[q0 + q] & 3); ++k) {   // multiply by i
+        const double t = re;
+        re = -im;
+        im = t;
+      }
+      o[2 * q] = re;
+      o[2 * q + 1] = im;
+    }
+    HIP_TRY(hipMemcpyAsync(out + 2 * q0, o, sizeof(double) * 2 * S.count, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return RYD_OK;
+}
+
+template <class V>
+int do_corr(const ryd_plan* P, const void* x, double* out, void* scratch, size_t bytes,
+            hipStream_t st) {
+  const ObsBasis B = obs_basis(P);
+  const int n = P->n;
+  const int width = n * (n + 1) / 2 + 1;
+  int64_t g0 = (P->dim + 4095) / 4096;
+  const int64_t cap = (int64_t)(bytes / (sizeof(double) * (size_t)width)) - 1;   // + the sums
+  if (g0 > cap) g0 = cap;
+  if (g0 > 512) g0 = 512;
+  if (g0 < 1) return fail(RYD_ERR_INVALID, "scratch too small for the correlation sums");
+  const unsigned g = (unsigned)g0;
+  double* part = static_cast<double*>(scratch);
+  double* sums = part + (size_t)g * width;
+  k_corr<V><<<g, 256, 0, st>>>(B, static_cast<const V*>(x), part);
+  LAUNCH_CHECK();
+  k_sum_columns<<<(width + 255) / 256, 256, 0, st>>>(part, (int)g, width, sums);
+  LAUNCH_CHECK();
+  k_corr_square<<<(n * n + 256) / 256, 256, 0, st>>>(sums, n, out);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+}  // namespace
+
+extern "C" int ryd_pauli_expect(const ryd_plan* P, int dtype, const void* x, int count,
+                                const uint64_t* flip, const uint64_t* sign, const int32_t* ny,
+                                double* out, void* ws, void* stream) {
+  if (!P || !x || !out || !ws || count < 0 || (count > 0 && (!flip || !sign || !ny)))
+    return fail(RYD_ERR_INVALID, "bad argument to ryd_pauli_expect");
+  for (int q = 0; q < count; ++q)
+    if ((P->n < 64 && ((flip[q] | sign[q]) >> P->n)) || ny[q] < 0)
+      return fail(RYD_ERR_INVALID, "Pauli string outside the plan's sites");
+  if (count == 0) return RYD_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case RYD_F32: return do_pauli<float>(P, x, count, flip, sign, ny, out, ws, st);
+    case RYD_F64: return do_pauli<double>(P, x, count, flip, sign, ny, out, ws, st);
+    case RYD_C64: return do_pauli<c32>(P, x, count, flip, sign, ny, out, ws, st);
+    case RYD_C128: return do_pauli<c64>(P, x, count, flip, sign, ny, out, ws, st);
+    default: return fail(RYD_ERR_INVALID, "unknown dtype");
+  }
+}
+
+extern "C" int ryd_site_correlations(const ryd_plan* P, int dtype, const void* x, double* out,
+                                     void* scratch, size_t scratch_bytes, void* stream) {
+  if (!P || !x || !out || !scratch) return fail(RYD_ERR_INVALID, "bad argument to ryd_site_correlations");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case RYD_F32: return do_corr<float>(P, x, out, scratch, scratch_bytes, st);
+    case RYD_F64: return do_corr<double>(P, x, out, scratch, scratch_bytes, st);
+    case RYD_C64: return do_corr<c32>(P, x, out, scratch, scratch_bytes, st);
+    case RYD_C128: return do_corr<c64>(P, x, out, scratch, scratch_bytes, st);
     default: return fail(RYD_ERR_INVALID, "unknown dtype");
   }
 }

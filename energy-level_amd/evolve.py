@@ -300,7 +300,7 @@ def segment_drive(t, segments, n, interp='left'):
 def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='left',
                            dtype='complex128', device='cuda', return_states=False,
                            basis_probabilities_max_dim=4096, basis_indices=None,
-                           blockade_edges=None, plan=None):
+                           blockade_edges=None, plan=None, pauli_strings=None, corr=False):
     """Exact piecewise-constant evolution, output at the times ``t`` (ns):
     every output interval is split at the segment starts and each piece
     [a, b] is propagated by exp(−i·CONV·H·(b−a)) with the drive of
@@ -318,9 +318,14 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
     restricted to the basis and normalised by its full norm (weight outside
     the basis is an error, as state.py:112-140).
 
+    ``pauli_strings`` (specs as ``normalize_pauli_strings``) and ``corr``
+    add the per-output-time reductions of engine_ed.py:259-281 on the device:
+    ``pauli`` (len(t), strings) complex and ``corr`` (len(t), n, n).
+
     Returns {t, pops_atom (len(t), n), norms (len(t),), psi (final state,
     host complex128, basis order), method='manybody', backend='hip-chebyshev',
-    basis_states (codes, projected runs only)[, states][, basis_probabilities]}."""
+    basis_states (codes, projected runs only)[, states][, basis_probabilities]
+    [, pauli, pauli_labels][, corr]}."""
     import torch
     from .lanczos import site_occupations, restricted_plan
     from .basis import sector_codes
@@ -340,9 +345,12 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
     projected = basis_indices is not None or blockade_edges is not None
     tdt = torch.complex64 if dtype == 'complex64' else torch.complex128
     dv = torch.device(device)
+    from .observables import normalize_pauli_strings, PauliPlanHip, corr_matrix_hip
+    specs = normalize_pauli_strings(pauli_strings, n) if pauli_strings is not None else None
     plans = {}
-    pops, norms, states, probs = [], [], [], []
+    pops, norms, states, probs, pauli_series, corr_series = [], [], [], [], [], []
     order = None            # plan order -> caller order of basis_indices
+    pauli_plan = None
 
     def plan_for(om, de):
         nonlocal order
@@ -357,8 +365,15 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
         return plans[key]
 
     def record(p, x):
+        nonlocal pauli_plan
         norms.append(float(torch.linalg.vector_norm(x.to(torch.complex128)).item()))
         pops.append(np.asarray(site_occupations(p, x)))
+        if specs:
+            if pauli_plan is None:
+                pauli_plan = PauliPlanHip(p, specs)
+            pauli_series.append(pauli_plan.expect(x))
+        if corr:
+            corr_series.append(corr_matrix_hip(p, x)[0])
         if return_states or (x.numel() <= basis_probabilities_max_dim):
             v = x.to(torch.complex128).cpu().numpy()
             if order is not None:
@@ -420,6 +435,11 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
             out['basis_states'] = (np.asarray(basis_indices.cpu() if hasattr(basis_indices, 'cpu')
                                               else basis_indices, np.int64)
                                    if basis_indices is not None else codes)
+        if specs is not None:
+            out['pauli'] = np.asarray(pauli_series, complex).reshape(len(t), len(specs))
+            out['pauli_labels'] = [s_['label'] for s_ in specs]
+        if corr:
+            out['corr'] = np.asarray(corr_series)
         if return_states:
             out['states'] = np.asarray(states)
         if probs:

@@ -175,6 +175,24 @@ int ryd_cheb_moments(const ryd_plan* plan, int dtype, const uint32_t* probe, voi
 int ryd_site_occupation(const ryd_plan* plan, int dtype, const void* x, double* out, void* ws,
                         void* stream);
 
+/* Pauli-string expectations <x|P_q|x> for q < count on the plan's basis
+ * (any kind).  String q: flip[q] = X|Y sites, sign[q] = Y|Z sites, as bit
+ * masks by bit position (site i = bit n-1-i), ny[q] = number of Y sites;
+ * out (device FP64, [count][2] = re, im).  The basis path of
+ * PauliPlan.expect (observables.py:564-622): targets outside the basis
+ * contribute nothing.  ws: the plan's workspace (ryd_workspace_bytes). */
+int ryd_pauli_expect(const ryd_plan* plan, int dtype, const void* x, int count,
+                     const uint64_t* flip, const uint64_t* sign, const int32_t* ny,
+                     double* out, void* ws, void* stream);
+
+/* Site correlation sums on the plan's basis: out[i*n + j] = Σ_s |x_s|^2
+ * b_i(s) b_j(s) (both triangles), out[n*n] = Σ_s |x_s|^2 (device FP64,
+ * n*n + 1 entries); C_ij = out[i,j]/out[n*n] - <n_i><n_j> as corr_matrix
+ * (observables.py:40-47).  scratch: device buffer of scratch_bytes >=
+ * (n(n+1)/2 + 1)·8 bytes (more bytes: more blocks, up to 512). */
+int ryd_site_correlations(const ryd_plan* plan, int dtype, const void* x, double* out,
+                          void* scratch, size_t scratch_bytes, void* stream);
+
 /* Restricted-basis enumeration (chain sector, codes and blockade-graph
  * plans): codes[k] = code of rank ranks[k] (or of rank first + k when ranks
  * is NULL); ranks outside [0, dim) give -1.  Codes come out in the

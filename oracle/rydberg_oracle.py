@@ -594,3 +594,51 @@ def from_positions(positions, coupling):
                 V, J, A = 0.0, c, 0.0
             pairs.append({'i': i, 'j': j, 'V': float(V), 'J': float(J), 'A': float(A)})
     return pairs
+
+
+# ---------------------------------------------------------------------------
+# Pauli strings  (reference observables.py:461-622, PauliPlan)
+# ---------------------------------------------------------------------------
+PAULI_1 = {'x': np.array([[0, 1], [1, 0]], complex),
+           'y': np.array([[0, -1j], [1j, 0]], complex),      # rows/cols (g, r): <r|y|g> = +i
+           'z': np.array([[1, 0], [0, -1]], complex)}        # |r> has -1
+
+
+def pauli_expect(n, vec, terms, codes=None):
+    """<v|P|v> for P = Π σ^{axis}_{site} on the full basis or a code list:
+    the basis path of PauliPlan.expect (observables.py:564-622) — every
+    source state s maps to t = s ^ (X|Y) with the source-bit phase
+    (σz: −1 on |r>, σy: −i from |r>, +i from |g>); targets outside the basis
+    contribute nothing."""
+    v = np.asarray(vec, complex).reshape(-1)
+    codes = np.arange(1 << n, dtype=np.int64) if codes is None else np.asarray(codes, np.int64)
+    pos = {int(c): k for k, c in enumerate(codes)}
+    total = 0j
+    for k, c in enumerate(codes):
+        c = int(c)
+        t, ph = c, 1 + 0j
+        for site, axis in terms:
+            b = (c >> (n - 1 - site)) & 1
+            if axis == 'z':
+                ph *= -1 if b else 1
+            elif axis == 'y':
+                ph *= -1j if b else 1j
+            if axis != 'z':
+                t ^= 1 << (n - 1 - site)
+        r = pos.get(t)
+        if r is not None:
+            total += np.conj(v[r]) * ph * v[k]
+    return total
+
+
+def pauli_dense(n, terms):
+    """The same operator as a dense 2^n matrix by Kronecker products
+    (site 0 = most significant factor) — an independent construction."""
+    ops = [np.eye(2, dtype=complex)] * n
+    ops = list(ops)
+    for site, axis in terms:
+        ops[site] = PAULI_1[axis]
+    M = np.ones((1, 1), complex)
+    for o in ops:
+        M = np.kron(M, o)
+    return M

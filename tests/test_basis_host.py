@@ -133,3 +133,15 @@ def test_validate_edges_and_plan_edges_mirror_reference(pkg):
         pkg.blockade_edges(calib, {'kind': 'blockade', 'edges': 'all'})
     with pytest.raises(ValueError):
         pkg.build_plan_hip(calib, constraint={'kind': 'ring'})
+
+
+def test_normalize_pauli_strings_mirrors_reference(pkg):
+    out = pkg.normalize_pauli_strings([[[0, 'x'], [2, 'σy']], {'label': 'zz', 'terms': [[1, 'Z'], [3, 'sz']]}], 4)
+    assert out == [{'label': 'x@0 y@2', 'terms': [(0, 'x'), (2, 'y')]},
+                   {'label': 'zz', 'terms': [(1, 'z'), (3, 'z')]}]
+    assert pkg.normalize_pauli_strings([]) == []
+    assert pkg.normalize_pauli_strings(None) is None
+    for bad in ([[]], [[[0, 'x'], [0, 'z']]], [[[5, 'x']]], [[[0, 'w']]], [[[True, 'x']]],
+                [{'label': 'a'}], 'x@0'):
+        with pytest.raises(ValueError):
+            pkg.normalize_pauli_strings(bad, 4)

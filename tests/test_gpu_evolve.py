@@ -96,7 +96,7 @@ def test_adjoint_vs_dense_expm_autograd(pkg, dev):
         psi = torch.linalg.matrix_exp(-1j * pkg.CONV * dt * H) @ psi
     loss = 1 - torch.abs(psi[tgt]) ** 2
     loss.backward()
-    assert abs(float(loss) - L) < 1e-12
+    assert abs(float(loss.detach()) - L) < 1e-12
     assert np.max(np.abs(om_t.grad.numpy() - gO)) < 1e-10 * max(1, np.max(np.abs(gO)))
     assert np.max(np.abs(de_t.grad.numpy() - gD)) < 1e-10 * max(1, np.max(np.abs(gD)))
 

@@ -198,3 +198,41 @@ def test_c_oracle_matches_numpy(n):
     xs = rng.normal(size=len(codes))
     refs = orc.channel_matvec(n, Om, delta, pairs, ['I1'], xs, codes)
     assert np.max(np.abs(hv_ref.hv_fib(n, Om, ds, codes, xs) - refs)) < 1e-12 * np.max(np.abs(refs))
+
+
+def test_pauli_expect_vs_kronecker_and_projection():
+    """The oracle's Pauli-string expectation (PauliPlan basis path) equals the
+    dense Kronecker operator on the full basis, and on a code list equals the
+    dense operator applied to the state embedded with zeros outside the list."""
+    rng = np.random.default_rng(4)
+    for n in (1, 3, 5, 7):
+        v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        codes = np.sort(rng.choice(1 << n, max(1, (1 << n) // 3), replace=False))
+        w = rng.normal(size=len(codes)) + 1j * rng.normal(size=len(codes))
+        emb = np.zeros(1 << n, complex)
+        emb[codes] = w
+        for _ in range(6):
+            k = int(rng.integers(1, n + 1))
+            sites = rng.choice(n, k, replace=False)
+            terms = [(int(s), str(rng.choice(['x', 'y', 'z']))) for s in sites]
+            M = orc.pauli_dense(n, terms)
+            assert abs(orc.pauli_expect(n, v, terms) - np.vdot(v, M @ v)) < 1e-10 * np.vdot(v, v).real
+            assert abs(orc.pauli_expect(n, w, terms, codes) - np.vdot(emb, M @ emb)) < 1e-10 * np.vdot(w, w).real
+    # single-qubit known answers: <g|z|g> = 1, <r|z|r> = -1, <+|x|+> = 1, y on (|g> + i|r>)/sqrt2 = 1
+    s2 = 1 / np.sqrt(2)
+    assert orc.pauli_expect(1, [1, 0], [(0, 'z')]) == 1
+    assert orc.pauli_expect(1, [0, 1], [(0, 'z')]) == -1
+    assert abs(orc.pauli_expect(1, [s2, s2], [(0, 'x')]) - 1) < 1e-15
+    assert abs(orc.pauli_expect(1, [s2, 1j * s2], [(0, 'y')]) - 1) < 1e-15
+
+
+def test_corr_matrix_oracle_known_answers():
+    """C_ij of a product state is zero; of (|gg> + |rr>)/sqrt2 it is 1/4."""
+    n = 3
+    v = np.zeros(8, complex)
+    v[0b101] = 1.0
+    assert np.allclose(orc.corr_matrix(n, v), 0.0)
+    v = np.zeros(4, complex)
+    v[0] = v[3] = 1 / np.sqrt(2)
+    C = orc.corr_matrix(2, v)
+    assert np.allclose(C, [[0.25, 0.25], [0.25, 0.25]])
