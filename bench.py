@@ -28,9 +28,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes of one FP64 H.v at N=28 (both launches), measured with rocprofv3
-# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 6.39 + 2.27 GB,
+# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 9.20 + 2.28 GB,
 # pass B 4.30 + 2.15 GB
-MEASURED_TRAFFIC_F64_N28 = 15.11e9
+MEASURED_TRAFFIC_F64_N28 = 17.92e9
 N_SITES = 28
 
 
@@ -100,7 +100,9 @@ def _run_configs(dev, which):
     """The five configurations (summaries only; big arrays dropped)."""
     from energy_level_amd import workloads as W
     out = {}
+    import torch
     for name in which:
+        torch.cuda.empty_cache()          # each workload starts from the same free memory
         t0 = time.perf_counter()
         r = getattr(W, 'run_' + name)(dev)
         for k in ('calib', 'mu', 'E0_batched', 'ns'):

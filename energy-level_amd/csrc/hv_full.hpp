@@ -238,15 +238,33 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
 
 // Pass B of the split H·v for large n: the top TOPB bits (n-TOPB .. n-1).
 // A block holds every combination of those bits for 2^CB consecutive codes
-// (one 128-byte run per combination), so each top-bit flip is an LDS or
-// register exchange.  It reads the pass-A partial sum from y and finishes
+// (one 128- or 256-byte run per combination), so each top-bit flip is an LDS
+// or register exchange.  It reads the pass-A partial sum from y and finishes
 //   y = ch·(y_A + Σ_top flips of s·x) + cx·(s·x) + cz·z  with the reductions.
 // Thread u (0..511): c = u & (2^CB-1) selects the code inside the run, u >> CB
 // gives the low (9-CB) top bits, register slot j the remaining top bits.
-constexpr int TOPB = 9;
+// Measured on MI355X at n = 28 (FP64 / FP32 ms per H·v): 9 top bits with
+// 128-byte runs 3.01 / 1.72; 8 bits with 256-byte FP64 and 128-byte FP32 runs
+// 2.95 / 1.52; 7 or 6 top bits lose in FP64 (pass A's far partners leave L2).
+#ifndef RYD_TOPB
+#define RYD_TOPB 8
+#endif
+#ifndef RYD_RUNB4
+#define RYD_RUNB4 5
+#endif
+#ifndef RYD_RUNB8
+#define RYD_RUNB8 5
+#endif
+#ifndef RYD_RUNB16
+#define RYD_RUNB16 3
+#endif
+constexpr int TOPB = RYD_TOPB;
 template <class V> constexpr int run_bits() {
-  return sizeof(V) == 4 ? 5 : (sizeof(V) == 16 ? 3 : 4);
+  return sizeof(V) == 4 ? RYD_RUNB4 : (sizeof(V) == 16 ? RYD_RUNB16 : RYD_RUNB8);
 }
+static_assert(TOPB >= 1 && TOPB <= 9, "top pass: 1..9 bits");
+static_assert(9 - RYD_RUNB4 <= TOPB && 9 - RYD_RUNB8 <= TOPB && 9 - RYD_RUNB16 <= TOPB,
+              "top pass: the thread index holds at most TOPB top bits");
 
 template <class V>
 __global__ __launch_bounds__(512) void k_full_top(FullModel M, ApplyIO<V> io, Reduce rd) {

@@ -93,9 +93,12 @@ RITZ_ROWS = 2048            # coefficients per ryd_ritz_stored call
 
 
 def _fits(plan, nbytes, dt):
-    """True when ``nbytes`` more device memory leave the working vectors room."""
+    """True when ``nbytes`` more device memory leave the working vectors room
+    (memory cached by torch's allocator but unused counts as free: the
+    allocator releases it before failing)."""
     import torch
     free, _ = torch.cuda.mem_get_info(plan.device)
+    free += torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
     itemsize = torch.tensor([], dtype=dt).element_size()
     reserve = 8 * plan.dim * itemsize + (4 << 30)
     return nbytes + reserve <= free
@@ -309,6 +312,7 @@ def _solve(P, start, m, cycles, tol, want, stats, key, dtype64):
     for attempt in range(cycles):
         R = lanczos_run(P, start, m, cycles, tol, want=want)
         vecs = ritz_vectors(P, R)
+        R.chunks, R.bufs = [], None          # the stored basis is not needed past here
         stats[key] += R.j
         stats['restarts'] += R.cycles - 1 + (1 if attempt else 0)
         evals, res = _certify(P, vecs, R, dtype64)
