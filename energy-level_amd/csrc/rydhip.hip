@@ -57,9 +57,11 @@ struct ryd_plan {
   bool cplx = false;                 // Ω has an imaginary part
   double ar[MAXN] = {0}, ai[MAXN] = {0};   // per bit position, already halved
   double* d_vb = nullptr;            // [n*n] V by bit position
-  double* d_elo[2] = {nullptr, nullptr};   // tables for tb = tbits[k]
-  double* d_ehi[2] = {nullptr, nullptr};
-  int tbits[2] = {0, 0};
+  // diagonal tables per tile width tb = 10 + k (built for the widths the
+  // element types use; tbits[k] = min(n, 10 + k))
+  double* d_elo[4] = {nullptr, nullptr, nullptr, nullptr};
+  double* d_ehi[4] = {nullptr, nullptr, nullptr, nullptr};
+  int tbits[4] = {0, 0, 0, 0};
   size_t ws_bytes = 0;
   // chain-blockade sector
   uint32_t* d_lo_code = nullptr;     // [F(L+2)] low-part codes
@@ -101,8 +103,16 @@ struct ryd_plan {
 
 namespace {
 
+int tile_bits_of(int dtype) {
+  return dtype == RYD_F32 ? tile_bits<float>()
+         : dtype == RYD_F64 ? tile_bits<double>()
+         : dtype == RYD_C64 ? tile_bits<c32>()
+                            : tile_bits<c64>();
+}
+
 int tb_index(const ryd_plan* P, int dtype) {
-  return (dtype == RYD_C128) ? 1 : 0;   // 16-byte elements use 11-bit tiles
+  (void)P;
+  return tile_bits_of(dtype) - 10;
 }
 
 FullModel full_model(const ryd_plan* P, int dtype) {
@@ -350,7 +360,7 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
 }
 
 unsigned apply_grid(const ryd_plan* P, int dtype) {
-  const int TB = (dtype == RYD_C128) ? 11 : 12;
+  const int TB = tile_bits_of(dtype);
   if (P->n >= TB) return (unsigned)(1ull << (P->n - TB));
   return (unsigned)(((1ull << P->n) + 255) / 256);
 }
@@ -999,9 +1009,10 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
     *out = P;
     return RYD_OK;
   }
-  const int tbs[2] = {12, 11};
-  for (int k = 0; k < 2; ++k) {
-    const int tb = n < tbs[k] ? n : tbs[k];
+  for (int dt = RYD_F32; dt <= RYD_C128; ++dt) {
+    const int k = tb_index(P, dt);
+    if (P->d_elo[k]) continue;
+    const int tb = n < 10 + k ? n : 10 + k;
     P->tbits[k] = tb;
     std::vector<double> elo, ehi;
     build_tables(n, tb, dl, vb, elo, ehi);
@@ -1049,7 +1060,7 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_rcode) (void)hipFree(P->d_rcode);
   if (P->d_rdiag) (void)hipFree(P->d_rdiag);
   if (P->d_lrow) (void)hipFree(P->d_lrow);
-  for (int k = 0; k < 2; ++k) {
+  for (int k = 0; k < 4; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
   }
