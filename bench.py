@@ -100,9 +100,7 @@ def _run_configs(dev, which):
     """The five configurations (summaries only; big arrays dropped)."""
     from energy_level_amd import workloads as W
     out = {}
-    import torch
     for name in which:
-        torch.cuda.empty_cache()          # each workload starts from the same free memory
         t0 = time.perf_counter()
         r = getattr(W, 'run_' + name)(dev)
         for k in ('calib', 'mu', 'E0_batched', 'ns'):

@@ -194,7 +194,7 @@ def spectrum_lanczos_hip(plan, phi, n_krylov=200, tiny=1e-12):
     if k < 1:
         raise ValueError('n_krylov must be positive')
     k = min(k, plan.dim)
-    R = lanczos_run(plan, v, k, 1, 0.0, want=1, tiny=tiny * norm)
+    R = lanczos_run(plan, v, k, 1, 0.0, want=1, tiny=tiny * norm, store=False)
     j = int(R.j)
     alphas = np.asarray(R.a[:j], float)
     betas = np.asarray(R.b[:max(j - 1, 0)], float)

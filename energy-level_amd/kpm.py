@@ -48,7 +48,7 @@ def spectral_bounds(plan, m=120, seed=0, tol=1e-8, margin=1e-3):
     """(emin, emax, c, r) from one FP64 device Lanczos run (both ends converge)."""
     import torch
     v0 = _start_vector(plan.dim, seed, not plan.is_real, plan.device)
-    R = lanczos_run(plan, v0, m, 8, tol, want=1)
+    R = lanczos_run(plan, v0, m, 8, tol, want=1, store=False)    # no Ritz vectors needed
     # Ritz values lie inside the spectrum; widen each end by its residual
     # bound |beta_last · y[-1]| so that [emin, emax] covers the spectrum
     lo, hi = int(np.argmin(R.theta)), int(np.argmax(R.theta))

@@ -7,17 +7,19 @@ arguments, same result keys, backend ``'hip-matrixfree-lanczos'``.
 Algorithm (not the reference's block Krylov; only converged outputs are
 compared):
 
-* the three-term recurrence runs on the device in cycles of ``m`` steps with
-  FP64 alpha/beta and no host synchronisation inside a cycle
-  (``ryd_lanczos``); the recurrence is continued, not restarted, between
-  cycles, up to ``m·(max_restarts+1)`` steps;
+* the three-term recurrence runs on the device with FP64 alpha/beta and no
+  host synchronisation between convergence checks (every 20 steps,
+  ``ryd_lanczos``); the recurrence is continued, not restarted, up to
+  ``m·(max_restarts+1)`` steps;
 * after each cycle the FP64 tridiagonal matrix is diagonalised on the host;
   duplicate and Cullum–Willoughby spurious Ritz values are dropped; the two
   lowest genuine ones are tested with the bound |beta_k · y_k[-1]|;
-* for dim >= 2^22 the Krylov basis stays in HBM (one chunk per cycle) and a
-  Ritz vector is one streaming combination of it; below that, or when free
-  memory runs short, the recurrence keeps three vectors and Ritz vectors
-  come from a replay with the stored coefficients;
+* the Krylov basis can stay in HBM (blocks of rows; automatically for
+  dim >= 2^22 when torch's allocator holds unused blocks, e.g. from a previous
+  solve) and a Ritz vector is then one streaming combination of it;
+  otherwise the recurrence keeps three vectors and Ritz vectors come from a
+  replay with the stored coefficients (only the unstored tail when storage
+  stopped partway);
 * FP32 bulk (``dtype='float32'``/``'complex64'``, ``refine=True``): the FP32
   Ritz pair seeds a short FP64 recurrence, so E0 and the residuals are
   certified in FP64 (``precision`` in the result says which parts ran in
@@ -90,33 +92,52 @@ class _Run:
 
 STORE_MIN_DIM = 1 << 22     # below this the replay of a cycle costs less than a stored basis
 RITZ_ROWS = 2048            # coefficients per ryd_ritz_stored call
+CHUNK_BYTES = 32 << 30      # stored-basis block: one size for every element type, so torch's
+                            # cache reuses the blocks of an FP32 run for the FP64 refinement
 
 
-def _fits(plan, nbytes, dt):
+CHECK_STEPS = 20            # steps between convergence checks (the budget stays m·cycles)
+
+
+def _fits(plan, nbytes, dt, cached_only=False):
     """True when ``nbytes`` more device memory leave the working vectors room
-    (memory cached by torch's allocator but unused counts as free: the
-    allocator releases it before failing)."""
+    (blocks cached but unused by torch's allocator count as free).
+    ``cached_only``: the bytes must come from those cached blocks.  The
+    automatic mode stores only there: on MI355X a run that allocated
+    hundreds of GB of new blocks (130 GB requests after releasing ~200 GB
+    took 1.9-3.2 s each) lost more than the replay of its Ritz vectors
+    costs, while blocks held in torch's cache come for free."""
     import torch
-    free, _ = torch.cuda.mem_get_info(plan.device)
-    free += torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
     itemsize = torch.tensor([], dtype=dt).element_size()
-    reserve = 8 * plan.dim * itemsize + (4 << 30)
-    return nbytes + reserve <= free
+    cached = torch.cuda.memory_reserved(plan.device) - torch.cuda.memory_allocated(plan.device)
+    if cached_only and nbytes > cached:
+        return False
+    free, _ = torch.cuda.mem_get_info(plan.device)
+    return nbytes + 8 * plan.dim * itemsize + (4 << 30) <= free + cached
 
 
 def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
     """Cycles of ``m`` device-side Lanczos steps from ``v0`` until the ``want``
     lowest genuine Ritz values pass the bound test or the budget is spent.
 
-    ``store``: keep every Lanczos vector in HBM (one chunk per cycle, while
-    free memory allows) so that Ritz vectors are one streaming combination
-    instead of a replay of the recurrence; 'auto' stores for dim >= 2^22."""
+    ``store``: keep every Lanczos vector in HBM (chunks of rows: a whole cycle
+    when it is small, else fixed 32 GiB blocks whose spare rows the next cycle
+    uses) so that Ritz vectors are one streaming combination instead of a
+    replay of the recurrence.  True: allocate as memory allows; 'auto'
+    (dim >= 2^22): only into blocks torch's allocator already holds unused
+    (a repeated solve reuses the blocks of the previous one); False: three
+    rotating vectors.  When storage stops, the rest of the run continues with
+    three vectors and only that tail is replayed.
+
+    Convergence is tested every ``CHECK_STEPS`` steps (at most ``m``); the
+    step budget is ``m·max_cycles``."""
     import ctypes
     import torch
     L = nat.lib()
     dim = plan.dim
     dt = v0.dtype
     itemsize = torch.tensor([], dtype=dt).element_size()
+    row_bytes = dim * itemsize
     kmax = int(min(max(1, m) * max(1, max_cycles), dim))
     R = _Run()
     R.v0 = v0
@@ -125,7 +146,10 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
     R.state = torch.zeros(int(L.ryd_lanczos_state_bytes()), dtype=torch.uint8, device=plan.device)
     R.chunks = []                 # (jbase, rows tensor) of a stored basis
     R.bufs = None
+    R.split = None                # first step run without storage (None: all stored)
     storing = (dim >= STORE_MIN_DIM) if store == 'auto' else bool(store)
+    cached_only = store == 'auto'
+    check = max(1, min(int(m), CHECK_STEPS))
     code = nat.dtype_code(dt)
     ws = plan.workspace()
     st = nat.stream_ptr()
@@ -133,7 +157,8 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
         tiny = 1e-13 if dt in (torch.float64, torch.complex128) else 1e-6
 
     def three_buffers(j):
-        """Switch to (or start) the rotating three-vector recurrence at step j."""
+        """Switch to (or start) the rotating three-vector recurrence at step j;
+        the stored chunks stay (they hold u_0 .. u_j for the Ritz vectors)."""
         if R.bufs is None:
             R.bufs = [torch.empty(dim, dtype=dt, device=plan.device) for _ in range(3)]
         if R.chunks:
@@ -141,34 +166,65 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
             R.bufs[j % 3].copy_(rows[j - jb])
             if j > 0:
                 R.bufs[(j - 1) % 3].copy_(rows[j - 1 - jb])
-            R.chunks = []
         elif j == 0:
             R.bufs[0].copy_(v0)
+
+    def new_chunk(j, want_rows):
+        """A chunk starting at row jb = j-1 (0 at j = 0) holding u_{jb}, u_j;
+        None when memory does not allow it."""
+        jb = j - 1 if j > 0 else 0
+        nbytes = want_rows * row_bytes
+        if nbytes > CHUNK_BYTES:             # one block size for every element type
+            nbytes = CHUNK_BYTES if 3 * row_bytes <= CHUNK_BYTES else 3 * row_bytes
+        if not _fits(plan, nbytes, dt, cached_only):
+            return None
+        try:
+            buf = torch.empty(nbytes, dtype=torch.uint8, device=plan.device)
+        except torch.OutOfMemoryError:
+            return None
+        cap = (nbytes // row_bytes)
+        rows = buf[:cap * row_bytes].view(dt).view(cap, dim)
+        if j == 0:
+            rows[0].copy_(v0)
+        else:
+            pjb, prev = R.chunks[-1]
+            rows[0].copy_(prev[j - 1 - pjb])
+            rows[1].copy_(prev[j - pjb])
+        R.chunks.append((jb, rows))
+        return jb, rows
 
     j, cycles = 0, 0
     stopped = False
     R.stored = storing
     while True:
-        steps = int(min(m, kmax - j))
+        steps = int(min(check, kmax - j))
         if storing:
-            jb = j - 1 if j > 0 else 0
-            nrows = j + steps + 1 - jb
-            if _fits(plan, nrows * dim * itemsize, dt):
-                rows = torch.empty((nrows, dim), dtype=dt, device=plan.device)
-                if j == 0:
-                    rows[0].copy_(v0)
-                else:
-                    pjb, prev = R.chunks[-1]
-                    rows[0].copy_(prev[j - 1 - pjb])
-                    rows[1].copy_(prev[j - pjb])
-                R.chunks.append((jb, rows))
-                nat.check(L.ryd_lanczos_stored(plan.handle, code, nat.ptr(rows), dim, jb, j, steps,
+            left, jj = steps, j
+            while left > 0:
+                cur = R.chunks[-1] if R.chunks else None
+                room = cur[0] + cur[1].shape[0] - 1 - jj if cur is not None else 0
+                if room <= 0:
+                    cur = new_chunk(jj, min(left, kmax - jj) + 2)
+                    if cur is None:
+                        break
+                    room = cur[0] + cur[1].shape[0] - 1 - jj
+                k = min(left, room)
+                jb, rows = cur
+                nat.check(L.ryd_lanczos_stored(plan.handle, code, nat.ptr(rows), dim, jb, jj, k,
                                                nat.ptr(R.alpha), nat.ptr(R.nrm), nat.ptr(R.state),
                                                float(tiny), nat.ptr(ws), st))
-            else:
-                storing = R.stored = False
-                three_buffers(j)
-        if not storing:
+                jj += k
+                left -= k
+            if left > 0:
+                # memory ran out inside the cycle: keep what is stored, go on with three vectors
+                storing = False
+                R.split = jj
+                three_buffers(jj)
+                nat.check(L.ryd_lanczos(plan.handle, code, nat.ptr(R.bufs[0]), nat.ptr(R.bufs[1]),
+                                        nat.ptr(R.bufs[2]), jj, left, nat.ptr(R.alpha),
+                                        nat.ptr(R.nrm), nat.ptr(R.state), float(tiny), None,
+                                        None, None, None, 0, nat.ptr(ws), st))
+        else:
             if R.bufs is None:
                 three_buffers(j)
             nat.check(L.ryd_lanczos(plan.handle, code, nat.ptr(R.bufs[0]), nat.ptr(R.bufs[1]),
@@ -202,14 +258,14 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
 
 
 def lanczos_vector(R, j):
-    """u_j (unnormalised, |u_j| = R.nrm[j]) when the run still holds it, else None."""
-    if R.chunks:
-        for jb, rows in reversed(R.chunks):
-            if jb <= j < jb + rows.shape[0]:
-                return rows[j - jb]
-        return None
-    if R.bufs is not None:
-        return R.bufs[j % 3]
+    """u_j (unnormalised, |u_j| = R.nrm[j]) when the run still holds it, else None:
+    from the stored chunks, or from the three rotating vectors for the last
+    steps of a run that was not (or no longer) stored."""
+    if R.bufs is not None and (not R.chunks or (R.split is not None and j >= R.split)):
+        return R.bufs[j % 3] if j >= R.j - 1 else None
+    for jb, rows in reversed(R.chunks):
+        if jb <= j < jb + rows.shape[0] and (R.split is None or j <= R.split):
+            return rows[j - jb]
     return None
 
 
@@ -225,9 +281,10 @@ def ritz_vectors(plan, R, which=None):
     if R.stored and R.chunks:
         ys = [np.ascontiguousarray(R.Y[:, i], dtype=np.float64) for i in which]
         first = True
+        jend = R.j if R.split is None else min(R.split, R.j)   # stored rows used: j < jend
         for jb, rows in R.chunks:
             j0 = jb if jb == 0 else jb + 1          # rows 0/1 repeat the previous chunk
-            j1 = min(jb + rows.shape[0] - 1, R.j)   # u_j used for j < R.j
+            j1 = min(jb + rows.shape[0] - 1, jend)
             for a in range(j0, j1, RITZ_ROWS):      # the kernel keeps <= RITZ_ROWS coefficients
                 b = min(a + RITZ_ROWS, j1)
                 base = rows[a - jb]
@@ -240,6 +297,23 @@ def ritz_vectors(plan, R, which=None):
                                             nat.ptr(out[1]) if len(out) > 1 else None,
                                             nat.ptr(plan.workspace()), nat.stream_ptr()))
                 first = False
+        if R.split is not None and R.split < R.j:
+            # the steps run without storage: replay them from the stored u_{split-1}, u_split
+            sp = R.split
+            jb, rows = [c for c in R.chunks if c[0] <= sp < c[0] + c[1].shape[0]][-1]
+            B = [torch.empty(plan.dim, dtype=dt, device=plan.device) for _ in range(3)]
+            B[sp % 3].copy_(rows[sp - jb])
+            if sp > 0:
+                B[(sp - 1) % 3].copy_(rows[sp - 1 - jb])
+            yt = [torch.as_tensor(y, dtype=torch.float64, device=plan.device) for y in ys]
+            nat.check(L.ryd_lanczos(plan.handle, nat.dtype_code(dt), nat.ptr(B[0]), nat.ptr(B[1]),
+                                    nat.ptr(B[2]), sp, R.j - sp, nat.ptr(R.alpha), nat.ptr(R.nrm),
+                                    nat.ptr(R.state), 0.0,
+                                    nat.ptr(out[0]) if len(out) > 0 else None,
+                                    nat.ptr(out[1]) if len(out) > 1 else None,
+                                    nat.ptr(yt[0]) if len(yt) > 0 else None,
+                                    nat.ptr(yt[1]) if len(yt) > 1 else None, 1,
+                                    nat.ptr(plan.workspace()), nat.stream_ptr()))
         return out
     if R.bufs is None:
         R.bufs = [torch.empty(plan.dim, dtype=dt, device=plan.device) for _ in range(3)]
@@ -314,7 +388,7 @@ def _solve(P, start, m, cycles, tol, want, stats, key, dtype64):
         vecs = ritz_vectors(P, R)
         R.chunks, R.bufs = [], None          # the stored basis is not needed past here
         stats[key] += R.j
-        stats['restarts'] += R.cycles - 1 + (1 if attempt else 0)
+        stats['restarts'] += max(0, -(-R.j // m) - 1) + (1 if attempt else 0)   # m-step cycles
         evals, res = _certify(P, vecs, R, dtype64)
         if len(evals) == want and np.all(res <= tol * np.maximum(1.0, np.abs(evals))):
             break

@@ -132,8 +132,9 @@ def test_kpm_bounds_cover_spectrum(pkg, dev):
 
 @pytest.mark.parametrize('dtype', ['float64', 'complex64'])
 def test_stored_basis_ritz_equals_replay(pkg, dev, dtype, monkeypatch):
-    """Ritz vectors from the HBM-resident Krylov basis equal the replayed ones,
-    also when memory runs out after the first cycle (fallback to the replay)."""
+    """Ritz vectors from the HBM-resident Krylov basis equal the replayed ones:
+    one chunk per run, small blocks spanning cycles, and memory running out
+    after the first chunk (stored rows plus a replayed tail)."""
     import torch
     from energy_level_amd import lanczos as lz
     n = 14
@@ -146,19 +147,29 @@ def test_stored_basis_ritz_equals_replay(pkg, dev, dtype, monkeypatch):
     Rr = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=False)
     xr = lz.ritz_vectors(P, Rr)
     Rs = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=True)
-    assert Rs.stored and len(Rs.chunks) == Rs.cycles and Rs.j == Rr.j
+    assert Rs.stored and len(Rs.chunks) >= 1 and Rs.j == Rr.j
     xs = lz.ritz_vectors(P, Rs)
     for a, b in zip(xr, xs):
         assert (a - b).abs().max().item() <= tol * a.abs().max().item()
+    # small 8-row blocks: several chunks per run, spare rows carried into the next cycle
+    monkeypatch.setattr(lz, 'CHUNK_BYTES', 8 * P.dim * v0.element_size())
+    Rc = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=True)
+    assert Rc.stored and len(Rc.chunks) > Rc.cycles and Rc.j == Rr.j
+    assert np.array_equal(Rc.a, Rr.a) and np.array_equal(Rc.b, Rr.b)
+    for a, b in zip(xr, lz.ritz_vectors(P, Rc)):
+        assert (a - b).abs().max().item() <= tol * a.abs().max().item()
     calls = {'n': 0}
 
-    def one_chunk(plan, nbytes, dt):
+    def one_chunk(plan, nbytes, dt, *rest):
         calls['n'] += 1
         return calls['n'] == 1
     monkeypatch.setattr(lz, '_fits', one_chunk)
     Rf = lz.lanczos_run(P, v0, 30, 4, 0.0, want=2, store=True)
-    assert not Rf.stored and Rf.j == Rr.j
+    assert Rf.split is not None and 0 < Rf.split < Rf.j and Rf.j == Rr.j
     assert np.allclose(Rf.a, Rr.a, rtol=0, atol=0) and np.allclose(Rf.b, Rr.b, rtol=0, atol=0)
-    xf = lz.ritz_vectors(P, Rf)
+    xf = lz.ritz_vectors(P, Rf)              # stored rows + replayed tail
     for a, b in zip(xr, xf):
-        assert (a - b).abs().max().item() == 0.0
+        assert (a - b).abs().max().item() <= tol * a.abs().max().item()
+    u = lz.lanczos_vector(Rf, Rf.j)
+    assert u is not None and abs(torch.linalg.vector_norm(u).item() - Rf.nrm[Rf.j].item()) <= \
+        1e-5 * Rf.nrm[Rf.j].item()
