@@ -36,6 +36,7 @@ struct GraphModel {
   const double* rdiag;          // [rows] diagonal of those bits alone
   const uint32_t* lrow;         // [L][rows] segment position of the row with bit q flipped
   const double* vb;             // [n*n] V by bit position
+  int hfar;                     // high bits 0 .. hfar-1 are applied here (H: all)
   double dl[MAXN];              // delta by bit position
   double ar[MAXN], ai[MAXN];    // Ω/2 by bit position
 };
@@ -133,12 +134,13 @@ __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io
           vcfma(acc, (RT)(M.ar[q] * s), (RT)((((l >> q) & 1u) ? aim : -aim) * s), xv[u]);
         }
       }
-      for (int p0 = 0; p0 < H; p0 += GR_FLIPS) {
+      const int hf = M.hfar;
+      for (int p0 = 0; p0 < hf; p0 += GR_FLIPS) {
         int64_t r2[GR_FLIPS];
 #pragma unroll
         for (int u = 0; u < GR_FLIPS; ++u) {
           const int p = p0 + u;
-          bool ok = p < H;
+          bool ok = p < hf;
           if (ok) {
             const int64_t o2 = off2[p];
             ok = o2 >= 0 && (((h >> p) & 1u) || !(l & f2[p])) &&
@@ -158,6 +160,10 @@ __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io
           vcfma(acc, (RT)(M.ar[p + L] * s), (RT)((((h >> p) & 1u) ? aim : -aim) * s), xv[u]);
         }
       }
+      if (io.partial) {                 // pass A of a split H·v: the raw sum
+        nt_store(io.y + row, acc);
+        continue;
+      }
       V v = vscale(acc, chr);
       vfma(v, cxr, xe);
       if (io.z) vfma(v, czr, io.z[row]);
@@ -167,6 +173,115 @@ __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io
         s1 += nrm2(v);
         s2 += nrm2(xe);
         if (io.probe) s3 += zdot(((io.probe[row >> 5] >> (row & 31)) & 1u) ? 1.0 : -1.0, v);
+      }
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// ---- pass B of a split graph H·v: the top GT bits of the high part ---------------
+// When the top K high bits are sites with no neighbour among the low bits,
+// every valid top pattern g over a body high part b (the other high bits)
+// has the class of b, so row j of the segment of h = g·2^(H-K) | b holds the
+// same low set for every g.  A block takes one body and a run of GR_RUN
+// positions j, stages s·x of that run for every valid g in LDS, and adds the
+// top flips (g ^ 2^k, an LDS read) to the pass-A partial, then finishes
+//   y = ch·(y_A + Σ top flips) + cx·(s·x) + cz·z  with the reductions.
+constexpr int GR_TOPMAX = 128;      // valid top patterns
+constexpr int GR_KMAX = 10;         // top bits
+constexpr int GR_HNEAR = 12;        // split only when pass A keeps at most this many high bits
+// segment positions per block: 64 KB of LDS for 128 patterns
+template <class V> constexpr int gr_run() { return 32; }
+
+struct GraphTop {
+  int H, K, nt;                 // high bits, top bits, valid top patterns
+  int L;
+  int64_t nbody;                // valid high parts with the top bits clear (hlist prefix)
+  int64_t tl;                   // low sets (the longest segment)
+  const uint32_t* hlist;
+  const uint64_t* hpk;
+  const uint32_t* tp;           // [nt] valid top patterns, ascending (top-bit coordinates)
+  const int16_t* tpk;           // [nt][K] index of tp ^ 2^k in tp, -1 if not independent
+  double ar[GR_KMAX], ai[GR_KMAX];   // Ω/2 of the top bits
+};
+
+template <class V>
+__global__ __launch_bounds__(256) void k_graph_top(GraphTop T, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int GR_RUN = gr_run<V>();
+  __shared__ V xs[GR_TOPMAX * GR_RUN];
+  __shared__ int64_t voff[GR_TOPMAX];
+  __shared__ int16_t pk[GR_TOPMAX * GR_KMAX];     // partner index, -1 when not valid for the body
+  __shared__ uint32_t tpat[GR_TOPMAX];
+  const int64_t nruns = (T.tl + GR_RUN - 1) / GR_RUN;
+  for (int i = threadIdx.x; i < T.nt; i += 256) tpat[i] = T.tp[i];
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s, chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  const int tid = threadIdx.x;
+  const int HB = T.H - T.K;
+  const int items = T.nt * GR_RUN;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const int64_t units = T.nbody * nruns;
+  // contiguous runs of units per XCD
+  int64_t u0, u1, ustep;
+  if (gridDim.x >= 8 && (gridDim.x & 7) == 0) {
+    const int64_t chunk = (units + 7) / 8;
+    u0 = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+    u1 = ((int64_t)(blockIdx.x & 7) + 1) * chunk;
+    if (u1 > units) u1 = units;
+    ustep = gridDim.x >> 3;
+  } else {
+    u0 = blockIdx.x;
+    u1 = units;
+    ustep = gridDim.x;
+  }
+  for (int64_t u = u0; u < u1; u += ustep) {
+    const int64_t bi = u / nruns;
+    const int64_t j0 = (u - bi * nruns) * GR_RUN;
+    const uint32_t hb = T.hlist[bi];
+    const int64_t cnt = (int64_t)(T.hpk[hb + 1] >> 24) - (int64_t)(T.hpk[hb] >> 24);
+    if (j0 >= cnt) continue;                       // block-uniform
+    __syncthreads();                               // LDS of the previous unit is free
+    if (tid < T.nt) {
+      const uint64_t e = T.hpk[((uint32_t)tpat[tid] << HB) | hb];
+      voff[tid] = (e & GR_CLS) == GR_CLS ? -1 : (int64_t)(e >> 24);
+    }
+    __syncthreads();
+    for (int i = tid; i < T.nt * T.K; i += 256) {
+      const int pv = T.tpk[i];
+      pk[i] = (int16_t)((pv >= 0 && voff[pv] >= 0) ? pv : -1);
+    }
+    for (int i = tid; i < items; i += 256) {
+      const int v = i / GR_RUN, jj = i - v * GR_RUN;
+      const int64_t o = voff[v];
+      xs[i] = (o >= 0 && j0 + jj < cnt) ? vscale(io.x[o + j0 + jj], sr) : vzero(xs[0]);
+    }
+    __syncthreads();
+    for (int i = tid; i < items; i += 256) {
+      const int v = i / GR_RUN, jj = i - v * GR_RUN;
+      const int64_t o = voff[v];
+      if (o < 0 || j0 + jj >= cnt) continue;
+      const int64_t row = o + j0 + jj;
+      V acc = io.y[row];                           // pass-A partial
+      const uint32_t g = tpat[v];
+      for (int k = 0; k < T.K; ++k) {
+        const int pv = pk[v * T.K + k];
+        if (pv < 0) continue;
+        const int bit = (int)((g >> k) & 1u);
+        vcfma(acc, (RT)T.ar[k], (RT)(bit ? T.ai[k] : -T.ai[k]), xs[pv * GR_RUN + jj]);
+      }
+      const V xe = xs[i];
+      V y = vscale(acc, chr);
+      vfma(y, cxr, xe);
+      if (io.z) vfma(y, czr, io.z[row]);
+      io.y[row] = y;
+      if (io.reduce) {
+        s0 += rdot(xe, y);
+        s1 += nrm2(y);
+        s2 += nrm2(xe);
+        if (io.probe) s3 += zdot(((io.probe[row >> 5] >> (row & 31)) & 1u) ? 1.0 : -1.0, y);
       }
     }
   }

@@ -2,6 +2,7 @@
 // logic and the device-resident Krylov drivers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -97,6 +98,12 @@ struct ryd_plan {
   uint32_t* d_rcode = nullptr;       // [rows]
   double* d_rdiag = nullptr;         // [rows]
   uint32_t* d_lrow = nullptr;        // [L][rows]
+  // top pass of a split graph H·v (gr_K = 0: one pass)
+  int gr_K = 0;
+  int gr_nt = 0;
+  int64_t gr_nbody = 0;
+  uint32_t* d_tp = nullptr;          // [nt] valid top patterns
+  int16_t* d_tpk = nullptr;          // [nt][K] partner indices
   int fib_L = 0;
   int64_t fib_cnt[2] = {0, 0};
 };
@@ -249,12 +256,41 @@ GraphModel graph_model(const ryd_plan* P) {
   M.rdiag = P->d_rdiag;
   M.lrow = P->d_lrow;
   M.vb = P->d_vb;
+  M.hfar = M.H - P->gr_K;
   for (int p = 0; p < MAXN; ++p) {
     M.dl[p] = P->dl[p];
     M.ar[p] = P->ar[p];
     M.ai[p] = P->ai[p];
   }
   return M;
+}
+
+GraphTop graph_top(const ryd_plan* P) {
+  GraphTop T;
+  std::memset(&T, 0, sizeof(T));
+  T.L = P->gr_L;
+  T.H = P->n - P->gr_L;
+  T.K = P->gr_K;
+  T.nt = P->gr_nt;
+  T.nbody = P->gr_nbody;
+  T.tl = P->gr_tl;
+  T.hlist = P->d_hlist;
+  T.hpk = P->d_hpk;
+  T.tp = P->d_tp;
+  T.tpk = P->d_tpk;
+  for (int k = 0; k < T.K; ++k) {
+    T.ar[k] = P->ar[T.L + T.H - T.K + k];
+    T.ai[k] = P->ai[T.L + T.H - T.K + k];
+  }
+  return T;
+}
+
+template <class V>
+unsigned graph_top_grid(const ryd_plan* P) {
+  const int64_t units = P->gr_nbody * ((P->gr_tl + gr_run<V>() - 1) / gr_run<V>());
+  if (units < 64) return (unsigned)(units < 1 ? 1 : units);
+  const int64_t g = 8 * ((units + 7) / 8);
+  return (unsigned)(g < CHAN_MAX_BLOCKS ? g : CHAN_MAX_BLOCKS);
 }
 
 // k_graph_apply: whole XCD groups of blocks, at most CHAN_MAX_BLOCKS
@@ -288,10 +324,17 @@ template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
   if (graph_kernel(P)) {
     io.nfar = 0;
-    io.partial = 0;
-    const unsigned grid = graph_grid(P);
+    const bool split = P->gr_K > 0;
+    io.partial = split ? 1 : 0;
+    unsigned grid = graph_grid(P);
     k_graph_apply<V><<<dim3(grid), dim3(256), 0, st>>>(graph_model(P), io, rd);
     LAUNCH_CHECK();
+    if (split) {
+      io.partial = 0;
+      grid = graph_top_grid<V>(P);
+      k_graph_top<V><<<dim3(grid), dim3(256), 0, st>>>(graph_top(P), io, rd);
+      LAUNCH_CHECK();
+    }
     if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
     LAUNCH_CHECK();
     return RYD_OK;
@@ -569,6 +612,17 @@ int gr_write(F f, W w, int64_t count, const int64_t* bsum) {
   return RYD_OK;
 }
 
+// Blockade-graph bases from this dimension on split H·v into two passes (the
+// top high bits in the second); RYD_GRAPH_SPLIT_MIN_DIM overrides it.
+int64_t gr_split_min_dim() {
+  if (const char* env = std::getenv("RYD_GRAPH_SPLIT_MIN_DIM")) {
+    char* end = nullptr;
+    const long long v = std::strtoll(env, &end, 10);
+    if (end != env) return (int64_t)v;
+  }
+  return 1ll << 22;
+}
+
 // Bytes allowed for the per-class prefix table: beyond this the ranks fall
 // back to a binary search over the code list.  Default: twice the code list,
 // at least 256 MiB; RYD_GRAPH_PREFIX_MAX_BYTES overrides it (0 = always search).
@@ -731,6 +785,45 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
       HIP_TRY(hipMalloc(&P->d_hlist, sizeof(uint32_t) * (size_t)nv));
       if (int rc = gr_write(GrValidFlag{P->d_hpk}, GrListWrite{P->d_hlist}, nhi, bsum)) return rc;
       P->gr_nv = nv;
+      // top pass: the top K high bits when none of them has a low neighbour
+      // and their independent patterns fit a block (large bases only)
+      std::vector<uint32_t> tp;
+      int K = 0;
+      if (dim >= gr_split_min_dim()) {
+        for (int k = 3; k <= GR_KMAX && k < H; ++k) {
+          bool clear = true;
+          for (int q = H - k; q < H; ++q) clear = clear && !(nbb[q + L] & low);
+          if (!clear) break;
+          std::vector<uint32_t> cand;
+          for (uint32_t g = 0; g < (1u << k); ++g) {
+            bool ind = true;
+            for (int a = 0; a < k && ind; ++a)
+              if (((g >> a) & 1u) && (g & (G.nb_hi[H - k + a] >> (H - k)))) ind = false;
+            if (ind) cand.push_back(g);
+          }
+          if ((int)cand.size() > GR_TOPMAX) break;
+          K = k;
+          tp.swap(cand);
+        }
+      }
+      if (K >= 3 && H - K <= GR_HNEAR) {
+        std::vector<int16_t> tpk(tp.size() * (size_t)K, -1);
+        for (size_t t = 0; t < tp.size(); ++t)
+          for (int k = 0; k < K; ++k) {
+            const uint32_t g2 = tp[t] ^ (1u << k);
+            auto it = std::lower_bound(tp.begin(), tp.end(), g2);
+            if (it != tp.end() && *it == g2) tpk[t * K + k] = (int16_t)(it - tp.begin());
+          }
+        std::vector<uint32_t> hl((size_t)nv);
+        HIP_TRY(hipMemcpy(hl.data(), P->d_hlist, sizeof(uint32_t) * (size_t)nv, hipMemcpyDeviceToHost));
+        P->gr_nbody = std::lower_bound(hl.begin(), hl.end(), 1u << (H - K)) - hl.begin();
+        HIP_TRY(hipMalloc(&P->d_tp, sizeof(uint32_t) * tp.size()));
+        HIP_TRY(hipMemcpy(P->d_tp, tp.data(), sizeof(uint32_t) * tp.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&P->d_tpk, sizeof(int16_t) * tpk.size()));
+        HIP_TRY(hipMemcpy(P->d_tpk, tpk.data(), sizeof(int16_t) * tpk.size(), hipMemcpyHostToDevice));
+        P->gr_K = K;
+        P->gr_nt = (int)tp.size();
+      }
     }
     HIP_TRY(hipMalloc(&P->d_codes, sizeof(int64_t) * (size_t)dim));
     const unsigned eg = (unsigned)(nhi < 65536 ? nhi : 65536);
@@ -1060,6 +1153,8 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_rcode) (void)hipFree(P->d_rcode);
   if (P->d_rdiag) (void)hipFree(P->d_rdiag);
   if (P->d_lrow) (void)hipFree(P->d_lrow);
+  if (P->d_tp) (void)hipFree(P->d_tp);
+  if (P->d_tpk) (void)hipFree(P->d_tpk);
   for (int k = 0; k < 4; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);

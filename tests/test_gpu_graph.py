@@ -289,3 +289,46 @@ def test_generic_evolution_matches_fused_full_basis(pkg, dev):
     codes = np.arange(1 << n, dtype=np.int64)
     gen = pkg.propagate_segments_hip(calib, segs, t, device=dev, basis_indices=codes)
     assert np.max(np.abs(full['psi'] - gen['psi'])) < 1e-12
+
+
+@pytest.mark.parametrize('w,h,diag', [(5, 4, False), (6, 4, True), (5, 5, False)])
+def test_split_graph_hv_vs_c_oracle(pkg, dev, w, h, diag, monkeypatch):
+    """The two-pass graph H·v (top high bits in a second pass), forced on at
+    small sizes, against the oracle's projected H·v, and against the one-pass
+    kernel on the same plan."""
+    import torch
+    n = w * h
+    calib = _lattice_calib(w, h)
+    edges = pkg.blockade_edges(calib, {'kind': 'blockade', 'threshold_mhz': 2.0 if diag else 10.0})
+    rng = np.random.default_rng(n)
+    Om, delta = rng.uniform(0.5, 1.5, n), rng.uniform(-2, 2, n)
+    codes = hv_ref.indep_codes(n, edges)
+    x = rng.normal(size=len(codes))
+    ref = hv_ref.hv_codes(n, Om, hv_ref.diag(n, delta, calib['pairs'], codes), codes, x)
+    out = {}
+    for key, mind in (('split', '0'), ('one', str(1 << 40))):
+        monkeypatch.setenv('RYD_GRAPH_SPLIT_MIN_DIM', mind)
+        P = pkg.Plan(pkg.from_calib(calib, Om, delta), basis='graph', edges=edges, device=dev)
+        assert P.dim == len(codes)
+        out[key] = P.hv(torch.as_tensor(x, device=dev)).cpu().numpy()
+        P.close()
+    for key, y in out.items():
+        assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref)), key
+    assert np.max(np.abs(out['split'] - out['one'])) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_split_graph_ground_state_vs_eigsh(pkg, dev, monkeypatch):
+    """Lanczos through the split H·v (three-term steps with z and fused
+    reductions in the second pass) on the 5x5 lattice basis (D = 55,447)."""
+    monkeypatch.setenv('RYD_GRAPH_SPLIT_MIN_DIM', '0')
+    n = 25
+    calib = _lattice_calib(5, 5)
+    edges = pkg.blockade_edges(calib, {'kind': 'blockade', 'threshold_mhz': 10.0})
+    Om, delta = np.ones(n), 1.2 * np.ones(n)
+    codes = hv_ref.indep_codes(n, edges)
+    H = orc.build_H_sparse(n, Om, delta, calib['pairs'], ['I1'], codes=codes)
+    ref = orc.ground_state_eigsh(H, k=2)
+    out = pkg.hip_lanczos_ground_state(calib, Om, delta, blockade_edges=edges, device=dev)
+    assert out['converged'] and out['dim'] == len(codes)
+    assert abs(out['E0'] - ref['E0']) <= 1e-9 * max(1.0, abs(ref['E0']))
+    assert abs(out['gap'] - ref['gap']) <= 1e-8
