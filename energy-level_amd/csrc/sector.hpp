@@ -109,6 +109,9 @@ __device__ inline void fib_cross_tables(const FibModel& M, uint64_t h, double* F
 // partner offset and coefficient, an outer or high flip is allowed or not for
 // the whole group (bit IB adds the lane's inner bit IB-1), and all LDS and
 // global accesses of a group are contiguous across lanes.
+#ifndef RYD_FIB_BATCH
+#define RYD_FIB_BATCH 1
+#endif
 constexpr int FIB_IB = 8;          // inner bits: groups of F(10) = 55 or F(9) = 34 offsets
 constexpr int FIB_THREADS = 512;   // 8 wavefronts per block
 constexpr int FIB_WAVES = FIB_THREADS / 64;
@@ -223,6 +226,14 @@ __global__ __launch_bounds__(FIB_THREADS, 4) void k_fib_apply(FibModel M, ApplyI
   }
   auto car = [&](int q) -> RT { if constexpr (CONV) return oar[q]; else return (RT)M.ar[IB + q]; };
   auto cai = [&](int q) -> RT { if constexpr (CONV) return oai[q]; else return (RT)M.ai[IB + q]; };
+  // FP32: every flip of a group predicated and its LDS reads batched (wider
+  // types would spill at 128 registers and keep the branch form)
+  constexpr bool BATCH = RYD_FIB_BATCH && sizeof(V) == 4;
+  int Wq[BATCH ? NOUT : 1];      // outer-bit weights held for the whole block
+  if constexpr (BATCH) {
+#pragma unroll
+    for (int q = 0; q < NOUT; ++q) Wq[q] = (int)M.W[IB + q < MAXN ? IB + q : 0];
+  }
   const double eh = H.ehi;
   V acc[FIB_GPW];
   int koff[FIB_GPW];   // this lane's offset in the block for each group of the wave
@@ -246,6 +257,36 @@ __global__ __launch_bounds__(FIB_THREADS, 4) void k_fib_apply(FibModel M, ApplyI
                             : 0.0;
     acc[i] = vscale(xt[k], (RT)d);
     if (!gv) continue;
+    if constexpr (BATCH) {
+      // every flip of the group predicated (a blocked flip reads the lane's
+      // own slot with coefficient 0); all LDS reads issued before the FMAs
+      constexpr int NF = FIB_IB + NOUT;
+      int fo[NF];
+      RT fr[NF], fi[NF];
+#pragma unroll
+      for (int p = 0; p < FIB_IB; ++p) {
+        const bool skip = p == IB - 1 && small;          // uniform
+        fo[p] = skip ? 0 : off[p];
+        fr[p] = skip ? (RT)0 : cfr[p];
+        fi[p] = skip ? (RT)0 : cfi[p];
+      }
+#pragma unroll
+      for (int q = 0; q < NOUT; ++q) {
+        const bool bit = (o >> q) & 1u;
+        bool use = q < nout && ((om >> q) & 1u);         // uniform
+        if (q == 0 && !bit) use = use && !top_in;        // setting bit IB: per lane
+        const int W = Wq[BATCH ? q : 0];
+        fo[FIB_IB + q] = use ? (bit ? -W : W) : 0;
+        fr[FIB_IB + q] = use ? car(q) : (RT)0;
+        fi[FIB_IB + q] = use ? (bit ? cai(q) : -cai(q)) : (RT)0;
+      }
+      V xv[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) xv[f] = xt[k + fo[f]];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) vcfma(acc[i], fr[f], fi[f], xv[f]);
+      continue;
+    }
     // inner flips: fixed per lane; in a small group bit IB-1 cannot be set
 #pragma unroll
     for (int p = 0; p < FIB_IB; ++p) {
