@@ -22,6 +22,7 @@ from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip,
     spectrum_lanczos_hip, quadrature, lorentz_spectrum
 from .observables import normalize_pauli_strings, PauliPlanHip, measure_pauli_strings_hip, \
     corr_matrix_hip, site_correlation_sums
+from .manybody import evolve_manybody_hip, state_vector_np, product_label
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
@@ -39,4 +40,5 @@ __all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_
            'scan_ground_sectors_hip', 'validate_edges', 'graph_plan', 'independent_states_hip',
            'independent_set_count_hip', 'blockade_edges', 'build_plan_hip',
            'normalize_pauli_strings', 'PauliPlanHip', 'measure_pauli_strings_hip',
-           'corr_matrix_hip', 'site_correlation_sums']
+           'corr_matrix_hip', 'site_correlation_sums', 'evolve_manybody_hip', 'state_vector_np',
+           'product_label']

@@ -300,7 +300,8 @@ def segment_drive(t, segments, n, interp='left'):
 def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='left',
                            dtype='complex128', device='cuda', return_states=False,
                            basis_probabilities_max_dim=4096, basis_indices=None,
-                           blockade_edges=None, plan=None, pauli_strings=None, corr=False):
+                           blockade_edges=None, plan=None, pauli_strings=None, corr=False,
+                           on_state=None, psi0_plan_order=False):
     """Exact piecewise-constant evolution, output at the times ``t`` (ns):
     every output interval is split at the segment starts and each piece
     [a, b] is propagated by exp(−i·CONV·H·(b−a)) with the drive of
@@ -321,6 +322,8 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
     ``pauli_strings`` (specs as ``normalize_pauli_strings``) and ``corr``
     add the per-output-time reductions of engine_ed.py:259-281 on the device:
     ``pauli`` (len(t), strings) complex and ``corr`` (len(t), n, n).
+    ``on_state(k, plan, x)`` is called at every output time k with the device
+    state in the plan's (ascending) basis order.
 
     Returns {t, pops_atom (len(t), n), norms (len(t),), psi (final state,
     host complex128, basis order), method='manybody', backend='hip-chebyshev',
@@ -374,6 +377,8 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
             pauli_series.append(pauli_plan.expect(x))
         if corr:
             corr_series.append(corr_matrix_hip(p, x)[0])
+        if on_state is not None:
+            on_state(len(norms) - 1, p, x)
         if return_states or (x.numel() <= basis_probabilities_max_dim):
             v = x.to(torch.complex128).cpu().numpy()
             if order is not None:
@@ -401,7 +406,7 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
             v = psi0.detach().cpu().numpy() if torch.is_tensor(psi0) else np.asarray(psi0)
             v = np.asarray(v, complex).reshape(-1)
             if v.size == dim:
-                if order is not None:
+                if order is not None and not psi0_plan_order:
                     v = v[order]
             elif projected and v.size == (1 << n):
                 norm = np.linalg.norm(v)

@@ -642,3 +642,85 @@ def pauli_dense(n, terms):
     for o in ops:
         M = np.kron(M, o)
     return M
+
+
+# ---------------------------------------------------------------------------
+# spin observables and entanglement  (reference observables.py:50-204)
+# ---------------------------------------------------------------------------
+def spin_observables(n, vec, codes=None):
+    """sx, sy, sz, sp = <σ+>, sm = <σ->, xy_corr[i,j] = <σ+_i σ-_j> (diagonal
+    <n_i>), phase = arg(<σ->) (NaN where |<σ->| < 1e-14); σ+ = |r><g|, on the
+    full basis or a code list (observables.py:117-204)."""
+    v = np.asarray(vec, complex).reshape(-1)
+    codes = np.arange(1 << n, dtype=np.int64) if codes is None else np.asarray(codes, np.int64)
+    pos = {int(c): k for k, c in enumerate(codes)}
+    sz = np.zeros(n)
+    sm = np.zeros(n, complex)
+    xy = np.zeros((n, n), complex)
+    for i in range(n):
+        mi = 1 << (n - 1 - i)
+        for k, c in enumerate(codes):
+            c = int(c)
+            if c & mi:
+                sz[i] -= abs(v[k]) ** 2
+                kg = pos.get(c ^ mi)
+                if kg is not None:
+                    sm[i] += np.conj(v[kg]) * v[k]
+            else:
+                sz[i] += abs(v[k]) ** 2
+    for i in range(n):
+        mi = 1 << (n - 1 - i)
+        for j in range(n):
+            mj = 1 << (n - 1 - j)
+            for k, c in enumerate(codes):
+                c = int(c)
+                if i == j:
+                    if c & mi:
+                        xy[i, i] += abs(v[k]) ** 2
+                elif not (c & mi) and (c & mj):
+                    kd = pos.get((c | mi) ^ mj)
+                    if kd is not None:
+                        xy[i, j] += np.conj(v[kd]) * v[k]
+    phase = np.angle(sm)
+    phase[np.abs(sm) < 1e-14] = np.nan
+    return dict(sx=2 * sm.real, sy=2 * sm.imag, sz=sz, sp=np.conj(sm), sm=sm, xy_corr=xy,
+                phase=phase)
+
+
+def sigma_pm_dense(n, i, j):
+    """σ+_i σ-_j as a dense 2^n matrix by Kronecker products (site 0 most
+    significant; σ+ = |r><g| has its 1 in row r, column g)."""
+    sp = np.array([[0, 0], [1, 0]], complex)
+    sm = np.array([[0, 1], [0, 0]], complex)
+    ops = [np.eye(2, dtype=complex) for _ in range(n)]
+    if i == j:
+        ops[i] = sp @ sm
+    else:
+        ops[i] = sp
+        ops[j] = sm
+    M = np.ones((1, 1), complex)
+    for o in ops:
+        M = np.kron(M, o)
+    return M
+
+
+def entropy_at_cut(n, vec, cut, codes=None):
+    """Von Neumann entropy (nat) of sites [0, cut) from the singular values of
+    the Schmidt matrix; on a code list the matrix rows/columns are the
+    occupied left/right parts (observables.py:50-115)."""
+    v = np.asarray(vec, complex).reshape(-1)
+    if n < 2:
+        return 0.0
+    if codes is None:
+        w = np.linalg.svd(v.reshape(1 << cut, 1 << (n - cut)), compute_uv=False) ** 2
+        w = w[w > 1e-15]
+        return float(-np.sum(w * np.log(w))) if w.size else 0.0
+    codes = np.asarray(codes, np.int64)
+    _, li = np.unique(codes >> (n - cut), return_inverse=True)
+    _, ri = np.unique(codes & ((1 << (n - cut)) - 1), return_inverse=True)
+    M = np.zeros((li.max() + 1, ri.max() + 1), complex)
+    M[li, ri] = v
+    w = np.linalg.svd(M, compute_uv=False) ** 2
+    w = w / w.sum()
+    w = w[w > 0]
+    return float(-np.sum(w * np.log(w)))

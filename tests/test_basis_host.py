@@ -145,3 +145,30 @@ def test_normalize_pauli_strings_mirrors_reference(pkg):
                 [{'label': 'a'}], 'x@0'):
         with pytest.raises(ValueError):
             pkg.normalize_pauli_strings(bad, 4)
+
+
+def test_state_vector_mirrors_reference(pkg):
+    """Initial states as state.py:112-210: patterns, index, superposition,
+    product dict, list of local states, 2^N vector; projected weight outside
+    the basis is an error."""
+    n = 3
+    v = pkg.state_vector_np('grg', n)
+    assert v[0b010] == 1 and np.count_nonzero(v) == 1
+    assert np.array_equal(pkg.state_vector_np(5, n), np.eye(8)[5])
+    s = pkg.state_vector_np(('sup', [('ggg', 1.0), ('rrr', 1.0)]), n)
+    assert np.allclose(s[[0, 7]], 1 / np.sqrt(2)) and np.count_nonzero(s) == 2
+    p = pkg.state_vector_np({'kind': 'product', 'amplitudes': [[1, 1], 'g', [0, 1]]}, n)
+    assert np.allclose(p[[0b001, 0b101]], 1 / np.sqrt(2))
+    q = pkg.state_vector_np(['r', 'g', 'g'], n)
+    assert q[0b100] == 1
+    full = np.arange(8, dtype=complex)
+    assert np.allclose(pkg.state_vector_np(full, n), full / np.linalg.norm(full))
+    codes = [0b000, 0b001, 0b100, 0b101]
+    assert np.allclose(pkg.state_vector_np('rgr', n, codes), [0, 0, 0, 1])
+    with pytest.raises(ValueError):
+        pkg.state_vector_np('rrg', n, codes)          # outside the basis
+    with pytest.raises(ValueError):
+        pkg.state_vector_np('gx', n)
+    with pytest.raises(ValueError):
+        pkg.state_vector_np('ground', n)
+    assert pkg.product_label(3, 0b101) == '|rgr>'

@@ -116,7 +116,7 @@ def test_autograd_function_and_complex64(pkg, dev):
     loss = pkg.PulseEvolution.apply(om_t, de_t, p64, psi0, tgt)
     loss.backward()
     L32, g32, d32, _ = p32.loss_and_grad(psi0, Om, De, tgt)
-    assert abs(L32 - float(loss)) < 1e-5
+    assert abs(L32 - float(loss.detach())) < 1e-5
     assert np.max(np.abs(g32 - om_t.grad.numpy())) < 1e-4 * max(1, np.max(np.abs(g32)))
     assert np.max(np.abs(d32 - de_t.grad.numpy())) < 1e-4 * max(1, np.max(np.abs(d32)))
 

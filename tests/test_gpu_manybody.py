@@ -1,0 +1,100 @@
+"""evolve_manybody on the device (the result contract of engine_ed.py:153-372)
+against the oracle's exact propagation and its restatements of the
+observables (spin observables, entropies, C_ij, Pauli strings)."""
+import numpy as np
+import pytest
+
+from oracle import rydberg_oracle as orc
+from test_gpu_graph import _lattice_calib
+
+pytestmark = pytest.mark.gpu
+
+
+def _segments(n, rng):
+    return [dict(t0=t0, Om=rng.uniform(0.5, 3.0, n), phase=rng.uniform(0, 2 * np.pi, n),
+                 delta=rng.uniform(-2, 2, n)) for t0 in (0.0, 55.0)]
+
+
+def _check(out, ref_states, n, codes, cut, pauli=None, link=None, tol=1e-9):
+    t = out['t']
+    for k in range(len(t)):
+        v = ref_states[k]
+        p = np.abs(v) ** 2
+        assert abs(out['norms'][k] - p.sum()) < tol
+        assert np.max(np.abs(out['pops_atom'][k] - orc.site_occupations(n, v, codes))) < tol
+        assert abs(out['S'][k] - orc.entropy_at_cut(n, v, cut, codes)) < 1e-8
+        ob = orc.spin_observables(n, v, codes)
+        O = out['observables']
+        for key in ('sx', 'sy', 'sz'):
+            assert np.max(np.abs(O[key][k] - ob[key])) < tol, key
+        assert np.max(np.abs(O['sm'][k] - ob['sm'])) < tol
+        if pauli:
+            for q, s in enumerate(pauli):
+                assert abs(O['pauli_strings'][k, q] - orc.pauli_expect(n, v, s['terms'], codes)) < tol
+        if link:
+            for q, (a, b) in enumerate(link):
+                assert abs(O['link_corr'][k, q] - ob['xy_corr'][a, b]) < tol
+    last = spin = orc.spin_observables(n, ref_states[-1], codes)
+    assert np.max(np.abs(out['observables']['xy_corr'] - last['xy_corr'])) < tol
+    assert np.max(np.abs(out['corr'] - orc.corr_matrix(n, ref_states[-1], codes))) < tol
+    assert np.allclose(out['loss'], 1 - out['norms'])
+
+
+def test_evolve_manybody_full_basis(pkg, dev):
+    n = 8
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0))
+    rng = np.random.default_rng(5)
+    segs = _segments(n, rng)
+    t = np.array([0.0, 30.0, 90.0])
+    pauli = [[[0, 'x'], [1, 'y']], [[2, 'z']]]
+    link = [(0, 1), (3, 5)]
+    out = pkg.evolve_manybody_hip(calib, psi0='grgggggg', t_grid=t, segments=segs,
+                                  pauli_strings=pauli, link_pairs=link, record_psi=True,
+                                  diag=True, device=dev)
+    psi0 = orc.propagate_segments(n, calib['pairs'], ['I1'], segs,
+                                  pkg.state_vector_np('grgggggg', n), t)
+    assert np.max(np.abs(out['psi_amp'] - psi0[-1])) < 1e-9
+    norm = pkg.normalize_pauli_strings(pauli, n)
+    _check(out, psi0, n, None, n // 2, pauli=norm, link=link)
+    assert out['pops'].shape == (len(t), 1 << n) and out['states'][5]['label'] == '|gggggrgr>'
+    assert out['backend'] == 'hip-ed' and out['method'] == 'manybody'
+    assert out['basis'] == dict(kind='full', bit_order='site0-msb', n_atoms=n, dim=1 << n,
+                                full_dim=1 << n)
+    assert out['error_report']['max_norm_dev'] < 1e-9 and out['diag']['dim'] == 1 << n
+
+
+@pytest.mark.parametrize('via', ['edges', 'states'])
+def test_evolve_manybody_projected_plan(pkg, dev, via):
+    n = 12
+    calib = _lattice_calib(4, 3, c6=30.0)
+    cons = {'kind': 'blockade', 'threshold_mhz': 10.0}
+    plan = pkg.build_plan_hip(calib, constraint=cons, device=dev)
+    if via == 'states':
+        plan = dict(plan, states=[int(c) for c in plan['states'].cpu().numpy()])
+    codes = plan['states'].cpu().numpy() if via == 'edges' else np.asarray(plan['states'])
+    rng = np.random.default_rng(6)
+    segs = _segments(n, rng)
+    t = np.array([0.0, 40.0, 100.0])
+    out = pkg.evolve_manybody_hip(calib, t_grid=t, segments=segs, plan=plan, device=dev,
+                                  entropy_cut=5)
+    psi0 = pkg.state_vector_np(None, n, codes)
+    ref = orc.propagate_segments(n, calib['pairs'], ['I1'], segs, psi0, t, codes=codes)
+    _check(out, ref, n, codes, 5)
+    assert out['backend'] == 'hip-ed-plan-blockade'
+    assert np.array_equal(np.asarray(out['basis_states']), codes)
+    assert out['pops'].shape == (len(t), n)                      # per-atom marginals
+
+
+def test_evolve_manybody_ground_start_static_drive(pkg, dev):
+    """psi0='ground' at a static drive stays stationary (up to phase)."""
+    n = 8
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0))
+    Om, delta = np.ones(n), -1.0 * np.ones(n)
+    out = pkg.evolve_manybody_hip(calib, psi0='ground', t_grid=[0.0, 20.0, 50.0], Om=Om,
+                                  delta=delta, device=dev)
+    assert np.max(np.abs(out['pops_atom'] - out['pops_atom'][0])) < 1e-8
+    assert np.max(np.abs(out['S'] - out['S'][0])) < 1e-7
+    with pytest.raises(NotImplementedError):
+        pkg.evolve_manybody_hip(calib, t_grid=[0.0, 1.0], gamma=np.ones(n), device=dev)
+    with pytest.raises(ValueError):
+        pkg.evolve_manybody_hip(calib, t_grid=[0.0, 1.0], entropy_cut=n, device=dev)

@@ -236,3 +236,47 @@ def test_corr_matrix_oracle_known_answers():
     v[0] = v[3] = 1 / np.sqrt(2)
     C = orc.corr_matrix(2, v)
     assert np.allclose(C, [[0.25, 0.25], [0.25, 0.25]])
+
+
+def test_spin_observables_vs_dense_operators():
+    """The oracle's spin observables against dense Kronecker operators
+    (σ+_i σ-_j, X, Y, Z) on random states, full basis and a code list
+    (state embedded with zeros outside)."""
+    rng = np.random.default_rng(11)
+    for n in (1, 3, 5):
+        v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        v /= np.linalg.norm(v)
+        codes = np.sort(rng.choice(1 << n, max(2, (1 << n) // 2), replace=False))
+        w = rng.normal(size=len(codes)) + 1j * rng.normal(size=len(codes))
+        w /= np.linalg.norm(w)
+        emb = np.zeros(1 << n, complex)
+        emb[codes] = w
+        for vec, cd, full in ((v, None, v), (w, codes, emb)):
+            ob = orc.spin_observables(n, vec, cd)
+            for i in range(n):
+                assert abs(ob['sx'][i] - np.vdot(full, orc.pauli_dense(n, [(i, 'x')]) @ full).real) < 1e-12
+                assert abs(ob['sy'][i] - np.vdot(full, orc.pauli_dense(n, [(i, 'y')]) @ full).real) < 1e-12
+                assert abs(ob['sz'][i] - np.vdot(full, orc.pauli_dense(n, [(i, 'z')]) @ full).real) < 1e-12
+                for j in range(n):
+                    ref = np.vdot(full, orc.sigma_pm_dense(n, i, j) @ full)
+                    assert abs(ob['xy_corr'][i, j] - ref) < 1e-12
+
+
+def test_entropy_vs_reduced_density_matrix():
+    """Entropy from Schmidt values equals -Tr ρ_A ln ρ_A from the eigenvalues
+    of ρ_A = M M†; Bell pair ln 2, product state 0."""
+    rng = np.random.default_rng(12)
+    for n, cut in ((4, 1), (5, 2), (6, 3)):
+        v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        v /= np.linalg.norm(v)
+        M = v.reshape(1 << cut, 1 << (n - cut))
+        lam = np.linalg.eigvalsh(M @ M.conj().T)
+        lam = lam[lam > 1e-15]
+        assert abs(orc.entropy_at_cut(n, v, cut) + np.sum(lam * np.log(lam))) < 1e-12
+        codes = np.arange(1 << n)
+        assert abs(orc.entropy_at_cut(n, v, cut, codes) - orc.entropy_at_cut(n, v, cut)) < 1e-12
+    bell = np.zeros(4, complex)
+    bell[0] = bell[3] = 1 / np.sqrt(2)
+    assert abs(orc.entropy_at_cut(2, bell, 1) - np.log(2)) < 1e-14
+    prod = np.kron([0.6, 0.8], [1.0, 0.0]).astype(complex)
+    assert abs(orc.entropy_at_cut(2, prod, 1)) < 1e-14
