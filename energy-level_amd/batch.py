@@ -173,7 +173,7 @@ def disorder_ensemble(count=512, nmin=10, nmax=20, sigma=0.1, deltas=None, c6=10
     for i in range(int(count)):
         n = int(rng.integers(nmin, nmax + 1))
         pos = chain_positions(n) + rng.normal(0.0, sigma, size=(n, 3))
-        calib = from_positions(pos, c6_coupling(c6))
+        calib = from_positions(pos, c6_coupling(c6), terms=('I1',))
         Delta = deltas[i % len(deltas)]
         out.append(from_calib(calib, omega * np.ones(n), -Delta * np.ones(n)))
     return out

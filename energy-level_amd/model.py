@@ -132,17 +132,63 @@ def c6_coupling(c6):
     return lambda r: {'V': c6 / r ** 6, 'J': 0.0, 'A': 0.0}
 
 
-def from_positions(positions, coupling, terms=('I1',)):
-    """Point cloud + coupling kernel → calib dict with pairs i<j."""
+def _coupling_triplet(value):
+    """(V, J, A) from a coupling kernel's return value: a dict, a (V, J, A)
+    tuple, or a scalar (pure exchange J), as calib_build.py:32-41."""
+    if isinstance(value, dict):
+        return (float(value.get('V') or 0.0), float(value.get('J') or 0.0),
+                float(value.get('A') or 0.0))
+    if isinstance(value, (tuple, list)):
+        if len(value) == 3:
+            return tuple(float(x) for x in value)
+        raise ValueError('a coupling tuple is (V, J, A), got %r' % (value,))
+    return (0.0, float(value), 0.0)
+
+
+def from_positions(positions, coupling, *, terms=('I2',), cutoff_um=None, periods_um=None,
+                   images=0, source='calib_build.from_positions'):
+    """Point cloud (N, 3) µm + coupling kernel → calib dict, as
+    calib_build.from_positions (calib_build.py:121-166): pairs i < j with
+    V, J, A summed over the periodic images (ix·Lx, iy·Ly), |ix|, |iy| <=
+    ``images`` when ``periods_um`` = (Lx, Ly) is given, images beyond
+    ``cutoff_um`` dropped (and counted), zero-distance self images skipped."""
     pts = np.asarray(positions, float)
-    pairs = []
-    for i in range(len(pts)):
-        for j in range(i + 1, len(pts)):
-            r = float(np.linalg.norm(pts[j] - pts[i]))
-            if r <= 0.0:
-                continue
-            c = coupling(r)
-            pairs.append({'i': i, 'j': j, 'V': float(c.get('V', 0.0)),
-                          'J': float(c.get('J', 0.0)), 'A': float(c.get('A', 0.0))})
+    if pts.ndim != 2 or pts.shape[1] != 3:
+        raise ValueError('positions must be an (N, 3) point cloud in µm, got %r' % (pts.shape,))
+    if pts.shape[0] == 0:
+        raise ValueError('positions is empty: a calib needs at least one atom')
+    shifts = [(0.0, 0.0)]
+    if periods_um is not None:
+        periods = np.asarray(periods_um, float)
+        if periods.shape != (2,) or np.any(periods <= 0):
+            raise ValueError('periods_um must be two positive numbers (Lx, Ly)')
+        k = int(images)
+        if k < 0:
+            raise ValueError('images must be non-negative')
+        shifts = [(ix * periods[0], iy * periods[1]) for ix in range(-k, k + 1)
+                  for iy in range(-k, k + 1)]
+    cutoff = None if cutoff_um is None else float(cutoff_um)
+    n = pts.shape[0]
+    pairs, dropped = [], 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            acc = np.zeros(3)
+            touched = False
+            for sx, sy in shifts:
+                r = float(np.linalg.norm(pts[j] - pts[i] + np.array([sx, sy, 0.0])))
+                if r <= 0.0:
+                    continue
+                if cutoff is not None and r > cutoff:
+                    dropped += 1
+                    continue
+                acc += np.asarray(_coupling_triplet(coupling(r)), float)
+                touched = True
+            if touched:
+                pairs.append({'i': i, 'j': j, 'V': float(acc[0]), 'J': float(acc[1]),
+                              'A': float(acc[2])})
     return {'atoms': [{'id': i, 'pos': [float(x) for x in p]} for i, p in enumerate(pts)],
-            'pairs': pairs, 'terms': list(terms)}
+            'pairs': pairs, 'terms': list(terms), 'source': str(source),
+            'dropped': [], 'dropped_amp': {},
+            'notes': {'n_dropped_by_cutoff': dropped,
+                      'periods_um': None if periods_um is None else list(map(float, periods_um)),
+                      'images': int(0 if periods_um is None else images)}}

@@ -19,7 +19,7 @@ __all__ = ['chain_calib', 'lattice_calib', 'warm_device_memory', 'run_c1', 'run_
 
 
 def chain_calib(n, c6):
-    return from_positions(chain_positions(n), c6_coupling(c6))
+    return from_positions(chain_positions(n), c6_coupling(c6), terms=('I1',))
 
 
 def _sync():
@@ -94,7 +94,7 @@ def run_c2(device='cuda', num=1024, nvec=16, seed=0, margin=0.01, dtypes=('float
     from .plan import Plan
     from .kpm import kpm_moments_hip, spectral_bounds, device_probes
     n = 25
-    calib = from_positions(square_positions(n), c6_coupling(20.0))
+    calib = from_positions(square_positions(n), c6_coupling(20.0), terms=('I1',))
     model = from_calib(calib, np.ones(n), -0.8 * np.ones(n))
     P = Plan(model, device=device)
     _sync()

@@ -172,3 +172,36 @@ def test_state_vector_mirrors_reference(pkg):
     with pytest.raises(ValueError):
         pkg.state_vector_np('ground', n)
     assert pkg.product_label(3, 0b101) == '|rgr>'
+
+
+def test_from_positions_mirrors_reference(pkg):
+    """Pairs from a point cloud as calib_build.from_positions: scalar kernel is
+    pure J, tuples are (V, J, A), periodic images summed per pair, images
+    past the cutoff dropped and counted, default terms ('I2',)."""
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 2.0, 0.0]])
+    c = pkg.from_positions(pos, lambda r: 1.0 / r)
+    assert c['terms'] == ['I2'] and len(c['pairs']) == 3
+    assert c['pairs'][0] == {'i': 0, 'j': 1, 'V': 0.0, 'J': 1.0, 'A': 0.0}
+    c = pkg.from_positions(pos, lambda r: (2.0 / r, 0.0, 0.5), terms=('I1', 'I3'))
+    assert c['pairs'][1]['V'] == 1.0 and c['pairs'][1]['A'] == 0.5
+    # periodic images: Lx = 3, Ly = 4, one image ring; sum of 1/r^2 by hand
+    L = (3.0, 4.0)
+    c = pkg.from_positions(pos, lambda r: {'V': 1.0 / r ** 2}, periods_um=L, images=1)
+    want = 0.0
+    for ix in (-1, 0, 1):
+        for iy in (-1, 0, 1):
+            d = pos[1] - pos[0] + np.array([ix * L[0], iy * L[1], 0.0])
+            want += 1.0 / float(d @ d)
+    assert abs(c['pairs'][0]['V'] - want) < 1e-14
+    assert c['notes']['images'] == 1 and c['notes']['periods_um'] == [3.0, 4.0]
+    c = pkg.from_positions(pos, lambda r: 1.0, cutoff_um=1.5)
+    assert [(p['i'], p['j']) for p in c['pairs']] == [(0, 1)]
+    assert c['notes']['n_dropped_by_cutoff'] == 2
+    for bad in (dict(periods_um=(1.0,)), dict(periods_um=(1.0, -1.0)),
+                dict(periods_um=(1.0, 1.0), images=-1)):
+        with pytest.raises(ValueError):
+            pkg.from_positions(pos, lambda r: 1.0, **bad)
+    with pytest.raises(ValueError):
+        pkg.from_positions(np.zeros((0, 3)), lambda r: 1.0)
+    with pytest.raises(ValueError):
+        pkg.from_positions(np.zeros((3, 2)), lambda r: 1.0)

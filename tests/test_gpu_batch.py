@@ -13,7 +13,7 @@ def _models(pkg, ns, seed=0):
     out, calibs = [], []
     for i, n in enumerate(ns):
         pos = pkg.chain_positions(n) + rng.normal(0, 0.1, size=(n, 3))
-        calib = pkg.from_positions(pos, pkg.c6_coupling(10.0))
+        calib = pkg.from_positions(pos, pkg.c6_coupling(10.0), terms=('I1',))
         De = np.linspace(-2, 4, 32)[i % 32]
         out.append(pkg.from_calib(calib, np.ones(n), -De * np.ones(n)))
         calibs.append((calib, De))

@@ -121,7 +121,7 @@ def test_unsorted_basis_indices_ground_state(pkg, dev):
     codes = np.asarray(orc.independent_states(n, edges), np.int64)
     perm = rng.permutation(len(codes))
     idx = codes[perm]
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(5.0))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(5.0), terms=('I1',))
     Om, delta = np.ones(n), -1.0 * np.ones(n)
     out = pkg.hip_lanczos_ground_state(calib, Om, delta, basis_indices=idx, return_state=True,
                                        device=dev)

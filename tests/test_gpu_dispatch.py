@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _system(pkg, n, c6=10.0, De=1.2, dev='cuda:0'):
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6), terms=('I1',))
     return dict(calib=calib, Om=np.ones(n), delta=-De * np.ones(n), device=dev)
 
 

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _model(pkg, n, c6=10.0):
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6), terms=('I1',))
     return calib, pkg.from_calib(calib, np.ones(n), np.zeros(n))
 
 
@@ -128,7 +128,7 @@ def test_segments_match_exact_exponentials(pkg, dev, interp, dtype, tol):
     """Per-site complex drives in three segments, output grid off the jumps."""
     n = 6
     rng = np.random.default_rng(11)
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
     segs = [dict(t0=t0, Om=rng.uniform(0.5, 3.0, n), phase=rng.uniform(0, 2 * np.pi, n),
                  delta=rng.uniform(-2, 2, n)) for t0 in (0.0, 70.0, 150.0)]
     t = np.array([0.0, 40.0, 100.0, 160.0, 230.0])

@@ -42,7 +42,7 @@ def _check(out, ref_states, n, codes, cut, pauli=None, link=None, tol=1e-9):
 
 def test_evolve_manybody_full_basis(pkg, dev):
     n = 8
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
     rng = np.random.default_rng(5)
     segs = _segments(n, rng)
     t = np.array([0.0, 30.0, 90.0])
@@ -88,7 +88,7 @@ def test_evolve_manybody_projected_plan(pkg, dev, via):
 def test_evolve_manybody_ground_start_static_drive(pkg, dev):
     """psi0='ground' at a static drive stays stationary (up to phase)."""
     n = 8
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
     Om, delta = np.ones(n), -1.0 * np.ones(n)
     out = pkg.evolve_manybody_hip(calib, psi0='ground', t_grid=[0.0, 20.0, 50.0], Om=Om,
                                   delta=delta, device=dev)

@@ -117,7 +117,7 @@ def test_correlations_64_sites_and_large_sector(pkg, dev):
 
 def test_evolution_records_pauli_and_corr(pkg, dev):
     n = 8
-    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0))
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
     rng = np.random.default_rng(9)
     segs = [dict(t0=t0, Om=rng.uniform(0.5, 2.0, n), phase=rng.uniform(0, 6.3, n),
                  delta=rng.uniform(-1, 1, n)) for t0 in (0.0, 40.0)]

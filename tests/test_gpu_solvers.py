@@ -8,7 +8,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _chain_calib(pkg, n, c6=10.0):
-    return pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6))
+    return pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(c6), terms=('I1',))
 
 
 def test_c1_chain12_ground_state_fp64(pkg, dev):
