@@ -21,12 +21,12 @@ from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip
 from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip, \
     spectrum_lanczos_hip, quadrature, lorentz_spectrum
 from .observables import normalize_pauli_strings, PauliPlanHip, measure_pauli_strings_hip, \
-    corr_matrix_hip, site_correlation_sums
+    corr_matrix_hip, site_correlation_sums, loop_observables, link_loop_observables
 from .manybody import evolve_manybody_hip, state_vector_np, product_label
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
     dos_from_moments, probe_bits
 
-__all__ = ['RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_positions',
+__all__ = ['loop_observables', 'link_loop_observables', 'RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_positions',
            'from_positions', 'c6_coupling', 'Plan', 'cast', 'hip_lanczos_ground_state',
            'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
            'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',

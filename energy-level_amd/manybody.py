@@ -16,8 +16,9 @@ matrix.  Initial states follow ``state_vector`` (state.py:112-160): None
 product dict or list of local states, or 'ground' (the device Lanczos ground
 state of the drive at t[0]).
 
-Not supported (NotImplementedError): a non-Hermitian drive (gamma != 0) and
-``loops`` (the loop-phase post-processing of observables.py:206-300).
+``loops`` are post-processed on the host from the measured bonds
+(observables.py:206-300).  Not supported (NotImplementedError): a
+non-Hermitian drive (gamma != 0).
 """
 from __future__ import annotations
 
@@ -201,7 +202,8 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     needs neither sectors nor a dense fallback)."""
     import torch
     from .evolve import propagate_segments_hip
-    from .observables import normalize_pauli_strings, PauliPlanHip, corr_matrix_hip
+    from .observables import normalize_pauli_strings, PauliPlanHip, corr_matrix_hip, \
+        loop_observables, link_loop_observables
     from .model import active_terms
     from .basis import sector_codes
     log = log or (lambda *a: None)
@@ -209,8 +211,6 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     n = len(atoms)
     if n == 0:
         raise ValueError('many-body evolution: calib has no atoms')
-    if loops is not None:
-        raise NotImplementedError('loop observables are not computed by the HIP engine')
     if link_pairs is not None:
         link_pairs = [(int(a), int(b)) for a, b in link_pairs]
         if any(a < 0 or a >= n or b < 0 or b >= n for a, b in link_pairs):
@@ -350,9 +350,15 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     use = set(active_terms(calib) if terms is None else terms)
     obs = None
     if want_observables:
-        obs = dict(sx=sx, sy=sy, sz=sz, sp=spv, sm=smv, phase=phase, xy_corr=state['xyp'],
-                   loop_phase=[], loop_winding=[], loop_phase_series=np.empty((0, 0)),
-                   loop_winding_series=np.empty((0, 0)), link_corr=link_series,
+        xy_last = state['xyp']
+        lp, lw = (loop_observables(xy_last, loops) if xy_last is not None and loops is not None
+                  else ([], []))
+        lps, lws = (link_loop_observables(link_series, link_pairs, loops)
+                    if link_series is not None and loops is not None
+                    else (np.empty((0, 0)), np.empty((0, 0))))
+        obs = dict(sx=sx, sy=sy, sz=sz, sp=spv, sm=smv, phase=phase, xy_corr=xy_last,
+                   loop_phase=lp, loop_winding=lw, loop_phase_series=lps,
+                   loop_winding_series=lws, link_corr=link_series,
                    link_pairs=link_pairs, pauli_strings=out.get('pauli'),
                    pauli_string_labels=out.get('pauli_labels', []))
     result = dict(

@@ -3,8 +3,10 @@
 Mirrors tomisnot/energy-level src/core/manybody/observables.py:
 ``corr_matrix`` (:40-47) as ``corr_matrix_hip``, ``normalize_pauli_strings``
 (:315-377), ``PauliPlan`` (:486-622, the full and basis paths) as
-``PauliPlanHip`` and ``measure_pauli_strings`` (:624-661, 1-D states) as
-``measure_pauli_strings_hip``.  The sums run on the device
+``PauliPlanHip``, ``measure_pauli_strings`` (:624-661, 1-D states) as
+``measure_pauli_strings_hip`` and the loop post-processing of :206-300
+(``loop_observables``, ``link_loop_observables``) on the host, over the
+link correlations the device measured.  The sums run on the device
 (``ryd_pauli_expect``, ``ryd_site_correlations``); no 2^n bit matrix or
 position table is built.
 """
@@ -15,7 +17,11 @@ import numpy as np
 from . import _native as nat
 
 __all__ = ['normalize_pauli_strings', 'PauliPlanHip', 'measure_pauli_strings_hip',
-           'corr_matrix_hip', 'site_correlation_sums']
+           'corr_matrix_hip', 'site_correlation_sums', 'loop_observables',
+           'link_loop_observables']
+
+# below this modulus a bond correlation has no phase (NaN), as observables.py:225
+PHASE_FLOOR = 1e-14
 
 _AXIS_ALIASES = {'x': 'x', 'y': 'y', 'z': 'z', 'sx': 'x', 'sy': 'y', 'sz': 'z',
                  'σx': 'x', 'σy': 'y', 'σz': 'z', 'sigmax': 'x', 'sigmay': 'y', 'sigmaz': 'z'}
@@ -154,3 +160,69 @@ def corr_matrix_hip(plan, x):
     nn = S / norm
     ni = np.diag(nn).copy()
     return nn - np.outer(ni, ni), ni
+
+
+def _loop_sites(raw, n=None):
+    q = [int(x) for x in raw]
+    if len(q) < 2 or (n is not None and any(not 0 <= x < n for x in q)):
+        raise ValueError('a loop needs at least two valid atom indices, got %r' % (list(raw),))
+    return q
+
+
+def loop_observables(xy_corr, loops):
+    """(phases, windings) of closed loops over a bond matrix
+    ``xy_corr[a, b] = <σ+_a σ−_b>`` (observables.py:206-229).
+
+    A loop lists atoms without repeating the first; its closing bond is
+    implied.  The phase is the sum of arg(xy_corr[a, b]) over its bonds (not
+    wrapped), the winding that sum over 2π; a bond of modulus at most
+    ``PHASE_FLOOR`` makes both NaN."""
+    if loops is None:
+        return [], []
+    xy = np.asarray(xy_corr, complex)
+    phases, windings = [], []
+    for raw in loops:
+        q = _loop_sites(raw, xy.shape[0])
+        z = xy[q, q[1:] + q[:1]]
+        if np.all(np.abs(z) > PHASE_FLOOR):
+            total = float(np.sum(np.angle(z)))
+            phases.append(total)
+            windings.append(total / (2.0 * np.pi))
+        else:
+            phases.append(np.nan)
+            windings.append(np.nan)
+    return phases, windings
+
+
+def link_loop_observables(link_corr, link_pairs, loops):
+    """(phase, winding) arrays of shape (times, loops) from a series of
+    measured links (observables.py:232-279).
+
+    ``link_corr[t, q] = <σ+_a σ−_b>`` for ``link_pairs[q] = (a, b)``; a bond
+    walked as (b, a) uses the conjugate, the first listed pair wins, and a
+    loop with a bond that no pair covers stays NaN.  At a time where any of
+    its bonds is at most ``PHASE_FLOOR`` in modulus the loop is NaN."""
+    if loops is None:
+        return np.empty((0, 0), float), np.empty((0, 0), float)
+    where = {}
+    for q, (a, b) in enumerate((int(a), int(b)) for a, b in (link_pairs or [])):
+        where.setdefault((a, b), (q, False))
+        where.setdefault((b, a), (q, True))
+    arr = np.asarray(link_corr, complex)
+    if arr.ndim == 1:
+        arr = arr[None, :]
+    phase = np.full((arr.shape[0], len(loops)), np.nan)
+    winding = np.full_like(phase, np.nan)
+    for ell, raw in enumerate(loops):
+        q = _loop_sites(raw)
+        bonds = [where.get(e) for e in zip(q, q[1:] + q[:1])]
+        if any(b is None for b in bonds):
+            continue
+        cols = arr[:, [c for c, _ in bonds]]
+        flip = np.array([f for _, f in bonds])
+        cols = np.where(flip[None, :], np.conj(cols), cols)
+        ok = np.all(np.abs(cols) > PHASE_FLOOR, axis=1)
+        total = np.angle(cols).sum(axis=1)
+        phase[ok, ell] = total[ok]
+        winding[ok, ell] = total[ok] / (2.0 * np.pi)
+    return phase, winding

@@ -205,3 +205,55 @@ def test_from_positions_mirrors_reference(pkg):
         pkg.from_positions(np.zeros((0, 3)), lambda r: 1.0)
     with pytest.raises(ValueError):
         pkg.from_positions(np.zeros((3, 2)), lambda r: 1.0)
+
+
+def _vortex_bonds(n, winding):
+    """<σ+_a σ−_b> of the product state ⊗_k (|g> + e^{iφ_k}|r>)/√2 with
+    φ_k = 2π·winding·k/n: (1/4)·e^{i(φ_b − φ_a)} for a ≠ b."""
+    phi = 2 * np.pi * winding * np.arange(n) / n
+    return 0.25 * np.exp(1j * (phi[None, :] - phi[:, None]))
+
+
+@pytest.mark.parametrize('w', [1, -1, 2])
+def test_loop_observables_count_vortex_windings(pkg, w):
+    n = 6
+    xy = _vortex_bonds(n, w)
+    ring = list(range(n))
+    ph, wd = pkg.loop_observables(xy, [ring, ring[::-1], [0, 3]])
+    # every bond's phase step 2πw/6 lies in (−π, π] for |w| ≤ 2, so the
+    # wrapped sum around the ring is exactly 2πw, reversed −2πw; a
+    # two-site loop goes there and back (zero)
+    assert np.allclose(ph, [2 * np.pi * w, -2 * np.pi * w, 0.0], atol=1e-12)
+    assert np.allclose(wd, [w, -w, 0.0], atol=1e-12)
+    xy[2, 3] = 1e-15
+    ph, wd = pkg.loop_observables(xy, [ring, [0, 1, 2]])
+    assert np.isnan(ph[0]) and np.isnan(wd[0]) and np.isfinite(ph[1])
+    assert pkg.loop_observables(xy, None) == ([], [])
+    for bad in ([0], [0, n], [-1, 2]):
+        with pytest.raises(ValueError):
+            pkg.loop_observables(xy, [bad])
+
+
+def test_link_loop_observables_orientation_and_gaps(pkg):
+    n = 4
+    xys = [_vortex_bonds(n, 1), _vortex_bonds(n, -1)]
+    xys[1][3, 0] = 0.0                 # no phase on the closing bond at t1
+    pairs = [(0, 1), (2, 1), (2, 3), (3, 0), (1, 0)]
+    series = np.array([[xy[a, b] for a, b in pairs] for xy in xys])
+    loops = [[0, 1, 2, 3], [3, 2, 1, 0], [0, 2, 3]]
+    ph, wd = pkg.link_loop_observables(series, pairs, loops)
+    assert ph.shape == wd.shape == (2, 3)
+    # (2, 1) walked as 1 -> 2 takes the conjugate; (0, 1) is listed before
+    # (1, 0), so the bond 1 -> 0 is conj(<σ+_0 σ−_1>)
+    assert np.allclose(wd[0, :2], [1, -1], atol=1e-12)
+    assert np.isnan(wd[1, :2]).all()            # a bond of zero modulus
+    assert np.isnan(ph[:, 2]).all()             # bond 0 -> 2 not measured
+    # the same loops from the full matrix agree where every bond is measured
+    ph0, _ = pkg.loop_observables(xys[0], loops[:2])
+    assert np.allclose(ph[0, :2], ph0, atol=1e-12)
+    one, _ = pkg.link_loop_observables(series[0], pairs, loops[:1])
+    assert one.shape == (1, 1) and np.isclose(one[0, 0], 2 * np.pi)
+    e1, e2 = pkg.link_loop_observables(series, pairs, None)
+    assert e1.shape == (0, 0) and e2.shape == (0, 0)
+    with pytest.raises(ValueError):
+        pkg.link_loop_observables(series, pairs, [[1]])

@@ -98,3 +98,32 @@ def test_evolve_manybody_ground_start_static_drive(pkg, dev):
         pkg.evolve_manybody_hip(calib, t_grid=[0.0, 1.0], gamma=np.ones(n), device=dev)
     with pytest.raises(ValueError):
         pkg.evolve_manybody_hip(calib, t_grid=[0.0, 1.0], entropy_cut=n, device=dev)
+
+
+def test_evolve_manybody_loops_vortex(pkg, dev):
+    """loops: a vortex product state winds once at t0 (closed form); later
+    times follow the oracle's propagated state."""
+    n = 4
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
+    phi = 2 * np.pi * np.arange(n) / n
+    local = [np.array([1.0, np.exp(1j * p)]) / np.sqrt(2) for p in phi]
+    psi0 = local[0]
+    for v in local[1:]:
+        psi0 = np.kron(psi0, v)
+    rng = np.random.default_rng(9)
+    segs = _segments(n, rng)
+    t = np.array([0.0, 0.2, 0.5])
+    loops = [[0, 1, 2, 3], [3, 2, 1, 0], [0, 2]]
+    pairs = [(0, 1), (2, 1), (2, 3), (3, 0)]
+    out = pkg.evolve_manybody_hip(calib, psi0=psi0, t_grid=t, segments=segs, loops=loops,
+                                  link_pairs=pairs, device=dev)
+    O = out['observables']
+    assert np.allclose(O['loop_winding_series'][0, :2], [1.0, -1.0], atol=1e-9)
+    assert np.isnan(O['loop_winding_series'][:, 2]).all()       # (0, 2) is not a listed link
+    ref = orc.propagate_segments(n, calib['pairs'], ['I1'], segs, psi0, t)
+    for k in range(len(t)):
+        xy = orc.spin_observables(n, ref[k])['xy_corr']
+        ph, _ = pkg.loop_observables(xy, loops[:2])
+        assert np.allclose(O['loop_phase_series'][k, :2], ph, atol=1e-8)
+    ph, wd = pkg.loop_observables(orc.spin_observables(n, ref[-1])['xy_corr'], loops)
+    assert np.allclose(O['loop_phase'], ph, atol=1e-8) and np.allclose(O['loop_winding'], wd, atol=1e-8)
