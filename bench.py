@@ -28,9 +28,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes of one FP64 H.v at N=28 (both launches), measured with rocprofv3
-# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 9.20 + 2.28 GB,
+# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 4.03 + 2.21 GB,
 # pass B 4.30 + 2.15 GB
-MEASURED_TRAFFIC_F64_N28 = 17.92e9
+MEASURED_TRAFFIC_F64_N28 = 12.68e9
 N_SITES = 28
 
 
@@ -208,7 +208,7 @@ def main():
                      'traffic_source': ('rocprofv3 --pmc FETCH_SIZE x2.00 + WRITE_SIZE x1.00 '
                                         '(plain-copy scale), both launches of one H.v'
                                         if n == 28 else None),
-                     'kernel': 'k_full_tiled<double> + k_full_top<double> (one H.v)',
+                     'kernel': 'k_full_tiled<double, 10> + k_full_top<double> (one H.v)',
                      'alg_bytes_per_launch': f64['alg_bytes'],
                      'kernel_ms': f64['kernel_ms']},
         'fp32': {'hv_per_s': res['f32']['hv_per_s'], 'ms_per_step': res['f32']['ms_per_step'],

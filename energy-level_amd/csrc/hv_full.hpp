@@ -9,11 +9,11 @@
 // :114-118 (Ω flips: target bit 1 takes Ω/2, target bit 0 takes conj(Ω)/2).
 // Bit position p (LSB = 0) is site n-1-p (site 0 is the MSB).
 //
-// Tiling (per 256-thread block, one tile of 2^TB consecutive codes):
-//   element e = r·1024 + t·4 + c  (t = thread, c = bits 0..1, r = bits 10..TB-1)
-//   bits 0..1 and 10..TB-1 flip inside a thread's registers, bits 2..9 flip
-//   across threads through LDS, bits >= TB read the partner tile through
-//   L2/MALL.  The diagonal is rebuilt on the fly:
+// Tiling (per block of 2^TT threads, one tile of 2^TB consecutive codes):
+//   element e = r·4·2^TT + t·4 + c  (t = thread, c = bits 0..1, r = bits
+//   2+TT..TB-1); bits 0..1 and the r bits flip inside a thread's registers,
+//   bits 2..TT+1 flip across threads through LDS, bits >= TB read the partner
+//   tile through L2/MALL.  The diagonal is rebuilt on the fly:
 //     d(e) = E_hi[tile] + E_lo[e] + T0[e & 255] + T1[e >> 8]
 //   where T0/T1 hold the cross field Σ_{q>=TB} V_pq b_q of this tile on the
 //   low bits, built per tile in LDS.  Tiles are mapped XCD-aware: block b runs
@@ -81,23 +81,46 @@ __device__ __forceinline__ void apply_scalars(const ApplyIO<V>& io, double& s, d
 #ifndef RYD_FAR_UNROLL
 #define RYD_FAR_UNROLL 1
 #endif
+// Pass A of a split H·v (n >= 23) runs blocks of 2^TT threads, so its tiles
+// are 2^(TT-8) times the single-pass tile and fewer bits are far: 1024
+// threads give 15-bit FP32, 14-bit FP64 and 13-bit complex128 tiles (128 KB
+// of LDS, one block per CU).  Each far bit re-reads x through L2 (about
+// 0.085 ms per bit for FP64 at n = 28), which the wider tile saves twice
+// over: 1024 threads beat 256 by 0.07 ms (FP64) and 0.10 ms (FP32) per H·v.
+#ifndef RYD_PASSA_TT4
+#define RYD_PASSA_TT4 10
+#endif
+#ifndef RYD_PASSA_TT8
+#define RYD_PASSA_TT8 10
+#endif
+#ifndef RYD_PASSA_TT16
+#define RYD_PASSA_TT16 10
+#endif
+template <class V> constexpr int passa_tt() {
+  return sizeof(V) == 4 ? RYD_PASSA_TT4 : (sizeof(V) == 16 ? RYD_PASSA_TT16 : RYD_PASSA_TT8);
+}
+constexpr int NTBW = 6;          // diagonal table widths 10..15
+
 template <class V> constexpr int tile_r() {
   return sizeof(V) >= 16 ? 2 : (sizeof(V) == 8 ? RYD_TILE_R8 : RYD_TILE_R4);
 }
-template <class V> constexpr int tile_bits() {
-  return 10 + (tile_r<V>() >= 8 ? 3 : (tile_r<V>() >= 4 ? 2 : (tile_r<V>() >= 2 ? 1 : 0)));
+// Tile bits of a block of 2^TT threads (TT = 8: the single-pass tile).
+template <class V, int TT = 8> constexpr int tile_bits() {
+  return 2 + TT + (tile_r<V>() >= 8 ? 3 : (tile_r<V>() >= 4 ? 2 : (tile_r<V>() >= 2 ? 1 : 0)));
 }
 
-template <class V>
-__global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, Reduce rd) {
+template <class V, int TT = 8>
+__global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
   constexpr int R = tile_r<V>();
-  constexpr int TB = tile_bits<V>();
-  constexpr int TILE = 1024 * R;
+  constexpr int TB = tile_bits<V, TT>();
+  constexpr int GS = 4 << TT;                  // codes per register group
+  constexpr int TILE = GS * R;
+  static_assert(TT >= 8 && TT <= 10 && TB <= 15, "tiles: 256..1024 threads, at most 2^15 codes");
   __shared__ V lds[TILE];
   __shared__ double Fx[16];
   __shared__ double T0[256];
-  __shared__ double T1[16];
+  __shared__ double T1[1 << (TB - 8)];
 
   if (io.stop && load_coherent(io.stop)) return;
   double s, ch, cx, cz;
@@ -115,13 +138,13 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
   V xs[R][4];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + base + r * 1024 + t * 4);
+    V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + base + r * GS + t * 4);
 #pragma unroll
     for (int c = 0; c < 4; ++c) xs[r][c] = vscale(q.v[c], sr);
     V4<V> w;
 #pragma unroll
     for (int c = 0; c < 4; ++c) w.v[c] = xs[r][c];
-    *reinterpret_cast<V4<V>*>(lds + r * 1024 + t * 4) = w;
+    *reinterpret_cast<V4<V>*>(lds + r * GS + t * 4) = w;
   }
 
   // per-tile cross field of the high bits on the low bits
@@ -133,12 +156,14 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
     Fx[t] = f;
   }
   __syncthreads();
-  {
+  if (t < 256) {
     double v = 0.0;
 #pragma unroll
     for (int p = 0; p < 8; ++p)
       if ((t >> p) & 1) v += Fx[p];
     T0[t] = v;
+  }
+  {
     if (t < (1 << (TB - 8))) {
       double w = 0.0;
 #pragma unroll
@@ -154,7 +179,7 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
   V acc[R][4];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int e0 = r * 1024 + t * 4;
+    const int e0 = r * GS + t * 4;
     V4<double> el = *reinterpret_cast<const V4<double>*>(M.elo + e0);
     const double t1 = T1[e0 >> 8];
 #pragma unroll
@@ -176,10 +201,10 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
         vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r][c ^ (1 << p)]);
       }
   }
-  // bits 10..TB-1: across the thread's r groups
+  // bits 2+TT..TB-1: across the thread's r groups
 #pragma unroll
   for (int k = 0; (1 << k) < R; ++k) {
-    const int p = 10 + k;
+    const int p = 2 + TT + k;
     const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -188,9 +213,9 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
       for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r ^ (1 << k)][c]);
     }
   }
-  // bits 2..9: partner thread through LDS
+  // bits 2..TT+1: partner thread through LDS
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < TT; ++k) {
     const int p = 2 + k;
     const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
     const int bit = (t >> k) & 1;
@@ -198,7 +223,7 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
     const int tp = t ^ (1 << k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      V4<V> q = *reinterpret_cast<const V4<V>*>(lds + r * 1024 + tp * 4);
+      V4<V> q = *reinterpret_cast<const V4<V>*>(lds + r * GS + tp * 4);
 #pragma unroll
       for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
     }
@@ -213,7 +238,7 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
     const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + pbase + r * 1024 + t * 4);
+      V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + pbase + r * GS + t * 4);
 #pragma unroll
       for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
     }
@@ -223,7 +248,7 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) nt_store(io.y + base + r * 1024 + t * 4 + c, acc[r][c]);
+      for (int c = 0; c < 4; ++c) nt_store(io.y + base + r * GS + t * 4 + c, acc[r][c]);
     return;
   }
   // combine, store, reduce
@@ -231,7 +256,7 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
   const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint64_t g0 = base + r * 1024 + t * 4;
+    const uint64_t g0 = base + r * GS + t * 4;
     V4<V> zq;
     if (io.z) zq = *reinterpret_cast<const V4<V>*>(io.z + g0);
     uint32_t pw = 0;
@@ -260,16 +285,38 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
 // (one 128- or 256-byte run per combination), so each top-bit flip is an LDS
 // or register exchange.  It reads the pass-A partial sum from y and finishes
 //   y = ch·(y_A + Σ_top flips of s·x) + cx·(s·x) + cz·z  with the reductions.
-// Thread u (0..511): c = u & (2^CB-1) selects the code inside the run, u >> CB
-// gives the low (9-CB) top bits, register slot j the remaining top bits.
-// Measured on MI355X at n = 28 (FP64 / FP32 ms per H·v): 9 top bits with
-// 128-byte runs 3.01 / 1.72; 8 bits with 256-byte FP64 and 128-byte FP32 runs
-// 2.95 / 1.52; 7 or 6 top bits lose in FP64 (pass A's far partners leave L2).
-#ifndef RYD_TOPB
-#define RYD_TOPB 8
+// Thread u (0..2^UB-1): c = u & (2^CB-1) selects the code inside the run,
+// u >> CB gives the low (UB-CB) top bits, register slot j the remaining top
+// bits.  Per element size (4 / 8 / 16 bytes): TOPB, UB and CB below.
+// Measured on MI355X at n = 28, FP64 / FP32 ms per H·v, with 14-bit FP64 and
+// 15-bit FP32 pass-A tiles (CB = 5 unless stated): 8 top bits, 512 threads
+// 2.88 / 1.37; 8 bits, 1024 threads 2.80 / 1.44; 9 bits, 1024 threads
+// 2.64 / 1.40; 9 bits, 512 threads 3.11 / 1.58 (one block per CU); FP64 with
+// 128-byte runs (CB = 4): 9 bits 2.79 (1024 threads), 2.84 (512); FP32 with
+// 256-byte runs (CB = 6): 8 bits 1.32 (1024 threads), 1.52 (512).  With 12-
+// and 13-bit pass-A tiles: 8 bits, 512 threads 2.95 / 1.48; 9 bits with
+// 128-byte runs 3.01 / 1.72; 7 or 6 top bits lose (pass A's far partners
+// leave L2).
+#ifndef RYD_TOPB4
+#define RYD_TOPB4 8
+#endif
+#ifndef RYD_TOPB8
+#define RYD_TOPB8 9
+#endif
+#ifndef RYD_TOPB16
+#define RYD_TOPB16 9
+#endif
+#ifndef RYD_TOP_UB4
+#define RYD_TOP_UB4 10
+#endif
+#ifndef RYD_TOP_UB8
+#define RYD_TOP_UB8 10
+#endif
+#ifndef RYD_TOP_UB16
+#define RYD_TOP_UB16 10
 #endif
 #ifndef RYD_RUNB4
-#define RYD_RUNB4 5
+#define RYD_RUNB4 6
 #endif
 #ifndef RYD_RUNB8
 #define RYD_RUNB8 5
@@ -277,21 +324,29 @@ __global__ __launch_bounds__(256) void k_full_tiled(FullModel M, ApplyIO<V> io, 
 #ifndef RYD_RUNB16
 #define RYD_RUNB16 3
 #endif
-constexpr int TOPB = RYD_TOPB;
+template <class V> constexpr int top_bits() {
+  return sizeof(V) == 4 ? RYD_TOPB4 : (sizeof(V) == 16 ? RYD_TOPB16 : RYD_TOPB8);
+}
+template <class V> constexpr int top_ub() {
+  return sizeof(V) == 4 ? RYD_TOP_UB4 : (sizeof(V) == 16 ? RYD_TOP_UB16 : RYD_TOP_UB8);
+}
 template <class V> constexpr int run_bits() {
   return sizeof(V) == 4 ? RYD_RUNB4 : (sizeof(V) == 16 ? RYD_RUNB16 : RYD_RUNB8);
 }
-static_assert(TOPB >= 1 && TOPB <= 9, "top pass: 1..9 bits");
-static_assert(9 - RYD_RUNB4 <= TOPB && 9 - RYD_RUNB8 <= TOPB && 9 - RYD_RUNB16 <= TOPB,
-              "top pass: the thread index holds at most TOPB top bits");
 
 template <class V>
-__global__ __launch_bounds__(512) void k_full_top(FullModel M, ApplyIO<V> io, Reduce rd) {
+__global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
+  constexpr int TOPB = top_bits<V>();
+  constexpr int UB = top_ub<V>();
+  constexpr int NU = 1 << UB;                // threads
   constexpr int CB = run_bits<V>();
-  constexpr int TU = 9 - CB;                 // top bits carried by the thread index
+  constexpr int TU = UB - CB;                // top bits carried by the thread index
+  static_assert(TOPB >= 1 && TOPB <= 10 && UB >= 8 && UB <= 10, "top pass: 1..10 bits, 256..1024 threads");
+  static_assert(TU >= 0 && TU <= TOPB, "top pass: the thread index holds at most TOPB top bits");
   constexpr int J = 1 << (TOPB - TU);        // register slots per thread
-  __shared__ V lds[512 * J];
+  static_assert((size_t)NU * J * sizeof(V) <= 160 * 1024, "top pass: LDS");
+  __shared__ V lds[NU * J];
   if (io.stop && load_coherent(io.stop)) return;
   double s, ch, cx, cz;
   apply_scalars(io, s, ch, cx, cz);
@@ -307,7 +362,7 @@ __global__ __launch_bounds__(512) void k_full_top(FullModel M, ApplyIO<V> io, Re
   const uint64_t jstride = 1ull << (S + TU);
   // stage the scaled x of all top-bit combinations in LDS
 #pragma unroll 8
-  for (int j = 0; j < J; ++j) lds[u + 512 * j] = vscale(io.x[ebase + j * jstride], sr);
+  for (int j = 0; j < J; ++j) lds[u + NU * j] = vscale(io.x[ebase + j * jstride], sr);
   __syncthreads();
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
@@ -320,16 +375,16 @@ __global__ __launch_bounds__(512) void k_full_top(FullModel M, ApplyIO<V> io, Re
       const int p = S + k;
       const RT ai = (RT)M.ai[p];
       const int bit = (int)((tlo >> k) & 1ull);
-      vcfma(a, (RT)M.ar[p], bit ? ai : -ai, lds[(u ^ (1 << (CB + k))) + 512 * j]);
+      vcfma(a, (RT)M.ar[p], bit ? ai : -ai, lds[(u ^ (1 << (CB + k))) + NU * j]);
     }
 #pragma unroll
     for (int k = 0; k < TOPB - TU; ++k) {    // partner in another register slot
       const int p = S + TU + k;
       const RT ai = (RT)M.ai[p];
       const int bit = (j >> k) & 1;
-      vcfma(a, (RT)M.ar[p], bit ? ai : -ai, lds[u + 512 * (j ^ (1 << k))]);
+      vcfma(a, (RT)M.ar[p], bit ? ai : -ai, lds[u + NU * (j ^ (1 << k))]);
     }
-    const V xj = lds[u + 512 * j];
+    const V xj = lds[u + NU * j];
     V v = vscale(a, chr);
     vfma(v, cxr, xj);
     if (io.z) vfma(v, czr, io.z[e]);

@@ -59,10 +59,10 @@ struct ryd_plan {
   double ar[MAXN] = {0}, ai[MAXN] = {0};   // per bit position, already halved
   double* d_vb = nullptr;            // [n*n] V by bit position
   // diagonal tables per tile width tb = 10 + k (built for the widths the
-  // element types use; tbits[k] = min(n, 10 + k))
-  double* d_elo[4] = {nullptr, nullptr, nullptr, nullptr};
-  double* d_ehi[4] = {nullptr, nullptr, nullptr, nullptr};
-  int tbits[4] = {0, 0, 0, 0};
+  // element types and split passes use; tbits[k] = min(n, 10 + k))
+  double* d_elo[NTBW] = {};
+  double* d_ehi[NTBW] = {};
+  int tbits[NTBW] = {};
   size_t ws_bytes = 0;
   // chain-blockade sector
   uint32_t* d_lo_code = nullptr;     // [F(L+2)] low-part codes
@@ -110,6 +110,7 @@ struct ryd_plan {
 
 namespace {
 
+// tile bits of the single-pass kernel and of pass A of a split H·v
 int tile_bits_of(int dtype) {
   return dtype == RYD_F32 ? tile_bits<float>()
          : dtype == RYD_F64 ? tile_bits<double>()
@@ -117,14 +118,21 @@ int tile_bits_of(int dtype) {
                             : tile_bits<c64>();
 }
 
-int tb_index(const ryd_plan* P, int dtype) {
-  (void)P;
-  return tile_bits_of(dtype) - 10;
+int passa_bits_of(int dtype) {
+  return dtype == RYD_F32 ? tile_bits<float, passa_tt<float>()>()
+         : dtype == RYD_F64 ? tile_bits<double, passa_tt<double>()>()
+         : dtype == RYD_C64 ? tile_bits<c32, passa_tt<c32>()>()
+                            : tile_bits<c64, passa_tt<c64>()>();
 }
 
-FullModel full_model(const ryd_plan* P, int dtype) {
+size_t elem_bytes(int dtype) {
+  return dtype == RYD_F32 ? 4 : (dtype == RYD_C128 ? 16 : 8);
+}
+
+// diagonal tables of tile width `width`
+FullModel full_model(const ryd_plan* P, int width) {
   FullModel M;
-  const int k = tb_index(P, dtype);
+  const int k = width - 10;
   M.n = P->n;
   M.tb = P->tbits[k];
   M.elo = P->d_elo[k];
@@ -159,7 +167,7 @@ void build_tables(int n, int tb, const std::vector<double>& dl, const std::vecto
   }
 }
 
-// Split threshold: from this n on, the top TOPB bits run in a second pass
+// Split threshold: from this n on, the top bits run in a second pass
 // (k_full_top) so that pass A's partner tiles stay within the caches.  The
 // split costs a write and a read of the partial: it pays from n = 23 for
 // 8- and 16-byte elements, from n = 25 for an FP32 H·v, and only beyond
@@ -168,7 +176,7 @@ void build_tables(int n, int tb, const std::vector<double>& dl, const std::vecto
 #define RYD_SPLIT_MIN_N 23
 #endif
 constexpr int SPLIT_MIN_N = RYD_SPLIT_MIN_N;
-inline int split_min_n(size_t elem, bool with_z) {
+constexpr int split_min_n(size_t elem, bool with_z) {
   if (elem >= 8) return SPLIT_MIN_N;
   return with_z ? SPLIT_MIN_N + 8 : SPLIT_MIN_N + 2;
 }
@@ -370,26 +378,41 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
     LAUNCH_CHECK();
     return RYD_OK;
   }
-  FullModel M = full_model(P, dtype);
   constexpr int TB = tile_bits<V>();
   if (P->n >= TB) {
-    const uint64_t ntiles = 1ull << (P->n - TB);
     const bool split = P->n >= split_min_n(sizeof(V), io.z != nullptr);
-    io.nfar = split ? P->n - TOPB - TB : P->n - TB;
-    io.partial = split ? 1 : 0;
-    k_full_tiled<V><<<dim3((unsigned)ntiles), dim3(256), 0, st>>>(M, io, rd);
-    LAUNCH_CHECK();
-    unsigned grid = (unsigned)ntiles;
+    unsigned grid;
     if (split) {
+      // pass A: tiles of 2^PTB codes, far bits up to the top pass
+      constexpr int PTT = passa_tt<V>();
+      constexpr int PTB = tile_bits<V, PTT>();
+      static_assert(PTB + top_bits<V>() <= split_min_n(sizeof(V), false),
+                    "split: pass-A tile and top bits fit the smallest split n");
+      FullModel M = full_model(P, PTB);
+      io.nfar = P->n - top_bits<V>() - PTB;
+      io.partial = 1;
+      k_full_tiled<V, PTT><<<dim3((unsigned)(1ull << (P->n - PTB))), dim3(1 << PTT), 0, st>>>(M, io, rd);
+      LAUNCH_CHECK();
+    } else {
+      FullModel M = full_model(P, TB);
+      io.nfar = P->n - TB;
       io.partial = 0;
-      const uint64_t nb = 1ull << (P->n - TOPB - run_bits<V>());
-      k_full_top<V><<<dim3((unsigned)nb), dim3(512), 0, st>>>(M, io, rd);
+      grid = (unsigned)(1ull << (P->n - TB));
+      k_full_tiled<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+      LAUNCH_CHECK();
+    }
+    if (split) {
+      FullModel M = full_model(P, TB);
+      io.partial = 0;
+      const uint64_t nb = 1ull << (P->n - top_bits<V>() - run_bits<V>());
+      k_full_top<V><<<dim3((unsigned)nb), dim3(1 << top_ub<V>()), 0, st>>>(M, io, rd);
       LAUNCH_CHECK();
       grid = (unsigned)nb;
     }
     if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
     LAUNCH_CHECK();
   } else {
+    FullModel M = full_model(P, TB);
     const uint64_t D = 1ull << P->n;
     const unsigned grid = (unsigned)((D + 255) / 256);
     io.nfar = 0;
@@ -1102,9 +1125,15 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
     *out = P;
     return RYD_OK;
   }
+  bool want[NTBW] = {};
   for (int dt = RYD_F32; dt <= RYD_C128; ++dt) {
-    const int k = tb_index(P, dt);
-    if (P->d_elo[k]) continue;
+    want[tile_bits_of(dt) - 10] = true;
+    const size_t eb = elem_bytes(dt);
+    const int smin = std::min(split_min_n(eb, false), split_min_n(eb, true));
+    if (n >= smin) want[passa_bits_of(dt) - 10] = true;
+  }
+  for (int k = 0; k < NTBW; ++k) {
+    if (!want[k]) continue;
     const int tb = n < 10 + k ? n : 10 + k;
     P->tbits[k] = tb;
     std::vector<double> elo, ehi;
@@ -1155,7 +1184,7 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_lrow) (void)hipFree(P->d_lrow);
   if (P->d_tp) (void)hipFree(P->d_tp);
   if (P->d_tpk) (void)hipFree(P->d_tpk);
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < NTBW; ++k) {
     if (P->d_elo[k]) (void)hipFree(P->d_elo[k]);
     if (P->d_ehi[k]) (void)hipFree(P->d_ehi[k]);
   }
@@ -1543,8 +1572,8 @@ namespace {
 
 template <class V>
 int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t st) {
-  FullModel M = full_model(P, dtype);
   constexpr int TB = tile_bits<V>();
+  FullModel M = full_model(P, TB);
   unsigned grid;
   if (P->n >= TB) {
     // below ~1000 tiles one tile per block leaves most CUs idle: spread each

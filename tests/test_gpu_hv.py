@@ -174,3 +174,20 @@ def test_hv_split_complex_and_fused_sums(pkg, dev, n):
     assert np.max(np.abs(yr - ref)) <= 1e-13 * np.max(np.abs(ref))
     S = sums.cpu().numpy()
     assert np.allclose(S[:3], [(s * xr) @ ref, ref @ ref, (s * xr) @ (s * xr)], rtol=1e-11)
+
+
+@pytest.mark.parametrize('n', [23, 25])
+def test_hv_split_random_model_vs_c_oracle(pkg, dev, n):
+    """Two-pass H·v with every site's Ω, Δ and random couplings distinct, so
+    a flip or diagonal term on the wrong bit changes the result; n = 25 runs
+    the FP32 split as well."""
+    import torch
+    Om, delta, pairs = _rand_model(n, 40 + n)
+    P = _plan(pkg, dev, n, Om, delta, pairs)
+    d = hv_ref.diag(n, delta, pairs)
+    x = np.random.default_rng(n).standard_normal(1 << n)
+    ref = hv_ref.hv_full(n, Om, d, x)
+    y = P.hv(torch.as_tensor(x, device=dev)).cpu().numpy()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    y32 = P.hv(torch.as_tensor(x.astype(np.float32), device=dev)).cpu().numpy()
+    assert np.max(np.abs(y32 - ref)) <= 1e-5 * np.max(np.abs(ref))
