@@ -85,6 +85,27 @@ __device__ __forceinline__ void nt_store(c64* p, c64 v) {
   __builtin_nontemporal_store(v.im, &p->im);
 }
 
+// Raw buffer loads over [p, p + bytes): the hardware range check returns zero
+// for an offset at or past the end, so a partner range shorter than the
+// block's own needs neither a clamp nor a branch.  (Buffer descriptor word 3
+// of the gfx9 family: 32-bit data format, no swizzle.)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <class V> __device__ __forceinline__ V buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ c32 buf_load<c32>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(c32, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <> __device__ __forceinline__ c64 buf_load<c64>(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(c64, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -186,6 +186,9 @@ constexpr int split_min_n(size_t elem, bool with_z) {
 #define RYD_FIB_SPLIT_MIN_N 38
 #endif
 constexpr int FIB_SPLIT_MIN_N = RYD_FIB_SPLIT_MIN_N;
+#ifndef RYD_FIB_SPLIT4
+#define RYD_FIB_SPLIT4 0
+#endif
 
 int64_t fib(int k) {
   int64_t a = 0, b = 1;
@@ -361,11 +364,14 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
     io.nfar = 0;
     // the top-bit pass pays for 8- and 16-byte elements only (FP32 blocks
     // read their far partners fast enough in one pass)
-    const bool split = P->fib_split && sizeof(V) >= 8;
+    const bool split = P->fib_split && (sizeof(V) >= 8 || RYD_FIB_SPLIT4);
     FibModel M = fib_model(P);
     if (!split) M.far_end = P->n;
     io.partial = split ? 1 : 0;
-    k_fib_apply<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
+    if (P->fib_L == FIB_L && RYD_FIB_FL)
+      k_fib_apply_fl<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
+    else
+      k_fib_apply<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
     LAUNCH_CHECK();
     unsigned grid = fib_grid(P);
     if (split) {

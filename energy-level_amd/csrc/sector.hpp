@@ -112,6 +112,12 @@ __device__ inline void fib_cross_tables(const FibModel& M, uint64_t h, double* F
 #ifndef RYD_FIB_BATCH
 #define RYD_FIB_BATCH 1
 #endif
+#ifndef RYD_FIB_FL
+#define RYD_FIB_FL 1
+#endif
+#ifndef RYD_FIB_FL_WAVES
+#define RYD_FIB_FL_WAVES(V) (sizeof(V) >= 8 ? 2 : 4)
+#endif
 constexpr int FIB_IB = 8;          // inner bits: groups of F(10) = 55 or F(9) = 34 offsets
 constexpr int FIB_THREADS = 512;   // 8 wavefronts per block
 constexpr int FIB_WAVES = FIB_THREADS / 64;
@@ -352,6 +358,214 @@ __global__ __launch_bounds__(FIB_THREADS, 4) void k_fib_apply(FibModel M, ApplyI
       s2 += nrm2(xe);
       if (io.probe) {
         const int64_t r = R + koff[i];
+        s3 += zdot(((io.probe[r >> 5] >> (r & 31)) & 1u) ? 1.0 : -1.0, v);
+      }
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// The same H·v for blocks with the full FIB_L low bits (n >= FIB_L), written
+// so that a group costs no scalar instructions.  The setup tabulates, per
+// group, the LDS base of every outer-bit partner run, or the base of a
+// 64-slot zero stripe when the group may not flip that bit; a blocked flip
+// then reads zeros with the flip's own coefficient, so the loop over a
+// wave's groups is straight-line vector and LDS code.  The diagonal splits
+// as lo_diag[k] + T0[inner] (per lane) + T1, T2 of the outer code (per
+// group).  High-bit partner loads are clamped into the partner range instead
+// of predicated.
+constexpr int FIB_NOUT = FIB_L - FIB_IB;          // 10 outer bits
+constexpr int FIB_NGMAX = 144;                    // F(12): groups of a block
+constexpr int FIB_ZB = FIB_TMAX + FIB_PAD;        // first slot of the zero stripe
+constexpr int FIB_PBW = 12;                       // ints per group row: 10 bases, base, code
+static_assert(FIB_NGMAX <= FIB_WAVES * FIB_GPW, "groups per block");
+
+template <class V>
+__global__ __launch_bounds__(FIB_THREADS, RYD_FIB_FL_WAVES(V)) void k_fib_apply_fl(FibModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr bool CPLX = VT<V>::cplx;
+  constexpr int IB = FIB_IB;
+  __shared__ V xt[FIB_ZB + 64];
+  __shared__ double Fx[FIB_L];
+  __shared__ double T0[256], T1[256], T2[4];
+  __shared__ __align__(16) int gpb[FIB_NGMAX * FIB_PBW];
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int64_t hb = fib_block_hi(M, blockIdx.x);
+  if (hb >= M.nh) {
+    if (io.reduce) block_reduce_store(0.0, 0.0, 0.0, 0.0, rd.partials);
+    return;
+  }
+  const FibHi H = M.hi[hb];
+  const uint64_t h = H.code;
+  const int hL = (int)((h >> FIB_L) & 1ull);
+  const int cnt = (int)M.cnt_lo[hL];
+  const int64_t R = H.R;
+  const int ng = (int)(hL ? M.W[FIB_NOUT - 1] : M.W[FIB_NOUT]);   // F(11) or F(12)
+  {
+    V v[FIB_SEPT];
+#pragma unroll
+    for (int j = 0; j < FIB_SEPT; ++j) {
+      const int k = j * FIB_THREADS + t;
+      v[j] = k < cnt ? io.x[R + k] : vzero(v[0]);
+    }
+#pragma unroll
+    for (int j = 0; j < FIB_SEPT; ++j) {
+      const int k = j * FIB_THREADS + t;
+      if (k < cnt) xt[k] = vscale(v[j], sr);
+    }
+  }
+  if (t < 64) xt[FIB_ZB + t] = vzero(xt[0]);
+  for (int g = t; g < ng; g += FIB_THREADS) {
+    uint32_t o = 0;
+    int r = g, prev = 0, base = 0;
+    for (int q = FIB_NOUT - 1; q >= 0; --q) {
+      if (!prev && r >= (int)M.W[q]) {
+        o |= 1u << q;
+        r -= (int)M.W[q];
+        base += (int)M.W[IB + q];
+        prev = 1;
+      } else {
+        prev = 0;
+      }
+    }
+    int* row = gpb + g * FIB_PBW;
+    for (int q = 0; q < FIB_NOUT; ++q) {
+      const bool bit = (o >> q) & 1u;
+      const bool up = q + 1 < FIB_NOUT ? ((o >> (q + 1)) & 1u) != 0 : hL != 0;
+      const bool dn = q > 0 && ((o >> (q - 1)) & 1u);
+      const int W = (int)M.W[IB + q];
+      row[q] = bit ? base - W : (!(up || dn) ? base + W : FIB_ZB);
+    }
+    row[FIB_NOUT] = base;
+    row[FIB_NOUT + 1] = (int)o;
+  }
+  fib_cross_tables(M, h, Fx, T0, T1, T2);     // its barriers publish xt, the stripe and gpb
+  // per-lane inner code, flip offsets and coefficients (upper neighbour of
+  // bit IB-1 taken clear; a set bit IB is the group's "small" flag)
+  uint32_t cin = 0;
+  {
+    int r = lane, prev = 0;
+    for (int p = IB - 1; p >= 0; --p) {
+      if (!prev && r >= (int)M.W[p]) { cin |= 1u << p; r -= (int)M.W[p]; prev = 1; }
+      else prev = 0;
+    }
+  }
+  RT cfr[IB], cfi[IB];
+  int off[IB];
+#pragma unroll
+  for (int p = 0; p < IB; ++p) {
+    const int bit = (int)((cin >> p) & 1u);
+    const bool fr = !((cin >> (p + 1)) & 1u) && !(p > 0 && ((cin >> (p - 1)) & 1u));
+    const bool ok = bit || fr;
+    off[p] = ok ? (bit ? -(int)M.W[p] : (int)M.W[p]) : 0;   // blocked: own slot, coefficient 0
+    cfr[p] = ok ? (RT)M.ar[p] : (RT)0;
+    cfi[p] = ok ? (RT)(bit ? M.ai[p] : -M.ai[p]) : (RT)0;
+  }
+  const int zin = ((cin >> (IB - 1)) & 1u) ? FIB_ZB : 0;   // setting bit IB: blocked for this lane
+  const double t0c = T0[cin];
+  RT oar[FIB_NOUT], oai[FIB_NOUT];
+#pragma unroll
+  for (int q = 0; q < FIB_NOUT; ++q) {
+    oar[q] = (RT)M.ar[IB + q];
+    oai[q] = (RT)M.ai[IB + q];
+  }
+  const double eh = H.ehi;
+  V acc[FIB_GPW];
+  uint32_t kb[FIB_GPW];     // byte offset of this lane's code in the block, per group of the wave
+  uint32_t actm = 0;        // bit i: group i of the wave holds this lane
+#pragma unroll
+  for (int i = 0; i < FIB_GPW; ++i) {
+    const int g = wv + FIB_WAVES * i;
+    if (g >= ng) {                                 // uniform
+      kb[i] = (uint32_t)lane * (uint32_t)sizeof(V);
+      acc[i] = vzero(acc[0]);
+      continue;
+    }
+    const int4* row = reinterpret_cast<const int4*>(gpb + g * FIB_PBW);
+    const int4 r0 = row[0], r1 = row[1], r2 = row[2];
+    const int pb[FIB_NOUT] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y};
+    const int base = r2.z;
+    const uint32_t o = (uint32_t)r2.w;
+    const bool small = o & 1u;
+    const int k = base + lane;
+    kb[i] = (uint32_t)k * (uint32_t)sizeof(V);
+    if (lane < (small ? (int)M.W[IB - 1] : (int)M.W[IB])) actm |= 1u << i;
+    const int kd = k < FIB_TMAX ? k : FIB_TMAX - 1;
+    const double d = eh + M.lo_diag[kd] + t0c + T1[o & 255u] + T2[(o >> 8) & 3u];
+    V a = vscale(xt[k], (RT)d);
+    {
+      V xv[IB];
+#pragma unroll
+      for (int p = 0; p < IB - 1; ++p) xv[p] = xt[k + off[p]];
+      xv[IB - 1] = xt[small ? FIB_ZB + lane : k + off[IB - 1]];
+#pragma unroll
+      for (int p = 0; p < IB; ++p) vcfma(a, cfr[p], cfi[p], xv[p]);
+    }
+    {
+      V xv[FIB_NOUT];
+      xv[0] = xt[((o & 1u) ? pb[0] : (zin ? FIB_ZB : pb[0])) + lane];
+#pragma unroll
+      for (int q = 1; q < FIB_NOUT; ++q) xv[q] = xt[pb[q] + lane];
+#pragma unroll
+      for (int q = 0; q < FIB_NOUT; ++q) {
+        RT ci = (RT)0;
+        if constexpr (CPLX) ci = ((o >> q) & 1u) ? oai[q] : -oai[q];
+        vcfma(a, oar[q], ci, xv[q]);
+      }
+    }
+    acc[i] = a;
+  }
+  // high bits: one contiguous partner range per allowed bit, read through a
+  // buffer descriptor of the range's length (lanes past it read zero), every
+  // load of the wave in flight together
+  for (int p = FIB_L; p < M.far_end; ++p) {
+    const int bit = (int)((h >> p) & 1ull);
+    const bool set_ok = !((h >> (p + 1)) & 1ull) && (p == FIB_L || !((h >> (p - 1)) & 1ull));
+    if (!bit && !set_ok) continue;
+    const bool need_lo = !bit && p == FIB_L;     // setting bit L: low bit L-1 must be clear
+    const uint32_t lim = need_lo ? (uint32_t)M.cnt_lo[1] : (uint32_t)cnt;
+    const auto rs = buf_rsrc(io.x + (R + (bit ? -M.W[p] : M.W[p])), lim * (uint32_t)sizeof(V));
+    const RT ar = (RT)(M.ar[p] * s), ai = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
+    // in two halves of the wave's groups (register pressure of 8-byte types)
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+      constexpr int HG = FIB_GPW / 2;
+      V q[HG];
+#pragma unroll
+      for (int j = 0; j < HG; ++j) q[j] = buf_load<V>(rs, kb[hf * HG + j]);
+#pragma unroll
+      for (int j = 0; j < HG; ++j) vcfma(acc[hf * HG + j], ar, ai, q[j]);
+    }
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  char* py = reinterpret_cast<char*>(io.y + R);
+  const char* pz = io.z ? reinterpret_cast<const char*>(io.z + R) : nullptr;
+  const char* pxt = reinterpret_cast<const char*>(xt);
+#pragma unroll
+  for (int i = 0; i < FIB_GPW; ++i) {
+    if (!((actm >> i) & 1u)) continue;
+    V* yi = reinterpret_cast<V*>(py + kb[i]);
+    if (io.partial) {
+      *yi = acc[i];
+      continue;
+    }
+    const V xe = *reinterpret_cast<const V*>(pxt + kb[i]);
+    V v = vscale(acc[i], chr);
+    vfma(v, cxr, xe);
+    if (io.z) vfma(v, czr, *reinterpret_cast<const V*>(pz + kb[i]));
+    *yi = v;
+    if (io.reduce) {
+      s0 += rdot(xe, v);
+      s1 += nrm2(v);
+      s2 += nrm2(xe);
+      if (io.probe) {
+        const int64_t r = R + kb[i] / (uint32_t)sizeof(V);
         s3 += zdot(((io.probe[r >> 5] >> (r & 31)) & 1u) ? 1.0 : -1.0, v);
       }
     }

@@ -154,3 +154,42 @@ def test_sector_hv_hermitian_large(pkg, dev, n, dtype):
              torch.linalg.vector_norm(z.double() if not cplx else z.to(torch.complex128))).item()
     tol = 1e-5 if dtype in ('complex64', 'float32') else 1e-12
     assert abs(a - b) <= tol * scale
+
+
+@pytest.mark.parametrize('n', [18, 19, 20, 23])
+def test_sector_hv_random_complex_vs_channel_oracle(pkg, dev, n):
+    """Blocks with all 18 low bits: every site's Ω (complex) and Δ distinct,
+    so a flip or diagonal term on the wrong bit changes the result."""
+    import torch
+    rng = np.random.default_rng(100 + n)
+    Om = rng.uniform(0.5, 1.5, n) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    delta = rng.normal(size=n)
+    pairs = [{'i': i, 'j': j, 'V': float(rng.uniform(0.1, 5.0)) / (j - i) ** 3}
+             for i in range(n) for j in range(i + 1, n)]
+    codes = hv_ref.fib_codes(n)
+    x = rng.normal(size=len(codes)) + 1j * rng.normal(size=len(codes))
+    ref = orc.channel_matvec(n, Om, delta, pairs, ['I1'], x, codes)
+    P = _plan(pkg, dev, n, Om, delta, pairs)
+    y = P.hv(torch.as_tensor(x, device=dev)).cpu().numpy()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    y64 = P.hv(torch.as_tensor(x.astype(np.complex64), device=dev)).cpu().numpy()
+    assert np.max(np.abs(y64 - ref)) <= 2e-6 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize('n', [21, 30, 38])
+def test_sector_hv_random_real_vs_c_oracle(pkg, dev, n):
+    """Real drive with distinct per-site Ω and Δ against the C oracle, FP64
+    and FP32 (n = 38 runs the FP64 two-pass H·v)."""
+    import torch
+    rng = np.random.default_rng(200 + n)
+    Om, delta = rng.uniform(0.5, 1.5, n), rng.normal(size=n)
+    pairs = _pairs(n)
+    codes = hv_ref.fib_codes(n)
+    d = hv_ref.diag(n, delta, pairs, codes)
+    x = rng.standard_normal(len(codes))
+    ref = hv_ref.hv_fib(n, Om, d, codes, x)
+    P = _plan(pkg, dev, n, Om, delta, pairs)
+    y = P.hv(torch.as_tensor(x, device=dev)).cpu().numpy()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    y32 = P.hv(torch.as_tensor(x.astype(np.float32), device=dev)).cpu().numpy()
+    assert np.max(np.abs(y32 - ref)) <= 1e-5 * np.max(np.abs(ref))
