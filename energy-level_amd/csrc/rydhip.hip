@@ -369,7 +369,7 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
     if (!split) M.far_end = P->n;
     io.partial = split ? 1 : 0;
     if (P->fib_L == FIB_L && RYD_FIB_FL)
-      k_fib_apply_fl<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
+      k_fib_apply_fl<V><<<dim3(fib_grid(P)), dim3(64 * fib_fl_waves<V>()), 0, st>>>(M, io, rd);
     else
       k_fib_apply<V><<<dim3(fib_grid(P)), dim3(FIB_THREADS), 0, st>>>(M, io, rd);
     LAUNCH_CHECK();

@@ -63,10 +63,13 @@ struct FibModel {
   int64_t nbody[2];           // F(S+2), F(S+1): body ranks, all / with bit S-1 clear
 };
 
+#ifndef RYD_FIB_XCD
+#define RYD_FIB_XCD 0
+#endif
 __device__ __forceinline__ int64_t fib_block_hi(const FibModel& M, int64_t b) {
   // XCD-aware: XCD b%8 sweeps a contiguous eighth of the high parts
   const int64_t per = (M.nh + 7) / 8;
-  return M.nh >= 64 ? ((b & 7) * per + (b >> 3)) : b;
+  return (RYD_FIB_XCD && M.nh >= 64) ? ((b & 7) * per + (b >> 3)) : b;
 }
 
 // per-block cross field of the high part on the low bits, as byte tables
@@ -117,6 +120,12 @@ __device__ inline void fib_cross_tables(const FibModel& M, uint64_t h, double* F
 #endif
 #ifndef RYD_FIB_FL_WAVES
 #define RYD_FIB_FL_WAVES(V) (sizeof(V) >= 8 ? 2 : 4)
+#endif
+#ifndef RYD_FIB_FL_NW4
+#define RYD_FIB_FL_NW4 8
+#endif
+#ifndef RYD_FIB_FL_NW8
+#define RYD_FIB_FL_NW8 8
 #endif
 constexpr int FIB_IB = 8;          // inner bits: groups of F(10) = 55 or F(9) = 34 offsets
 constexpr int FIB_THREADS = 512;   // 8 wavefronts per block
@@ -378,11 +387,19 @@ constexpr int FIB_NOUT = FIB_L - FIB_IB;          // 10 outer bits
 constexpr int FIB_NGMAX = 144;                    // F(12): groups of a block
 constexpr int FIB_ZB = FIB_TMAX + FIB_PAD;        // first slot of the zero stripe
 constexpr int FIB_PBW = 12;                       // ints per group row: 10 bases, base, code
-static_assert(FIB_NGMAX <= FIB_WAVES * FIB_GPW, "groups per block");
 
-template <class V>
-__global__ __launch_bounds__(FIB_THREADS, RYD_FIB_FL_WAVES(V)) void k_fib_apply_fl(FibModel M, ApplyIO<V> io, Reduce rd) {
+// NW wavefronts per block (8 or 16): 16 halves the groups a wave holds, and
+// with them the registers of 8-byte types.
+template <class V> constexpr int fib_fl_waves() {
+  return sizeof(V) == 4 ? RYD_FIB_FL_NW4 : (sizeof(V) == 16 ? 8 : RYD_FIB_FL_NW8);
+}
+
+template <class V, int NW = fib_fl_waves<V>()>
+__global__ __launch_bounds__(NW * 64, NW >= 16 ? 4 : RYD_FIB_FL_WAVES(V)) void k_fib_apply_fl(FibModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
+  constexpr int NT = NW * 64;                       // threads
+  constexpr int GPW = (FIB_NGMAX + NW - 1) / NW;    // groups per wave
+  constexpr int SEPT = (FIB_TMAX + NT - 1) / NT;    // staged codes per thread
   constexpr bool CPLX = VT<V>::cplx;
   constexpr int IB = FIB_IB;
   __shared__ V xt[FIB_ZB + 64];
@@ -407,20 +424,20 @@ __global__ __launch_bounds__(FIB_THREADS, RYD_FIB_FL_WAVES(V)) void k_fib_apply_
   const int64_t R = H.R;
   const int ng = (int)(hL ? M.W[FIB_NOUT - 1] : M.W[FIB_NOUT]);   // F(11) or F(12)
   {
-    V v[FIB_SEPT];
+    V v[SEPT];
 #pragma unroll
-    for (int j = 0; j < FIB_SEPT; ++j) {
-      const int k = j * FIB_THREADS + t;
+    for (int j = 0; j < SEPT; ++j) {
+      const int k = j * NT + t;
       v[j] = k < cnt ? io.x[R + k] : vzero(v[0]);
     }
 #pragma unroll
-    for (int j = 0; j < FIB_SEPT; ++j) {
-      const int k = j * FIB_THREADS + t;
+    for (int j = 0; j < SEPT; ++j) {
+      const int k = j * NT + t;
       if (k < cnt) xt[k] = vscale(v[j], sr);
     }
   }
   if (t < 64) xt[FIB_ZB + t] = vzero(xt[0]);
-  for (int g = t; g < ng; g += FIB_THREADS) {
+  for (int g = t; g < ng; g += NT) {
     uint32_t o = 0;
     int r = g, prev = 0, base = 0;
     for (int q = FIB_NOUT - 1; q >= 0; --q) {
@@ -475,12 +492,12 @@ __global__ __launch_bounds__(FIB_THREADS, RYD_FIB_FL_WAVES(V)) void k_fib_apply_
     oai[q] = (RT)M.ai[IB + q];
   }
   const double eh = H.ehi;
-  V acc[FIB_GPW];
-  uint32_t kb[FIB_GPW];     // byte offset of this lane's code in the block, per group of the wave
+  V acc[GPW];
+  uint32_t kb[GPW];     // byte offset of this lane's code in the block, per group of the wave
   uint32_t actm = 0;        // bit i: group i of the wave holds this lane
 #pragma unroll
-  for (int i = 0; i < FIB_GPW; ++i) {
-    const int g = wv + FIB_WAVES * i;
+  for (int i = 0; i < GPW; ++i) {
+    const int g = wv + NW * i;
     if (g >= ng) {                                 // uniform
       kb[i] = (uint32_t)lane * (uint32_t)sizeof(V);
       acc[i] = vzero(acc[0]);
@@ -532,14 +549,16 @@ __global__ __launch_bounds__(FIB_THREADS, RYD_FIB_FL_WAVES(V)) void k_fib_apply_
     const auto rs = buf_rsrc(io.x + (R + (bit ? -M.W[p] : M.W[p])), lim * (uint32_t)sizeof(V));
     const RT ar = (RT)(M.ar[p] * s), ai = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
     // in two halves of the wave's groups (register pressure of 8-byte types)
+    constexpr int HG = (GPW + 1) / 2;
 #pragma unroll
     for (int hf = 0; hf < 2; ++hf) {
-      constexpr int HG = FIB_GPW / 2;
       V q[HG];
 #pragma unroll
-      for (int j = 0; j < HG; ++j) q[j] = buf_load<V>(rs, kb[hf * HG + j]);
+      for (int j = 0; j < HG; ++j)
+        if (hf * HG + j < GPW) q[j] = buf_load<V>(rs, kb[hf * HG + j]);
 #pragma unroll
-      for (int j = 0; j < HG; ++j) vcfma(acc[hf * HG + j], ar, ai, q[j]);
+      for (int j = 0; j < HG; ++j)
+        if (hf * HG + j < GPW) vcfma(acc[hf * HG + j], ar, ai, q[j]);
     }
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -548,7 +567,7 @@ __global__ __launch_bounds__(FIB_THREADS, RYD_FIB_FL_WAVES(V)) void k_fib_apply_
   const char* pz = io.z ? reinterpret_cast<const char*>(io.z + R) : nullptr;
   const char* pxt = reinterpret_cast<const char*>(xt);
 #pragma unroll
-  for (int i = 0; i < FIB_GPW; ++i) {
+  for (int i = 0; i < GPW; ++i) {
     if (!((actm >> i) & 1u)) continue;
     V* yi = reinterpret_cast<V*>(py + kb[i]);
     if (io.partial) {
