@@ -63,11 +63,14 @@ struct FibModel {
   int64_t nbody[2];           // F(S+2), F(S+1): body ranks, all / with bit S-1 clear
 };
 
+// Block order over the high parts.  RYD_FIB_XCD = 1 lets XCD b%8 sweep its
+// own contiguous eighth; the default sweeps them in rank order across all
+// XCDs, which keeps one compact front whose recent high parts the MALL still
+// holds when a far high-bit flip reads them (N=40 FP32 2.53 -> 2.41 ms).
 #ifndef RYD_FIB_XCD
 #define RYD_FIB_XCD 0
 #endif
 __device__ __forceinline__ int64_t fib_block_hi(const FibModel& M, int64_t b) {
-  // XCD-aware: XCD b%8 sweeps a contiguous eighth of the high parts
   const int64_t per = (M.nh + 7) / 8;
   return (RYD_FIB_XCD && M.nh >= 64) ? ((b & 7) * per + (b >> 3)) : b;
 }
@@ -381,8 +384,13 @@ __global__ __launch_bounds__(FIB_THREADS, 4) void k_fib_apply(FibModel M, ApplyI
 // then reads zeros with the flip's own coefficient, so the loop over a
 // wave's groups is straight-line vector and LDS code.  The diagonal splits
 // as lo_diag[k] + T0[inner] (per lane) + T1, T2 of the outer code (per
-// group).  High-bit partner loads are clamped into the partner range instead
-// of predicated.
+// group).  High-bit partner runs are read through a buffer descriptor of the
+// run's length, whose range check zeroes the lanes past it.
+// Measured at N=40 (FP32 / FP64 pass A + top pass, ms per H·v): the
+// branch-per-group kernel above 4.28 / ~6.0; this form 2.70 / 5.02; with
+// buffer loads 2.58 / 4.78; global block order 2.41 / 4.70; 16 waves per
+// block 3.22 / 4.79; an FP32 top pass 3.04.  Its FP32 form fetches about
+// 10× the vector from beyond L2 (high-bit partners), so it is bound there.
 constexpr int FIB_NOUT = FIB_L - FIB_IB;          // 10 outer bits
 constexpr int FIB_NGMAX = 144;                    // F(12): groups of a block
 constexpr int FIB_ZB = FIB_TMAX + FIB_PAD;        // first slot of the zero stripe
