@@ -13,6 +13,10 @@
 namespace ryd {
 
 constexpr int STREAM_THREADS = 256;
+#ifndef RYD_LZ_UNROLL
+#define RYD_LZ_UNROLL 4
+#endif
+constexpr int LZ_UNROLL = RYD_LZ_UNROLL;
 
 template <class V>
 __global__ __launch_bounds__(256) void k_lz_orth(const V* __restrict__ uj, V* __restrict__ w,
@@ -27,15 +31,28 @@ __global__ __launch_bounds__(256) void k_lz_orth(const V* __restrict__ uj, V* __
   const RT r0c = ritz0 ? (RT)(yc0[j] * inv) : (RT)0;
   const RT r1c = ritz1 ? (RT)(yc1[j] * inv) : (RT)0;
   double acc = 0.0;
+  // LZ_UNROLL elements per thread and pass, every load issued before the
+  // first store (more reads in flight per CU than one element at a time)
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
-    const V u = uj[i];
-    V v = w[i];
-    vfma(v, coef, u);
-    w[i] = v;
-    acc += nrm2(v);
-    if (ritz0) { V a = ritz0[i]; vfma(a, r0c, u); ritz0[i] = a; }
-    if (ritz1) { V a = ritz1[i]; vfma(a, r1c, u); ritz1[i] = a; }
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < D;
+       i0 += stride * LZ_UNROLL) {
+    V u[LZ_UNROLL], v[LZ_UNROLL];
+#pragma unroll
+    for (int k = 0; k < LZ_UNROLL; ++k) {
+      const int64_t i = i0 + k * stride;
+      u[k] = i < D ? uj[i] : vzero(u[0]);
+      v[k] = i < D ? w[i] : vzero(u[0]);
+    }
+#pragma unroll
+    for (int k = 0; k < LZ_UNROLL; ++k) {
+      const int64_t i = i0 + k * stride;
+      if (i >= D) break;
+      vfma(v[k], coef, u[k]);
+      w[i] = v[k];
+      acc += nrm2(v[k]);
+      if (ritz0) { V a = ritz0[i]; vfma(a, r0c, u[k]); ritz0[i] = a; }
+      if (ritz1) { V a = ritz1[i]; vfma(a, r1c, u[k]); ritz1[i] = a; }
+    }
   }
   if (reduce) block_reduce_store(0.0, acc, 0.0, 0.0, rd.partials);
 }
