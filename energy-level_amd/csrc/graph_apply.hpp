@@ -34,16 +34,19 @@ struct GraphModel {
   const int64_t* post_off;      // [nc] start of class c in the row tables
   const uint32_t* rcode;        // [rows] low bits of the row
   const double* rdiag;          // [rows] diagonal of those bits alone
-  const uint32_t* lrow;         // [L][rows] segment position of the row with bit q flipped
+  const void* lrow;             // [L][rows] segment position of the row with bit q flipped
+                                // (uint16_t when every segment is shorter than 65535 rows)
   const double* vb;             // [n*n] V by bit position
   int hfar;                     // high bits 0 .. hfar-1 are applied here (H: all)
   double dl[MAXN];              // delta by bit position
   double ar[MAXN], ai[MAXN];    // Ω/2 by bit position
 };
 
-template <class V>
+template <class V, class IT>
 __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
+  const IT* lrow = static_cast<const IT*>(M.lrow);
+  constexpr IT INONE = (IT)~(IT)0;
   __shared__ double wq[GR_MAXB];
   __shared__ double dhi_s;
   __shared__ int64_t off2[GR_MAXB];     // first rank of h ^ 2^p, -1 when not independent
@@ -120,8 +123,8 @@ __global__ __launch_bounds__(256) void k_graph_apply(GraphModel M, ApplyIO<V> io
 #pragma unroll
         for (int u = 0; u < GR_FLIPS; ++u) {
           const int q = q0 + u;
-          const uint32_t pos = q < L ? M.lrow[(size_t)q * (size_t)M.rows + cr] : GR_NONE;
-          r2[u] = (pos != GR_NONE && (M.ar[q] != 0.0 || M.ai[q] != 0.0)) ? off + pos : -1;
+          const IT pos = q < L ? lrow[(size_t)q * (size_t)M.rows + cr] : INONE;
+          r2[u] = (pos != INONE && (M.ar[q] != 0.0 || M.ai[q] != 0.0)) ? off + (int64_t)pos : -1;
         }
         V xv[GR_FLIPS];
 #pragma unroll
@@ -292,6 +295,7 @@ __global__ __launch_bounds__(256) void k_graph_top(GraphTop T, ApplyIO<V> io, Re
 // Per class c and segment position j (row cr = post_off[c] + j, t = post[cr]):
 // rcode = lowtab[t], rdiag = diagonal of those low bits, lrow[q][cr] =
 // pre_c[index of lowtab[t] ^ 2^q] or GR_NONE.  One block per class.
+template <class IT>
 __global__ __launch_bounds__(256) void k_gr_rows(int n, int L, const uint32_t* __restrict__ lowtab,
                                                  int64_t tl, const uint32_t* __restrict__ lo_idx,
                                                  const uint32_t* __restrict__ fc, int64_t nc,
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(256) void k_gr_rows(int n, int L, const uint32_t* _
                                                  const double* __restrict__ vb,
                                                  uint32_t* __restrict__ rcode,
                                                  double* __restrict__ rdiag,
-                                                 uint32_t* __restrict__ lrow) {
+                                                 IT* __restrict__ lrow) {
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
     const uint32_t F = fc[c];
     const uint32_t* prec = pre + (size_t)c * (size_t)(tl + 1);
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(256) void k_gr_rows(int n, int L, const uint32_t* _
         const uint32_t l2 = l ^ (1u << q);
         const uint32_t t2 = lo_idx[l2];
         const bool ok = t2 != GR_NONE && !(l2 & F);
-        lrow[(size_t)q * (size_t)rows + cr] = ok ? prec[t2] : GR_NONE;
+        lrow[(size_t)q * (size_t)rows + cr] = ok ? (IT)prec[t2] : (IT)~(IT)0;
       }
     }
   }

@@ -97,7 +97,8 @@ struct ryd_plan {
   int64_t* d_post_off = nullptr;     // [nc]
   uint32_t* d_rcode = nullptr;       // [rows]
   double* d_rdiag = nullptr;         // [rows]
-  uint32_t* d_lrow = nullptr;        // [L][rows]
+  void* d_lrow = nullptr;            // [L][rows], uint16_t entries when gr_l16
+  bool gr_l16 = false;               // every segment shorter than 65535 rows
   // top pass of a split graph H·v (gr_K = 0: one pass)
   int gr_K = 0;
   int gr_nt = 0;
@@ -338,7 +339,10 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
     const bool split = P->gr_K > 0;
     io.partial = split ? 1 : 0;
     unsigned grid = graph_grid(P);
-    k_graph_apply<V><<<dim3(grid), dim3(256), 0, st>>>(graph_model(P), io, rd);
+    if (P->gr_l16)
+      k_graph_apply<V, uint16_t><<<dim3(grid), dim3(256), 0, st>>>(graph_model(P), io, rd);
+    else
+      k_graph_apply<V, uint32_t><<<dim3(grid), dim3(256), 0, st>>>(graph_model(P), io, rd);
     LAUNCH_CHECK();
     if (split) {
       io.partial = 0;
@@ -652,6 +656,14 @@ int64_t gr_split_min_dim() {
   return 1ll << 22;
 }
 
+// 16-bit row tables for k_graph_apply when every segment is shorter than
+// 65535 rows; RYD_GRAPH_WIDE_ROWS=1 keeps 32-bit entries.
+bool gr_row16(int64_t tl) {
+  if (const char* env = std::getenv("RYD_GRAPH_WIDE_ROWS"))
+    if (env[0] == '1') return false;
+  return tl < 65535;
+}
+
 // Bytes allowed for the per-class prefix table: beyond this the ranks fall
 // back to a binary search over the code list.  Default: twice the code list,
 // at least 256 MiB; RYD_GRAPH_PREFIX_MAX_BYTES overrides it (0 = always search).
@@ -803,10 +815,20 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
       const size_t rows1 = (size_t)(rows_all > 0 ? rows_all : 1);
       HIP_TRY(hipMalloc(&P->d_rcode, sizeof(uint32_t) * rows1));
       HIP_TRY(hipMalloc(&P->d_rdiag, sizeof(double) * rows1));
-      HIP_TRY(hipMalloc(&P->d_lrow, sizeof(uint32_t) * rows1 * (size_t)(L > 0 ? L : 1)));
-      k_gr_rows<<<dim3(cg), dim3(256)>>>(n, L, P->d_lowtab, tl, P->d_lo_idx, P->d_fc, nc, P->d_pre,
-                                          P->d_post, P->d_post_off, rows_all, dl_dev, P->d_vb,
-                                          P->d_rcode, P->d_rdiag, P->d_lrow);
+      // segment positions are < tl: 16-bit row tables halve their traffic
+      P->gr_l16 = gr_row16(tl);
+      const size_t lsz = P->gr_l16 ? sizeof(uint16_t) : sizeof(uint32_t);
+      HIP_TRY(hipMalloc(&P->d_lrow, lsz * rows1 * (size_t)(L > 0 ? L : 1)));
+      if (P->gr_l16)
+        k_gr_rows<uint16_t><<<dim3(cg), dim3(256)>>>(n, L, P->d_lowtab, tl, P->d_lo_idx, P->d_fc, nc,
+                                                     P->d_pre, P->d_post, P->d_post_off, rows_all,
+                                                     dl_dev, P->d_vb, P->d_rcode, P->d_rdiag,
+                                                     static_cast<uint16_t*>(P->d_lrow));
+      else
+        k_gr_rows<uint32_t><<<dim3(cg), dim3(256)>>>(n, L, P->d_lowtab, tl, P->d_lo_idx, P->d_fc, nc,
+                                                     P->d_pre, P->d_post, P->d_post_off, rows_all,
+                                                     dl_dev, P->d_vb, P->d_rcode, P->d_rdiag,
+                                                     static_cast<uint32_t*>(P->d_lrow));
       LAUNCH_CHECK();
       P->gr_rows = rows_all;
       int64_t nv = 0;

@@ -332,3 +332,26 @@ def test_split_graph_ground_state_vs_eigsh(pkg, dev, monkeypatch):
     assert out['converged'] and out['dim'] == len(codes)
     assert abs(out['E0'] - ref['E0']) <= 1e-9 * max(1.0, abs(ref['E0']))
     assert abs(out['gap'] - ref['gap']) <= 1e-8
+
+
+@pytest.mark.parametrize('wide', ['0', '1'])
+def test_graph_hv_row_table_widths(pkg, dev, wide, monkeypatch):
+    """16-bit (default below 65535 rows per segment) and 32-bit row tables of
+    the graph kernel against the C oracle, one-pass and split."""
+    import torch
+    monkeypatch.setenv('RYD_GRAPH_WIDE_ROWS', wide)
+    n = 30
+    calib = _lattice_calib(6, 5)
+    edges = pkg.blockade_edges(calib, {'kind': 'blockade', 'threshold_mhz': 10.0})
+    rng = np.random.default_rng(7)
+    Om, delta = rng.uniform(0.5, 1.5, n), rng.uniform(-2, 2, n)
+    codes = hv_ref.indep_codes(n, edges)
+    x = rng.normal(size=len(codes))
+    ref = hv_ref.hv_codes(n, Om, hv_ref.diag(n, delta, calib['pairs'], codes), codes, x)
+    for mind in ('0', str(1 << 40)):
+        monkeypatch.setenv('RYD_GRAPH_SPLIT_MIN_DIM', mind)
+        P = pkg.Plan(pkg.from_calib(calib, Om, delta), basis='graph', edges=edges, device=dev)
+        assert P.rank_mode == 'graph'
+        y = P.hv(torch.as_tensor(x, device=dev)).cpu().numpy()
+        P.close()
+        assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref)), mind
