@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, 'librydhip.so')
 
 RYD_OK, RYD_ERR_INVALID, RYD_ERR_MEMORY, RYD_ERR_DEVICE, RYD_ERR_UNSUPPORTED = range(5)
 RYD_F32, RYD_F64, RYD_C64, RYD_C128 = range(4)
-ABI_VERSION = 3
+ABI_VERSION = 4
 RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH = range(4)
 
 _vp = ctypes.c_void_p
@@ -70,6 +70,8 @@ SIGNATURES = {
     'ryd_sector_rank': (ctypes.c_int, [_vp, _vp, _i64, _vp, _vp]),
     'ryd_evolve_forward': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _i64, _vp, _vp, _vp, _vp,
                                           ctypes.c_int, _vp, _vp, _vp]),
+    'ryd_evolve_taylor': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _dbl, ctypes.c_int, ctypes.c_int,
+                                         _vp, _vp, _vp]),
     'ryd_evolve_adjoint': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
                                           ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'ryd_batch_create': (ctypes.c_int, [ctypes.POINTER(Model), ctypes.c_int32, ctypes.c_int,

@@ -248,4 +248,34 @@ __global__ __launch_bounds__(256) void k_caxpy(const V* __restrict__ x, V* __res
   }
 }
 
+// One Taylor term of psi <- exp(-tau·(i·H + G/2)) psi, the propagator of the
+// non-Hermitian H - (i/2)·G with the diagonal G = Σ_i gamma_i n_i:
+//   v_k = (tau/k)·(-i·(H v_{k-1}) - (G/2)·v_{k-1}),   out += v_k.
+// w holds H v_{k-1}; v is overwritten with v_k; gb[p] = gamma/2 of the site
+// at bit position p, summed over the set bits of each basis code.
+struct TaylorDiag {
+  double gb[MAXN];
+};
+
+template <class V>
+__global__ __launch_bounds__(256) void k_taylor_term(ObsBasis B, TaylorDiag T,
+                                                     const V* __restrict__ w, V* __restrict__ v,
+                                                     V* __restrict__ out, double f, int64_t D) {
+  using RT = typename VT<V>::R;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    double g = 0.0;
+    for (uint64_t b = obs_code(B, i); b; b &= b - 1) g += T.gb[__builtin_ctzll(b)];
+    const V hw = w[i], vv = v[i];
+    V r;
+    r.re = (RT)(f * ((double)hw.im - g * (double)vv.re));
+    r.im = (RT)(f * (-(double)hw.re - g * (double)vv.im));
+    v[i] = r;
+    V o = out[i];
+    o.re += r.re;
+    o.im += r.im;
+    out[i] = o;
+  }
+}
+
 }  // namespace ryd

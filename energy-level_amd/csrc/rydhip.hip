@@ -1716,6 +1716,47 @@ int do_evolve_generic(const ryd_plan* P, int dtype, void* psi_v, int64_t nslices
   return RYD_OK;
 }
 
+// psi <- exp(-tau·(i·H + G/2)) psi by nsub Taylor steps of degree m (tau =
+// CONV·dt of the whole interval); work: 2 vectors
+template <class V>
+int do_taylor(const ryd_plan* P, int dtype, void* psi_v, double tau, int nsub, int m,
+              const double* gamma, void* work_v, hipStream_t st) {
+  V* psi = static_cast<V*>(psi_v);
+  const int64_t D = P->dim;
+  V* v = static_cast<V*>(work_v);
+  V* w = v + D;
+  TaylorDiag T;
+  for (int p = 0; p < MAXN; ++p) T.gb[p] = 0.0;
+  for (int i = 0; i < P->n; ++i) T.gb[P->n - 1 - i] = 0.5 * gamma[i];
+  const ObsBasis B = obs_basis(P);
+  Reduce none = make_reduce(nullptr, POST_NONE, nullptr);
+  const unsigned sg = stream_grid(D);
+  const double h = tau / nsub;
+  for (int sub = 0; sub < nsub; ++sub) {
+    HIP_TRY(hipMemcpyAsync(v, psi, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+    for (int k = 1; k <= m; ++k) {
+      ApplyIO<V> io;
+      io.x = v;
+      io.z = nullptr;
+      io.y = w;
+      io.probe = nullptr;
+      io.smode = SM_CONST;
+      io.s = 1.0;
+      io.ch = 1.0;
+      io.cx = 0.0;
+      io.cz = 0.0;
+      io.nrm = nullptr;
+      io.j = 0;
+      io.stop = nullptr;
+      io.reduce = 0;
+      if (int rc = launch_apply<V>(P, dtype, io, none, st)) return rc;
+      k_taylor_term<V><<<sg, 256, 0, st>>>(B, T, w, v, psi, h / k, D);
+      LAUNCH_CHECK();
+    }
+  }
+  return RYD_OK;
+}
+
 // the fused slice kernels cover the full basis with Omega/delta/V up to n = 22
 bool evo_fused(const ryd_plan* P) {
   return P->basis == RYD_BASIS_FULL && !P->generic && P->n < SPLIT_MIN_N;
@@ -1805,6 +1846,21 @@ extern "C" int ryd_evolve_forward(const ryd_plan* P, int dtype, void* psi, int64
   if (dtype == RYD_C64)
     return do_evolve_forward<c32>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
   return do_evolve_forward<c64>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
+}
+
+extern "C" int ryd_evolve_taylor(const ryd_plan* P, int dtype, void* psi, double tau, int nsub,
+                                 int m, const double* gamma, void* work, void* stream) {
+  if (!P || !psi || !gamma || !work) return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_taylor");
+  if (dtype != RYD_C64 && dtype != RYD_C128)
+    return fail(RYD_ERR_INVALID, "non-Hermitian evolution needs a complex dtype");
+  if (!(tau >= 0.0) || !std::isfinite(tau)) return fail(RYD_ERR_INVALID, "tau must be finite and >= 0");
+  if (nsub < 1 || nsub > (1 << 24) || m < 1 || m > 64)
+    return fail(RYD_ERR_INVALID, "nsub must be in [1, 2^24] and m in [1, 64]");
+  for (int i = 0; i < P->n; ++i)
+    if (!std::isfinite(gamma[i])) return fail(RYD_ERR_INVALID, "gamma must be finite");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == RYD_C64) return do_taylor<c32>(P, dtype, psi, tau, nsub, m, gamma, work, st);
+  return do_taylor<c64>(P, dtype, psi, tau, nsub, m, gamma, work, st);
 }
 
 extern "C" int ryd_evolve_adjoint(const ryd_plan* P, int dtype, const void* psi, void* g,

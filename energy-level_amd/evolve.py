@@ -24,7 +24,9 @@ from .plan import Plan
 
 __all__ = ['CONV', 'cheb_coefficients', 'terms_for', 'PulseProblem', 'pulse_loss_and_grad',
            'PulseEvolution', 'propagate', 'z2_index', 'slice_bounds', 'segment_intervals',
-           'segment_drive', 'propagate_segments_hip']
+           'segment_drive', 'propagate_segments_hip', 'taylor_bound', 'propagate_lossy']
+
+TAYLOR_DEGREE = 20      # terms per Taylor step; truncation below 1/21! ≈ 2e-20 at ‖h·A‖ <= 1
 
 CONV = 2.0 * np.pi * 1e-3
 
@@ -297,11 +299,46 @@ def segment_drive(t, segments, n, interp='left'):
     return om, de
 
 
+def taylor_bound(model: RydModel, gamma):
+    """Row-sum (Gershgorin) bound of ‖H − (i/2)·Σ γ_i n_i‖ for the Taylor
+    step count: |diagonal| ≤ Σ|δ| + Σ|V| + Σγ/2, off-diagonal ≤ Σ|Ω|/2 +
+    Σ(|J| + |A|) per row."""
+    g = np.abs(np.asarray(gamma, float))
+    return float(np.sum(np.abs(model.delta)) + np.sum(np.abs(model.pair_v)) + 0.5 * np.sum(g)
+                 + 0.5 * np.sum(np.abs(model.Om)) + np.sum(np.abs(model.pair_J))
+                 + np.sum(np.abs(model.pair_A)))
+
+
+def propagate_lossy(plan, model, psi, dt, gamma, work=None):
+    """psi ← exp(−i·CONV·dt·(H − (i/2)·Σ_i g_i n_i)) psi on the device
+    (``ryd_evolve_taylor``), g_i = γ_i·1e-3/CONV: the non-Hermitian drive of
+    engine_ed._propagate_segments (engine_ed.py:130-147, expm_multiply with
+    the diagonal −(i/2)·γ_i·1e-3/CONV·n_i of channels.py:58-77; γ in 1/µs,
+    so |ψ|² of an excited atom decays as exp(−γ·t·1e-3), t in ns), by Taylor
+    steps with ‖h·(H − (i/2)G)‖ ≤ 1."""
+    import torch
+    gamma = np.asarray(gamma, np.float64)
+    if gamma.shape != (plan.n,):
+        raise ValueError('gamma needs one entry per atom')
+    gamma = np.ascontiguousarray(gamma * 1e-3 / CONV)      # MHz, like delta
+    code = plan.check_vec(psi)
+    if code not in (nat.RYD_C64, nat.RYD_C128):
+        raise ValueError('lossy evolution needs a complex vector')
+    tau = CONV * float(dt)
+    nsub = max(1, int(np.ceil(tau * taylor_bound(model, gamma))))
+    if work is None or work.numel() < 2 * plan.dim or work.dtype != psi.dtype:
+        work = torch.empty(2 * plan.dim, dtype=psi.dtype, device=psi.device)
+    nat.check(nat.lib().ryd_evolve_taylor(plan.handle, code, nat.ptr(psi), tau, nsub,
+                                          TAYLOR_DEGREE, gamma.ctypes.data, nat.ptr(work),
+                                          nat.stream_ptr()))
+    return psi
+
+
 def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='left',
                            dtype='complex128', device='cuda', return_states=False,
                            basis_probabilities_max_dim=4096, basis_indices=None,
                            blockade_edges=None, plan=None, pauli_strings=None, corr=False,
-                           on_state=None, psi0_plan_order=False):
+                           on_state=None, psi0_plan_order=False, gamma=None):
     """Exact piecewise-constant evolution, output at the times ``t`` (ns):
     every output interval is split at the segment starts and each piece
     [a, b] is propagated by exp(−i·CONV·H·(b−a)) with the drive of
@@ -323,7 +360,9 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
     add the per-output-time reductions of engine_ed.py:259-281 on the device:
     ``pauli`` (len(t), strings) complex and ``corr`` (len(t), n, n).
     ``on_state(k, plan, x)`` is called at every output time k with the device
-    state in the plan's (ascending) basis order.
+    state in the plan's (ascending) basis order.  ``gamma`` (per atom, MHz)
+    adds the loss diagonal −(i/2)Γ_i n_i; each piece is then propagated by
+    the device Taylor series (``propagate_lossy``) instead of Chebyshev.
 
     Returns {t, pops_atom (len(t), n), norms (len(t),), psi (final state,
     host complex128, basis order), method='manybody', backend='hip-chebyshev',
@@ -419,12 +458,19 @@ def propagate_segments_hip(calib, segments, t, psi0=None, terms=None, interp='le
                                  % (dim, ' (or 2^n)' if projected else ''))
             psi = torch.as_tensor(v, device=dv).to(tdt)
         record(p0, psi)
+        lossy = gamma is not None and bool(np.any(np.asarray(gamma, float) != 0.0))
+        work = None
         for k in range(1, len(t)):
             for a, b in segment_intervals(t[k - 1], t[k], segments):
                 if b <= a:
                     continue
                 om, de = segment_drive(0.5 * (a + b), segments, n, interp)
                 model, P = plan_for(om, de)
+                if lossy:
+                    if work is None:
+                        work = torch.empty(2 * P.dim, dtype=tdt, device=dv)
+                    psi = propagate_lossy(P, model, psi.contiguous(), b - a, gamma, work)
+                    continue
                 prob = PulseProblem(model, dt=b - a, K=15, dtype=dtype, plan=P,
                                     units='reference')
                 psi = prob.propagate(psi, [1.0], [0.0])

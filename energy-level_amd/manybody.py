@@ -17,8 +17,10 @@ product dict or list of local states, or 'ground' (the device Lanczos ground
 state of the drive at t[0]).
 
 ``loops`` are post-processed on the host from the measured bonds
-(observables.py:206-300).  Not supported (NotImplementedError): a
-non-Hermitian drive (gamma != 0).
+(observables.py:206-300).  A non-Hermitian drive (gamma != 0, the loss
+diagonal −(i/2)Γ_i n_i) is propagated by the device Taylor series
+(``ryd_evolve_taylor``); the norm then decays and ``loss`` = 1 − norm is the
+physical loss (engine_ed.py:311-318).
 """
 from __future__ import annotations
 
@@ -226,8 +228,12 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     t = np.asarray(t_grid, float) if t_grid is not None else None
     if t is None or t.ndim != 1 or len(t) < 2 or not np.isfinite(t).all() or np.any(np.diff(t) < 0):
         raise ValueError('t_grid needs at least two finite, non-decreasing times')
-    if gamma is not None and np.any(np.asarray(gamma, float) != 0):
-        raise NotImplementedError('the HIP engine propagates Hermitian drives only (gamma = 0)')
+    gamma = np.zeros(n) if gamma is None else np.asarray(gamma, float)
+    if gamma.ndim == 0:
+        gamma = np.full(n, float(gamma))
+    if gamma.shape != (n,) or not np.all(np.isfinite(gamma)):
+        raise ValueError('gamma needs one finite entry per atom')
+    hermitian = not bool(np.any(np.abs(gamma) > 0.0))
     Om = np.zeros(n, complex) if Om is None else np.asarray(Om, complex)
     delta = np.zeros(n) if delta is None else np.asarray(delta, float)
     if Om.shape != (n,) or delta.shape != (n,):
@@ -323,7 +329,7 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     out = propagate_segments_hip(calib, segs, t, psi0=psi0v, terms=terms, interp=drive_interp,
                                  dtype=dtype, device=device, return_states=False,
                                  basis_probabilities_max_dim=fullpop_max, pauli_strings=specs,
-                                 on_state=on_state, **kw)
+                                 on_state=on_state, gamma=None if hermitian else gamma, **kw)
     norms = out['norms'] ** 2                  # Σ|ψ|², as the reference's p.sum()
     pops_atom = out['pops_atom'] * norms[:, None]
     sz[:] = norms[:, None] - 2.0 * pops_atom
@@ -332,7 +338,7 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     corr = state['corr'] if want_corr else None
     joint = out.get('basis_probabilities')
     loss = 1.0 - norms
-    norm_dev = float(np.max(np.abs(norms - 1.0)))
+    norm_dev = float(np.max(np.abs(norms - 1.0))) if hermitian else 0.0
     if want_entropy and np.isnan(S).any():
         S_note = 'entropy not computed at some output times (Schmidt matrix budget); NaN there'
     capped = joint is None or projected
@@ -346,7 +352,8 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
                   for i, a in enumerate(atoms)]
         log('[manybody] pops are the per-atom marginals; joint probabilities in basis_probabilities')
     kind = (plan.get('constraint') or {}).get('kind', 'blockade') if projected else None
-    backend = 'hip-ed-plan-%s' % kind if projected else 'hip-ed'
+    backend = ('hip-ed-plan-%s' % kind if projected else
+               'hip-ed' if hermitian else 'hip-ed-nonhermitian')
     use = set(active_terms(calib) if terms is None else terms)
     obs = None
     if want_observables:
@@ -369,11 +376,12 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
         basis=dict(kind='projected' if projected else 'full', bit_order='site0-msb',
                    n_atoms=n, dim=dim, full_dim=1 << n),
         error_report=dict(backend=backend, solver='chebyshev', dim=dim, n_atoms=n, n_t=n_t,
-                          hermitian=True, used_sectors=False, full_pops=not capped,
+                          hermitian=hermitian, used_sectors=False, full_pops=not capped,
                           projected=projected, entropy_cut=cut if want_entropy else None,
                           entropy_note=S_note, max_norm_dev=norm_dev),
         diag=({'dim': dim, 'n_atoms': n, 'backend': backend, 'used_sectors': False,
-               'hermitian': True, 'terms': sorted(use), 'n_pairs': len(calib.get('pairs') or []),
+               'hermitian': hermitian, 'terms': sorted(use),
+               'n_pairs': len(calib.get('pairs') or []),
                'dropped_amp': calib.get('dropped_amp'),
                'entropy_cut': cut if want_entropy else None, 'full_pops': not capped}
               if diag else None),

@@ -152,14 +152,14 @@ def site_correlation_sums(plan, x):
 
 
 def corr_matrix_hip(plan, x):
-    """C_ij = <n_i n_j> − <n_i><n_j> of the normalised state (corr_matrix,
-    observables.py:40-47); also returns <n_i>."""
+    """C_ij = <n_i n_j> − <n_i><n_j> with the weights |x_s|^2 as they are
+    (corr_matrix, observables.py:40-47: a state of norm below one, as under
+    loss, is not renormalised); also returns <n_i>."""
     S, norm = site_correlation_sums(plan, x)
     if not norm > 0:
         raise ValueError('zero state')
-    nn = S / norm
-    ni = np.diag(nn).copy()
-    return nn - np.outer(ni, ni), ni
+    ni = np.diag(S).copy()
+    return S - np.outer(ni, ni), ni
 
 
 def _loop_sites(raw, n=None):

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 3
+#define RYD_ABI_VERSION 4
 
 enum ryd_status {
   RYD_OK = 0,
@@ -224,6 +224,19 @@ int ryd_sector_rank(const ryd_plan* plan, const int64_t* codes, int64_t count, i
 int ryd_evolve_forward(const ryd_plan* plan, int dtype, void* psi, int64_t nslices,
                        const double* om, const double* dadd, const double* c, const double* r,
                        int K, const double* coef, void* work, void* stream);
+
+/* psi <- exp(-tau·(i·H + G/2)) psi = exp(-i·CONV·dt·(H - (i/2)·G)) psi with
+ * tau = CONV·dt, for the plan's H on its basis and the loss diagonal
+ * G = Σ_i gamma[i]·n_i (host FP64 [n], by site; MHz like delta, i.e. the
+ * reference's rate γ_i·1e-3/CONV): the non-Hermitian
+ * (dissipative) drive of engine_ed._propagate_segments / _propagate_sparse
+ * (engine_ed.py:117-147, expm_multiply of −i·CONV·H with the −(i/2)Γ_i n_i
+ * diagonal of channels.py:58-77).  nsub Taylor steps of degree m, each term
+ * one H·v and one fused update; the caller picks nsub so that tau/nsub times
+ * a bound of ‖H − (i/2)G‖ stays below 1 (the truncation is then below
+ * 1/(m+1)! per step).  Complex dtypes; work: 2 vectors of dim elements. */
+int ryd_evolve_taylor(const ryd_plan* plan, int dtype, void* psi, double tau, int nsub, int m,
+                      const double* gamma, void* work, void* stream);
 
 /* Vector-Jacobian product of one slice of ryd_evolve_forward (the adjoint;
  * no reference counterpart — the reference has no gradient).  Given the

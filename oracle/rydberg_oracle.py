@@ -547,11 +547,12 @@ def segment_drive(t, segments, n, interp='left'):
 
 
 def propagate_segments(n, pairs, terms, segments, psi0, t, codes=None, interp='left',
-                       method='expm'):
+                       method='expm', gamma=None):
     """ψ(t_k) by exact exponentials per piece, exp(-i·CONV·H·Δt)
     (reference engine_ed.py:130-147).  ``method='expm_multiply'`` is the
     reference's own sparse stepping (engine_ed.py:145); ``'expm'`` uses a
-    dense exponential (small N)."""
+    dense exponential (small N).  ``gamma`` adds the loss diagonal
+    −(i/2)Γ_i n_i of build_H_sparse (non-Hermitian H)."""
     from scipy.sparse.linalg import expm_multiply
     psi = np.asarray(psi0, complex).copy()
     out = [psi.copy()]
@@ -561,7 +562,7 @@ def propagate_segments(n, pairs, terms, segments, psi0, t, codes=None, interp='l
             om, de = segment_drive(0.5 * (a + b), segments, n, interp)
             key = (tuple(om), tuple(de))
             if key not in cache:
-                H = build_H_sparse(n, om, de, pairs, terms, codes)
+                H = build_H_sparse(n, om, de, pairs, terms, codes, gamma=gamma)
                 cache[key] = H.toarray() if method == 'expm' else H.tocsr()
             if method == 'expm':
                 psi = sla.expm(-1j * CONV * (b - a) * cache[key]) @ psi

@@ -94,8 +94,6 @@ def test_evolve_manybody_ground_start_static_drive(pkg, dev):
                                   delta=delta, device=dev)
     assert np.max(np.abs(out['pops_atom'] - out['pops_atom'][0])) < 1e-8
     assert np.max(np.abs(out['S'] - out['S'][0])) < 1e-7
-    with pytest.raises(NotImplementedError):
-        pkg.evolve_manybody_hip(calib, t_grid=[0.0, 1.0], gamma=np.ones(n), device=dev)
     with pytest.raises(ValueError):
         pkg.evolve_manybody_hip(calib, t_grid=[0.0, 1.0], entropy_cut=n, device=dev)
 
@@ -127,3 +125,54 @@ def test_evolve_manybody_loops_vortex(pkg, dev):
         assert np.allclose(O['loop_phase_series'][k, :2], ph, atol=1e-8)
     ph, wd = pkg.loop_observables(orc.spin_observables(n, ref[-1])['xy_corr'], loops)
     assert np.allclose(O['loop_phase'], ph, atol=1e-8) and np.allclose(O['loop_winding'], wd, atol=1e-8)
+
+
+def test_evolve_manybody_lossy_full_basis(pkg, dev):
+    """gamma != 0: the device Taylor propagator of H − (i/2)·γ·1e-3/CONV·n
+    against the oracle's exact exponential of the same non-Hermitian H;
+    every observable is taken on the decaying (unnormalised) state, as the
+    reference does."""
+    n = 8
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
+    rng = np.random.default_rng(21)
+    segs = _segments(n, rng)
+    gamma = rng.uniform(0.5, 6.0, n)                 # 1/µs
+    t = np.array([0.0, 30.0, 90.0])
+    link = [(0, 1), (2, 5)]
+    out = pkg.evolve_manybody_hip(calib, psi0='grgrgggg', t_grid=t, segments=segs, gamma=gamma,
+                                  link_pairs=link, record_psi=True, diag=True, device=dev)
+    ref = orc.propagate_segments(n, calib['pairs'], ['I1'], segs,
+                                 pkg.state_vector_np('grgrgggg', n), t, gamma=gamma)
+    assert np.max(np.abs(out['psi_amp'] - ref[-1])) < 1e-10
+    _check(out, ref, n, None, n // 2, link=link)
+    assert np.all(np.diff(out['norms']) < 0) and out['norms'][-1] < 0.9
+    assert np.allclose(out['loss'], 1 - out['norms'])
+    assert out['backend'] == 'hip-ed-nonhermitian' and out['error_report']['hermitian'] is False
+    assert out['error_report']['max_norm_dev'] == 0.0 and out['diag']['hermitian'] is False
+    # a scalar rate applies to every atom
+    o2 = pkg.evolve_manybody_hip(calib, psi0='grgrgggg', t_grid=t, segments=segs, gamma=2.0,
+                                 device=dev, want_observables=False)
+    r2 = orc.propagate_segments(n, calib['pairs'], ['I1'], segs,
+                                pkg.state_vector_np('grgrgggg', n), t, gamma=np.full(n, 2.0))
+    assert np.max(np.abs(o2['norms'] - np.sum(np.abs(r2) ** 2, axis=1))) < 1e-10
+
+
+def test_evolve_manybody_lossy_projected_plan(pkg, dev):
+    """gamma != 0 on a blockade-graph basis (the loss diagonal summed over
+    each code's excited atoms on the device)."""
+    n = 12
+    calib = _lattice_calib(4, 3, c6=30.0)
+    cons = {'kind': 'blockade', 'threshold_mhz': 10.0}
+    plan = pkg.build_plan_hip(calib, constraint=cons, device=dev)
+    codes = plan['states'].cpu().numpy()
+    rng = np.random.default_rng(22)
+    segs = _segments(n, rng)
+    gamma = rng.uniform(0.5, 6.0, n)
+    t = np.array([0.0, 40.0, 100.0])
+    out = pkg.evolve_manybody_hip(calib, t_grid=t, segments=segs, plan=plan, gamma=gamma,
+                                  device=dev, entropy_cut=5)
+    psi0 = pkg.state_vector_np(None, n, codes)
+    ref = orc.propagate_segments(n, calib['pairs'], ['I1'], segs, psi0, t, codes=codes,
+                                 gamma=gamma)
+    _check(out, ref, n, codes, 5)
+    assert out['backend'] == 'hip-ed-plan-blockade' and out['error_report']['hermitian'] is False

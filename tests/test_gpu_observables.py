@@ -86,6 +86,10 @@ def test_correlations_all_plan_kinds(pkg, dev, dtype):
         assert np.max(np.abs(ni - orc.site_occupations(n, v, codes))) < tol
         C2, _ = pkg.corr_matrix_hip(P, x)
         assert np.array_equal(C, C2)                        # deterministic sums
+        # a state of norm 0.7 (as under loss) is not renormalised
+        C3, ni3 = pkg.corr_matrix_hip(P, 0.7 * x)
+        assert np.max(np.abs(C3 - orc.corr_matrix(n, 0.7 * v, codes))) < tol, name
+        assert np.max(np.abs(ni3 - 0.49 * orc.site_occupations(n, v, codes))) < tol
 
 
 def test_correlations_64_sites_and_large_sector(pkg, dev):

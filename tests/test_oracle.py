@@ -280,3 +280,23 @@ def test_entropy_vs_reduced_density_matrix():
     assert abs(orc.entropy_at_cut(2, bell, 1) - np.log(2)) < 1e-14
     prod = np.kron([0.6, 0.8], [1.0, 0.0]).astype(complex)
     assert abs(orc.entropy_at_cut(2, prod, 1)) < 1e-14
+
+
+def test_oracle_loss_diagonal_closed_form():
+    """γ convention of the oracle's non-Hermitian H (channels.py:58-77:
+    −(i/2)·γ·1e-3/CONV per excited atom): with no drive and no couplings,
+    |ψ|² of |r r g⟩ decays as exp(−(γ0 + γ1)·t·1e-3) (t in ns, γ in 1/µs),
+    and a ground-state atom does not decay."""
+    n = 3
+    gamma = np.array([2.0, 5.0, 11.0])
+    seg = [dict(t0=0.0, Om=np.zeros(n), phase=np.zeros(n), delta=np.zeros(n))]
+    t = np.array([0.0, 100.0, 350.0])
+    psi0 = np.zeros(1 << n, complex)
+    psi0[0b110] = 1.0
+    out = orc.propagate_segments(n, [], ['I1'], seg, psi0, t, gamma=gamma)
+    norms = np.sum(np.abs(out) ** 2, axis=1)
+    assert np.allclose(norms, np.exp(-(gamma[0] + gamma[1]) * t * 1e-3), rtol=1e-12, atol=0)
+    g0 = np.zeros(1 << n, complex)
+    g0[0] = 1.0
+    outg = orc.propagate_segments(n, [], ['I1'], seg, g0, t, gamma=gamma)
+    assert np.allclose(np.sum(np.abs(outg) ** 2, axis=1), 1.0, rtol=1e-14)
