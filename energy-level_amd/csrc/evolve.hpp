@@ -60,6 +60,10 @@ __device__ __forceinline__ void evo_finish(const EvoIO<V>& io, uint64_t g, V fl,
   }
 }
 
+#ifndef RYD_EVO_FAR_BATCH
+#define RYD_EVO_FAR_BATCH 8
+#endif
+
 // RR r-groups per block: RR = tile_r (one tile per block, bits 10.. of the
 // tile in registers) or RR = 1 (a tile spread over tile_r blocks, so a small
 // vector still fills the chip; the tile's bits 10.. then come from the other
@@ -172,6 +176,32 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
       for (int c = 0; c < 4; ++c) vcfma(fl[rr][c], ar, aim, qv.v[c]);
     }
   }
+  if constexpr (RR == 1 && RYD_EVO_FAR_BATCH > 1) {
+    // small vectors (a tile spread over several blocks): the far partners
+    // are L2 hits, so the loads of RYD_EVO_FAR_BATCH far bits go out
+    // together instead of one round trip per bit
+    constexpr int FB = RYD_EVO_FAR_BATCH;
+    for (int k0 = 0; k0 < nhi; k0 += FB) {
+      V4<V> q[FB];
+#pragma unroll
+      for (int u = 0; u < FB; ++u) {
+        const int k = k0 + u;
+        if (k < nhi)
+          q[u] = *reinterpret_cast<const V4<V>*>(io.x + (((tile ^ (1ull << k)) << TB) + r0 * 1024 + t * 4));
+      }
+#pragma unroll
+      for (int u = 0; u < FB; ++u) {
+        const int k = k0 + u;
+        if (k >= nhi) break;
+        const int p = TB + k;
+        const int bit = (int)((tile >> k) & 1ull);
+        const RT ar = (RT)(M.ar[p] * io.s);
+        const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * io.s);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) vcfma(fl[0][c], ar, aim, q[u].v[c]);
+      }
+    }
+  } else {
   for (int k = 0; k < nhi; ++k) {
     const int p = TB + k;
     const uint64_t pbase = (tile ^ (1ull << k)) << TB;
@@ -184,6 +214,7 @@ __global__ __launch_bounds__(256) void k_evo_tiled(FullModel M, EvoIO<V> io, Red
 #pragma unroll
       for (int c = 0; c < 4; ++c) vcfma(fl[rr][c], ar, aim, qv.v[c]);
     }
+  }
   }
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
