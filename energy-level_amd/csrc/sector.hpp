@@ -607,8 +607,14 @@ __global__ __launch_bounds__(NW * 64, NW >= 16 ? 4 : RYD_FIB_FL_WAVES(V)) void k
 // every valid pattern, so each top-bit flip (pattern rank ± F(q+2)) is an LDS
 // read.  It reads the pass-A partial from y and finishes
 //   y = ch·(y_A + Σ_top flips of s·x) + cx·(s·x) + cz·z  with the reductions.
-constexpr int FIB_K = 9;
-constexpr int FIB_NG = 89;                      // F(FIB_K + 2)
+// N=40 FP64 H·v (pass A + this pass): 8 top bits 4.82 ms, 9 bits 4.69 ms,
+// 10 bits 5.25 ms.
+#ifndef RYD_FIB_K
+#define RYD_FIB_K 9
+#endif
+constexpr int FIB_K = RYD_FIB_K;
+constexpr int fib_const(int k) { return k < 2 ? k : fib_const(k - 1) + fib_const(k - 2); }
+constexpr int FIB_NG = fib_const(FIB_K + 2);    // valid top patterns
 template <class V> constexpr int fib_run() { return 256 / (int)sizeof(V); }
 
 template <class V>
