@@ -43,3 +43,17 @@ def test_abi_version_and_error_path(pkg):
     assert 'n must be' in nat.last_error()
     with pytest.raises(ValueError):
         nat.check(rc)
+
+
+def test_argument_checks_before_device_work(pkg):
+    """Entry points reject null or out-of-range arguments before any device
+    call (so these run on a host without a GPU)."""
+    from energy_level_amd import _native as nat
+    L = nat.lib()
+    g = (ctypes.c_double * 4)(0.0, 0.0, 0.0, 0.0)
+    # ryd_evolve_taylor: null plan, then a real dtype on a null plan
+    assert L.ryd_evolve_taylor(None, nat.RYD_C128, None, 1.0, 1, 20, g, None, None) == nat.RYD_ERR_INVALID
+    assert 'null argument' in nat.last_error()
+    assert L.ryd_hv(None, nat.RYD_F64, None, None, None) == nat.RYD_ERR_INVALID
+    assert L.ryd_evolve_forward(None, nat.RYD_C64, None, 1, None, None, None, None, 4, None,
+                                None, None) == nat.RYD_ERR_INVALID
