@@ -15,7 +15,7 @@ import numpy as np
 from .model import from_positions, chain_positions, square_positions, c6_coupling, from_calib
 
 __all__ = ['chain_calib', 'lattice_calib', 'warm_device_memory', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5',
-           'run_g1', 'time_hv']
+           'run_g1', 'run_k1', 'time_hv']
 
 
 def chain_calib(n, c6):
@@ -330,4 +330,53 @@ def run_g1(device='cuda', w=6, h=7, diag=False, steps=10, seed=0, ground_state=T
         out['seconds_fp64_gs'] = time.perf_counter() - t0
         out['fp64'] = {k: r[k] for k in ('E0', 'gap', 'residuals', 'converged', 'krylov_dim')}
     P.close()
+    return out
+
+
+def run_k1(device='cuda', calib=None, steps=10, seed=0, ground_state=True):
+    """Exchange (J) channels: the dipolar XY kagome calib with 24 atoms held
+    by the reference's checks (checks/fixtures/kagome_n24.json; 276 pairs
+    with J, per-atom delta_mhz), Ω = 0.  Reports the H·v time of the
+    full-basis plan (D = 2^24) in FP64 and FP32, and the FP64 ground state of
+    the half-filled number sector (12 excitations, D = 2,704,156) on a code
+    list (the sector scan's basis)."""
+    import json
+    import os
+    import torch
+    from .basis import number_sector_basis
+    from .lanczos import hip_lanczos_ground_state
+    from .plan import Plan
+    if calib is None:
+        here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+        with open(os.path.join(here, 'tests', 'golden', 'kagome_n24.json')) as f:
+            calib = json.load(f)
+    n = len(calib['atoms'])
+    delta = np.array([float(a.get('delta_mhz', 0.0)) for a in calib['atoms']])
+    Om = np.zeros(n)
+    model = from_calib(calib, Om, delta)
+    P = Plan(model, device=device)
+    # off-diagonal entries per H·v: an exchange pair acts on the half of the
+    # codes where its two bits differ, a pair-creation pair on the other half
+    nj = int(np.count_nonzero(np.abs(np.asarray(model.pair_J)) > 0))
+    na = int(np.count_nonzero(np.abs(np.asarray(model.pair_A)) > 0))
+    offdiag = (nj + na) * (P.dim // 2)
+    out = dict(config='K1 kagome N=%d XY (J channels)' % n, n_sites=n,
+               pairs=len(calib['pairs']), dim=P.dim, offdiag_per_hv=offdiag)
+    for name, dt, b in (('f64', torch.float64, 8), ('f32', torch.float32, 4)):
+        ms = time_hv(P, dt, steps=steps, seed=seed)
+        out['hv_ms_' + name] = ms
+        out['hv_gbps_' + name] = 2.0 * P.dim * b / (ms * 1e-3) / 1e9
+        out['offdiag_per_s_' + name] = offdiag / (ms * 1e-3)
+    P.close()
+    if ground_state:
+        codes = number_sector_basis(n, n // 2)
+        _sync()
+        t0 = time.perf_counter()
+        r = hip_lanczos_ground_state(calib, Om, delta, basis_indices=codes, dtype='float64',
+                                     seed=seed, device=device)
+        _sync()
+        out['sector'] = n // 2
+        out['sector_dim'] = int(len(codes))
+        out['seconds_fp64_gs'] = time.perf_counter() - t0
+        out['fp64'] = {k: r[k] for k in ('E0', 'gap', 'residuals', 'converged', 'krylov_dim')}
     return out

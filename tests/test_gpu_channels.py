@@ -152,3 +152,31 @@ def test_sector_scan_kagome_xy(pkg, dev):
     assert abs(out['E0'] - best) < 1e-9 * max(1.0, abs(best))
     with pytest.raises(ValueError):
         pkg.scan_ground_sectors_hip(calib, delta=delta, Om=np.ones(n), device=dev)
+
+
+def test_kagome_n24_sector_is_the_restricted_full_hv(pkg, dev):
+    """Reference kagome N=24 XY calib (J channels conserve the excitation
+    number): the full-basis H·v (D = 2^24) of a vector supported on the
+    half-filled sector, restricted to that sector, equals the code-list H·v
+    of the sector (D = 2,704,156) — two different rank modes of the channel
+    kernel checked against each other at the full size."""
+    import torch
+    calib = json.load(open(os.path.join(HERE, 'golden', 'kagome_n24.json')))
+    n = len(calib['atoms'])
+    delta = np.array([a.get('delta_mhz', 0.0) for a in calib['atoms']])
+    model = pkg.from_calib(calib, np.zeros(n), delta)
+    codes = pkg.number_sector_basis(n, n // 2)
+    rng = np.random.default_rng(24)
+    xs = rng.normal(size=len(codes))
+    Ps = pkg.Plan(model, basis='codes', codes=codes, device=dev)
+    ys = Ps.hv(torch.as_tensor(xs, device=dev)).cpu().numpy()
+    Ps.close()
+    Pf = pkg.Plan(model, device=dev)
+    xf = torch.zeros(1 << n, dtype=torch.float64, device=dev)
+    ci = torch.as_tensor(codes, device=dev)
+    xf[ci] = torch.as_tensor(xs, device=dev)
+    yf = Pf.hv(xf)
+    Pf.close()
+    assert float(torch.linalg.vector_norm(yf).item() ** 2) == pytest.approx(
+        float(np.sum(ys ** 2)), rel=1e-12)                 # nothing leaves the sector
+    assert np.max(np.abs(yf[ci].cpu().numpy() - ys)) <= 1e-12 * np.max(np.abs(ys))
