@@ -133,12 +133,23 @@ __global__ __launch_bounds__(256) void k_chan_apply(ChanModel M, ApplyIO<V> io, 
     const uint64_t c = chan_code(M, k);
     const V xe = vscale(io.x[k], sr);
     V acc = vscale(xe, (RT)chan_diag(M, vs, c));
-    for (int p = 0; p < M.n; ++p) {
-      if (M.ar[p] == 0.0 && M.ai[p] == 0.0) continue;
-      const int64_t r = chan_rank(M, c ^ (1ull << p));
-      if (r < 0) continue;
-      const int bit = (int)((c >> p) & 1ull);
-      vcfma(acc, (RT)(M.ar[p] * s), (RT)((bit ? M.ai[p] : -M.ai[p]) * s), io.x[r]);
+    for (int p0 = 0; p0 < M.n; p0 += CHAN_PB) {    // Ω flips, batched like the pairs
+      int64_t r[CHAN_PB];
+#pragma unroll
+      for (int u = 0; u < CHAN_PB; ++u) {
+        const int p = p0 + u < M.n ? p0 + u : 0;
+        r[u] = (p0 + u < M.n && (M.ar[p] != 0.0 || M.ai[p] != 0.0)) ? chan_rank(M, c ^ (1ull << p)) : -1;
+      }
+      V xv[CHAN_PB];
+#pragma unroll
+      for (int u = 0; u < CHAN_PB; ++u) xv[u] = r[u] >= 0 ? io.x[r[u]] : vzero(xe);
+#pragma unroll
+      for (int u = 0; u < CHAN_PB; ++u) {
+        if (r[u] < 0) continue;
+        const int p = p0 + u;
+        const int bit = (int)((c >> p) & 1ull);
+        vcfma(acc, (RT)(M.ar[p] * s), (RT)((bit ? M.ai[p] : -M.ai[p]) * s), xv[u]);
+      }
     }
     // pair channels in groups of CHAN_PB: the ranks of a group first, then
     // all its gathers in flight together, then the FMAs
