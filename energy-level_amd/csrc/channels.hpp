@@ -33,6 +33,7 @@ struct PairChan {
 
 constexpr int CHAN_MAX_BLOCKS = 16384;
 constexpr int BINOM_LD = 65;          // binom[p * BINOM_LD + i] = C(p, i)
+constexpr int RS_MAXN = 32;           // number-sector split rank tables up to this n
 
 struct ChanModel {
   int n;
@@ -42,6 +43,8 @@ struct ChanModel {
   int64_t dim;
   const int64_t* codes;        // [dim] ascending (modes NUMBER, SEARCH)
   const int64_t* binom;        // [64][BINOM_LD]
+  int rsL;                     // RANK_NUMBER split tables (n <= RS_MAXN): low bits
+  const int64_t* rsplit;       // [2^rsL] low part, then [2^(n-rsL)][rsL+1] high part
   const PairChan* pairs;       // [npairs]
   const double* vb;            // [n*n] V by bit position
   // RANK_GRAPH tables (graph.hpp)
@@ -77,6 +80,10 @@ __device__ inline int64_t chan_rank(const M_& M, uint64_t s) {
   }
   if (M.mode == RANK_NUMBER) {
     if (__popcll(s) != M.m) return -1;
+    if (M.rsplit) {       // r = Σ_i C(p_i, i) split at bit rsL: two table loads
+      const uint64_t l = s & ((1ull << M.rsL) - 1ull);
+      return M.rsplit[l] + M.rsplit[(1ll << M.rsL) + (int64_t)(s >> M.rsL) * (M.rsL + 1) + __popcll(l)];
+    }
     int64_t r = 0;
     int i = 1;
     for (uint64_t c = s; c; c &= c - 1, ++i) r += M.binom[__builtin_ctzll(c) * BINOM_LD + i];

@@ -26,6 +26,8 @@ struct ObsBasis {
   int64_t dim, tl;
   const int64_t* codes;
   const int64_t* binom;
+  int rsL;
+  const int64_t* rsplit;
   const uint64_t* hpk;
   const uint32_t* lo_idx;
   const uint32_t* fc;

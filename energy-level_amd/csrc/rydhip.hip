@@ -77,6 +77,8 @@ struct ryd_plan {
   int popcount = 0;                  // number sector
   int64_t* d_codes = nullptr;        // [dim] ascending
   int64_t* d_binom = nullptr;        // [64][BINOM_LD]
+  int rs_L = 0;                      // number-sector split rank tables (RANK_NUMBER, n <= RS_MAXN)
+  int64_t* d_rsplit = nullptr;       // [2^rs_L] + [2^(n-rs_L)][rs_L+1]
   PairChan* d_pairs = nullptr;       // pair flip channels
   int npairs = 0;
   double dl[MAXN] = {0};             // delta by bit position
@@ -234,6 +236,8 @@ ChanModel chan_model(const ryd_plan* P) {
   M.dim = P->dim;
   M.codes = P->d_codes;
   M.binom = P->d_binom;
+  M.rsL = P->rs_L;
+  M.rsplit = P->d_rsplit;
   M.pairs = P->d_pairs;
   M.vb = P->d_vb;
   M.gl = P->gr_L;
@@ -1030,6 +1034,33 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
     e = hipMalloc(&P->d_binom, sizeof(int64_t) * binom.size());
     if (e == hipSuccess)
       e = hipMemcpy(P->d_binom, binom.data(), sizeof(int64_t) * binom.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && P->rank_mode == RANK_NUMBER && n <= RS_MAXN) {
+      // r = Σ_i C(p_i, i) over the set bits p_1 < p_2 < ...: the low rs_L bits
+      // give the first terms, the high bits the rest with indices shifted by
+      // the low popcount
+      const int L = n / 2, Hn = n - L;
+      std::vector<int64_t> rs(((size_t)1 << L) + ((size_t)1 << Hn) * (size_t)(L + 1), 0);
+      for (uint64_t l = 0; l < (1ull << L); ++l) {
+        int64_t r = 0;
+        int i = 1;
+        for (uint64_t c = l; c; c &= c - 1, ++i) r += binom[__builtin_ctzll(c) * BINOM_LD + i];
+        rs[l] = r;
+      }
+      for (uint64_t h = 0; h < (1ull << Hn); ++h)
+        for (int kl = 0; kl <= L; ++kl) {
+          int64_t r = 0;
+          int i = kl + 1;
+          for (uint64_t c = h; c; c &= c - 1, ++i) {
+            const int p = L + __builtin_ctzll(c);
+            r += i <= p ? binom[p * BINOM_LD + i] : 0;
+          }
+          rs[((size_t)1 << L) + h * (size_t)(L + 1) + (size_t)kl] = r;
+        }
+      P->rs_L = L;
+      e = hipMalloc(&P->d_rsplit, sizeof(int64_t) * rs.size());
+      if (e == hipSuccess)
+        e = hipMemcpy(P->d_rsplit, rs.data(), sizeof(int64_t) * rs.size(), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && !pc.empty()) e = hipMalloc(&P->d_pairs, sizeof(PairChan) * pc.size());
     if (e == hipSuccess && !pc.empty())
       e = hipMemcpy(P->d_pairs, pc.data(), sizeof(PairChan) * pc.size(), hipMemcpyHostToDevice);
@@ -1198,6 +1229,7 @@ int ryd_plan_destroy(ryd_plan* P) {
   if (P->d_top) (void)hipFree(P->d_top);
   if (P->d_codes) (void)hipFree(P->d_codes);
   if (P->d_binom) (void)hipFree(P->d_binom);
+  if (P->d_rsplit) (void)hipFree(P->d_rsplit);
   if (P->d_pairs) (void)hipFree(P->d_pairs);
   if (P->d_hpk) (void)hipFree(P->d_hpk);
   if (P->d_lo_idx) (void)hipFree(P->d_lo_idx);
@@ -1456,6 +1488,8 @@ ObsBasis obs_basis(const ryd_plan* P) {
     B.tl = P->gr_tl;
     B.codes = P->d_codes;
     B.binom = P->d_binom;
+    B.rsL = P->rs_L;
+    B.rsplit = P->d_rsplit;
     B.hpk = P->d_hpk;
     B.lo_idx = P->d_lo_idx;
     B.fc = P->d_fc;
