@@ -176,3 +176,42 @@ def test_evolve_manybody_lossy_projected_plan(pkg, dev):
                                  gamma=gamma)
     _check(out, ref, n, codes, 5)
     assert out['backend'] == 'hip-ed-plan-blockade' and out['error_report']['hermitian'] is False
+
+
+@pytest.mark.parametrize('basis', ['full', 'chain_fib', 'codes'])
+def test_propagate_lossy_every_basis_kind(pkg, dev, basis):
+    """The loss diagonal of the Taylor propagator reads each rank's code
+    through the plan's own basis (identity, Zeckendorf unranking, code
+    list): one piece against the oracle's exponential of H − (i/2)γ·n."""
+    import torch
+    from oracle import hv_ref
+    from energy_level_amd import evolve as ev
+    n = 10
+    rng = np.random.default_rng(31)
+    Om = rng.uniform(0.5, 2.0, n) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    delta = rng.normal(size=n)
+    pairs = [{'i': i, 'j': j, 'V': 8.0 / (j - i) ** 6} for i in range(n) for j in range(i + 1, n)]
+    calib = {'atoms': [{}] * n, 'pairs': pairs, 'terms': ['I1']}
+    gamma = rng.uniform(0.5, 8.0, n)
+    model = pkg.from_calib(calib, Om, delta)
+    if basis == 'full':
+        codes = None
+        P = pkg.Plan(model, device=dev)
+    elif basis == 'chain_fib':
+        codes = hv_ref.fib_codes(n)
+        P = pkg.Plan(model, basis='chain_fib', device=dev)
+    else:
+        codes = hv_ref.fib_codes(n)[::2].copy()        # an arbitrary sorted code list
+        P = pkg.Plan(model, basis='codes', codes=codes, device=dev)
+    dim = P.dim
+    x0 = rng.normal(size=dim) + 1j * rng.normal(size=dim)
+    x0 /= np.linalg.norm(x0)
+    psi = torch.as_tensor(x0, device=dev)
+    dt = 120.0
+    ev.propagate_lossy(P, model, psi, dt, gamma)
+    seg = [dict(t0=0.0, Om=np.abs(Om), phase=np.angle(Om), delta=delta)]
+    ref = orc.propagate_segments(n, pairs, ['I1'], seg, x0, np.array([0.0, dt]), codes=codes,
+                                 gamma=gamma)[-1]
+    assert np.max(np.abs(psi.cpu().numpy() - ref)) < 1e-11
+    assert np.linalg.norm(ref) < 0.95                    # the loss acted
+    P.close()
