@@ -72,6 +72,7 @@ SIGNATURES = {
                                           ctypes.c_int, _vp, _vp, _vp]),
     'ryd_evolve_taylor': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _dbl, ctypes.c_int, ctypes.c_int,
                                          _vp, _vp, _vp]),
+    'ryd_loss_axpy': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     'ryd_evolve_adjoint': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _dbl, _dbl, _dbl, _dbl,
                                           ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'ryd_batch_create': (ctypes.c_int, [ctypes.POINTER(Model), ctypes.c_int32, ctypes.c_int,

@@ -309,4 +309,24 @@ __global__ __launch_bounds__(256) void k_taylor_term(ObsBasis B, TaylorDiag T,
   }
 }
 
+// y += -(i/2)·G·x with G = Σ_i gamma_i n_i from each rank's code (gb[p] =
+// gamma/2 of the site at bit position p): the loss diagonal of a
+// non-Hermitian H added to a plan's H·v.
+template <class V>
+__global__ __launch_bounds__(256) void k_loss_axpy(ObsBasis B, TaylorDiag T,
+                                                   const V* __restrict__ x, V* __restrict__ y,
+                                                   int64_t D) {
+  using RT = typename VT<V>::R;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    double g = 0.0;
+    for (uint64_t b = obs_code(B, i); b; b &= b - 1) g += T.gb[__builtin_ctzll(b)];
+    const V xv = x[i];
+    V o = y[i];
+    o.re += (RT)(g * (double)xv.im);      // -(i)·g·(a + ib) = g·b - i·g·a
+    o.im -= (RT)(g * (double)xv.re);
+    y[i] = o;
+  }
+}
+
 }  // namespace ryd

@@ -1897,6 +1897,27 @@ extern "C" int ryd_evolve_taylor(const ryd_plan* P, int dtype, void* psi, double
   return do_taylor<c64>(P, dtype, psi, tau, nsub, m, gamma, work, st);
 }
 
+extern "C" int ryd_loss_axpy(const ryd_plan* P, int dtype, const void* x, void* y,
+                             const double* gamma, void* stream) {
+  if (!P || !x || !y || !gamma) return fail(RYD_ERR_INVALID, "null argument to ryd_loss_axpy");
+  if (dtype != RYD_C64 && dtype != RYD_C128)
+    return fail(RYD_ERR_INVALID, "the loss diagonal needs a complex dtype");
+  for (int i = 0; i < P->n; ++i)
+    if (!std::isfinite(gamma[i])) return fail(RYD_ERR_INVALID, "gamma must be finite");
+  TaylorDiag T;
+  for (int p = 0; p < MAXN; ++p) T.gb[p] = 0.0;
+  for (int i = 0; i < P->n; ++i) T.gb[P->n - 1 - i] = 0.5 * gamma[i];
+  const ObsBasis B = obs_basis(P);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const unsigned sg = stream_grid(P->dim);
+  if (dtype == RYD_C64)
+    k_loss_axpy<c32><<<sg, 256, 0, st>>>(B, T, static_cast<const c32*>(x), static_cast<c32*>(y), P->dim);
+  else
+    k_loss_axpy<c64><<<sg, 256, 0, st>>>(B, T, static_cast<const c64*>(x), static_cast<c64*>(y), P->dim);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
 extern "C" int ryd_evolve_adjoint(const ryd_plan* P, int dtype, const void* psi, void* g,
                                   double om, double dadd, double c, double r, int K,
                                   const double* coef, void* tstore, void* work, double* sums,

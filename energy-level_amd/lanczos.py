@@ -37,7 +37,8 @@ from .model import from_calib
 from .plan import Plan, cast
 
 __all__ = ['hip_lanczos_ground_state', 'lanczos_run', 'ritz_vectors', 'rayleigh_residual',
-           'site_occupations', 'restricted_plan', 'lanczos_vector']
+           'site_occupations', 'restricted_plan', 'lanczos_vector',
+           'hamiltonian_linear_operator_hip']
 
 
 def _tridiag(a, b):
@@ -552,3 +553,50 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
             v = back
         out['vec'] = v
     return out
+
+
+def hamiltonian_linear_operator_hip(calib, Om=None, delta=None, gamma=None, terms=None,
+                                    basis_indices=None, device='cuda'):
+    """``hamiltonian_linear_operator`` (matrix_free.py:51-62) on the device: a
+    scipy ``LinearOperator`` (dim × dim, complex128) whose matvec is the
+    plan's H·v, plus the loss diagonal −(i/2)·γ_i·1e-3/CONV·n_i of
+    channel_spec (channels.py:58-77) when ``gamma`` is not zero
+    (``ryd_loss_axpy``).  ``basis_indices``: the codes of a restricted basis,
+    vectors in their order, as ChannelKernel maps codes to basis indices.
+    Each matvec copies the vector to the device and back."""
+    import torch
+    from scipy.sparse.linalg import LinearOperator
+    from .evolve import CONV
+    n = len(calib.get('atoms') or [])
+    g = np.zeros(n) if gamma is None else np.asarray(gamma, float).reshape(-1)
+    if g.shape != (n,) or not np.all(np.isfinite(g)):
+        raise ValueError('gamma must be a finite array of length N')
+    model = from_calib(calib, Om, delta, None, terms)
+    if basis_indices is None:
+        P, order = Plan(model, device=device), None
+    else:
+        P, order = restricted_plan(model, basis_indices, None, device=device)
+    loss = np.ascontiguousarray(g * 1e-3 / CONV)
+    lossy = bool(np.any(g != 0.0))
+
+    def matvec(v):
+        v = np.asarray(v, complex).reshape(-1)
+        if v.size != P.dim:
+            raise ValueError('vector of length %d for a %d-dimensional operator' % (v.size, P.dim))
+        if order is not None:
+            v = v[order]
+        x = torch.as_tensor(np.ascontiguousarray(v), device=P.device)
+        y = P.hv(x)
+        if lossy:
+            nat.check(nat.lib().ryd_loss_axpy(P.handle, nat.RYD_C128, nat.ptr(x), nat.ptr(y),
+                                              loss.ctypes.data, nat.stream_ptr()))
+        out = y.cpu().numpy()
+        if order is not None:
+            back = np.empty_like(out)
+            back[order] = out
+            out = back
+        return out
+
+    op = LinearOperator((P.dim, P.dim), matvec=matvec, dtype=np.complex128)
+    op.plan = P
+    return op

@@ -238,6 +238,14 @@ int ryd_evolve_forward(const ryd_plan* plan, int dtype, void* psi, int64_t nslic
 int ryd_evolve_taylor(const ryd_plan* plan, int dtype, void* psi, double tau, int nsub, int m,
                       const double* gamma, void* work, void* stream);
 
+/* y += -(i/2)·G·x, G = Σ_i gamma[i]·n_i over the plan's basis (host FP64
+ * [n], by site, MHz like delta): with ryd_hv, the action of the
+ * non-Hermitian H of channel_spec with gamma != 0 (channels.py:58-77, the
+ * −(i/2)Γ_i n_i diagonal) for hamiltonian_linear_operator
+ * (matrix_free.py:51-62).  Complex dtypes. */
+int ryd_loss_axpy(const ryd_plan* plan, int dtype, const void* x, void* y, const double* gamma,
+                  void* stream);
+
 /* Vector-Jacobian product of one slice of ryd_evolve_forward (the adjoint;
  * no reference counterpart — the reference has no gradient).  Given the
  * slice input psi and g = dL/d(conj psi_out), overwrites g with
