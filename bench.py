@@ -201,7 +201,7 @@ def main():
         'dtype': 'f64',
         'data': 'synthetic',
         'config': {'workload': 'hv_full_chain_n%d' % n, 'n_sites': n, 'dim': D, 'basis': 'full',
-                   'model': 'chain a=1, V=10/r^6 all pairs, Omega=1, Delta=1.2',
+                   'hamiltonian': 'chain a=1, V=10/r^6 all pairs, Omega=1, Delta=1.2',
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': f64['achieved_gbps'], 'peak': HBM_PEAK_GBPS,
                      'unit': 'GB/s', 'frac': f64['achieved_gbps'] / HBM_PEAK_GBPS,
