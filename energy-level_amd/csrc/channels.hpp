@@ -202,6 +202,187 @@ __global__ __launch_bounds__(256) void k_chan_apply(ChanModel M, ApplyIO<V> io, 
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
+// Full basis with pair channels, tiled: a block holds 2^TB consecutive codes
+// (the single-pass tile width of the element type) scaled by s in LDS.
+// Diagonal from the plan's E_lo/E_hi tables of that width plus the cross
+// field of the tile's high bits (as k_full_tiled).  A flip or pair whose bits
+// all lie below TB reads LDS; one that touches high bits reads the partner
+// tile (tile ^ high part of the mask) at position e ^ (low part) through the
+// cache hierarchy.  A pair with both bits high has one coefficient for the
+// whole tile (its source bits are the tile's); a pair with one low and one
+// high bit loads only for the codes whose coefficient is non-zero.
+struct ChanFullModel {
+  int n;
+  int npairs;
+  const double* elo;           // [2^TB] diagonal of the bits below TB
+  const double* ehi;           // [2^(n-TB)] diagonal of the bits from TB up
+  const double* vb;            // [n*n] V by bit position
+  const PairChan* pairs;       // [npairs]
+  double ar[MAXN], ai[MAXN];   // Ω/2 by bit position (0: no Ω channel)
+};
+
+// gather-form coefficient of a pair for source bits (bi, bj) at (pi, pj):
+// exchange (1,0) conj(J), (0,1) J; pair creation (0,0) A, (1,1) conj(A)
+__device__ __forceinline__ void pair_coef(const PairChan& P, int bi, int bj, double& cr,
+                                          double& ci) {
+  const bool ex = bi != bj;
+  cr = ex ? P.jr : P.ar;
+  const double im = ex ? P.ji : P.ai;
+  ci = bi ? -im : im;
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_chan_full_tiled(ChanFullModel M, ApplyIO<V> io,
+                                                         Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int TB = tile_bits<V>();
+  constexpr int TILE = 1 << TB;
+  constexpr int R = TILE / 256;
+  __shared__ V lx[TILE];
+  __shared__ double Fx[16];
+  __shared__ double T0[256];
+  __shared__ double T1[1 << (TB - 8)];
+
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+  const int t = threadIdx.x;
+  const uint64_t ntiles = 1ull << (M.n - TB);
+  const uint64_t b = blockIdx.x;
+  const uint64_t tile = ntiles >= 8 ? ((b & 7ull) * (ntiles >> 3) + (b >> 3)) : b;
+  const uint64_t base = tile << TB;
+
+#pragma unroll
+  for (int r = 0; r < R; ++r) lx[r * 256 + t] = vscale(io.x[base + r * 256 + t], sr);
+  if (t < TB) {
+    double f = 0.0;
+    const double* row = M.vb + (size_t)t * M.n;
+    for (int q = TB; q < M.n; ++q)
+      if ((tile >> (q - TB)) & 1ull) f += row[q];
+    Fx[t] = f;
+  }
+  __syncthreads();
+  {
+    double v = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+      if ((t >> p) & 1) v += Fx[p];
+    T0[t] = v;
+    if (t < (1 << (TB - 8))) {
+      double w = 0.0;
+#pragma unroll
+      for (int p = 8; p < TB; ++p)
+        if ((t >> (p - 8)) & 1) w += Fx[p];
+      T1[t] = w;
+    }
+  }
+  const double eh = M.ehi[tile];
+  __syncthreads();
+
+  V acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int e = r * 256 + t;
+    const double d = eh + M.elo[e] + T0[e & 255] + T1[e >> 8];
+    acc[r] = vscale(lx[e], (RT)d);
+  }
+
+  // Ω flips
+  for (int p = 0; p < M.n; ++p) {
+    const double arp = M.ar[p], aip = M.ai[p];
+    if (arp == 0.0 && aip == 0.0) continue;
+    if (p < TB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int e = r * 256 + t;
+        const int bit = (e >> p) & 1;
+        vcfma(acc[r], (RT)arp, (RT)(bit ? aip : -aip), lx[e ^ (1 << p)]);
+      }
+    } else {
+      const int bit = (int)((tile >> (p - TB)) & 1ull);
+      const V* src = io.x + ((tile ^ (1ull << (p - TB))) << TB);
+      const RT a = (RT)(arp * s), ai = (RT)((bit ? aip : -aip) * s);
+      V xv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) xv[r] = src[r * 256 + t];
+#pragma unroll
+      for (int r = 0; r < R; ++r) vcfma(acc[r], a, ai, xv[r]);
+    }
+  }
+
+  // pair channels
+  for (int q = 0; q < M.npairs; ++q) {
+    const PairChan P = M.pairs[q];
+    const int lo = P.pi < P.pj ? P.pi : P.pj;
+    const int hi = P.pi < P.pj ? P.pj : P.pi;
+    if (hi < TB) {
+      const int m = (1 << P.pi) | (1 << P.pj);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int src = (r * 256 + t) ^ m;
+        double cr, ci;
+        pair_coef(P, (src >> P.pi) & 1, (src >> P.pj) & 1, cr, ci);
+        vcfma(acc[r], (RT)cr, (RT)ci, lx[src]);
+      }
+    } else if (lo >= TB) {
+      const uint64_t st = tile ^ ((1ull << (P.pi - TB)) | (1ull << (P.pj - TB)));
+      double cr, ci;
+      pair_coef(P, (int)((st >> (P.pi - TB)) & 1ull), (int)((st >> (P.pj - TB)) & 1ull), cr, ci);
+      if (cr == 0.0 && ci == 0.0) continue;
+      const V* src = io.x + (st << TB);
+      const RT a = (RT)(cr * s), ai = (RT)(ci * s);
+      V xv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) xv[r] = src[r * 256 + t];
+#pragma unroll
+      for (int r = 0; r < R; ++r) vcfma(acc[r], a, ai, xv[r]);
+    } else {
+      const uint64_t st = tile ^ (1ull << (hi - TB));
+      const int bh = (int)((st >> (hi - TB)) & 1ull);      // source bit of the high site
+      const V* src = io.x + (st << TB);
+      // coefficients for source bit 0 and 1 of the low site
+      double c0r, c0i, c1r, c1i;
+      pair_coef(P, P.pi == lo ? 0 : bh, P.pj == lo ? 0 : bh, c0r, c0i);
+      pair_coef(P, P.pi == lo ? 1 : bh, P.pj == lo ? 1 : bh, c1r, c1i);
+      const bool use0 = c0r != 0.0 || c0i != 0.0, use1 = c1r != 0.0 || c1i != 0.0;
+      if (!use0 && !use1) continue;
+      V xv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int e = r * 256 + t;
+        const int bl = ((e >> lo) & 1) ^ 1;                 // source bit of the low site
+        xv[r] = (bl ? use1 : use0) ? src[e ^ (1 << lo)] : vzero(acc[r]);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int bl = (((r * 256 + t) >> lo) & 1) ^ 1;
+        vcfma(acc[r], (RT)((bl ? c1r : c0r) * s), (RT)((bl ? c1i : c0i) * s), xv[r]);
+      }
+    }
+  }
+
+  // combine, store, reduce
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint64_t g = base + r * 256 + t;
+    const V xe = lx[r * 256 + t];
+    V v = vscale(acc[r], chr);
+    vfma(v, cxr, xe);
+    if (io.z) vfma(v, czr, io.z[g]);
+    io.y[g] = v;
+    if (io.reduce) {
+      s0 += rdot(xe, v);
+      s1 += nrm2(v);
+      s2 += nrm2(xe);
+      if (io.probe) s3 += zdot(((io.probe[g >> 5] >> (g & 31)) & 1u) ? 1.0 : -1.0, v);
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
 // site occupations over a code basis: partials[block][p] = Σ |x_k|^2 b_p(code_k),
 // partials[block][MAXN_OCC] = Σ |x_k|^2 (deterministic: fixed chunk per block)
 template <class V>

@@ -254,6 +254,31 @@ ChanModel chan_model(const ryd_plan* P) {
   return M;
 }
 
+// Full-basis pair-channel plans run the tiled kernel when the diagonal tables
+// of the element type's tile width exist (n >= that width).
+#ifndef RYD_CHAN_TILED
+#define RYD_CHAN_TILED 1
+#endif
+bool chan_full_tiled(const ryd_plan* P, int width) {
+  return RYD_CHAN_TILED && P->basis == RYD_BASIS_FULL && P->rank_mode == RANK_FULL &&
+         P->n >= width && P->d_elo[width - 10] && P->tbits[width - 10] == width;
+}
+
+ChanFullModel chan_full_model(const ryd_plan* P, int width) {
+  ChanFullModel M;
+  M.n = P->n;
+  M.npairs = P->npairs;
+  M.elo = P->d_elo[width - 10];
+  M.ehi = P->d_ehi[width - 10];
+  M.vb = P->d_vb;
+  M.pairs = P->d_pairs;
+  for (int p = 0; p < MAXN; ++p) {
+    M.ar[p] = P->ar[p];
+    M.ai[p] = P->ai[p];
+  }
+  return M;
+}
+
 GraphModel graph_model(const ryd_plan* P) {
   GraphModel M;
   M.n = P->n;
@@ -361,6 +386,15 @@ int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStre
   if (P->generic) {
     io.nfar = 0;
     io.partial = 0;
+    constexpr int TB = tile_bits<V>();
+    if (chan_full_tiled(P, TB)) {
+      const unsigned grid = (unsigned)(1ull << (P->n - TB));
+      k_chan_full_tiled<V><<<dim3(grid), dim3(256), 0, st>>>(chan_full_model(P, TB), io, rd);
+      LAUNCH_CHECK();
+      if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
+      LAUNCH_CHECK();
+      return RYD_OK;
+    }
     const unsigned grid = chan_grid(P);
     k_chan_apply<V><<<dim3(grid), dim3(256), 0, st>>>(chan_model(P), io, rd);
     LAUNCH_CHECK();

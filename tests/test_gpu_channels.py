@@ -32,8 +32,11 @@ def _plan(pkg, dev, n, Om, delta, pairs, terms, codes=None):
                     basis='codes' if codes is not None else 'full', codes=codes, device=dev)
 
 
-@pytest.mark.parametrize('n', [2, 5, 9, 12])
+@pytest.mark.parametrize('n', [2, 5, 9, 12, 15])
 def test_full_basis_with_exchange_and_pair_creation(pkg, dev, n):
+    """From n = 11/12 the full basis runs the tiled channel kernel (2^11
+    complex128 and 2^12 complex64 codes per tile): n = 15 has pairs with both
+    bits in the tile, one bit above it and both above it."""
     import torch
     rng = np.random.default_rng(n)
     terms = ['I1', 'I2', 'I3']
@@ -51,9 +54,9 @@ def test_full_basis_with_exchange_and_pair_creation(pkg, dev, n):
         P.hv(torch.as_tensor(x.real, device=dev))
 
 
-def test_real_exchange_model_real_vectors(pkg, dev):
+@pytest.mark.parametrize('n', [10, 17])
+def test_real_exchange_model_real_vectors(pkg, dev, n):
     import torch
-    n = 10
     rng = np.random.default_rng(3)
     pairs = _pairs(n, rng, A=False, cplx=False)
     P = _plan(pkg, dev, n, rng.uniform(0, 1, n), rng.uniform(-1, 1, n), pairs, ['I1', 'I2'])
@@ -152,6 +155,23 @@ def test_sector_scan_kagome_xy(pkg, dev):
     assert abs(out['E0'] - best) < 1e-9 * max(1.0, abs(best))
     with pytest.raises(ValueError):
         pkg.scan_ground_sectors_hip(calib, delta=delta, Om=np.ones(n), device=dev)
+
+
+def test_full_basis_pair_channels_ground_state(pkg, dev):
+    """Lanczos on the full basis with J and A (the tiled channel kernel in
+    the fused three-term step) against eigsh of the oracle's CSR."""
+    from scipy.sparse.linalg import eigsh
+    n = 14
+    rng = np.random.default_rng(14)
+    terms = ['I1', 'I2', 'I3']
+    pairs = _pairs(n, rng, cplx=False)
+    Om, delta = rng.uniform(0.5, 1.5, n), rng.uniform(-2, 2, n)
+    calib = {'atoms': [{}] * n, 'pairs': pairs, 'terms': terms}
+    out = pkg.hip_lanczos_ground_state(calib, Om, delta, terms=terms, device=dev)
+    H = orc.build_H_sparse(n, Om, delta, pairs, terms)
+    E0 = float(eigsh(H.real, k=1, which='SA', tol=1e-12)[0][0])
+    assert out['converged']
+    assert abs(out['E0'] - E0) <= 1e-10 * abs(E0)
 
 
 def test_kagome_n24_sector_is_the_restricted_full_hv(pkg, dev):
