@@ -288,24 +288,33 @@ __global__ __launch_bounds__(256) void k_chan_full_tiled(ChanFullModel M, ApplyI
     acc[r] = vscale(lx[e], (RT)d);
   }
 
+  // Element e = (r << 8) | t: a bit below 8 is the thread's, constant over r;
+  // a bit from 8 up is r's, uniform per unrolled r.  Each case below keeps
+  // the per-element work to an LDS or buffer load and an FMA.
+  constexpr uint32_t VB = sizeof(V);
   // Ω flips
   for (int p = 0; p < M.n; ++p) {
     const double arp = M.ar[p], aip = M.ai[p];
     if (arp == 0.0 && aip == 0.0) continue;
-    if (p < TB) {
+    if (p < 8) {
+      const int bit = (t >> p) & 1;
+      const RT a = (RT)arp, ai = (RT)(bit ? aip : -aip);
+      const int tp = t ^ (1 << p);
+#pragma unroll
+      for (int r = 0; r < R; ++r) vcfma(acc[r], a, ai, lx[(r << 8) + tp]);
+    } else if (p < TB) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int e = r * 256 + t;
-        const int bit = (e >> p) & 1;
-        vcfma(acc[r], (RT)arp, (RT)(bit ? aip : -aip), lx[e ^ (1 << p)]);
+        const int bit = (r >> (p - 8)) & 1;
+        vcfma(acc[r], (RT)arp, (RT)(bit ? aip : -aip), lx[((r ^ (1 << (p - 8))) << 8) + t]);
       }
     } else {
       const int bit = (int)((tile >> (p - TB)) & 1ull);
-      const V* src = io.x + ((tile ^ (1ull << (p - TB))) << TB);
+      const auto rs = buf_rsrc(io.x + ((tile ^ (1ull << (p - TB))) << TB), TILE * VB);
       const RT a = (RT)(arp * s), ai = (RT)((bit ? aip : -aip) * s);
       V xv[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) xv[r] = src[r * 256 + t];
+      for (int r = 0; r < R; ++r) xv[r] = buf_load_so<V>(rs, t * VB, (r << 8) * VB);
 #pragma unroll
       for (int r = 0; r < R; ++r) vcfma(acc[r], a, ai, xv[r]);
     }
@@ -318,29 +327,49 @@ __global__ __launch_bounds__(256) void k_chan_full_tiled(ChanFullModel M, ApplyI
     const int hi = P.pi < P.pj ? P.pj : P.pi;
     if (hi < TB) {
       const int m = (1 << P.pi) | (1 << P.pj);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int src = (r * 256 + t) ^ m;
+      const int tl = t ^ (m & 255);              // thread part of the source index
+      const int mh = m >> 8;
+      if (hi < 8) {                              // one coefficient per thread
         double cr, ci;
-        pair_coef(P, (src >> P.pi) & 1, (src >> P.pj) & 1, cr, ci);
-        vcfma(acc[r], (RT)cr, (RT)ci, lx[src]);
+        pair_coef(P, (tl >> P.pi) & 1, (tl >> P.pj) & 1, cr, ci);
+#pragma unroll
+        for (int r = 0; r < R; ++r) vcfma(acc[r], (RT)cr, (RT)ci, lx[(r << 8) + tl]);
+      } else if (lo >= 8) {                      // one coefficient per r
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int rs = r ^ mh;
+          double cr, ci;
+          pair_coef(P, (rs >> (P.pi - 8)) & 1, (rs >> (P.pj - 8)) & 1, cr, ci);
+          if (cr != 0.0 || ci != 0.0) vcfma(acc[r], (RT)cr, (RT)ci, lx[(rs << 8) + tl]);
+        }
+      } else {                                   // low site in t, high site in r
+        const int bt = (tl >> lo) & 1;
+        double c0r, c0i, c1r, c1i;
+        pair_coef(P, P.pi == lo ? bt : 0, P.pj == lo ? bt : 0, c0r, c0i);
+        pair_coef(P, P.pi == lo ? bt : 1, P.pj == lo ? bt : 1, c1r, c1i);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int rs = r ^ mh;
+          const int bh = (rs >> (hi - 8)) & 1;
+          vcfma(acc[r], (RT)(bh ? c1r : c0r), (RT)(bh ? c1i : c0i), lx[(rs << 8) + tl]);
+        }
       }
     } else if (lo >= TB) {
       const uint64_t st = tile ^ ((1ull << (P.pi - TB)) | (1ull << (P.pj - TB)));
       double cr, ci;
       pair_coef(P, (int)((st >> (P.pi - TB)) & 1ull), (int)((st >> (P.pj - TB)) & 1ull), cr, ci);
       if (cr == 0.0 && ci == 0.0) continue;
-      const V* src = io.x + (st << TB);
+      const auto rs = buf_rsrc(io.x + (st << TB), TILE * VB);
       const RT a = (RT)(cr * s), ai = (RT)(ci * s);
       V xv[R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) xv[r] = src[r * 256 + t];
+      for (int r = 0; r < R; ++r) xv[r] = buf_load_so<V>(rs, t * VB, (r << 8) * VB);
 #pragma unroll
       for (int r = 0; r < R; ++r) vcfma(acc[r], a, ai, xv[r]);
     } else {
       const uint64_t st = tile ^ (1ull << (hi - TB));
       const int bh = (int)((st >> (hi - TB)) & 1ull);      // source bit of the high site
-      const V* src = io.x + (st << TB);
+      const auto rs = buf_rsrc(io.x + (st << TB), TILE * VB);
       // coefficients for source bit 0 and 1 of the low site
       double c0r, c0i, c1r, c1i;
       pair_coef(P, P.pi == lo ? 0 : bh, P.pj == lo ? 0 : bh, c0r, c0i);
@@ -348,16 +377,30 @@ __global__ __launch_bounds__(256) void k_chan_full_tiled(ChanFullModel M, ApplyI
       const bool use0 = c0r != 0.0 || c0i != 0.0, use1 = c1r != 0.0 || c1i != 0.0;
       if (!use0 && !use1) continue;
       V xv[R];
+      if (lo < 8) {                              // source bit of the low site per thread
+        const int bl = ((t >> lo) & 1) ^ 1;
+        const bool use = bl ? use1 : use0;
+        const RT a = (RT)((bl ? c1r : c0r) * s), ai = (RT)((bl ? c1i : c0i) * s);
+        const uint32_t vo = (uint32_t)(t ^ (1 << lo)) * VB;
+        if (use) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int e = r * 256 + t;
-        const int bl = ((e >> lo) & 1) ^ 1;                 // source bit of the low site
-        xv[r] = (bl ? use1 : use0) ? src[e ^ (1 << lo)] : vzero(acc[r]);
-      }
+          for (int r = 0; r < R; ++r) xv[r] = buf_load_so<V>(rs, vo, (r << 8) * VB);
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int bl = (((r * 256 + t) >> lo) & 1) ^ 1;
-        vcfma(acc[r], (RT)((bl ? c1r : c0r) * s), (RT)((bl ? c1i : c0i) * s), xv[r]);
+          for (int r = 0; r < R; ++r) vcfma(acc[r], a, ai, xv[r]);
+        }
+      } else {                                   // per r
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int bl = ((r >> (lo - 8)) & 1) ^ 1;
+          xv[r] = (bl ? use1 : use0)
+                      ? buf_load_so<V>(rs, t * VB, ((r ^ (1 << (lo - 8))) << 8) * VB)
+                      : vzero(acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int bl = ((r >> (lo - 8)) & 1) ^ 1;
+          vcfma(acc[r], (RT)((bl ? c1r : c0r) * s), (RT)((bl ? c1i : c0i) * s), xv[r]);
+        }
       }
     }
   }

@@ -106,6 +106,26 @@ template <> __device__ __forceinline__ c64 buf_load<c64>(__amdgpu_buffer_rsrc_t 
   return __builtin_bit_cast(c64, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
+// the same with a wave-uniform byte offset (soffset) added to the lane's
+template <class V> __device__ __forceinline__ V buf_load_so(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                            uint32_t soff);
+template <> __device__ __forceinline__ float buf_load_so<float>(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                                uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+template <> __device__ __forceinline__ double buf_load_so<double>(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                                  uint32_t soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+template <> __device__ __forceinline__ c32 buf_load_so<c32>(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                            uint32_t soff) {
+  return __builtin_bit_cast(c32, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+template <> __device__ __forceinline__ c64 buf_load_so<c64>(__amdgpu_buffer_rsrc_t r, uint32_t off,
+                                                            uint32_t soff) {
+  return __builtin_bit_cast(c64, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
