@@ -1,5 +1,6 @@
-// Generic channel H·v: any basis given as sorted codes, and the full basis
-// when pair channels (J exchange, A pair creation) are present.
+// Channel H·v with pair channels (J exchange, A pair creation): any basis
+// given as sorted codes (k_chan_apply, a gather per channel and row), and the
+// full basis (k_chan_full_tiled, LDS tiles; k_chan_apply below the tile width).
 //
 // Channel semantics of tomisnot/energy-level src/core/manybody/channels.py:91-132,
 // in gather form for the row of code c (source s = c ^ mask):
