@@ -174,6 +174,46 @@ def test_full_basis_pair_channels_ground_state(pkg, dev):
     assert abs(out['E0'] - E0) <= 1e-10 * abs(E0)
 
 
+def test_full_basis_pair_channels_kpm(pkg, dev):
+    """KPM moments (the probe-dot reductions of the tiled channel kernel) on
+    the full basis with J and A against the oracle's estimator with the
+    same Rademacher probes."""
+    n = 13
+    rng = np.random.default_rng(13)
+    terms = ['I1', 'I2', 'I3']
+    pairs = _pairs(n, rng, cplx=False)
+    Om, delta = rng.uniform(0.5, 1.5, n), rng.uniform(-2, 2, n)
+    P = _plan(pkg, dev, n, Om, delta, pairs, terms)
+    H = orc.build_H_sparse(n, Om, delta, pairs, terms)
+    emin, emax, c, r = orc.kpm_bounds(H, margin=0.01)
+    mref, _, _ = orc.kpm_moments(H, 30, 2, seed=5, c=c, r=r)
+    mu, _, _ = pkg.kpm_moments_hip(plan=P, num=30, nvec=2, seed=5, c=c, r=r)
+    assert np.max(np.abs(mu - mref)) < 1e-10
+    mu32, _, _ = pkg.kpm_moments_hip(plan=P, num=30, nvec=2, seed=5, c=c, r=r, dtype='float32')
+    assert np.max(np.abs(mu32 - mref)) < 2e-4
+
+
+@pytest.mark.parametrize('dtype,tol', [('complex128', 1e-10), ('complex64', 2e-5)])
+def test_full_basis_pair_channels_segments(pkg, dev, dtype, tol):
+    """Piecewise-constant evolution under Ω, Δ, V, J and A on the full basis
+    (the Chebyshev recurrence through the tiled channel kernel) against the
+    oracle's sparse exponential steps."""
+    n = 13
+    rng = np.random.default_rng(31)
+    terms = ['I1', 'I2', 'I3']
+    pairs = _pairs(n, rng, cplx=False)
+    calib = {'atoms': [{}] * n, 'pairs': pairs, 'terms': terms}
+    segs = [dict(t0=t0, Om=rng.uniform(0.5, 3.0, n), phase=rng.uniform(0, 2 * np.pi, n),
+                 delta=rng.uniform(-2, 2, n)) for t0 in (0.0, 30.0)]
+    t = np.array([0.0, 20.0, 50.0])
+    out = pkg.propagate_segments_hip(calib, segs, t, terms=terms, dtype=dtype, device=dev,
+                                     return_states=True)
+    psi0 = np.zeros(1 << n, complex)
+    psi0[0] = 1.0
+    ref = orc.propagate_segments(n, pairs, terms, segs, psi0, t, method='expm_multiply')
+    assert np.max(np.abs(out['states'] - ref)) < tol
+
+
 def test_kagome_n24_sector_is_the_restricted_full_hv(pkg, dev):
     """Reference kagome N=24 XY calib (J channels conserve the excitation
     number): the full-basis H·v (D = 2^24) of a vector supported on the
