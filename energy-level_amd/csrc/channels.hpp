@@ -141,6 +141,17 @@ __global__ __launch_bounds__(256) void k_chan_apply(ChanModel M, ApplyIO<V> io, 
     const uint64_t c = chan_code(M, k);
     const V xe = vscale(io.x[k], sr);
     V acc = vscale(xe, (RT)chan_diag(M, vs, c));
+    // number sector with split rank tables: the row's own two terms, reused
+    // by the pairs whose bits lie on one side of the split
+    const bool nsplit = M.mode == RANK_NUMBER && M.rsplit != nullptr;
+    const uint64_t lmask = (1ull << M.rsL) - 1ull;
+    int64_t rlo = 0, rhi = 0;
+    int pcl = 0;
+    if (nsplit) {
+      pcl = __popcll(c & lmask);
+      rlo = M.rsplit[c & lmask];
+      rhi = M.rsplit[(1ll << M.rsL) + (int64_t)(c >> M.rsL) * (M.rsL + 1) + pcl];
+    }
     for (int p0 = 0; p0 < M.n; p0 += CHAN_PB) {    // Ω flips, batched like the pairs
       int64_t r[CHAN_PB];
 #pragma unroll
@@ -180,7 +191,18 @@ __global__ __launch_bounds__(256) void k_chan_apply(ChanModel M, ApplyIO<V> io, 
           are[u] = P.ar;
           aim[u] = bi ? -P.ai : P.ai;
         }
-        if (are[u] != 0.0 || aim[u] != 0.0) r[u] = chan_rank(M, src);
+        if (are[u] == 0.0 && aim[u] == 0.0) continue;
+        if (nsplit) {
+          // equal source bits change the popcount by 2: outside the sector
+          if (bi == bj) continue;
+          const uint64_t msk = (1ull << P.pi) | (1ull << P.pj);
+          const int64_t hb = (1ll << M.rsL) + (int64_t)(src >> M.rsL) * (M.rsL + 1);
+          if (!(msk & ~lmask)) r[u] = M.rsplit[src & lmask] + rhi;      // both bits low
+          else if (!(msk & lmask)) r[u] = rlo + M.rsplit[hb + pcl];     // both bits high
+          else r[u] = M.rsplit[src & lmask] + M.rsplit[hb + __popcll(src & lmask)];
+        } else {
+          r[u] = chan_rank(M, src);
+        }
       }
       V xv[CHAN_PB];
 #pragma unroll

@@ -67,12 +67,14 @@ def test_real_exchange_model_real_vectors(pkg, dev, n):
         assert np.max(np.abs(y - ref)) <= tol * np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize('n,k', [(10, 4), (12, 6), (9, 0), (9, 9)])
-def test_number_sector_closed_form_ranks(pkg, dev, n, k):
+@pytest.mark.parametrize('n,k,A', [(10, 4, False), (12, 6, False), (9, 0, False), (9, 9, False),
+                                   (11, 5, True)])
+def test_number_sector_closed_form_ranks(pkg, dev, n, k, A):
+    """With A pairs the pair-creation targets leave the sector and drop out."""
     import torch
     rng = np.random.default_rng(100 + n + k)
-    terms = ['I1', 'I2']
-    pairs = _pairs(n, rng, A=False)
+    terms = ['I1', 'I2', 'I3'] if A else ['I1', 'I2']
+    pairs = _pairs(n, rng, A=A)
     codes = pkg.number_sector_basis(n, k)
     delta = rng.uniform(-2, 2, n)
     P = _plan(pkg, dev, n, np.zeros(n), delta, pairs, terms, codes=codes)
