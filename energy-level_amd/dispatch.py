@@ -118,14 +118,26 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
                 raise ValueError('hip-batch takes the full basis')
             if return_state:
                 raise ValueError('the batched ground-state backend does not return states')
-            from .batch import batched_ground_state_hip
+            from .batch import BMAXN, batched_ground_state_hip
             from .model import from_calib
             model = from_calib(calib, system.get('Om'), system.get('delta'), gamma,
                                system.get('terms'))
-            rb = batched_ground_state_hip([model], dtype=dtype if dtype in ('float32', 'float64')
-                                          else 'float64', seed=seed, device=device)
-            result = dict(base, E0=float(rb['E0'][0]), gap=float(rb['gap'][0]),
-                          n_atom=rb['n_atom'][0])
+            if model.has_pair_flips or not model.is_real or model.n > BMAXN:
+                # the batch kernels take real Ω/Δ/V instances of up to BMAXN
+                # sites; the reference's batch takes any dense H, so one system
+                # of another kind is one device Lanczos solve (same keys)
+                from .lanczos import hip_lanczos_ground_state
+                rl = hip_lanczos_ground_state(
+                    calib, system.get('Om'), system.get('delta'), gamma, system.get('terms'),
+                    m=int(krylov_dim or system.get('krylov_dim', 48)), seed=seed,
+                    device=device, memory_budget_mb=memory_budget_mb)
+                result = dict(base, E0=float(rl['E0']), gap=float(rl['gap']),
+                              n_atom=np.asarray(rl['n_atom']))
+            else:
+                rb = batched_ground_state_hip([model], dtype=dtype if dtype in ('float32', 'float64')
+                                              else 'float64', seed=seed, device=device)
+                result = dict(base, E0=float(rb['E0'][0]), gap=float(rb['gap'][0]),
+                              n_atom=rb['n_atom'][0])
     else:
         from .kpm import kpm_moments_hip, spectral_bounds, thermal_from_moments, dos_from_moments
         from .model import from_calib

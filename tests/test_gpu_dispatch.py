@@ -57,6 +57,27 @@ def test_gs_hip_batch_single_system(pkg, dev):
     assert np.allclose(out['n_atom'], orc.site_occupations(n, ref['vec']), atol=1e-6)
 
 
+def test_gs_hip_batch_exchange_and_complex_drive(pkg, dev):
+    """A system the batch kernels do not take (J and A pairs, a complex
+    drive) still answers hip-batch, as the reference's dense batch does."""
+    n = 9
+    rng = np.random.default_rng(9)
+    terms = ['I1', 'I2', 'I3']
+    pairs = [{'i': i, 'j': j, 'V': 2.0 / (j - i) ** 6, 'J': 0.7 / (j - i) ** 3,
+              'A': 0.2 / (j - i) ** 3} for i in range(n) for j in range(i + 1, n)]
+    calib = {'atoms': [{}] * n, 'pairs': pairs, 'terms': terms}
+    Om = rng.uniform(0.5, 1.5, n) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    delta = rng.uniform(-1, 1, n)
+    sysd = dict(calib=calib, Om=Om, delta=delta, terms=terms, device=dev)
+    out = pkg.solve_equilibrium_hip(sysd, 'gs', backend='hip-batch')
+    H = orc.build_H_sparse(n, Om, delta, pairs, terms)
+    ref = orc.ground_state_dense(H)
+    assert out['backend'] == 'hip-batch' and out['n_sites'] == n
+    assert abs(out['E0'] - ref['E0']) < 1e-9 * abs(ref['E0'])
+    assert abs(out['gap'] - ref['gap']) < 1e-6
+    assert np.allclose(out['n_atom'], orc.site_occupations(n, ref['vec']), atol=1e-6)
+
+
 @pytest.mark.parametrize('want', ['thermal', 'dos'])
 def test_hip_kpm_thermal_and_dos_vs_oracle(pkg, dev, want):
     n = 8
