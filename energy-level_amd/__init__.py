@@ -8,7 +8,7 @@ points (tomisnot/energy-level src/core/manybody/) on top of that ABI.
 from . import _native
 from .model import RydModel, from_calib, active_terms, chain_positions, square_positions, \
     from_positions, c6_coupling
-from .plan import Plan, cast
+from .plan import Plan, CsrPlan, cast
 from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, rayleigh_residual, \
     site_occupations, hamiltonian_linear_operator_hip
 from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, sector_codes, \
@@ -24,10 +24,10 @@ from .observables import normalize_pauli_strings, PauliPlanHip, measure_pauli_st
     corr_matrix_hip, site_correlation_sums, loop_observables, link_loop_observables
 from .manybody import evolve_manybody_hip, state_vector_np, product_label
 from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_moments, \
-    dos_from_moments, probe_bits
+    dos_from_moments, probe_bits, kpm_moments_gpu_hip
 
 __all__ = ['hamiltonian_linear_operator_hip', 'loop_observables', 'link_loop_observables', 'RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_positions',
-           'from_positions', 'c6_coupling', 'Plan', 'cast', 'hip_lanczos_ground_state',
+           'from_positions', 'c6_coupling', 'Plan', 'CsrPlan', 'cast', 'kpm_moments_gpu_hip', 'hip_lanczos_ground_state',
            'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
            'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',
            'dos_from_moments', 'probe_bits', 'fibonacci', 'chain_blockade_dim',

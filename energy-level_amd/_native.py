@@ -15,8 +15,8 @@ LIB_PATH = os.path.join(_HERE, 'librydhip.so')
 
 RYD_OK, RYD_ERR_INVALID, RYD_ERR_MEMORY, RYD_ERR_DEVICE, RYD_ERR_UNSUPPORTED = range(5)
 RYD_F32, RYD_F64, RYD_C64, RYD_C128 = range(4)
-ABI_VERSION = 4
-RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH = range(4)
+ABI_VERSION = 5
+RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH, RYD_BASIS_CSR = range(5)
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -43,6 +43,8 @@ SIGNATURES = {
     'ryd_abi_version': (ctypes.c_int, []),
     'ryd_last_error': (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
     'ryd_plan_create': (ctypes.c_int, [ctypes.POINTER(Model), ctypes.POINTER(_vp)]),
+    'ryd_plan_create_csr': (ctypes.c_int, [ctypes.c_int64, _vp, _vp, _vp, _vp,
+                                           ctypes.POINTER(_vp)]),
     'ryd_plan_destroy': (ctypes.c_int, [_vp]),
     'ryd_plan_dim': (_i64, [_vp]),
     'ryd_plan_rank_mode': (ctypes.c_int, [_vp]),

@@ -449,6 +449,47 @@ __global__ __launch_bounds__(256) void k_chan_full_tiled(ChanFullModel M, ApplyI
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
 
+// H·v with a caller's sparse matrix in CSR form (rows and columns in the
+// order of the vectors): one row per thread, grid-stride, the same fused
+// scalars and FP64 reductions as the other apply kernels.
+struct CsrModel {
+  int64_t dim;
+  const int64_t* ptr;          // [dim + 1]
+  const int32_t* col;          // [nnz]
+  const double* vre;           // [nnz]
+  const double* vim;           // [nnz], null for a real matrix
+};
+
+template <class V>
+__global__ __launch_bounds__(256) void k_csr_apply(CsrModel M, ApplyIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  if (io.stop && load_coherent(io.stop)) return;
+  double s, ch, cx, cz;
+  apply_scalars(io, s, ch, cx, cz);
+  const RT sr = (RT)s;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < M.dim; k += stride) {
+    const V xe = vscale(io.x[k], sr);
+    V acc = vzero(xe);
+    const int64_t a = M.ptr[k], b = M.ptr[k + 1];
+    for (int64_t q = a; q < b; ++q)
+      vcfma(acc, (RT)(M.vre[q] * s), (RT)(M.vim ? M.vim[q] * s : 0.0), io.x[M.col[q]]);
+    V v = vscale(acc, chr);
+    vfma(v, cxr, xe);
+    if (io.z) vfma(v, czr, io.z[k]);
+    io.y[k] = v;
+    if (io.reduce) {
+      s0 += rdot(xe, v);
+      s1 += nrm2(v);
+      s2 += nrm2(xe);
+      if (io.probe) s3 += zdot(((io.probe[k >> 5] >> (k & 31)) & 1u) ? 1.0 : -1.0, v);
+    }
+  }
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
 // site occupations over a code basis: partials[block][p] = Σ |x_k|^2 b_p(code_k),
 // partials[block][MAXN_OCC] = Σ |x_k|^2 (deterministic: fixed chunk per block)
 template <class V>

@@ -76,6 +76,10 @@ struct ryd_plan {
   int rank_mode = 0;                 // RankMode
   int popcount = 0;                  // number sector
   int64_t* d_codes = nullptr;        // [dim] ascending
+  int64_t* d_csr_ptr = nullptr;      // RYD_BASIS_CSR: [dim + 1] row pointers
+  int32_t* d_csr_col = nullptr;      // [nnz] column indices
+  double* d_csr_re = nullptr;        // [nnz] values (real part)
+  double* d_csr_im = nullptr;        // [nnz] imaginary part, null for a real matrix
   int64_t* d_binom = nullptr;        // [64][BINOM_LD]
   int rs_L = 0;                      // number-sector split rank tables (RANK_NUMBER, n <= RS_MAXN)
   int64_t* d_rsplit = nullptr;       // [2^rs_L] + [2^(n-rs_L)][rs_L+1]
@@ -363,6 +367,17 @@ unsigned fib_top_grid(const ryd_plan* P, int run) {
 
 template <class V>
 int launch_apply(const ryd_plan* P, int dtype, ApplyIO<V> io, Reduce rd, hipStream_t st) {
+  if (P->basis == RYD_BASIS_CSR) {
+    io.nfar = 0;
+    io.partial = 0;
+    CsrModel M{P->dim, P->d_csr_ptr, P->d_csr_col, P->d_csr_re, P->d_csr_im};
+    const unsigned grid = chan_grid(P);
+    k_csr_apply<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
+    LAUNCH_CHECK();
+    if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   if (graph_kernel(P)) {
     io.nfar = 0;
     const bool split = P->gr_K > 0;
@@ -1254,8 +1269,60 @@ int ryd_plan_create(const ryd_model* m, ryd_plan** out) {
   return RYD_OK;
 }
 
+int ryd_plan_create_csr(int64_t dim, const int64_t* indptr, const int64_t* indices,
+                        const double* val_re, const double* val_im, ryd_plan** out) {
+  if (!out || !indptr) return fail(RYD_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (dim < 1 || dim > (int64_t)INT32_MAX) return fail(RYD_ERR_INVALID, "csr: dim must be 1..2^31-1");
+  if (indptr[0] != 0) return fail(RYD_ERR_INVALID, "csr: indptr[0] must be 0");
+  for (int64_t k = 0; k < dim; ++k)
+    if (indptr[k + 1] < indptr[k]) return fail(RYD_ERR_INVALID, "csr: indptr must not decrease");
+  const int64_t nnz = indptr[dim];
+  if (nnz > 0 && (!indices || !val_re)) return fail(RYD_ERR_INVALID, "null argument");
+  std::vector<int32_t> col((size_t)(nnz > 0 ? nnz : 1), 0);
+  bool cplx = false;
+  for (int64_t q = 0; q < nnz; ++q) {
+    if (indices[q] < 0 || indices[q] >= dim)
+      return fail(RYD_ERR_INVALID, "csr: column index out of range");
+    if (!std::isfinite(val_re[q]) || (val_im && !std::isfinite(val_im[q])))
+      return fail(RYD_ERR_INVALID, "csr: non-finite value");
+    col[(size_t)q] = (int32_t)indices[q];
+    if (val_im && val_im[q] != 0.0) cplx = true;
+  }
+  ryd_plan* P = new ryd_plan();
+  P->n = 0;
+  P->basis = RYD_BASIS_CSR;
+  P->dim = dim;
+  P->cplx = cplx;
+  P->generic = true;
+  (void)hipGetDevice(&P->device);
+  const size_t nb = (size_t)(nnz > 0 ? nnz : 1);
+  hipError_t e = hipMalloc(&P->d_csr_ptr, sizeof(int64_t) * (size_t)(dim + 1));
+  if (e == hipSuccess) e = hipMemcpy(P->d_csr_ptr, indptr, sizeof(int64_t) * (size_t)(dim + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&P->d_csr_col, sizeof(int32_t) * nb);
+  if (e == hipSuccess) e = hipMemcpy(P->d_csr_col, col.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&P->d_csr_re, sizeof(double) * nb);
+  if (e == hipSuccess && nnz > 0)
+    e = hipMemcpy(P->d_csr_re, val_re, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess && cplx) e = hipMalloc(&P->d_csr_im, sizeof(double) * nb);
+  if (e == hipSuccess && cplx)
+    e = hipMemcpy(P->d_csr_im, val_im, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    ryd_plan_destroy(P);
+    return fail(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
+                std::string("csr upload: ") + hipGetErrorString(e));
+  }
+  P->ws_bytes = WS_HEADER + (size_t)CHAN_MAX_BLOCKS * NRED * sizeof(double);
+  *out = P;
+  return RYD_OK;
+}
+
 int ryd_plan_destroy(ryd_plan* P) {
   if (!P) return RYD_OK;
+  if (P->d_csr_ptr) (void)hipFree(P->d_csr_ptr);
+  if (P->d_csr_col) (void)hipFree(P->d_csr_col);
+  if (P->d_csr_re) (void)hipFree(P->d_csr_re);
+  if (P->d_csr_im) (void)hipFree(P->d_csr_im);
   if (P->d_vb) (void)hipFree(P->d_vb);
   if (P->d_lo_code) (void)hipFree(P->d_lo_code);
   if (P->d_lo_diag) (void)hipFree(P->d_lo_diag);
@@ -1493,6 +1560,8 @@ int do_occ(const ryd_plan* P, const void* x, double* out, void* ws, hipStream_t 
 
 extern "C" int ryd_site_occupation(const ryd_plan* P, int dtype, const void* x, double* out,
                                    void* ws, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !x || !out || !ws) return fail(RYD_ERR_INVALID, "null argument to ryd_site_occupation");
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (dtype) {
@@ -1605,6 +1674,8 @@ int do_corr(const ryd_plan* P, const void* x, double* out, void* scratch, size_t
 extern "C" int ryd_pauli_expect(const ryd_plan* P, int dtype, const void* x, int count,
                                 const uint64_t* flip, const uint64_t* sign, const int32_t* ny,
                                 double* out, void* ws, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !x || !out || !ws || count < 0 || (count > 0 && (!flip || !sign || !ny)))
     return fail(RYD_ERR_INVALID, "bad argument to ryd_pauli_expect");
   for (int q = 0; q < count; ++q)
@@ -1623,6 +1694,8 @@ extern "C" int ryd_pauli_expect(const ryd_plan* P, int dtype, const void* x, int
 
 extern "C" int ryd_site_correlations(const ryd_plan* P, int dtype, const void* x, double* out,
                                      void* scratch, size_t scratch_bytes, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !x || !out || !scratch) return fail(RYD_ERR_INVALID, "bad argument to ryd_site_correlations");
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (dtype) {
@@ -1636,6 +1709,8 @@ extern "C" int ryd_site_correlations(const ryd_plan* P, int dtype, const void* x
 
 extern "C" int ryd_sector_codes(const ryd_plan* P, const int64_t* ranks, int64_t first,
                                 int64_t count, int64_t* codes, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !codes || count < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_sector_codes");
   if (P->basis == RYD_BASIS_FULL)
     return fail(RYD_ERR_UNSUPPORTED, "codes and ranks are defined for restricted bases");
@@ -1651,6 +1726,8 @@ extern "C" int ryd_sector_codes(const ryd_plan* P, const int64_t* ranks, int64_t
 
 extern "C" int ryd_sector_rank(const ryd_plan* P, const int64_t* codes, int64_t count,
                                int64_t* ranks, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !codes || !ranks || count < 0) return fail(RYD_ERR_INVALID, "bad argument to ryd_sector_rank");
   if (P->basis == RYD_BASIS_FULL)
     return fail(RYD_ERR_UNSUPPORTED, "codes and ranks are defined for restricted bases");
@@ -1892,6 +1969,8 @@ extern "C" int ryd_evolve_forward(const ryd_plan* P, int dtype, void* psi, int64
                                   const double* om, const double* dadd, const double* c,
                                   const double* r, int K, const double* coef, void* work,
                                   void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !psi || !om || !dadd || !c || !r || !coef || !work || nslices < 0)
     return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_forward");
   for (int64_t k = 0; k < nslices; ++k)
@@ -1918,6 +1997,8 @@ extern "C" int ryd_evolve_forward(const ryd_plan* P, int dtype, void* psi, int64
 
 extern "C" int ryd_evolve_taylor(const ryd_plan* P, int dtype, void* psi, double tau, int nsub,
                                  int m, const double* gamma, void* work, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !psi || !gamma || !work) return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_taylor");
   if (dtype != RYD_C64 && dtype != RYD_C128)
     return fail(RYD_ERR_INVALID, "non-Hermitian evolution needs a complex dtype");
@@ -1933,6 +2014,8 @@ extern "C" int ryd_evolve_taylor(const ryd_plan* P, int dtype, void* psi, double
 
 extern "C" int ryd_loss_axpy(const ryd_plan* P, int dtype, const void* x, void* y,
                              const double* gamma, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !x || !y || !gamma) return fail(RYD_ERR_INVALID, "null argument to ryd_loss_axpy");
   if (dtype != RYD_C64 && dtype != RYD_C128)
     return fail(RYD_ERR_INVALID, "the loss diagonal needs a complex dtype");
@@ -1956,6 +2039,8 @@ extern "C" int ryd_evolve_adjoint(const ryd_plan* P, int dtype, const void* psi,
                                   double om, double dadd, double c, double r, int K,
                                   const double* coef, void* tstore, void* work, double* sums,
                                   void* ws, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
   if (!P || !psi || !g || !coef || !tstore || !work || !sums || !ws)
     return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_adjoint");
   if (int rc = check_evo(P, dtype, K)) return rc;

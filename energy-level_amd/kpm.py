@@ -29,7 +29,8 @@ from .model import from_calib
 from .plan import Plan
 
 __all__ = ['kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'cheb_coeffs',
-           'thermal_from_moments', 'dos_from_moments', 'probe_bits', 'device_probes']
+           'thermal_from_moments', 'dos_from_moments', 'probe_bits', 'device_probes',
+           'kpm_moments_gpu_hip']
 
 
 def probe_bits(D, nvec, seed):
@@ -58,6 +59,57 @@ def spectral_bounds(plan, m=120, seed=0, tol=1e-8, margin=1e-3):
     c = 0.5 * (emin + emax)
     r = 0.5 * (emax - emin) * (1.0 + margin) + 1e-12
     return emin, emax, c, r
+
+
+def kpm_moments_gpu_hip(H, num=300, nvec=48, seed=0, device='cuda', chunk=32,
+                        memory_budget_mb=None, dtype='float64'):
+    """kpm_moments_gpu (tomisnot/energy-level src/core/manybody/kpm.py:26-75)
+    with its signature: H is an assembled Hermitian matrix (scipy sparse or
+    dense), copied once into a device CSR plan; → (mu[num+1], c, r).  The
+    probes are that function's stream: blocks of min(chunk, nvec) columns of
+    torch.randint(0, 2, (D, b)) from a device generator seeded with ``seed``
+    (:60-63), 1 → +1 and 0 → −1; c and r use its 1e-3 bound margin (:82-97),
+    here from the device Lanczos."""
+    import torch
+    from .plan import CsrPlan
+    nat.require_gpu()
+    if str(device).startswith('cpu'):
+        raise RuntimeError('HIP device unavailable (device=%s)' % device)
+    if dtype not in ('float32', 'float64'):
+        raise ValueError('dtype must be float32/float64')
+    if int(num) < 1 or int(nvec) < 1:
+        raise ValueError('num and nvec must be positive integers')
+    P = CsrPlan(H, device=device)
+    try:
+        D, nvec = P.dim, int(nvec)
+        if memory_budget_mb is not None:
+            item = 8 if dtype == 'float32' else 16
+            need = P.nnz * 20 + (D + 1) * 8 + 3 * D * item
+            if need > float(memory_budget_mb) * 2 ** 20:
+                raise MemoryError('KPM operator and vectors need %.1f MB > budget %.1f MB'
+                                  % (need / 2 ** 20, float(memory_budget_mb)))
+        gen = torch.Generator(device=P.device)
+        gen.manual_seed(int(seed))
+        chunk = max(1, min(int(chunk), nvec))
+        words = (D + 31) // 32
+        weight = torch.pow(2, torch.arange(32, device=P.device, dtype=torch.int64))
+        probes, done = [], 0
+        while done < nvec:
+            b = min(chunk, nvec - done)
+            z = torch.randint(0, 2, (D, b), device=P.device, generator=gen, dtype=torch.int8)
+            for j in range(b):
+                bits = torch.zeros(words * 32, dtype=torch.int64, device=P.device)
+                bits[:D] = z[:, j]
+                w = (bits.view(words, 32) * weight).sum(dim=1)
+                w = torch.where(w >= 2 ** 31, w - 2 ** 32, w)      # same bits as uint32
+                probes.append(w.to(torch.int32).contiguous())
+            done += b
+            del z
+        _, _, c, r = spectral_bounds(P, margin=1e-3)
+        return kpm_moments_hip(plan=P, num=num, nvec=nvec, seed=seed, dtype=dtype, c=c, r=r,
+                               probes=probes)
+    finally:
+        P.close()
 
 
 def device_probes(plan, nvec, seed):

@@ -487,7 +487,8 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
         raise ValueError('dtype must be float32/float64/complex64/complex128')
     if int(m) < 2 or int(max_restarts) < 0:
         raise ValueError('m must be >= 2 and max_restarts >= 0')
-    model = from_calib(calib, Om, delta, gamma, terms)
+    # a given plan (a model plan or an assembled-matrix CsrPlan) carries H
+    model = from_calib(calib, Om, delta, gamma, terms) if plan is None else None
     order = None
     if plan is not None:
         P = plan
@@ -543,7 +544,8 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     if not converged:
         out['warning'] = 'lowest eigenpairs did not all reach the residual tolerance; gap not reported'
     x0 = vecs[0]
-    out['n_atom'] = site_occupations(P, x0)
+    if getattr(P, 'basis', None) != 'csr':       # a matrix plan has no sites
+        out['n_atom'] = site_occupations(P, x0)
     if return_state:
         x = x0 / torch.linalg.vector_norm(x0)
         v = x.cpu().numpy().astype(np.complex128)

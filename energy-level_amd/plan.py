@@ -161,6 +161,51 @@ class Plan:
         return sums
 
 
+class CsrPlan(Plan):
+    """``ryd_plan`` over a caller's sparse Hermitian matrix H (scipy sparse or
+    dense; rows and columns in the order of the vectors), copied once to the
+    device.  Serves hv, apply, the Lanczos and Chebyshev drivers; entry points
+    that need site codes refuse it."""
+
+    def __init__(self, H, device='cuda'):
+        import scipy.sparse as sp
+        import torch
+        nat.require_gpu()
+        L = nat.lib()
+        Hc = sp.csr_matrix(H, copy=True)
+        if Hc.ndim != 2 or Hc.shape[0] != Hc.shape[1] or Hc.shape[0] < 1:
+            raise ValueError('H must be a non-empty square matrix')
+        Hc.sum_duplicates()
+        Hc.sort_indices()
+        self.model = None
+        self.n = 0
+        self.basis = 'csr'
+        dv = torch.device(device)
+        if dv.type != 'cuda':
+            raise RuntimeError('HIP device unavailable (device=%s)' % device)
+        self.device = torch.device('cuda', torch.cuda.current_device() if dv.index is None
+                                   else dv.index)
+        D = int(Hc.shape[0])
+        indptr = np.ascontiguousarray(Hc.indptr, np.int64)
+        idx = np.ascontiguousarray(Hc.indices, np.int64)
+        data = np.asarray(Hc.data)
+        re = np.ascontiguousarray(data.real, np.float64)
+        im = (np.ascontiguousarray(data.imag, np.float64)
+              if np.iscomplexobj(data) and np.any(data.imag != 0) else None)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            nat.check(L.ryd_plan_create_csr(D, indptr.ctypes.data, idx.ctypes.data,
+                                            re.ctypes.data,
+                                            None if im is None else im.ctypes.data,
+                                            ctypes.byref(handle)))
+        self._h = handle
+        self._keep = []
+        self.dim = D
+        self.nnz = int(Hc.nnz)
+        self.is_real = im is None
+        self._ws = None
+
+
 def cast(x, dtype):
     """Precision cast of a state vector through the library (F32<->F64, C64<->C128)."""
     import torch

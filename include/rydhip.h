@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 4
+#define RYD_ABI_VERSION 5
 
 enum ryd_status {
   RYD_OK = 0,
@@ -47,10 +47,11 @@ enum ryd_basis_kind {
   RYD_BASIS_CODES = 1,      /* explicit ascending int64 codes (number
                                sectors, any subset of codes)               */
   RYD_BASIS_CHAIN_FIB = 2,  /* chain nearest-neighbour blockade sector      */
-  RYD_BASIS_GRAPH = 3       /* independent sets of a blockade graph (edges
+  RYD_BASIS_GRAPH = 3,      /* independent sets of a blockade graph (edges
                                below), enumerated on the device in
                                ascending order as independent_states,
                                basis.py:24-47; n <= 60, no 2^20 cap        */
+  RYD_BASIS_CSR = 4         /* a caller's sparse matrix (ryd_plan_create_csr) */
 };
 
 /* Model description (host memory, copied at plan creation). */
@@ -95,6 +96,18 @@ int ryd_last_error(char* buf, size_t size);
  * :135-175 (ChannelKernel.__init__) and matrix_free.py:65-115
  * (_channel_arrays / _group_channels, uploaded at matrix_free.py:185-198). */
 int ryd_plan_create(const ryd_model* model, ryd_plan** out);
+
+/* A plan over a caller's sparse Hermitian matrix H (CSR, host arrays: row
+ * pointers indptr[dim+1] from 0, column indices[nnz] in [0, dim), values
+ * val_re/val_im[nnz], val_im NULL for a real H), copied to the device.
+ * Replaces the COO upload of kpm_moments_gpu (src/core/manybody/kpm.py:26-75,
+ * `torch.sparse_coo_tensor` at :40-47), which takes an assembled H: the plan
+ * serves ryd_hv, ryd_apply, ryd_lanczos, ryd_lanczos_stored and
+ * ryd_cheb_moments; the entry points that need site codes (occupations,
+ * Pauli strings, correlations, codes/ranks, evolution) refuse it.
+ * dim must be below 2^31. */
+int ryd_plan_create_csr(int64_t dim, const int64_t* indptr, const int64_t* indices,
+                        const double* val_re, const double* val_im, ryd_plan** out);
 int ryd_plan_destroy(ryd_plan* plan);
 
 /* Basis dimension of a plan (0 on a NULL plan). */

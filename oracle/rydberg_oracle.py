@@ -367,11 +367,12 @@ def rademacher_probes(D, nvec, seed):
         yield rng.choice(np.array([-1.0, 1.0]), size=int(D))
 
 
-def kpm_moments(H, num, nvec, seed=0, exact=False, c=None, r=None, margin=1e-3):
+def kpm_moments(H, num, nvec, seed=0, exact=False, c=None, r=None, margin=1e-3, probes=None):
     """μ_n = (1/D)Tr T_n(H̃) by Hutchinson Z2 probes (reference kpm.py:115-151).
 
     ``c``/``r`` may be given explicitly (the eigsh bounds are the one part
-    that is iteration-path dependent)."""
+    that is iteration-path dependent); ``probes`` (nvec arrays of ±1) replaces
+    the CPU probe stream, e.g. by the stream of the GPU path (kpm.py:60-63)."""
     D = H.shape[0]
     if c is None or r is None:
         _, _, c, r = kpm_bounds(H, margin)
@@ -390,7 +391,8 @@ def kpm_moments(H, num, nvec, seed=0, exact=False, c=None, r=None, margin=1e-3):
     Hc = H if sp.issparse(H) else sp.csr_matrix(H)
     Ht = (Hc - c * sp.eye(D, format='csr')) * (1.0 / r)
     acc = np.zeros(num + 1)
-    for z in rademacher_probes(D, nvec, seed):
+    for z in (rademacher_probes(D, nvec, seed) if probes is None else probes):
+        z = np.asarray(z, float)
         tp, tc = z.copy(), Ht @ z
         acc[0] += (tp @ z).real
         if num >= 1:

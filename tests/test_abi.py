@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(pkg):
 def test_abi_version_and_error_path(pkg):
     from energy_level_amd import _native as nat
     L = nat.lib()
-    assert L.ryd_abi_version() == 4
+    assert L.ryd_abi_version() == 5
     # invalid model is rejected before any device work
     handle = ctypes.c_void_p()
     m = nat.Model()
@@ -55,5 +55,15 @@ def test_argument_checks_before_device_work(pkg):
     assert L.ryd_evolve_taylor(None, nat.RYD_C128, None, 1.0, 1, 20, g, None, None) == nat.RYD_ERR_INVALID
     assert 'null argument' in nat.last_error()
     assert L.ryd_hv(None, nat.RYD_F64, None, None, None) == nat.RYD_ERR_INVALID
+    # ryd_plan_create_csr validates the host arrays before any device call
+    h = ctypes.c_void_p()
+    ptr = (ctypes.c_int64 * 3)(0, 1, 2)
+    col = (ctypes.c_int64 * 2)(0, 5)
+    val = (ctypes.c_double * 2)(1.0, 2.0)
+    assert L.ryd_plan_create_csr(2, ptr, col, val, None, ctypes.byref(h)) == nat.RYD_ERR_INVALID
+    assert 'column index out of range' in nat.last_error()
+    bad = (ctypes.c_int64 * 3)(0, 2, 1)
+    assert L.ryd_plan_create_csr(2, bad, col, val, None, ctypes.byref(h)) == nat.RYD_ERR_INVALID
+    assert L.ryd_plan_create_csr(0, ptr, col, val, None, ctypes.byref(h)) == nat.RYD_ERR_INVALID
     assert L.ryd_evolve_forward(None, nat.RYD_C64, None, 1, None, None, None, None, 4, None,
                                 None, None) == nat.RYD_ERR_INVALID

@@ -1,0 +1,79 @@
+"""A plan over an assembled sparse matrix (ryd_plan_create_csr): H·v against
+scipy, the kpm_moments_gpu mirror (kpm.py:26-75) against the oracle's
+recurrence on the same device-drawn probes, and the Lanczos ground state
+against eigsh."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import rydberg_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _hermitian(D, density, rng, cplx):
+    A = sp.random(D, D, density=density, random_state=rng, format='csr')
+    if cplx:
+        B = sp.random(D, D, density=density, random_state=rng, format='csr')
+        A = A + 1j * B
+    return ((A + A.conj().T) * 0.5 + sp.diags(rng.normal(size=D))).tocsr()
+
+
+@pytest.mark.parametrize('cplx', [False, True])
+def test_csr_hv(pkg, dev, cplx):
+    import torch
+    rng = np.random.default_rng(3)
+    D = 3001
+    H = _hermitian(D, 0.004, rng, cplx)
+    P = pkg.CsrPlan(H, device=dev)
+    assert P.dim == D and P.is_real == (not cplx)
+    x = rng.normal(size=D) + (1j * rng.normal(size=D) if cplx else 0)
+    ref = H @ x
+    dt = torch.complex128 if cplx else torch.float64
+    y = P.hv(torch.as_tensor(x, device=dev).to(dt)).cpu().numpy()
+    assert np.max(np.abs(y - ref)) <= 1e-13 * np.max(np.abs(ref))
+    dt32 = torch.complex64 if cplx else torch.float32
+    y32 = P.hv(torch.as_tensor(x, device=dev).to(dt32)).cpu().numpy()
+    assert np.max(np.abs(y32 - ref)) <= 1e-5 * np.max(np.abs(ref))
+    with pytest.raises(NotImplementedError):              # no site codes
+        pkg.site_occupations(P, torch.as_tensor(x, device=dev).to(dt))
+    P.close()
+
+
+def test_kpm_moments_gpu_mirror(pkg, dev):
+    import torch
+    rng = np.random.default_rng(5)
+    D, num, nvec, seed, chunk = 2000, 40, 5, 11, 2
+    H = _hermitian(D, 0.005, rng, True)
+    mu, c, r = pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, seed=seed, device=dev,
+                                       chunk=chunk)
+    # the reference's probe draw (kpm.py:60-63), repeated here on the device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    probes, done = [], 0
+    while done < nvec:
+        b = min(chunk, nvec - done)
+        z = torch.randint(0, 2, (D, b), device=dev, generator=gen, dtype=torch.int8)
+        z = z.to(torch.float64).mul_(2).sub_(1).cpu().numpy()
+        probes += [z[:, j] for j in range(b)]
+        done += b
+    w = np.linalg.eigvalsh(H.toarray())
+    assert c - r <= w[0] and c + r >= w[-1]
+    assert abs(r - (w[-1] - w[0]) / 2 * (1 + 1e-3) - 1e-12) < 1e-8 * r
+    ref, _, _ = orc.kpm_moments(H, num, nvec, c=c, r=r, probes=probes)
+    assert mu.shape == (num + 1,)
+    assert np.max(np.abs(mu - ref)) < 1e-10
+    mu32, _, _ = pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, seed=seed, device=dev,
+                                         chunk=chunk, dtype='float32')
+    assert np.max(np.abs(mu32 - ref)) < 2e-4
+
+
+def test_csr_lanczos_ground_state(pkg, dev):
+    from scipy.sparse.linalg import eigsh
+    rng = np.random.default_rng(8)
+    H = _hermitian(4000, 0.003, rng, False)
+    P = pkg.CsrPlan(H, device=dev)
+    out = pkg.hip_lanczos_ground_state(None, plan=P, device=dev)
+    E0 = float(eigsh(H, k=1, which='SA', tol=1e-12)[0][0])
+    assert out['converged'] and abs(out['E0'] - E0) <= 1e-9 * abs(E0)
+    P.close()
