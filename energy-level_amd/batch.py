@@ -198,3 +198,46 @@ def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device=
     finally:
         B.close()
     return out
+
+
+def batched_ground_state_dense_hip(H_batch, backend='auto', vram_budget=1.2e9, device='cuda'):
+    """batched_ground_state (tomisnot/energy-level
+    src/core/manybody/equilibrium.py:251-311) with its signature: H_batch is a
+    (B, dim, dim) Hermitian stack with dim = 2^n; → dict(E0 (B,), gap (B,),
+    n_atom (B, n), backend).  Each H is copied into a device CSR plan (its
+    non-zeros) and solved by the device Lanczos; <n_i> of the ground state
+    comes from the full-basis occupation kernel of an n-site plan (site 0 is
+    the most significant bit, as bit_matrix).  ``backend`` and
+    ``vram_budget`` are accepted for the signature; one instance is resident
+    at a time."""
+    from .lanczos import hip_lanczos_ground_state, site_occupations
+    from .model import from_calib
+    from .plan import CsrPlan, Plan
+    import torch
+    Hb = np.asarray(H_batch)
+    if Hb.ndim != 3 or Hb.shape[1] != Hb.shape[2]:
+        raise ValueError('H_batch must be shaped (B, dim, dim), got %r' % (Hb.shape,))
+    B, dim, _ = Hb.shape
+    n = int(round(np.log2(dim))) if dim > 0 else 0
+    if dim < 1 or (1 << n) != dim:
+        raise ValueError('dim=%d is not 2^N' % dim)
+    occ_plan = Plan(from_calib({'atoms': [{}] * n, 'pairs': []}, np.zeros(n), np.zeros(n)),
+                    device=device)
+    E0 = np.zeros(B)
+    gap = np.zeros(B)
+    n_atom = np.zeros((B, n))
+    try:
+        for b in range(B):
+            P = CsrPlan(Hb[b], device=device)
+            try:
+                out = hip_lanczos_ground_state(None, plan=P, device=device, return_state=True)
+            finally:
+                P.close()
+            E0[b], gap[b] = out['E0'], out['gap']
+            v = torch.as_tensor(np.asarray(out['vec']), device=occ_plan.device)
+            v = v.to(torch.complex128 if torch.is_complex(v) else torch.float64)
+            n_atom[b] = np.asarray(site_occupations(occ_plan, v))
+    finally:
+        occ_plan.close()
+    return dict(E0=E0, gap=gap, n_atom=n_atom, backend='hip-csr-lanczos')
+

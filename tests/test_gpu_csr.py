@@ -77,3 +77,24 @@ def test_csr_lanczos_ground_state(pkg, dev):
     E0 = float(eigsh(H, k=1, which='SA', tol=1e-12)[0][0])
     assert out['converged'] and abs(out['E0'] - E0) <= 1e-9 * abs(E0)
     P.close()
+
+
+def test_batched_ground_state_dense_signature(pkg, dev):
+    """batched_ground_state (equilibrium.py:251-311) on a dense Hermitian
+    stack: E0, gap and <n_i> (site 0 = most significant bit) against eigh."""
+    rng = np.random.default_rng(12)
+    B, n = 3, 7
+    dim = 1 << n
+    Hs = []
+    for _ in range(B):
+        A = rng.normal(size=(dim, dim)) + 1j * rng.normal(size=(dim, dim))
+        Hs.append((A + A.conj().T) / 2)
+    out = pkg.batched_ground_state_dense_hip(np.array(Hs), device=dev)
+    bits = np.array([[(s >> (n - 1 - i)) & 1 for s in range(dim)] for i in range(n)], float)
+    for b, H in enumerate(Hs):
+        w, V = np.linalg.eigh(H)
+        assert abs(out['E0'][b] - w[0]) <= 1e-9 * abs(w[0])
+        assert abs(out['gap'][b] - (w[1] - w[0])) <= 1e-7
+        assert np.allclose(out['n_atom'][b], bits @ np.abs(V[:, 0]) ** 2, atol=1e-7)
+    with pytest.raises(ValueError):
+        pkg.batched_ground_state_dense_hip(np.zeros((2, 6, 6)), device=dev)
