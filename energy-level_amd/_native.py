@@ -83,6 +83,9 @@ SIGNATURES = {
     'ryd_batch_offset': (_i64, [_vp, ctypes.c_int32]),
     'ryd_batch_run': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     'ryd_batch_autotune': (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp]),
+    'ryd_ritz_lowest': (ctypes.c_int, [ctypes.c_int32, _vp, _vp, _i64, _vp, _i64, _vp, _dbl, _dbl,
+                                       _vp, _vp, _vp, _i64]),
+    'ryd_batch_set_tile_bits': (ctypes.c_int, [_vp, ctypes.c_int32]),
     'ryd_batch_results': (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     'ryd_batch_ritz': (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     'ryd_dots': (ctypes.c_int, [ctypes.c_int, _vp, _vp, _i64, _vp, _vp, _vp]),

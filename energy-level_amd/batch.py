@@ -15,10 +15,10 @@ import ctypes
 import numpy as np
 
 from . import _native as nat
-from .lanczos import _genuine_lowest, _tridiag
 from .model import RydModel, from_positions, chain_positions, c6_coupling, from_calib
 
-__all__ = ['BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip']
+__all__ = ['BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip',
+           'certified_ground_states']
 
 BMAXN = 24            # largest instance n (occupation rows hold BMAXN sites + the norm)
 
@@ -95,6 +95,12 @@ class BatchLanczos:
         return torch.randn(self.total, dtype=torch.float64, device=self.device,
                            generator=g).to(self.tdt)
 
+    def set_tile_bits(self, bits):
+        """Use tiles of 2^bits codes (10, 11 or 12), e.g. the autotuner's
+        choice of another batch over the same instances."""
+        nat.check(nat.lib().ryd_batch_set_tile_bits(self._h, int(bits)))
+        self.tile_bits = int(bits)
+
     def autotune(self, v0):
         tb = ctypes.c_int32(0)
         ms = (ctypes.c_double * 3)()
@@ -118,48 +124,99 @@ class BatchLanczos:
                                               stop.ctypes.data))
         return alpha, nrm, stop
 
-    def _lowest(self, alpha, nrm, stop, b):
-        k = self.steps
-        if stop[b]:
-            # first step whose new norm fell below the breakdown threshold
-            tiny = 1e-6 if self.dtype == 'float32' else 1e-13
-            bad = np.nonzero(~(nrm[b, 1:] > tiny))[0]
-            k = int(bad[0]) + 1 if bad.size else self.steps
-        a, bb = alpha[b, :k], nrm[b, 1:k]
-        theta, Y = _tridiag(a, bb)
-        return k, theta, Y, _genuine_lowest(a, bb, theta, 2)
-
-    def occupations(self, v0):
-        """<n_i> of each instance's lowest Ritz vector (list of arrays, site
-        order), replaying the recurrence from the same start vectors ``v0``."""
+    def instance_index(self):
+        """int16 tensor of the instance of every element of the concatenation."""
         import torch
-        alpha, nrm, stop = self.coefficients()
-        ycoef = np.zeros((self.count, self.steps))
-        for b in range(self.count):
-            k, theta, Y, picks = self._lowest(alpha, nrm, stop, b)
-            ycoef[b, :k] = Y[:, picks[0]]
+        if getattr(self, '_inst_el', None) is None:
+            sizes = torch.as_tensor(np.diff(self.offsets), device=self.device)
+            self._inst_el = torch.repeat_interleave(
+                torch.arange(self.count, dtype=torch.int16, device=self.device), sizes)
+        return self._inst_el
+
+    def ritz(self, v0, ycoef):
+        """Σ_j ycoef[b, j]·q_j of every instance (q_j the normalised Lanczos
+        vectors of the last run from ``v0``, replayed) and the unnormalised
+        site weights of each result (occupation rows: BMAXN sites + |x|²)."""
+        import torch
+        ycoef = np.ascontiguousarray(ycoef, np.float64)
+        if ycoef.shape != (self.count, self.steps):
+            raise ValueError('ycoef must be shaped (count, steps)')
         ritz = torch.empty(self.total, dtype=self.tdt, device=self.device)
         occ = torch.empty((self.count, BMAXN + 1), dtype=torch.float64, device=self.device)
         nat.check(nat.lib().ryd_batch_ritz(self._h, nat.ptr(v0.contiguous()), ycoef.ctypes.data,
                                            nat.ptr(ritz), nat.ptr(occ), nat.stream_ptr()))
+        return ritz, occ
+
+    def occupations(self, v0):
+        """<n_i> of each instance's lowest Ritz vector (list of arrays, site
+        order), replaying the recurrence from the same start vectors ``v0``."""
+        self.ground_states()
+        _, occ = self.ritz(v0, self.restart_coefficients(np.zeros(self.count, np.int64)))
         occ = occ.cpu().numpy()
         return [occ[b, :int(m.n)] / occ[b, BMAXN] for b, m in enumerate(self.models)]
 
-    def ground_states(self):
-        """E0, gap (lowest two genuine Ritz values) and Ritz bound per instance."""
+    def _steps_of(self, nrm, stop):
+        """Valid steps per instance: all, or up to the breakdown."""
+        k = np.full(self.count, self.steps, np.int32)
+        tiny = 1e-6 if self.dtype == 'float32' else 1e-13
+        for b in np.nonzero(stop)[0]:
+            bad = np.nonzero(~(nrm[b, 1:] > tiny))[0]
+            k[b] = int(bad[0]) + 1 if bad.size else self.steps
+        return k
+
+    def ground_states(self, tol=None, active=None, merge_rel=None):
+        """E0, gap (lowest two genuine Ritz values, ``ryd_ritz_lowest``), their
+        Ritz bounds and, with ``tol``, converged flags per instance:
+        ``converged_E0`` (the E0 bound <= tol·max(1,|E0|)) and ``converged``
+        (both bounds, the rule of torch_lanczos_ground_state,
+        matrix_free.py:258-266).  ``active`` (bool mask) limits the work to
+        those instances; the others are left at zero.  The two Ritz vectors'
+        coefficients are kept for :meth:`restart_coefficients`."""
         alpha, nrm, stop = self.coefficients()
-        E0 = np.zeros(self.count)
-        gap = np.full(self.count, np.nan)
-        bound = np.zeros(self.count)
-        steps = np.zeros(self.count, np.int64)
-        for b in range(self.count):
-            k, theta, Y, picks = self._lowest(alpha, nrm, stop, b)
-            E0[b] = theta[picks[0]]
-            if len(picks) > 1:
-                gap[b] = theta[picks[1]] - theta[picks[0]]
-            bound[b] = 0.0 if stop[b] else abs(nrm[b, k] * Y[-1, picks[0]])
-            steps[b] = k
-        return dict(E0=E0, gap=gap, ritz_bound=bound, steps=steps)
+        idx = np.arange(self.count) if active is None else np.nonzero(active)[0]
+        k = self._steps_of(nrm, stop)
+        E = np.full((self.count, 2), np.nan)
+        bnd = np.full((self.count, 2), np.inf)
+        if getattr(self, '_ycoef', None) is None:
+            self._ycoef = np.zeros((self.count, 2, self.steps))
+        if idx.size:
+            a = np.ascontiguousarray(alpha[idx])
+            o = np.ascontiguousarray(nrm[idx, 1:])
+            kk = np.ascontiguousarray(k[idx])
+            bl = np.where(stop[idx] != 0, 0.0, nrm[idx, kk])
+            bl = np.ascontiguousarray(bl, np.float64)
+            Es = np.zeros((idx.size, 2))
+            Bs = np.zeros((idx.size, 2))
+            Ys = np.zeros((idx.size, 2, self.steps))
+            nat.check(nat.lib().ryd_ritz_lowest(
+                int(idx.size), kk.ctypes.data, a.ctypes.data, self.steps, o.ctypes.data,
+                self.steps, bl.ctypes.data, float(tol if tol is not None else 0.0),
+                float(merge_rel if merge_rel is not None else
+                      (1e-6 if self.dtype == 'float32' else 1e-10)), Es.ctypes.data, Bs.ctypes.data, Ys.ctypes.data, self.steps))
+            E[idx], bnd[idx] = Es, Bs
+            self._ycoef[idx] = Ys
+        out = dict(E0=np.where(np.isnan(E[:, 0]), 0.0, E[:, 0]), gap=E[:, 1] - E[:, 0],
+                   ritz_bound=np.where(np.isinf(bnd[:, 0]), 0.0, bnd[:, 0]),
+                   ritz_bound_gap=np.where(stop != 0, 0.0, bnd[:, 1]),
+                   steps=np.where(np.isin(np.arange(self.count), idx), k, 0),
+                   stopped=stop.astype(bool))
+        if tol is not None:
+            E0, E1 = out['E0'], out['E0'] + np.nan_to_num(out['gap'])
+            c0 = out['ritz_bound'] <= tol * np.maximum(1.0, np.abs(E0))
+            out['converged_E0'] = c0
+            out['converged'] = c0 & (out['ritz_bound_gap'] <= tol * np.maximum(1.0, np.abs(E1)))
+        return out
+
+    def restart_coefficients(self, which):
+        """ycoef rows from the last ``ground_states`` call: for instance b,
+        ``which[b]`` = 0 → its lowest Ritz vector, 1 → the sum of the two
+        lowest (the restart seed; the lowest alone when there is no second),
+        -1 → zeros."""
+        w = np.asarray(which)
+        y = self._ycoef
+        out = np.where((w >= 0)[:, None], y[:, 0], 0.0)
+        two = (w == 1)[:, None] & np.isfinite(y[:, 1]).all(axis=1)[:, None]
+        return np.ascontiguousarray(out + np.where(two, y[:, 1], 0.0))
 
 
 def disorder_ensemble(count=512, nmin=10, nmax=20, sigma=0.1, deltas=None, c6=10.0, omega=1.0,
@@ -179,24 +236,204 @@ def disorder_ensemble(count=512, nmin=10, nmax=20, sigma=0.1, deltas=None, c6=10
     return out
 
 
-def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device='cuda',
-                             autotune=True, use_graph=True, observables=True):
-    """Dict like batched_ground_state (tomisnot/energy-level
-    src/core/manybody/equilibrium.py:251-311): E0 (B,), gap (B,), n_atom (a
-    list, one array of <n_i> per instance since N may differ) and backend,
-    for uneven-N instances.  ``observables=False`` skips the Ritz replay."""
+def _masked(B, x, keep):
+    """x with the instances where ``keep`` is False set to zero (device op)."""
+    import torch
+    m = torch.as_tensor(np.asarray(keep, bool), device=B.device)[B.instance_index().long()]
+    return torch.where(m, x, torch.zeros((), dtype=x.dtype, device=x.device))
+
+
+def _segments(B, x, local):
+    """The concatenated segments of x (B's layout) of B's instances ``local``."""
+    import torch
+    return torch.cat([x[int(B.offsets[i]):int(B.offsets[i + 1])] for i in local])
+
+
+def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph, tile_bits,
+           device, occ_rows=None, seeds=None, finish_last=True):
+    """Batched Lanczos cycles over the instances ``idx`` (indices into
+    ``models``) from their concatenated start vectors ``v``, until each
+    instance's two lowest genuine Ritz pairs have bounds <= tol·max(1,|E|)
+    or ``cycles`` are spent.
+
+    A finished instance records E0, gap, bounds and flags in ``out`` (at its
+    index) and then its lowest Ritz vector's site weights (``occ_rows``) or,
+    with ``seeds`` (a dict), the sum of its two lowest Ritz vectors as the
+    seed of a next stage.  The others restart from that sum: while they are
+    at least half of the batch they stay in it (a finished instance gets a
+    zero start vector, which the batch kernels skip: its norm is zero, so its
+    stop flag is set before the first step); below that the cycle continues
+    on a new batch of those instances only, so that the launches shrink with
+    the work.  With ``finish_last`` False the instances still unconverged
+    after the last cycle are not recorded.  Returns (cycles run, indices
+    still unconverged, their restart vectors concatenated, or None)."""
+    idx = np.asarray(idx, np.int64)
+    B = BatchLanczos([models[i] for i in idx], steps=steps, dtype=dtype, device=device)
+    try:
+        if tile_bits is not None:
+            B.set_tile_bits(tile_bits)
+        act = np.ones(len(idx), bool)
+        done = 0
+        for cyc in range(int(cycles)):
+            B.run(v, use_graph=use_graph)
+            done += 1
+            gs = B.ground_states(tol, act, merge_rel)
+            last = cyc == cycles - 1
+            fin = act & (gs['converged'] | (last and finish_last))
+            g = idx[fin]
+            for key in ('E0', 'gap', 'ritz_bound', 'ritz_bound_gap', 'converged', 'converged_E0'):
+                out[key][g] = gs[key][fin]
+            out['steps_' + dtype][idx[act]] += gs['steps'][act]
+            rest = act & ~fin
+            which = np.where(rest, 1, np.where(fin, 1 if seeds is not None else 0, -1))
+            if not (np.any(rest) or seeds is not None or occ_rows is not None):
+                return done, idx[:0], None
+            x, occ = B.ritz(v, B.restart_coefficients(which))
+            if occ_rows is not None and np.any(fin):
+                occ_np = occ.cpu().numpy()
+                for l in np.nonzero(fin)[0]:
+                    occ_rows[idx[l]] = occ_np[l].copy()
+            if seeds is not None:
+                for l in np.nonzero(fin)[0]:
+                    seeds[int(idx[l])] = x[int(B.offsets[l]):int(B.offsets[l + 1])].clone()
+            if not np.any(rest):
+                return done, idx[:0], None
+            if last:
+                loc = np.nonzero(rest)[0]
+                return done, idx[loc], _segments(B, x, loc)
+            if 2 * int(rest.sum()) <= len(idx):
+                loc = np.nonzero(rest)[0]
+                v = _segments(B, x, loc)
+                idx = idx[loc]
+                B.close()
+                B = BatchLanczos([models[i] for i in idx], steps=steps, dtype=dtype, device=device)
+                if tile_bits is not None:
+                    B.set_tile_bits(tile_bits)
+                act = np.ones(len(idx), bool)
+            else:
+                v = _masked(B, x, rest)
+                act = rest
+        return done, idx[:0], None
+    finally:
+        B.close()
+
+
+def certified_ground_states(models, steps=150, dtype='float32', refine=True, bulk_tol=1e-5,
+                            tol=1e-9, bulk_cycles=2, refine_schedule=((50, 1), (200, 2), (1000, 2)),
+                            seed=0, device='cuda', autotune=True, use_graph=True, observables=True,
+                            tile_bits=None, seed_noise=1e-3):
+    """Batched ground states with a convergence test per instance.
+
+    Bulk: ``steps``-step batched Lanczos cycles in ``dtype`` (one captured
+    graph) from seeded start vectors; an instance leaves when its two lowest
+    genuine Ritz pairs have bounds <= tol·max(1,|E|), the others restart from
+    the sum of their two lowest Ritz vectors (see :func:`_stage`).  With an
+    FP32 bulk and ``refine``, the bulk stops at ``bulk_tol`` and FP64 cycles
+    take over to ``tol`` from the bulk Ritz vectors (plus a seeded random
+    part of relative size ``seed_noise``, so that a direction the bulk lost
+    is present again), following ``refine_schedule`` = ((steps, cycles), ...):
+    a short cycle first (an instance the bulk brought to its FP32 limit
+    needs no more), then longer ones for what is left, the last ones long
+    and continued (few restarts) for the slowest instances, which by then
+    form a small batch of their own.  E0, the gap and their bounds are then
+    FP64 results.  Ghost copies merge within 1e-6·|T| in FP32 cycles and
+    1e-10·|T| in FP64 ones (their rounding).  An FP64 bulk (or no
+    refinement) runs ``bulk_cycles`` cycles and then the schedule's cycle
+    counts as further cycles of ``steps`` steps.
+
+    Returns E0, gap, ritz_bound (E0), ritz_bound_gap (E1), converged (both
+    pairs, the final stage), converged_E0, bulk_converged,
+    steps_float32/steps_float64 (steps run per instance), cycles_* and, with
+    ``observables``, n_atom (<n_i> of each instance's final lowest Ritz
+    vector).  ``tile_bits`` skips the autotuner."""
+    import torch
+    count = len(models)
+    two_stage = bool(refine) and dtype == 'float32'
+    out = dict(E0=np.zeros(count), gap=np.full(count, np.nan), ritz_bound=np.zeros(count),
+               ritz_bound_gap=np.zeros(count), converged=np.zeros(count, bool),
+               converged_E0=np.zeros(count, bool), steps_float32=np.zeros(count, np.int64),
+               steps_float64=np.zeros(count, np.int64))
+    occ_rows = [None] * count if observables else None
+    merge = {'float32': 1e-6, 'float64': 1e-10}
+    schedule = [(int(st), int(cy)) for st, cy in refine_schedule if int(cy) > 0]
     B = BatchLanczos(models, steps=steps, dtype=dtype, device=device)
     try:
         v0 = B.start_vectors(seed)
-        if autotune:
-            B.autotune(v0)
-        B.run(v0, use_graph=use_graph)
-        out = B.ground_states()
-        if observables:
-            out['n_atom'] = B.occupations(v0)
-        out.update(backend='hip-batched-lanczos', tile_bits=B.tile_bits, dtype=dtype)
+        if tile_bits is not None:
+            B.set_tile_bits(tile_bits)
+        elif autotune:
+            tile_bits = B.autotune(v0)
+        out['autotune_ms'] = B.autotune_ms
+        out['tile_bits'] = B.tile_bits
     finally:
         B.close()
+    all_idx = np.arange(count)
+    # one stage (an FP64 bulk, or no refinement) takes the whole cycle budget
+    budget = bulk_cycles if two_stage else bulk_cycles + sum(cy for _, cy in schedule)
+    seeds = {} if two_stage else None
+    done, _, _ = _stage(models, all_idx, v0, dtype, steps, bulk_tol if two_stage else tol,
+                        merge[dtype], budget, out, use_graph, tile_bits, device,
+                        None if two_stage else occ_rows, seeds)
+    del v0
+    out['cycles_' + dtype] = done
+    out['bulk_converged'] = out['converged'].copy()
+    if two_stage:
+        out['converged'][:] = False
+        out['converged_E0'][:] = False
+        gen = torch.Generator(device=seeds[0].device)
+        gen.manual_seed(int(seed) + 1)
+        parts = []
+        for b in range(count):
+            x = seeds.pop(b).to(torch.float64)
+            d = x.numel()
+            noise = torch.randn(d, dtype=torch.float64, device=x.device, generator=gen)
+            parts.append(x / torch.linalg.vector_norm(x) + (seed_noise / np.sqrt(d)) * noise)
+        vrest = torch.cat(parts)
+        del parts
+        done64, rest = 0, all_idx
+        for k, (st, cy) in enumerate(schedule):
+            if not len(rest):
+                break
+            d, rest, vrest = _stage(models, rest, vrest, 'float64', st, tol, merge['float64'], cy,
+                                    out, use_graph, tile_bits, device, occ_rows,
+                                    finish_last=(k == len(schedule) - 1))
+            done64 += d
+        out['cycles_float64'] = done64
+    if observables:
+        out['n_atom'] = [np.asarray(occ_rows[b][:int(m.n)]) / occ_rows[b][BMAXN]
+                         for b, m in enumerate(models)]
+    out['tolerance'] = tol if (two_stage or dtype == 'float64') else bulk_tol
+    return out
+
+
+def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device='cuda',
+                             autotune=True, use_graph=True, observables=True, refine=True,
+                             tol=1e-9, bulk_tol=1e-5, bulk_cycles=2,
+                             refine_schedule=((50, 1), (200, 2), (1000, 2))):
+    """Dict like batched_ground_state (tomisnot/energy-level
+    src/core/manybody/equilibrium.py:251-311): E0 (B,), gap (B,), n_atom (a
+    list, one array of <n_i> per instance since N may differ) and backend,
+    for uneven-N instances; plus converged (B,), converged_E0 and the Ritz
+    bounds of E0 and E1.  Instances run batched Lanczos cycles until their
+    two lowest Ritz pairs converge (FP32 bulk to ``bulk_tol``, then an FP64
+    refinement to ``tol``, see :func:`certified_ground_states`); an instance
+    that does not converge within the cycle budget is flagged and reported
+    in ``warning``.  ``observables=False`` skips <n_i>."""
+    import warnings
+    out = certified_ground_states(models, steps=steps, dtype=dtype, refine=refine,
+                                  bulk_tol=bulk_tol, tol=tol, bulk_cycles=bulk_cycles,
+                                  refine_schedule=refine_schedule, seed=seed, device=device,
+                                  autotune=autotune, use_graph=use_graph,
+                                  observables=observables)
+    out.update(backend='hip-batched-lanczos', dtype=dtype)
+    bad = np.nonzero(~out['converged'])[0]
+    if bad.size:
+        bad0 = np.nonzero(~out['converged_E0'])[0]
+        out['warning'] = ('%d of %d instances did not converge to tol=%g within the cycle budget '
+                          '(%d of them in E0; the others in E1, so their gap is an estimate '
+                          'within ritz_bound_gap); indices %s'
+                          % (bad.size, len(models), out['tolerance'], bad0.size, bad[:16].tolist()))
+        warnings.warn(out['warning'], RuntimeWarning)
     return out
 
 

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rydhip.h"
@@ -21,6 +22,7 @@
 #include "sector.hpp"
 #include "evolve.hpp"
 #include "batch.hpp"
+#include "tridiag.hpp"
 
 using namespace ryd;
 
@@ -2322,6 +2324,50 @@ extern "C" int ryd_batch_autotune(ryd_batch* B, const void* v0, int32_t* chosen_
   (void)hipEventDestroy(b);
   B->choice = bestk;
   if (chosen_tile_bits) *chosen_tile_bits = 10 + bestk;
+  return RYD_OK;
+}
+
+extern "C" int ryd_ritz_lowest(int32_t count, const int32_t* steps, const double* alpha, int64_t lda,
+                               const double* offd, int64_t ldo, const double* beta_last,
+                               double conv_tol, double merge_rel, double* E, double* bound, double* y,
+                               int64_t ldy) {
+  if (count < 0 || (count > 0 && (!steps || !alpha || !offd || !beta_last || !E || !bound)))
+    return fail(RYD_ERR_INVALID, "ryd_ritz_lowest: null argument");
+  if (!(merge_rel >= 0.0)) return fail(RYD_ERR_INVALID, "ryd_ritz_lowest: merge_rel must be >= 0");
+  for (int32_t b = 0; b < count; ++b) {
+    const int32_t k = steps[b];
+    if (k < 0 || k > lda || (k > 1 && k - 1 > ldo) || (y && k > ldy))
+      return fail(RYD_ERR_INVALID, "ryd_ritz_lowest: steps exceed the leading dimensions");
+  }
+  // instances are independent: split them over host threads
+  auto run = [&](int32_t b0, int32_t b1) {
+    tri::Work w;
+    for (int32_t b = b0; b < b1; ++b) {
+      double* y0 = y ? y + (size_t)2 * b * ldy : nullptr;
+      double* y1 = y ? y0 + ldy : nullptr;
+      tri::lowest_pairs(alpha + (size_t)b * lda, offd + (size_t)b * ldo, steps[b], beta_last[b],
+                        conv_tol, merge_rel, E + 2 * (size_t)b, bound + 2 * (size_t)b, y0, y1, w);
+    }
+  };
+  const int hw = (int)std::thread::hardware_concurrency();
+  const int nt = std::max(1, std::min({hw > 0 ? hw : 1, 16, (int)(count / 8)}));
+  if (nt <= 1) {
+    run(0, count);
+    return RYD_OK;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nt; ++t) {
+    const int32_t b0 = (int32_t)((int64_t)count * t / nt), b1 = (int32_t)((int64_t)count * (t + 1) / nt);
+    pool.emplace_back(run, b0, b1);
+  }
+  for (auto& th : pool) th.join();
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_set_tile_bits(ryd_batch* B, int32_t tile_bits) {
+  if (!B || tile_bits < 10 || tile_bits > 12)
+    return fail(RYD_ERR_INVALID, "ryd_batch_set_tile_bits: tile bits must be 10, 11 or 12");
+  B->choice = tile_bits - 10;
   return RYD_OK;
 }
 

@@ -137,7 +137,12 @@ def solve_equilibrium_hip(system, want='gs', *, backend='hip-lanczos', betas=Non
                 rb = batched_ground_state_hip([model], dtype=dtype if dtype in ('float32', 'float64')
                                               else 'float64', seed=seed, device=device)
                 result = dict(base, E0=float(rb['E0'][0]), gap=float(rb['gap'][0]),
-                              n_atom=rb['n_atom'][0])
+                              n_atom=rb['n_atom'][0], converged=bool(rb['converged'][0]),
+                              ritz_bound=float(rb['ritz_bound'][0]),
+                              ritz_bound_gap=float(rb['ritz_bound_gap'][0]),
+                              tolerance=float(rb['tolerance']))
+                if 'warning' in rb:
+                    result['warning'] = rb['warning']
     else:
         from .kpm import kpm_moments_hip, spectral_bounds, thermal_from_moments, dos_from_moments
         from .model import from_calib

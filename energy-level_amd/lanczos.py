@@ -56,17 +56,37 @@ def _tridiag(a, b):
         return np.linalg.eigh(np.diag(a) + np.diag(b, 1) + np.diag(b, -1))
 
 
-def _genuine_lowest(a, b, theta, want=2):
+def _genuine_lowest(a, b, theta, want=2, bounds=None, conv_tol=None, that=None, merge_rel=1e-10):
     """Indices (into theta) of the ``want`` lowest distinct, non-spurious Ritz values.
 
-    Duplicates (ghost copies of converged values) merge; a simple value that
+    Duplicates (ghost copies of converged values, equal to ``merge_rel``·|T|:
+    the rounding of the recurrence, 1e-10 for FP64 vectors and 1e-6 for FP32
+    ones) merge; a simple value that
     is also an eigenvalue of T with its first row/column removed is spurious
-    (Cullum–Willoughby)."""
+    (Cullum–Willoughby), with two exceptions that test cannot see:
+
+    * the lowest Ritz value is always kept.  A spurious value is a copy in
+      the making of an eigenvalue that has already converged, so it never
+      lies below the lowest genuine one; the test itself misfires on a
+      genuine value whose eigenvector has a tiny component in the start
+      vector, and dropping the lowest value then reports the first excited
+      level as E0;
+    * a value whose residual bound ``bounds[i]`` (|beta_k · y_k[-1]|) is at
+      most ``conv_tol``·max(1, |theta_i|) has converged, so it lies within
+      that bound of an eigenvalue of H (Paige) and is kept.
+
+    ``theta`` may hold only the lowest eigenvalues of T, with ``that`` the
+    lowest eigenvalues of T-hat that interlace them."""
     from scipy.linalg import eigvalsh_tridiagonal
     k = len(a)
     scale = max(1.0, float(np.max(np.abs(theta))))
-    tol = max(1e-10 * scale, 100.0 * k * np.finfo(float).eps * scale)
-    if k > 2:
+    if that is not None and k > 1:
+        # only the lowest values are at hand: bound |T| by Gershgorin
+        scale = max(scale, float(np.max(np.abs(a)) + 2.0 * np.max(np.abs(b))))
+    tol = max(merge_rel * scale, 100.0 * k * np.finfo(float).eps * scale)
+    if that is not None:
+        pass
+    elif k > 2:
         that = eigvalsh_tridiagonal(a[1:], b[1:])
     elif k == 2:
         that = a[1:]
@@ -79,12 +99,20 @@ def _genuine_lowest(a, b, theta, want=2):
         if kept and abs(th - theta[kept[-1]]) <= tol:
             continue
         mult = int(np.sum(np.abs(theta - th) <= tol))
-        if mult == 1 and that.size and np.min(np.abs(that - th)) <= tol:
-            continue
+        if kept and mult == 1 and that.size and np.min(np.abs(that - th)) <= tol:
+            converged = (bounds is not None and conv_tol is not None and
+                         bounds[i] <= conv_tol * max(1.0, abs(th)))
+            if not converged:
+                continue
         kept.append(int(i))
         if len(kept) == want:
             break
     return kept
+
+
+def _ritz_bounds(Y, beta_last):
+    """|beta_k · y_k[-1]| of every Ritz pair of the k-step tridiagonal."""
+    return np.abs(float(beta_last) * np.asarray(Y)[-1, :])
 
 
 class _Run:
@@ -156,6 +184,8 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
     st = nat.stream_ptr()
     if tiny is None:
         tiny = 1e-13 if dt in (torch.float64, torch.complex128) else 1e-6
+    # ghost copies of a value agree to the recurrence's own rounding
+    merge_rel = 1e-10 if dt in (torch.float64, torch.complex128) else 1e-6
 
     def three_buffers(j):
         """Switch to (or start) the rotating three-vector recurrence at step j;
@@ -247,8 +277,9 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
         nr = R.nrm[:j + 1].cpu().numpy()
         b = nr[1:j]
         theta, Y = _tridiag(a, b)
-        picks = _genuine_lowest(a, b, theta, want)
         blast = 0.0 if stopped else float(nr[j])
+        picks = _genuine_lowest(a, b, theta, want, _ritz_bounds(Y, blast),
+                                max(float(tol), 1e-12), merge_rel=merge_rel)
         est = [abs(blast * Y[-1, i]) for i in picks]
         ok = all(e <= tol * max(1.0, abs(theta[i])) for e, i in zip(est, picks))
         if ok or stopped or j >= kmax:

@@ -294,6 +294,27 @@ int ryd_batch_run(ryd_batch* batch, const void* v0, int use_graph, void* stream)
  * fastest; ms_out (host, 3 entries) may be NULL. */
 int ryd_batch_autotune(ryd_batch* batch, const void* v0, int32_t* chosen_tile_bits,
                        double* ms_out);
+/* Host only (no device work): the two lowest genuine Ritz pairs of `count`
+ * Lanczos tridiagonals, replacing the dense tridiagonal eigensolve and the
+ * ghost filter the reference's Rayleigh-Ritz step does not need
+ * (matrix_free.py:251-266 diagonalises its projected block instead).
+ * Tridiagonal b: diagonal alpha[b*lda + i] (i < steps[b]), off-diagonal
+ * offd[b*ldo + i] (i < steps[b]-1), beta_last[b] = the next Lanczos norm
+ * (0 after a breakdown).  Eigenvalues by Sturm bisection; a value within
+ * merge_rel·|T| of the previous kept one merges (a ghost copy: 1e-10 for a
+ * recurrence in FP64 vectors, 1e-6 in FP32 ones); a simple value
+ * that T without its first row and column also has is spurious
+ * (Cullum-Willoughby) unless its bound |beta_last · y[k-1]| is at most
+ * conv_tol·max(1,|E|); the lowest value is always kept.  Outputs (host):
+ * E[b*2 + q], bound[b*2 + q] (q = 0: E0, 1: E1; NaN / inf when absent) and,
+ * if y is non-null, the unit eigenvectors y[(b*2 + q)*ldy + i], i < steps[b]. */
+int ryd_ritz_lowest(int32_t count, const int32_t* steps, const double* alpha, int64_t lda,
+                    const double* offd, int64_t ldo, const double* beta_last, double conv_tol,
+                    double merge_rel, double* E, double* bound, double* y, int64_t ldy);
+/* Uses tiles of 2^tile_bits codes (10, 11 or 12) without timing, e.g. the
+ * autotuner's choice for another batch of the same instances (an FP64
+ * refinement of an FP32 bulk run).  No reference counterpart. */
+int ryd_batch_set_tile_bits(ryd_batch* batch, int32_t tile_bits);
 /* Host copies: alpha [count][steps], nrm [count][steps+1] (beta_j = nrm[j+1]),
  * stop [count] (1 where the recurrence broke down). */
 int ryd_batch_results(const ryd_batch* batch, double* alpha, double* nrm, int32_t* stop);

@@ -129,6 +129,58 @@ def test_jackson_kernel_closed_form():
     assert np.all(np.diff(g) < 0)
 
 
+def _kpm_selfcheck_model():
+    # the random complex N=8 model with J and A of reference
+    # checks/equibench_kpm_selfcheck.py:39-49 (nearest and next-nearest pairs)
+    rng = np.random.default_rng(7)
+    n = 8
+    pairs = [{'i': i, 'j': i + 1, 'V': float(rng.uniform(0, 2)),
+              'J': complex(rng.normal() + 1j * rng.normal()),
+              'A': complex(rng.normal() + 1j * rng.normal())} for i in range(n - 1)]
+    pairs += [{'i': i, 'j': i + 2, 'V': float(rng.uniform(0, 1)),
+               'J': complex(rng.normal(0, .5)), 'A': complex(rng.normal(0, .5))} for i in range(n - 2)]
+    Om = rng.uniform(0, 1, n)
+    delta = rng.uniform(-1, 1, n)
+    return n, orc.build_H_sparse(n, Om, delta, pairs, ['I1', 'I2', 'I3'])
+
+
+def test_thermal_from_moments_vs_exact_spectrum():
+    # reference checks/equibench_kpm_selfcheck.py:52-63: KPM thermodynamics from
+    # exact moments equals the spectral thermodynamics, E < 1e-5, C < 1e-4
+    n, H = _kpm_selfcheck_model()
+    D = 1 << n
+    evals = np.linalg.eigvalsh(H.toarray())
+    betas = np.array([0.1, 0.5, 2.0])
+    mu, c, r = orc.kpm_moments(H, 512, 1, exact=True)
+    out = orc.thermal_from_moments(D, betas, mu, c, r, 512)
+    for i, b in enumerate(betas):
+        w = np.exp(-b * (evals - evals[0]))
+        Z = w.sum()
+        E = float(evals @ w / Z)
+        C = float(b * b * ((evals ** 2) @ w / Z - E * E))
+        lnZ = float(np.log(Z) - b * evals[0])
+        assert abs(out['E'][i] - E) < 1e-5, (b, out['E'][i], E)
+        assert abs(out['C'][i] - C) < 1e-4, (b, out['C'][i], C)
+        assert abs(out['lnZ'][i] - lnZ) < 1e-6 * max(1.0, abs(lnZ)), (b, out['lnZ'][i], lnZ)
+        assert abs(out['F'][i] + lnZ / b) < 1e-6 * max(1.0, abs(lnZ / b)), (b, out['F'][i], -lnZ / b)
+    # the estimator from a few probes is deterministic for a seed
+    m1, _, _ = orc.kpm_moments(H, 64, 4, seed=3, c=c, r=r)
+    m2, _, _ = orc.kpm_moments(H, 64, 4, seed=3, c=c, r=r)
+    assert np.array_equal(m1, m2)
+
+
+def test_thermal_high_temperature_limit():
+    # β → 0: E → Tr(H)/D and C/β² → Var of the spectrum (free of any interpolation)
+    n, H = _kpm_selfcheck_model()
+    D = 1 << n
+    evals = np.linalg.eigvalsh(H.toarray())
+    mu, c, r = orc.kpm_moments(H, 128, 1, exact=True)
+    b = 1e-4
+    out = orc.thermal_from_moments(D, [b], mu, c, r, 128)
+    assert abs(out['E'][0] - evals.mean()) < 1e-3 * max(1.0, np.ptp(evals))
+    assert abs(out['C'][0] / b ** 2 - evals.var()) < 1e-2 * evals.var()
+
+
 def test_dos_normalisation():
     lam = np.linspace(-2.0, 3.0, 64)
     mu, c, r = orc.kpm_moments(np.diag(lam), 200, 1, exact=True)
