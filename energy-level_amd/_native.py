@@ -63,6 +63,8 @@ SIGNATURES = {
                                        ctypes.c_int, _vp, _vp, _vp, _vp]),
     'ryd_cheb_moments': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _dbl,
                                         _dbl, _vp, _vp, _vp]),
+    'ryd_cheb_moments_noalloc': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _dbl,
+                                                _dbl, _vp, _vp]),
     'ryd_site_occupation': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]),
     'ryd_pauli_expect': (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                         _vp, _vp]),

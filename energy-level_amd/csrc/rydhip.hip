@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -1471,6 +1472,60 @@ int ryd_cheb_moments(const ryd_plan* P, int dtype, const uint32_t* probe, void* 
   if (int rc = check_dtype(P, dtype)) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
   DISPATCH(dtype, do_cheb, P, dtype, probe, t0, t1, t2, nsteps, c, r, sums, ws, st);
+}
+
+}  // extern "C"
+
+// Allocation-free variant of ryd_cheb_moments: the reduction partials live in
+// this file-scope device array instead of a caller workspace.  Each GPU holds
+// its own copy until the process ends; host code uses the per-device address
+// from hipGetSymbolAddress (the array's name is only a host-side stand-in).
+// Every call on a GPU shares that GPU's array, so calls that overlap on one
+// GPU race.  Every block writes its partial before the finishing launch reads
+// it, so nothing in the array needs clearing between calls.
+constexpr size_t SCRATCH_BYTES = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// the address of `scratch` on the current device, looked up once per device
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(RYD_ERR_DEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(RYD_ERR_DEVICE, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RYD_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ryd_cheb_moments_noalloc(const ryd_plan* P, int dtype, const uint32_t* probe, void* t0,
+                             void* t1, void* t2, int64_t nsteps, double c, double r, double* sums,
+                             void* stream) {
+  if (!P || !probe || !t0 || !t1 || !t2 || !sums)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_cheb_moments_noalloc");
+  if (P->ws_bytes > SCRATCH_BYTES)
+    return fail(RYD_ERR_INVALID, "ryd_cheb_moments_noalloc: the plan's reductions need " +
+                                     std::to_string(P->ws_bytes) + " bytes of scratch, more than the " +
+                                     std::to_string(SCRATCH_BYTES) + " held for allocation-free calls");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != P->device)
+    return fail(RYD_ERR_INVALID, "ryd_cheb_moments_noalloc: the plan's device must be current");
+  void* ws = nullptr;
+  if (int rc = scratch_on_current_device(&ws)) return rc;
+  return ryd_cheb_moments(P, dtype, probe, t0, t1, t2, nsteps, c, r, sums, ws, stream);
 }
 
 }  // extern "C"

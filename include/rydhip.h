@@ -181,6 +181,21 @@ int ryd_cheb_moments(const ryd_plan* plan, int dtype, const uint32_t* probe, voi
                      void* t1, void* t2, int64_t nsteps, double c, double r, double* sums,
                      void* ws, void* stream);
 
+/* Allocation-free form of ryd_cheb_moments (same arguments without the
+ * workspace, same results bit for bit): the reduction partials live in a
+ * device array the library holds for the life of the process, 8 MiB per GPU
+ * (plans whose ryd_workspace_bytes exceeds that, e.g. a full basis beyond
+ * n = 31, are refused with RYD_ERR_INVALID).  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  It calls no allocating routine.  The plan's device must be
+ * current.  All calls on one GPU share that GPU's array: calls that overlap
+ * on one GPU race.  sums accumulate as in ryd_cheb_moments (the caller
+ * clears them). */
+int ryd_cheb_moments_noalloc(const ryd_plan* plan, int dtype, const uint32_t* probe, void* t0,
+                             void* t1, void* t2, int64_t nsteps, double c, double r,
+                             double* sums, void* stream);
+
 /* out[i] = Σ_s |x_s|^2 b_i(s) for every site i, out[n] = Σ_s |x_s|^2 (device FP64,
  * n+1 entries; divide by out[n] for <n_i>).  Replaces _gs_observables' n_atom
  * (eq_route.py:108-122) and the bits @ |psi|^2 reduction of
