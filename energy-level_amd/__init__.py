@@ -20,7 +20,8 @@ from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolu
 from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip, \
     batched_ground_state_dense_hip
 from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip, \
-    spectrum_lanczos_hip, quadrature, lorentz_spectrum
+    spectrum_lanczos_hip, quadrature, lorentz_spectrum, source_spectrum_hip, resolve_source
+from .results import describe_result
 from .observables import normalize_pauli_strings, PauliPlanHip, measure_pauli_strings_hip, \
     corr_matrix_hip, site_correlation_sums, loop_observables, link_loop_observables
 from .manybody import evolve_manybody_hip, state_vector_np, product_label
@@ -36,7 +37,8 @@ __all__ = ['hamiltonian_linear_operator_hip', 'loop_observables', 'link_loop_obs
            'cheb_coefficients', 'terms_for', 'PulseProblem', 'PulseEvolution', 'z2_index',
            'slice_bounds', 'BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip',
            'EQ_BACKENDS_HIP', 'normalize_backend', 'solve_equilibrium_hip', 'spectrum_lanczos_hip',
-           'quadrature', 'lorentz_spectrum', 'segment_intervals', 'segment_drive',
+           'quadrature', 'lorentz_spectrum', 'source_spectrum_hip', 'resolve_source',
+           'describe_result', 'segment_intervals', 'segment_drive',
            'propagate_segments_hip', 'number_sector_basis', 'check_number_conserving',
            'scan_ground_sectors_hip', 'validate_edges', 'graph_plan', 'independent_states_hip',
            'independent_set_count_hip', 'blockade_edges', 'build_plan_hip',

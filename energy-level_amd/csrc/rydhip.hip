@@ -24,6 +24,7 @@
 #include "evolve.hpp"
 #include "batch.hpp"
 #include "tridiag.hpp"
+#include "reortho.hpp"
 
 using namespace ryd;
 
@@ -666,6 +667,74 @@ int do_cheb(const ryd_plan* P, int dtype, const uint32_t* probe, void* t0, void*
       rc = do_apply<V>(P, dtype, B[(k - 1) % 3], B[(k - 2) % 3], B[k % 3], 1.0, 2.0 / r,
                        -2.0 * c / r, -1.0, probe, sums + 4 * k, 1, ws, st);
     if (rc) return rc;
+  }
+  return RYD_OK;
+}
+
+// Lanczos with two classical Gram-Schmidt passes against every stored row
+// (reortho.hpp); basis rows are unit vectors, row 0 is phi·phi_scale on exit.
+template <class V>
+int do_lanczos_reortho(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, void* w_v,
+                       int64_t k, double phi_scale, double* alpha, double* nrm, void* state,
+                       double tiny, int passes, double* gs_ws, void* ws, hipStream_t st) {
+  V* basis = static_cast<V*>(basis_v);
+  V* w = static_cast<V*>(w_v);
+  LzState* S = static_cast<LzState*>(state);
+  const int64_t D = P->dim;
+  const unsigned sg = stream_grid(D);
+  const int64_t nchunk = (D + GS_CHUNK - 1) / GS_CHUNK;
+  double* partials = gs_ws;
+  double* coef = gs_ws + (size_t)2 * k * nchunk;
+  k_lz_init<<<1, 1, 0, st>>>(S, tiny);
+  LAUNCH_CHECK();
+  k_axpby<V><<<sg, 256, 0, st>>>(nullptr, basis, D, 0.0, phi_scale);
+  LAUNCH_CHECK();
+  for (int64_t j = 0; j < k; ++j) {
+    V* qj = basis + j * ld;
+    ApplyIO<V> io;
+    io.x = qj;
+    io.z = nullptr;
+    io.y = w;
+    io.probe = nullptr;
+    io.smode = SM_CONST;
+    io.s = 1.0; io.ch = 1.0; io.cx = 0.0; io.cz = 0.0;
+    io.nrm = nullptr;
+    io.j = 0;
+    io.stop = &S->stop;
+    io.reduce = 1;
+    Reduce ra = make_reduce(ws, POST_LZ_ALPHA, nullptr);
+    ra.alpha = alpha;
+    ra.st = S;
+    ra.j = (int)j;
+    if (int rc = launch_apply<V>(P, dtype, io, ra, st)) return rc;
+    k_rl_three_term<V><<<sg, 256, 0, st>>>(w, qj, j > 0 ? qj - ld : qj, D, alpha, nrm, (int)j,
+                                           &S->stop);
+    LAUNCH_CHECK();
+    const int rows = (int)(j + 1);
+    for (int pass = 0; pass < passes; ++pass) {
+      k_gs_dots<V><<<(unsigned)nchunk, 256, 0, st>>>(basis, ld, D, rows, w, partials, &S->stop);
+      LAUNCH_CHECK();
+      k_gs_finish<<<(unsigned)rows, 256, 0, st>>>(partials, nchunk, coef, &S->stop);
+      LAUNCH_CHECK();
+      for (int r0 = 0; r0 < rows; r0 += GS_MAX_ROWS) {
+        const int cnt = std::min(GS_MAX_ROWS, rows - r0);
+        k_gs_axpy<V><<<sg, 256, 0, st>>>(basis + (int64_t)r0 * ld, ld, D, cnt, coef + 2 * r0, w,
+                                         &S->stop);
+        LAUNCH_CHECK();
+      }
+    }
+    Reduce rb = make_reduce(ws, POST_LZ_NORM, nullptr);
+    rb.nrm = nrm;
+    rb.st = S;
+    rb.j = (int)j;
+    k_dots<V><<<sg, 256, 0, st>>>(nullptr, w, D, rb);
+    LAUNCH_CHECK();
+    k_reduce_finish<<<1, 256, 0, st>>>(rb, sg);
+    LAUNCH_CHECK();
+    if (j + 1 < k) {
+      k_rl_scale<V><<<sg, 256, 0, st>>>(qj + ld, w, D, nrm, (int)j, &S->stop);
+      LAUNCH_CHECK();
+    }
   }
   return RYD_OK;
 }
@@ -1449,6 +1518,28 @@ int ryd_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int6
   hipStream_t st = static_cast<hipStream_t>(stream);
   DISPATCH(dtype, do_lanczos, P, dtype, b0, b1, b2, j0, steps, alpha, nrm, state, tiny, ritz0,
            ritz1, y0, y1, replay, ws, st);
+}
+
+size_t ryd_reortho_workspace_bytes(int64_t dim, int64_t k) {
+  if (dim < 1 || k < 1) return 0;
+  const int64_t nchunk = (dim + GS_CHUNK - 1) / GS_CHUNK;
+  return sizeof(double) * (size_t)(2 * k * nchunk + 2 * k);
+}
+
+int ryd_lanczos_reortho(const ryd_plan* P, int dtype, void* basis, int64_t ld, void* w, int64_t k,
+                        double phi_scale, double* alpha, double* nrm, void* state, double tiny,
+                        int passes, void* gs_ws, void* ws, void* stream) {
+  if (!P || !basis || !w || !alpha || !nrm || !state || !gs_ws || !ws)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_lanczos_reortho");
+  if (k < 1 || k > P->dim) return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho: need 1 <= k <= dim");
+  if (ld < P->dim) return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho: row stride below dim");
+  if (passes < 0 || passes > 4) return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho: passes in 0..4");
+  if (!(phi_scale > 0.0) || !std::isfinite(phi_scale))
+    return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho: phi_scale must be finite and > 0");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_lanczos_reortho, P, dtype, basis, ld, w, k, phi_scale, alpha, nrm, state,
+           tiny, passes, static_cast<double*>(gs_ws), ws, st);
 }
 
 int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream) {

@@ -388,4 +388,10 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
         observables=obs)
     if record_psi:
         result['psi_amp'] = out['psi']
-    return result
+    if loops is not None:
+        result['loops'] = [list(loop) for loop in loops]
+    # data descriptors as time_result attaches them (results.py:273-287,
+    # engine_ed.py:153): the loss is a physical loss under gamma, else the
+    # numerical norm drift
+    from .results import describe_result
+    return describe_result(result, calib, loss_kind='norm_drift' if hermitian else 'physical_loss')

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 5
+#define RYD_ABI_VERSION 6
 
 enum ryd_status {
   RYD_OK = 0,
@@ -152,6 +152,24 @@ int ryd_lanczos(const ryd_plan* plan, int dtype, void* b0, void* b1, void* b2, i
                 void* ritz0, void* ritz1, const double* y0, const double* y1, int replay,
                 void* ws, void* stream);
 size_t ryd_lanczos_state_bytes(void);
+/* Lanczos from a source vector with full reorthogonalisation, the
+ * recurrence of spectrum.lanczos with reortho=True
+ * (src/core/manybody/spectrum.py:34-103): on entry row 0 of `basis` (k rows,
+ * row stride ld >= dim, the plan's element type) holds phi; it is scaled by
+ * phi_scale (= 1/|phi|) in place and rows 1..k-1 receive the unit Lanczos
+ * vectors.  Step j: w = H q_j, alpha[j] = Re<q_j, w>, w -= alpha_j q_j +
+ * beta_{j-1} q_{j-1}, then `passes` classical Gram-Schmidt sweeps of w
+ * against rows 0..j (the reference sweeps twice), nrm[j+1] = beta_j = |w|,
+ * q_{j+1} = w / beta_j.  The run stops when beta_j <= tiny (absolute; the
+ * rows are unit vectors) and `state` (ryd_lanczos_state_bytes) records the
+ * steps, read with ryd_lanczos_status.  w: one vector (the last step's
+ * residual on return); gs_ws: ryd_reortho_workspace_bytes(dim, k) bytes of
+ * device memory; ws: the plan's workspace.  No host synchronisation. */
+size_t ryd_reortho_workspace_bytes(int64_t dim, int64_t k);
+int ryd_lanczos_reortho(const ryd_plan* plan, int dtype, void* basis, int64_t ld, void* w,
+                        int64_t k, double phi_scale, double* alpha, double* nrm, void* state,
+                        double tiny, int passes, void* gs_ws, void* ws, void* stream);
+
 /* Reads the state written by ryd_lanczos (synchronises the stream). */
 int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream);
 

@@ -98,21 +98,3 @@ def test_hip_kpm_thermal_and_dos_vs_oracle(pkg, dev, want):
         assert out['energies'].shape == (400,)
         rho = orc.dos_from_moments(mu, c, r, out['energies'], num)
         assert np.allclose(out['rho'], rho, rtol=1e-8, atol=1e-10)
-
-
-def test_spectrum_lanczos_matches_reference_recurrence(pkg, dev):
-    import torch
-    n = 9
-    sysd = _system(pkg, n, dev=dev)
-    P = pkg.Plan(pkg.from_calib(sysd['calib'], sysd['Om'], sysd['delta']), device=dev)
-    H = orc.build_H_sparse(n, sysd['Om'], sysd['delta'], sysd['calib']['pairs'], ['I1'])
-    phi = np.random.default_rng(7).normal(size=1 << n)
-    out = pkg.spectrum_lanczos_hip(P, torch.as_tensor(phi, device=dev), n_krylov=40)
-    a, b, norm2 = orc.lanczos(lambda v: H @ v, phi, 40)
-    assert out['n'] == 40 and np.isclose(out['norm2'], norm2)
-    # early coefficients agree before loss of orthogonality separates the runs
-    assert np.allclose(out['alphas'][:15], a[:15], rtol=1e-9, atol=1e-9)
-    assert np.allclose(out['betas'][:15], b[:15], rtol=1e-9, atol=1e-9)
-    nodes, w = pkg.quadrature(out['alphas'], out['betas'], out['norm2'])
-    assert np.isclose(w.sum(), norm2)
-    assert nodes[0] >= orc.ground_state_dense(H)['E0'] - 1e-9

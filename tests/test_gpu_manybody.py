@@ -61,6 +61,10 @@ def test_evolve_manybody_full_basis(pkg, dev):
     assert out['basis'] == dict(kind='full', bit_order='site0-msb', n_atoms=n, dim=1 << n,
                                 full_dim=1 << n)
     assert out['error_report']['max_norm_dev'] < 1e-9 and out['diag']['dim'] == 1 << n
+    q = out['data_schema']['quantities']
+    assert q['pops']['quantity'] == 'basis_probability' and q['pops']['axes'] == ['time', 'basis']
+    assert q['loss']['alias_of'] == 'norm_drift' and np.array_equal(out['norm_drift'], out['loss'])
+    assert q['observables.sx']['axes'] == ['time', 'site'] and len(out['sites']) == n
 
 
 @pytest.mark.parametrize('via', ['edges', 'states'])
@@ -149,6 +153,8 @@ def test_evolve_manybody_lossy_full_basis(pkg, dev):
     assert np.allclose(out['loss'], 1 - out['norms'])
     assert out['backend'] == 'hip-ed-nonhermitian' and out['error_report']['hermitian'] is False
     assert out['error_report']['max_norm_dev'] == 0.0 and out['diag']['hermitian'] is False
+    assert out['data_schema']['quantities']['loss']['alias_of'] == 'physical_loss'
+    assert np.array_equal(out['physical_loss'], out['loss'])
     # a scalar rate applies to every atom
     o2 = pkg.evolve_manybody_hip(calib, psi0='grgrgggg', t_grid=t, segments=segs, gamma=2.0,
                                  device=dev, want_observables=False)
