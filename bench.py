@@ -147,10 +147,10 @@ def main():
                     help='time one element type only (profiling runs)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run only a plain device copy of one FP64 N-site vector (counter scale)')
-    ap.add_argument('--configs', default='c1,c2,c3,c4,c5,g1,k1',
+    ap.add_argument('--configs', default='c1,c2,c3,c4,c5,c5r,g1,k1',
                     help='comma list of the configurations to run after the headline: the '
-                         'five workloads, g1 (blockade-graph lattice) and k1 (kagome XY, J '
-                         'channels) ("" for none)')
+                         'five workloads, c5r (C5 timed in the reference units), g1 (blockade-graph '
+                         'lattice) and k1 (kagome XY, J channels) ("" for none)')
     ap.add_argument('--graph-lattice', default=None, metavar='WxH[d]',
                     help='time only the blockade-graph H·v on a w x h square lattice '
                          '("6x7"; suffix d adds diagonal edges) and print its summary')

@@ -7,6 +7,7 @@
 // One launch computes, for x~ = s·x,
 //   y    = ch·(om·F x~ + D x~ + dadd·N x~) + cx·x~ + cz·z + cw·w
 //   acc += ca·y                                   (complex cw, ca; optional)
+//   acc  = c0·x~ + ca·y                           (acc_init: the first term)
 //   sums = {Re<q, F x~>, Re<q, N x~>, |y|^2, Re<q, y>}   (optional q)
 // Forward slice: t_{j+1} = 2H~t_j − t_{j−1}, ψ_out = Σ a_j t_j (acc).
 // Adjoint slice: b_j = conj(a_j) g + 2H~ b_{j+1} − b_{j+2} with q = t_j gives
@@ -29,6 +30,8 @@ struct EvoIO {
   const V* q;
   double om, dadd, s, ch, cx, cz;
   double cw_re, cw_im, ca_re, ca_im;
+  double c0_re, c0_im;      // acc_init: acc starts from c0·x~ instead of its old value
+  int acc_init;
   int reduce;
 };
 
@@ -45,7 +48,13 @@ __device__ __forceinline__ void evo_finish(const EvoIO<V>& io, uint64_t g, V fl,
   if (io.w) vcfma(v, (RT)io.cw_re, (RT)io.cw_im, io.w[g]);
   io.y[g] = v;
   if (io.acc) {
-    V a = io.acc[g];
+    V a;
+    if (io.acc_init) {
+      a = vzero(xe);
+      vcfma(a, (RT)io.c0_re, (RT)io.c0_im, xe);
+    } else {
+      a = io.acc[g];
+    }
     vcfma(a, (RT)io.ca_re, (RT)io.ca_im, v);
     io.acc[g] = a;
   }
@@ -251,6 +260,183 @@ __global__ __launch_bounds__(256) void k_evo_small(FullModel M, EvoIO<V> io, Red
     evo_finish(io, g, fl, xe, d, (double)__popcll(g), s0, s1, s2, s3);
   }
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// Small complex64 vectors (n in [11, 21]): one 2^11-code tile per
+// 1024-thread block.  A thread owns 2 consecutive codes (bit 0 in
+// registers), bits 1–10 go through LDS, and the far partners (bits 11..n−1,
+// at most EVO_WIDE_FAR) are requested together with the thread's own codes,
+// before the block's barrier, and folded into the flip sum at once; the z,
+// w, q and acc operands are requested right after the first barrier, so
+// their latency hides behind the LDS phase.  A launch then waits on about
+// two memory round trips instead of one per far bit (the 256-thread tiles
+// spread over several blocks also fetch bits 10–11 from other blocks).
+// Tile map XCD-aware as in k_evo_tiled.
+constexpr int EVO_WIDE_TB = 11;
+constexpr int EVO_WIDE_FAR = 10;
+constexpr int EVO_WIDE_MAXN = EVO_WIDE_TB + EVO_WIDE_FAR;
+
+template <class V> struct alignas(2 * sizeof(V)) V2 { V v[2]; };
+
+template <class V>
+__global__ __launch_bounds__(1024) void k_evo_wide(FullModel M, EvoIO<V> io, Reduce rd) {
+  using RT = typename VT<V>::R;
+  constexpr int TB = EVO_WIDE_TB;
+  __shared__ V2<V> lds[1024];
+  __shared__ double Fx[TB];
+  __shared__ double T0[256];
+  __shared__ double T1[1 << (TB - 8)];
+  const int t = threadIdx.x;
+  const int nhi = M.n - TB;
+  const uint64_t nblk = 1ull << nhi;
+  const uint64_t b = blockIdx.x;
+  const uint64_t tile = nblk >= 8 ? ((b & 7ull) * (nblk >> 3) + (b >> 3)) : b;
+  const uint64_t base = tile << TB;
+  const uint64_t g0 = base + (uint64_t)t * 2;
+  const RT sr = (RT)io.s;
+  const V2<V> own = *reinterpret_cast<const V2<V>*>(io.x + g0);
+  // far partners: bit TB+k of the tile flipped; beyond nhi the load rereads
+  // the thread's own codes (in bounds) and its coefficient is zero
+  V2<V> far[EVO_WIDE_FAR];
+#pragma unroll
+  for (int k = 0; k < EVO_WIDE_FAR; ++k) {
+    const uint64_t src = k < nhi ? (((tile ^ (1ull << k)) << TB) + (uint64_t)t * 2) : g0;
+    far[k] = *reinterpret_cast<const V2<V>*>(io.x + src);
+  }
+  V xs[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) xs[c] = vscale(own.v[c], sr);
+  {
+    V2<V> wv;
+    wv.v[0] = xs[0];
+    wv.v[1] = xs[1];
+    lds[t] = wv;
+  }
+  if (t < TB) {
+    double f = 0.0;
+    const double* row = M.vb + (size_t)t * M.n;
+    for (int q = TB; q < M.n; ++q)
+      if ((tile >> (q - TB)) & 1ull) f += row[q];
+    Fx[t] = f;
+  }
+  V fl[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) fl[c] = vzero(xs[c]);
+#pragma unroll
+  for (int k = 0; k < EVO_WIDE_FAR; ++k) {
+    const int p = k < nhi ? TB + k : 0;
+    const int bit = (int)((tile >> k) & 1ull);
+    const double on = k < nhi ? io.s : 0.0;
+    const RT ar = (RT)(M.ar[p] * on);
+    const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * on);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) vcfma(fl[c], ar, aim, far[k].v[c]);
+  }
+  // bit 0 in registers
+  {
+    const RT ar = (RT)M.ar[0], ai = (RT)M.ai[0];
+    vcfma(fl[0], ar, -ai, xs[1]);
+    vcfma(fl[1], ar, ai, xs[0]);
+  }
+  __syncthreads();
+  V2<V> zv, wv, qv, av;
+  if (io.z) zv = *reinterpret_cast<const V2<V>*>(io.z + g0);
+  if (io.w) wv = *reinterpret_cast<const V2<V>*>(io.w + g0);
+  if (io.q) qv = *reinterpret_cast<const V2<V>*>(io.q + g0);
+  if (io.acc && !io.acc_init) av = *reinterpret_cast<const V2<V>*>(io.acc + g0);
+  if (t < 256) {
+    double v = 0.0;
+#pragma unroll
+    for (int p = 0; p < 8; ++p)
+      if ((t >> p) & 1) v += Fx[p];
+    T0[t] = v;
+  }
+  if (t < (1 << (TB - 8))) {
+    double w = 0.0;
+#pragma unroll
+    for (int p = 8; p < TB; ++p)
+      if ((t >> (p - 8)) & 1) w += Fx[p];
+    T1[t] = w;
+  }
+  // bits 1–10 through LDS
+#pragma unroll
+  for (int k = 0; k < TB - 1; ++k) {
+    const int p = 1 + k;
+    const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
+    const RT aim = ((t >> k) & 1) ? ai : -ai;
+    const V2<V> pv = lds[t ^ (1 << k)];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) vcfma(fl[c], ar, aim, pv.v[c]);
+  }
+  __syncthreads();
+  const double eh = M.ehi[tile];
+  const int pop_hi = __popcll(tile);
+  const int e0 = t * 2;
+  const double el0 = M.elo[e0], el1 = M.elo[e0 + 1];
+  const double t1 = T1[e0 >> 8];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  V2<V> yv, ov;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int e = e0 + c;
+    const double d = eh + (c ? el1 : el0) + T0[e & 255] + t1;
+    const double pop = (double)(pop_hi + __popc((unsigned)e));
+    V hx = vscale(fl[c], (RT)io.om);
+    vfma(hx, (RT)(d + io.dadd * pop), xs[c]);
+    V v = vscale(hx, (RT)io.ch);
+    vfma(v, (RT)io.cx, xs[c]);
+    if (io.z) vfma(v, (RT)io.cz, zv.v[c]);
+    if (io.w) vcfma(v, (RT)io.cw_re, (RT)io.cw_im, wv.v[c]);
+    yv.v[c] = v;
+    if (io.acc) {
+      V a;
+      if (io.acc_init) {
+        a = vzero(v);
+        vcfma(a, (RT)io.c0_re, (RT)io.c0_im, xs[c]);
+      } else {
+        a = av.v[c];
+      }
+      vcfma(a, (RT)io.ca_re, (RT)io.ca_im, v);
+      ov.v[c] = a;
+    }
+    if (io.reduce) {
+      s2 += nrm2(v);
+      if (io.q) {
+        s0 += rdot(qv.v[c], fl[c]);
+        s1 += pop * rdot(qv.v[c], xs[c]);
+        s3 += rdot(qv.v[c], v);
+      }
+    }
+  }
+  *reinterpret_cast<V2<V>*>(io.y + g0) = yv;
+  if (io.acc) *reinterpret_cast<V2<V>*>(io.acc + g0) = ov;
+  if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
+}
+
+// Fixed-order sums of per-term block partials: block j sums the nblocks
+// partials of term j (partials + j·nblocks·NRED) into out[j·NRED ..].
+__global__ __launch_bounds__(256) void k_reduce_terms(const double* __restrict__ partials,
+                                                      unsigned nblocks, double* __restrict__ out) {
+  __shared__ double red_lds[4 * NRED];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double* p = partials + (size_t)blockIdx.x * nblocks * NRED;
+  double s[NRED] = {0.0, 0.0, 0.0, 0.0};
+  for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < NRED; ++k) s[k] += p[(size_t)b * NRED + k];
+  }
+#pragma unroll
+  for (int k = 0; k < NRED; ++k) s[k] = wave_sum(s[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NRED; ++k) red_lds[wid * NRED + k] = s[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < NRED) {
+    const int k = threadIdx.x;
+    out[(size_t)blockIdx.x * NRED + k] =
+        red_lds[k] + red_lds[NRED + k] + red_lds[2 * NRED + k] + red_lds[3 * NRED + k];
+  }
 }
 
 // y = a·x (complex a; y may alias nothing)

@@ -1891,12 +1891,29 @@ extern "C" int ryd_sector_rank(const ryd_plan* P, const int64_t* codes, int64_t 
 
 namespace {
 
+// Launch one fused evolution term; *grid_out receives the block count (the
+// number of reduction partials when io.reduce).  The caller finishes the
+// reduction.
 template <class V>
-int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t st) {
+int launch_evo_term(const ryd_plan* P, EvoIO<V> io, Reduce rd, hipStream_t st, unsigned* grid_out) {
   constexpr int TB = tile_bits<V>();
-  FullModel M = full_model(P, TB);
   unsigned grid;
+  if constexpr (sizeof(V) == 8) {
+    if (P->n >= EVO_WIDE_TB && P->n <= EVO_WIDE_MAXN) {
+      // complex64 up to 2^21 codes: one 2048-code tile per 1024-thread
+      // block, every far partner requested at once
+      FullModel M = full_model(P, EVO_WIDE_TB);
+      if (M.tb != EVO_WIDE_TB || !M.elo)
+        return fail(RYD_ERR_DEVICE, "evolution tables: unexpected tile width");
+      grid = (unsigned)(1ull << (P->n - EVO_WIDE_TB));
+      k_evo_wide<V><<<dim3(grid), dim3(1024), 0, st>>>(M, io, rd);
+      LAUNCH_CHECK();
+      *grid_out = grid;
+      return RYD_OK;
+    }
+  }
   if (P->n >= TB) {
+    FullModel M = full_model(P, TB);
     // below ~1000 tiles one tile per block leaves most CUs idle: spread each
     // tile over tile_r blocks
     constexpr int R = tile_r<V>();
@@ -1908,13 +1925,36 @@ int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t
       k_evo_tiled<V, R><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
     }
   } else {
+    FullModel M = full_model(P, TB);
     grid = (unsigned)(((1ull << P->n) + 255) / 256);
     k_evo_small<V><<<dim3(grid), dim3(256), 0, st>>>(M, io, rd);
   }
   LAUNCH_CHECK();
+  *grid_out = grid;
+  return RYD_OK;
+}
+
+template <class V>
+int launch_evo(const ryd_plan* P, int dtype, EvoIO<V> io, Reduce rd, hipStream_t st) {
+  (void)dtype;
+  unsigned grid = 0;
+  if (int rc = launch_evo_term<V>(P, io, rd, st, &grid)) return rc;
   if (io.reduce) k_reduce_finish<<<1, 256, 0, st>>>(rd, grid);
   LAUNCH_CHECK();
   return RYD_OK;
+}
+
+// Blocks of one evolution term (the reduction partials it writes).
+template <class V>
+unsigned evo_term_blocks(const ryd_plan* P) {
+  constexpr int TB = tile_bits<V>();
+  if (sizeof(V) == 8 && P->n >= EVO_WIDE_TB && P->n <= EVO_WIDE_MAXN)
+    return (unsigned)(1ull << (P->n - EVO_WIDE_TB));
+  if (P->n >= TB) {
+    const unsigned grid = (unsigned)(1ull << (P->n - TB));
+    return grid < 1024 ? grid * tile_r<V>() : grid;
+  }
+  return (unsigned)(((1ull << P->n) + 255) / 256);
 }
 
 template <class V>
@@ -1928,31 +1968,58 @@ EvoIO<V> evo_io(double om, double dadd) {
   io.ch = 1.0;
   io.cx = io.cz = 0.0;
   io.cw_re = io.cw_im = io.ca_re = io.ca_im = 0.0;
+  io.c0_re = io.c0_im = 0.0;
+  io.acc_init = 0;
   io.reduce = 0;
   return io;
 }
 
+// Forward slices with the fused kernel.  Slice k: t_0 = ψ_k, t_{j+1} =
+// (1|2)·H~ t_j − t_{j−1}, ψ_{k+1} = Σ_j a_j t_j accumulated by the term
+// launches themselves (the first term initialises it), so a slice is K
+// launches and ψ alternates between the caller's buffer and one work vector
+// (one copy at the end when the result sits in the work vector).
+// tstore != null keeps t_0..t_{K−1} of slice k at tstore + k·K·D (the
+// adjoint's operands) and ψ_k itself in that slot's t_0; work then needs one
+// vector (t_K), else four (three t's and the ψ ping-pong).
 template <class V>
 int do_evolve_forward(const ryd_plan* P, int dtype, void* psi_v, int64_t nslices,
                       const double* om, const double* dadd, const double* c, const double* r,
-                      int K, const double* coef, void* work_v, hipStream_t st) {
+                      int K, const double* coef, void* work_v, void* tstore_v, hipStream_t st) {
   V* psi = static_cast<V*>(psi_v);
   const int64_t D = P->dim;
   V* W = static_cast<V*>(work_v);
-  V* T[3] = {W, W + D, W + 2 * D};
-  V* acc = W + 3 * D;
+  V* TS = static_cast<V*>(tstore_v);
   Reduce none = make_reduce(nullptr, POST_NONE, nullptr);
+  if (nslices <= 0) return RYD_OK;
+  V* cur = psi;
+  if (TS) {
+    HIP_TRY(hipMemcpyAsync(TS, psi, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+    cur = TS;
+  }
+  V* pong = W + 3 * D;
   for (int64_t k = 0; k < nslices; ++k) {
     const double* a = coef + (size_t)k * (K + 1) * 2;
     const double inv = 1.0 / r[k];
-    k_cscale<V><<<stream_grid(D), 256, 0, st>>>(psi, acc, D, a[0], a[1]);
-    LAUNCH_CHECK();
+    V* slot = TS ? TS + (size_t)k * K * D : nullptr;
+    V* nxt;
+    if (TS)
+      nxt = k + 1 < nslices ? TS + (size_t)(k + 1) * K * D : psi;
+    else
+      nxt = cur == psi ? pong : psi;
+    auto tvec = [&](int j) -> V* {       // t_j, j >= 1
+      if (TS) return j < K ? slot + (size_t)j * D : W;
+      return W + (size_t)((j - 1) % 3) * D;
+    };
     for (int j = 0; j < K; ++j) {   // produce t_{j+1}
       EvoIO<V> io = evo_io<V>(om[k], dadd[k]);
-      io.x = j == 0 ? psi : T[(j - 1) % 3];
-      io.z = j == 0 ? nullptr : (j == 1 ? psi : T[(j - 2) % 3]);
-      io.y = T[j % 3];
-      io.acc = acc;
+      io.x = j == 0 ? cur : tvec(j);
+      io.z = j == 0 ? nullptr : (j == 1 ? cur : tvec(j - 1));
+      io.y = tvec(j + 1);
+      io.acc = nxt;
+      io.acc_init = j == 0;
+      io.c0_re = a[0];
+      io.c0_im = a[1];
       io.ch = (j == 0 ? 1.0 : 2.0) * inv;
       io.cx = -(j == 0 ? 1.0 : 2.0) * c[k] * inv;
       io.cz = j == 0 ? 0.0 : -1.0;
@@ -1960,8 +2027,57 @@ int do_evolve_forward(const ryd_plan* P, int dtype, void* psi_v, int64_t nslices
       io.ca_im = a[2 * (j + 1) + 1];
       if (int rc = launch_evo<V>(P, dtype, io, none, st)) return rc;
     }
-    HIP_TRY(hipMemcpyAsync(psi, acc, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+    cur = nxt;
   }
+  if (cur != psi) HIP_TRY(hipMemcpyAsync(psi, cur, sizeof(V) * D, hipMemcpyDeviceToDevice, st));
+  return RYD_OK;
+}
+
+// Adjoint of nslices stored slices (t_0..t_{K−1} of slice k at tstore +
+// k·K·D, from do_evolve_forward), last slice first: Clenshaw b_K = conj(a_K)
+// g, b_j = conj(a_j) g + (2|1)·H~ b_{j+1} − b_{j+2}, b_0 → g (the cotangent
+// of ψ_k), with the gradient dots of every term written as block partials
+// and summed per slice by one launch into sums + k·K·4 (device memory).
+// work: 3 vectors; partials: K·evo_term_blocks doubles × 4.
+template <class V>
+int do_evolve_adjoint_stored(const ryd_plan* P, int dtype, void* g_v, int64_t nslices,
+                             const double* om, const double* dadd, const double* c,
+                             const double* r, int K, const double* coef, const void* tstore_v,
+                             void* work_v, double* sums, double* partials, hipStream_t st) {
+  V* g = static_cast<V*>(g_v);
+  const int64_t D = P->dim;
+  const V* TS = static_cast<const V*>(tstore_v);
+  V* W = static_cast<V*>(work_v);
+  V* B[3] = {W, W + D, W + 2 * D};
+  const unsigned nb = evo_term_blocks<V>(P);
+  for (int64_t k = nslices - 1; k >= 0; --k) {
+    const double* a = coef + (size_t)k * (K + 1) * 2;
+    const double inv = 1.0 / r[k];
+    const V* slot = TS + (size_t)k * K * D;
+    k_cscale<V><<<stream_grid(D), 256, 0, st>>>(g, B[K % 3], D, a[2 * K], -a[2 * K + 1]);
+    LAUNCH_CHECK();
+    for (int j = K - 1; j >= 0; --j) {
+      EvoIO<V> io = evo_io<V>(om[k], dadd[k]);
+      io.x = B[(j + 1) % 3];
+      io.z = j + 2 <= K ? B[(j + 2) % 3] : nullptr;
+      io.w = g;
+      io.y = j == 0 ? g : B[j % 3];
+      io.q = slot + (size_t)j * D;
+      io.ch = (j == 0 ? 1.0 : 2.0) * inv;
+      io.cx = -(j == 0 ? 1.0 : 2.0) * c[k] * inv;
+      io.cz = io.z ? -1.0 : 0.0;
+      io.cw_re = a[2 * j];
+      io.cw_im = -a[2 * j + 1];
+      io.reduce = 1;
+      Reduce rd = make_reduce(nullptr, POST_NONE, nullptr);
+      rd.partials = partials + (size_t)j * nb * NRED;
+      unsigned grid = 0;
+      if (int rc = launch_evo_term<V>(P, io, rd, st, &grid)) return rc;
+    }
+    k_reduce_terms<<<K, 256, 0, st>>>(partials, nb, sums + (size_t)k * K * NRED);
+    LAUNCH_CHECK();
+  }
+  (void)dtype;
   return RYD_OK;
 }
 
@@ -2139,8 +2255,55 @@ extern "C" int ryd_evolve_forward(const ryd_plan* P, int dtype, void* psi, int64
   }
   if (int rc = check_evo(P, dtype, K)) return rc;
   if (dtype == RYD_C64)
-    return do_evolve_forward<c32>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
-  return do_evolve_forward<c64>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, st);
+    return do_evolve_forward<c32>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, nullptr, st);
+  return do_evolve_forward<c64>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, nullptr, st);
+}
+
+extern "C" int ryd_evolve_forward_store(const ryd_plan* P, int dtype, void* psi, int64_t nslices,
+                                        const double* om, const double* dadd, const double* c,
+                                        const double* r, int K, const double* coef, void* tstore,
+                                        void* work, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
+  if (!P || !psi || !om || !dadd || !c || !r || !coef || !tstore || !work || nslices < 0)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_forward_store");
+  if (int rc = check_evo(P, dtype, K)) return rc;
+  for (int64_t k = 0; k < nslices; ++k)
+    if (!(r[k] > 0.0)) return fail(RYD_ERR_INVALID, "spectral radius r must be > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == RYD_C64)
+    return do_evolve_forward<c32>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, tstore, st);
+  return do_evolve_forward<c64>(P, dtype, psi, nslices, om, dadd, c, r, K, coef, work, tstore, st);
+}
+
+extern "C" int64_t ryd_evolve_partials_bytes(const ryd_plan* P, int dtype, int K) {
+  if (!P || P->basis != RYD_BASIS_FULL || P->generic || P->n >= SPLIT_MIN_N || K < 1 || K > 4096)
+    return -1;
+  const unsigned nb = dtype == RYD_C64 ? evo_term_blocks<c32>(P)
+                                       : (dtype == RYD_C128 ? evo_term_blocks<c64>(P) : 0u);
+  if (!nb) return -1;
+  return (int64_t)K * nb * NRED * (int64_t)sizeof(double);
+}
+
+extern "C" int ryd_evolve_adjoint_stored(const ryd_plan* P, int dtype, void* g, int64_t nslices,
+                                         const double* om, const double* dadd, const double* c,
+                                         const double* r, int K, const double* coef,
+                                         const void* tstore, void* work, double* sums,
+                                         double* partials, void* stream) {
+  if (P && P->basis == RYD_BASIS_CSR)
+    return fail(RYD_ERR_UNSUPPORTED, "a sparse-matrix plan has no site codes");
+  if (!P || !g || !om || !dadd || !c || !r || !coef || !tstore || !work || !sums || !partials ||
+      nslices < 0)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_evolve_adjoint_stored");
+  if (int rc = check_evo(P, dtype, K)) return rc;
+  for (int64_t k = 0; k < nslices; ++k)
+    if (!(r[k] > 0.0)) return fail(RYD_ERR_INVALID, "spectral radius r must be > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == RYD_C64)
+    return do_evolve_adjoint_stored<c32>(P, dtype, g, nslices, om, dadd, c, r, K, coef, tstore,
+                                         work, sums, partials, st);
+  return do_evolve_adjoint_stored<c64>(P, dtype, g, nslices, om, dadd, c, r, K, coef, tstore,
+                                       work, sums, partials, st);
 }
 
 extern "C" int ryd_evolve_taylor(const ryd_plan* P, int dtype, void* psi, double tau, int nsub,

@@ -9,10 +9,10 @@ CONV = 2π·1e-3 (H in MHz, t in ns).  Each slice applies a Chebyshev series
 
 τ = CONV·dt, with [c - r, c + r] a Gershgorin enclosure of the slice's
 spectrum.  The adjoint is the Clenshaw-form reverse recurrence of that same
-series (``ryd_evolve_adjoint``), so gradients are those of the discrete
-propagator; (c, r) are constants of the slice (no gradient flows through
-them).  Checkpoints are kept every ``checkpoint`` slices and the slices
-between them are recomputed during the backward pass.
+series (``ryd_evolve_adjoint_stored``), so gradients are those of the
+discrete propagator; (c, r) are constants of the slice (no gradient flows
+through them).  Checkpoints are kept every ``checkpoint`` slices and the
+slices between them are recomputed once during the backward pass.
 """
 from __future__ import annotations
 
@@ -77,10 +77,16 @@ class PulseProblem:
     ``'natural'`` is exp(−iH·dt).  A slice whose phase span τ·r would leave
     a series tail above ``eps`` is integrated as several equal sub-steps of
     the same drive (gradients are summed back onto the slice).
+
+    The adjoint keeps the state at every ``checkpoint``-th slice; in the
+    backward sweep each segment between checkpoints is recomputed once with
+    its Chebyshev vectors kept in HBM (at most ``store_budget_bytes``, by
+    default half the free device memory; a longer segment is recomputed in
+    chunks from its checkpoint).
     """
 
     def __init__(self, model: RydModel, dt, K=15, dtype='complex64', device='cuda',
-                 checkpoint=20, plan=None, units='reference', eps=1e-13):
+                 checkpoint=20, plan=None, units='reference', eps=1e-13, store_budget_bytes=None):
         import torch
         nat.require_gpu()
         self.model = model
@@ -95,6 +101,7 @@ class PulseProblem:
         self.tdt = {'complex64': torch.complex64, 'complex128': torch.complex128}[dtype]
         self.dtype = dtype
         self.checkpoint = max(1, int(checkpoint))
+        self.store_budget_bytes = store_budget_bytes
         self.dim = self.plan.dim
 
     def _substeps(self, tau_r):
@@ -151,54 +158,87 @@ class PulseProblem:
         self._forward_slices(psi, params, 0, len(params[0]), work)
         return psi
 
-    def loss_and_grad(self, psi0, Om, Delta, target_index):
-        """L = 1 − |<target|ψ(T)>|² and (dL/dΩ_k, dL/dΔ_k) by the adjoint."""
+    def _segments(self):
+        """Checkpoint segments as sub-step ranges [s0, s1): a checkpoint at
+        every ``checkpoint``-th slice (sub-steps of one slice stay together)."""
+        owner = self._owner
+        starts = [0] + [i for i in range(1, len(owner))
+                        if owner[i] != owner[i - 1] and owner[i] % self.checkpoint == 0]
+        return list(zip(starts, starts[1:] + [len(owner)]))
+
+    def forward_checkpoints(self, psi0, params):
+        """ψ(T) and the checkpoint states {first sub-step of a segment: ψ}."""
+        import torch
+        psi = psi0.to(self.tdt).clone()
+        work = torch.empty(4 * self.dim, dtype=self.tdt, device=self.plan.device)
+        ckpts = {}
+        for s0, s1 in self._segments():
+            ckpts[s0] = psi.clone()
+            self._forward_slices(psi, params, s0, s1, work)
+        return psi, ckpts
+
+    def _store_steps(self, longest):
+        """Sub-steps whose Chebyshev vectors fit the store budget (≥ 1)."""
+        import torch
+        per = self.K * self.dim * (8 if self.tdt == torch.complex64 else 16)
+        budget = self.store_budget_bytes
+        if budget is None:
+            free, _ = torch.cuda.mem_get_info(self.plan.device)
+            budget = free // 2
+        return max(1, min(longest, int(budget // per)))
+
+    def adjoint(self, g, ckpts, params):
+        """Backward sweep from g = dL/d(conj ψ(T)) (device, overwritten with
+        dL/d(conj ψ0)).  Each checkpoint segment is recomputed once with its
+        Chebyshev vectors kept (``ryd_evolve_forward_store``) and then run
+        backwards (``ryd_evolve_adjoint_stored``); the gradient dots stay on
+        the device until the end.  Returns the (sub-steps, K, 4) FP64 sums."""
         import torch
         P = self.plan
-        params = self._slice_params(Om, Delta)
         Omv, dadd, cs, rs, coef = params
         M = len(Omv)
-        psi = psi0.to(self.tdt).clone()
-        work = torch.empty(4 * self.dim, dtype=self.tdt, device=P.device)
-        every = self.checkpoint
-        ckpts = {0: psi.clone()}
-        k = 0
-        while k < M:
-            k1 = min(M, k + every)
-            self._forward_slices(psi, params, k, k1, work)
-            k = k1
-            if k < M:
-                ckpts[k] = psi.clone()
-        o = psi[int(target_index)].item()
-        loss = 1.0 - abs(o) ** 2
-        # cotangent dL/d(conj psi_T) = -o e_target
-        g = torch.zeros(self.dim, dtype=self.tdt, device=P.device)
-        g[int(target_index)] = -o
-        tstore = torch.empty(self.K * self.dim, dtype=self.tdt, device=P.device)
-        vwork = torch.empty(3 * self.dim, dtype=self.tdt, device=P.device)
-        sums = torch.zeros(4 * self.K, dtype=torch.float64, device=P.device)
-        ws = P.workspace()
-        all_sums = np.zeros((M, self.K, 4))
         L = nat.lib()
         code = nat.dtype_code(self.tdt)
-        starts = sorted(ckpts)
-        seg_buf = torch.empty((every, self.dim), dtype=self.tdt, device=P.device)
-        for s0 in reversed(starts):
-            s1 = min(M, s0 + every)
-            # recompute slice inputs psi_{s0} .. psi_{s1-1}
-            cur = ckpts[s0].clone()
-            for kk in range(s0, s1):
-                seg_buf[kk - s0].copy_(cur)
-                if kk + 1 < s1:
-                    self._forward_slices(cur, params, kk, kk + 1, work)
-            for kk in range(s1 - 1, s0 - 1, -1):
-                cf = np.ascontiguousarray(coef[kk])
-                nat.check(L.ryd_evolve_adjoint(
-                    P.handle, code, nat.ptr(seg_buf[kk - s0]), nat.ptr(g), float(Omv[kk]),
-                    float(dadd[kk]), float(cs[kk]), float(rs[kk]), self.K, cf.ctypes.data,
-                    nat.ptr(tstore), nat.ptr(vwork), nat.ptr(sums), nat.ptr(ws),
+        segs = self._segments()
+        chunk = self._store_steps(max(s1 - s0 for s0, s1 in segs))
+        tstore = torch.empty(chunk * self.K * self.dim, dtype=self.tdt, device=P.device)
+        work = torch.empty(4 * self.dim, dtype=self.tdt, device=P.device)
+        nbytes = int(L.ryd_evolve_partials_bytes(P.handle, code, self.K))
+        if nbytes <= 0:
+            raise NotImplementedError('the adjoint runs on the fused full-basis kernels '
+                                      '(Omega/delta/V, n <= 22)')
+        partials = torch.empty(nbytes // 8, dtype=torch.float64, device=P.device)
+        sums = torch.zeros(M * self.K * 4, dtype=torch.float64, device=P.device)
+
+        def arr(x, lo, hi):
+            return np.ascontiguousarray(x[lo:hi])
+
+        for s0, s1 in reversed(segs):
+            # chunks of the segment from its end; a chunk that does not start
+            # at the checkpoint is reached by a plain forward from it
+            c1 = s1
+            while c1 > s0:
+                c0 = max(s0, c1 - chunk)
+                psi = ckpts[s0].clone()
+                if c0 > s0:
+                    self._forward_slices(psi, params, s0, c0, work)
+                om_a, da_a, c_a, r_a = (arr(Omv, c0, c1), arr(dadd, c0, c1), arr(cs, c0, c1),
+                                        arr(rs, c0, c1))
+                cf = np.ascontiguousarray(coef[c0:c1])
+                nat.check(L.ryd_evolve_forward_store(
+                    P.handle, code, nat.ptr(psi), c1 - c0, om_a.ctypes.data, da_a.ctypes.data,
+                    c_a.ctypes.data, r_a.ctypes.data, self.K, cf.ctypes.data, nat.ptr(tstore),
+                    nat.ptr(work), nat.stream_ptr()))
+                nat.check(L.ryd_evolve_adjoint_stored(
+                    P.handle, code, nat.ptr(g), c1 - c0, om_a.ctypes.data, da_a.ctypes.data,
+                    c_a.ctypes.data, r_a.ctypes.data, self.K, cf.ctypes.data, nat.ptr(tstore),
+                    nat.ptr(work), nat._vp(sums.data_ptr() + 8 * c0 * self.K * 4), nat.ptr(partials),
                     nat.stream_ptr()))
-                all_sums[kk] = sums.view(self.K, 4).cpu().numpy()
+                c1 = c0
+        return sums.view(M, self.K, 4).cpu().numpy()
+
+    def grad_from_sums(self, all_sums, rs):
+        """(dL/dΩ_k, dL/dΔ_k) per slice from the adjoint sums."""
         w = np.full(self.K, 2.0)
         w[0] = 1.0
         fac = 2.0 / rs
@@ -206,7 +246,21 @@ class PulseProblem:
         d_dadd_sub = fac * (all_sums[:, :, 1] @ w)
         d_om = np.bincount(self._owner, weights=d_om_sub, minlength=self._nslices)
         d_dadd = np.bincount(self._owner, weights=d_dadd_sub, minlength=self._nslices)
-        return loss, d_om, -d_dadd, psi
+        return d_om, -d_dadd
+
+    def loss_and_grad(self, psi0, Om, Delta, target_index):
+        """L = 1 − |<target|ψ(T)>|² and (dL/dΩ_k, dL/dΔ_k) by the adjoint."""
+        import torch
+        params = self._slice_params(Om, Delta)
+        psi, ckpts = self.forward_checkpoints(psi0, params)
+        o = psi[int(target_index)].item()
+        loss = 1.0 - abs(o) ** 2
+        # cotangent dL/d(conj psi_T) = -o e_target
+        g = torch.zeros(self.dim, dtype=self.tdt, device=self.plan.device)
+        g[int(target_index)] = -o
+        all_sums = self.adjoint(g, ckpts, params)
+        d_om, d_de = self.grad_from_sums(all_sums, params[3])
+        return loss, d_om, d_de, psi
 
 
 def pulse_loss_and_grad(problem, psi0, Om, Delta, target_index):
@@ -228,15 +282,30 @@ def _autograd_function():
 
         @staticmethod
         def forward(ctx, Om, Delta, problem, psi0, target_index):
-            loss, gO, gD, _ = problem.loss_and_grad(
-                psi0, Om.detach().cpu().numpy(), Delta.detach().cpu().numpy(), target_index)
-            ctx.save_for_backward(torch.as_tensor(gO, dtype=Om.dtype),
-                                  torch.as_tensor(gD, dtype=Delta.dtype))
-            return torch.tensor(loss, dtype=Om.dtype)
+            # forward sweep only: the loss and the checkpoint states; the
+            # adjoint runs in backward
+            params = problem._slice_params(Om.detach().cpu().numpy(),
+                                           Delta.detach().cpu().numpy())
+            psi, ckpts = problem.forward_checkpoints(psi0, params)
+            o = psi[int(target_index)].item()
+            ctx.problem = problem
+            ctx.params = params
+            ctx.ckpts = ckpts
+            ctx.o = o
+            ctx.target = int(target_index)
+            ctx.dtypes = (Om.dtype, Delta.dtype)
+            return torch.tensor(1.0 - abs(o) ** 2, dtype=Om.dtype)
 
         @staticmethod
         def backward(ctx, grad_out):
-            gO, gD = ctx.saved_tensors
+            problem = ctx.problem
+            g = torch.zeros(problem.dim, dtype=problem.tdt, device=problem.plan.device)
+            g[ctx.target] = -ctx.o
+            all_sums = problem.adjoint(g, ctx.ckpts, ctx.params)
+            ctx.ckpts = None
+            gO, gD = problem.grad_from_sums(all_sums, ctx.params[3])
+            gO = torch.as_tensor(gO, dtype=ctx.dtypes[0])
+            gD = torch.as_tensor(gD, dtype=ctx.dtypes[1])
             return grad_out * gO, grad_out * gD, None, None, None
 
     return PulseEvolution

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 6
+#define RYD_ABI_VERSION 7
 
 enum ryd_status {
   RYD_OK = 0,
@@ -303,6 +303,35 @@ int ryd_loss_axpy(const ryd_plan* plan, int dtype, const void* x, void* y, const
 int ryd_evolve_adjoint(const ryd_plan* plan, int dtype, const void* psi, void* g, double om,
                        double dadd, double c, double r, int K, const double* coef, void* tstore,
                        void* work, double* sums, void* ws, void* stream);
+
+/* ryd_evolve_forward (fused slice kernels only: full basis, Ω/Δ/V,
+ * n <= 22) that also keeps each slice's Chebyshev vectors for the adjoint:
+ * t_0..t_{K-1} of slice k (t_0 = the slice's input state) at
+ * tstore + k·K·dim elements.  tstore: nslices·K vectors; work: 1 vector.
+ * No reference counterpart (the reference has no gradient); it is the
+ * recompute step of a checkpointed adjoint. */
+int ryd_evolve_forward_store(const ryd_plan* plan, int dtype, void* psi, int64_t nslices,
+                             const double* om, const double* dadd, const double* c,
+                             const double* r, int K, const double* coef, void* tstore,
+                             void* work, void* stream);
+
+/* Bytes of the partials buffer ryd_evolve_adjoint_stored needs for K terms
+ * (-1 for a plan or dtype the fused kernels do not take).  No reference
+ * counterpart. */
+int64_t ryd_evolve_partials_bytes(const ryd_plan* plan, int dtype, int K);
+
+/* Adjoint of nslices slices stored by ryd_evolve_forward_store, last slice
+ * first (no reference counterpart — the reference has no gradient).  g =
+ * dL/d(conj psi_out) of the last slice on entry, dL/d(conj psi_in) of the
+ * first on exit.  For slice k and term j = 0..K-1 the FP64 sums of
+ * ryd_evolve_adjoint go to sums[(k·K + j)·4 ..] in DEVICE memory (no host
+ * round trip per slice).  work: 3 vectors; partials: device,
+ * ryd_evolve_partials_bytes; coef, om, dadd, c, r: host arrays as in
+ * ryd_evolve_forward. */
+int ryd_evolve_adjoint_stored(const ryd_plan* plan, int dtype, void* g, int64_t nslices,
+                              const double* om, const double* dadd, const double* c,
+                              const double* r, int K, const double* coef, const void* tstore,
+                              void* work, double* sums, double* partials, void* stream);
 
 /* Batched Lanczos over many independent full-basis instances of uneven n
  * (1 <= n <= 24, real drives), FP32 or FP64 vectors with FP64 alpha/beta.

@@ -33,7 +33,8 @@ def test_rabi_closed_form(pkg, dev):
 
 
 @pytest.mark.parametrize('n,dtype,tol', [(6, 'complex128', 1e-12), (13, 'complex128', 1e-11),
-                                         (13, 'complex64', 2e-5), (16, 'complex128', 1e-11)])
+                                         (11, 'complex64', 2e-5), (13, 'complex64', 2e-5),
+                                         (18, 'complex64', 2e-5), (16, 'complex128', 1e-11)])
 def test_forward_vs_exact_segments(pkg, dev, n, dtype, tol):
     import torch
     calib, model = _model(pkg, n)
@@ -70,6 +71,86 @@ def test_adjoint_vs_central_differences(pkg, dev):
                 - prob.loss_and_grad(psi0, Om, De - e, tgt)[0]) / (2 * h)
         assert abs(gO[k] - fd_o) < 1e-7 * max(1.0, abs(fd_o)), (k, gO[k], fd_o)
         assert abs(gD[k] - fd_d) < 1e-7 * max(1.0, abs(fd_d)), (k, gD[k], fd_d)
+
+
+def _per_slice_adjoint(pkg, prob, psi0, Om, De, tgt):
+    """Gradient by the one-slice entry point ryd_evolve_adjoint: every slice
+    input from a fresh forward, the adjoint sums copied to the host per
+    slice (the straightforward form the stored sweep must reproduce)."""
+    import torch
+    from energy_level_amd import _native as nat
+    P = prob.plan
+    params = prob._slice_params(Om, De)
+    Omv, dadd, cs, rs, coef = params
+    M, D, K = len(Omv), prob.dim, prob.K
+    work = torch.empty(4 * D, dtype=prob.tdt, device=dev_of(P))
+    states = []
+    psi = psi0.to(prob.tdt).clone()
+    for k in range(M):
+        states.append(psi.clone())
+        prob._forward_slices(psi, params, k, k + 1, work)
+    o = psi[tgt].item()
+    g = torch.zeros(D, dtype=prob.tdt, device=dev_of(P))
+    g[tgt] = -o
+    tstore = torch.empty(K * D, dtype=prob.tdt, device=dev_of(P))
+    vwork = torch.empty(3 * D, dtype=prob.tdt, device=dev_of(P))
+    sums = torch.zeros(4 * K, dtype=torch.float64, device=dev_of(P))
+    all_sums = np.zeros((M, K, 4))
+    code = nat.dtype_code(prob.tdt)
+    for k in range(M - 1, -1, -1):
+        cf = np.ascontiguousarray(coef[k])
+        nat.check(nat.lib().ryd_evolve_adjoint(
+            P.handle, code, nat.ptr(states[k]), nat.ptr(g), float(Omv[k]), float(dadd[k]),
+            float(cs[k]), float(rs[k]), K, cf.ctypes.data, nat.ptr(tstore), nat.ptr(vwork),
+            nat.ptr(sums), nat.ptr(P.workspace()), nat.stream_ptr()))
+        all_sums[k] = sums.view(K, 4).cpu().numpy()
+    gO, gD = prob.grad_from_sums(all_sums, rs)
+    return 1.0 - abs(o) ** 2, gO, gD, g
+
+
+def dev_of(P):
+    return P.device
+
+
+@pytest.mark.parametrize('n,dtype,tol', [(9, 'complex128', 1e-13), (12, 'complex64', 2e-6),
+                                         (14, 'complex128', 1e-13)])
+def test_stored_sweep_matches_per_slice_adjoint(pkg, dev, n, dtype, tol):
+    """The checkpointed sweep (segments recomputed once with their Chebyshev
+    vectors stored, sums kept on the device) against the one-slice adjoint
+    entry point, with whole segments and with the store forced down to one
+    sub-step (chunks recomputed from the checkpoint); natural units so that
+    slices take several sub-steps and checkpoints fall inside slices' runs."""
+    import torch
+    _, model = _model(pkg, n)
+    Om, De = _drive(17, 5)
+    psi0 = torch.zeros(1 << n, dtype=torch.complex128, device=dev)
+    psi0[0] = 1
+    psi0[3] = 0.5
+    psi0 /= torch.linalg.norm(psi0)
+    tgt = pkg.z2_index(n)
+    plan = pkg.Plan(model, device=dev)
+    full = pkg.PulseProblem(model, dt=0.05, K=15, dtype=dtype, plan=plan, checkpoint=5,
+                            units='natural')
+    tiny = pkg.PulseProblem(model, dt=0.05, K=15, dtype=dtype, plan=plan, checkpoint=5,
+                            units='natural', store_budget_bytes=1)
+    L, gO, gD, _ = full.loss_and_grad(psi0, Om, De, tgt)
+    assert len(full._owner) > 17          # sub-stepped slices
+    L1, gO1, gD1, _ = tiny.loss_and_grad(psi0, Om, De, tgt)
+    assert L1 == L and np.array_equal(gO1, gO) and np.array_equal(gD1, gD)
+    Lr, gOr, gDr, _ = _per_slice_adjoint(pkg, full, psi0, Om, De, tgt)
+    scale = max(np.max(np.abs(gOr)), np.max(np.abs(gDr)))
+    assert abs(Lr - L) <= tol
+    assert np.max(np.abs(gO - gOr)) <= tol * scale
+    assert np.max(np.abs(gD - gDr)) <= tol * scale
+    # torch autograd through the lazy backward equals the eager gradient
+    om_t = torch.tensor(Om, requires_grad=True)
+    de_t = torch.tensor(De, requires_grad=True)
+    loss = pkg.PulseEvolution.apply(om_t, de_t, full, psi0, tgt)
+    (2.0 * loss).backward()
+    assert float(loss.detach()) == L
+    assert np.array_equal(om_t.grad.numpy(), 2.0 * gO)
+    assert np.array_equal(de_t.grad.numpy(), 2.0 * gD)
+    plan.close()
 
 
 def test_adjoint_vs_dense_expm_autograd(pkg, dev):
