@@ -13,7 +13,10 @@ level).  ``roofline`` is for the dominant kernel (k_full_tiled) with
 algorithmic bytes 2·D·sizeof(element) (x read once, y written once) over
 its mean duration measured with HIP events on the launch stream.
 ``cpu_baseline`` times the oracle's OpenMP matrix-free H·v on the host
-cores for a bounded sample of the same N=28 rows.
+cores for a bounded sample of the same N=28 rows; its ``configs`` hold the
+reference's CPU algorithms for the headline H·v (per-channel numpy) and each
+configuration (eigsh, the KPM recurrence, the sector H·v, the per-instance
+eigsh loop, expm_multiply stepping), each on a bounded sample and scaled.
 """
 from __future__ import annotations
 
@@ -113,6 +116,140 @@ def _cpu_baseline(n, rows):
                        'oracle, median of 5 after warm-up, diagonal precomputed; scaled by %d'
                        % (rows, n, D // rows, D // rows),
                 seconds_per_sample=t)
+
+
+def _cpu_model_name():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _timed(fn):
+    t0 = time.perf_counter()
+    r = fn()
+    return time.perf_counter() - t0, r
+
+
+def _cpu_configs(which):
+    """The reference's CPU algorithms, as the oracle restates them, on bounded
+    samples of each configuration, scaled to the whole workload (seconds of
+    the GPU result's equivalent).  Each entry says what was timed and how it
+    was scaled; together they take well under a minute of host time."""
+    import numpy as np
+    from oracle import hv_ref
+    from oracle import rydberg_oracle as orc
+    from energy_level_amd import workloads as W
+    from energy_level_amd.batch import disorder_ensemble
+    thr = hv_ref.num_threads()
+    out = {}
+    if 'hv' in which:
+        # ChannelKernel.matvec (channels.py:210-221): one numpy gather/scatter
+        # per channel over the whole vector; the headline chain at N=20, work
+        # linear in D·N
+        n = 20
+        calib, Om, delta = _model(n)
+        x = np.random.default_rng(0).random(1 << n)
+        orc.channel_matvec(n, Om, delta, calib['pairs'], ['I1'], x)
+        t = _timed(lambda: orc.channel_matvec(n, Om, delta, calib['pairs'], ['I1'], x))[0]
+        scale = (2.0 ** 28 * 28) / (2.0 ** n * n)
+        out['hv_channel_matvec'] = dict(
+            value=1.0 / (t * scale), unit='H·v/s', cores=1, kind='port',
+            sample='the reference H·v algorithm (per-channel numpy gather/scatter, the oracle '
+                   'restatement of channels.py:210-221) on the same chain at N=20 after one '
+                   'warm-up, scaled by (2^28·28)/(2^20·20)', seconds_per_sample=t)
+    if 'c1' in which:
+        n = 12
+        calib = W.chain_calib(n, 10.0)
+        H = orc.build_H_sparse(n, np.ones(n), -1.2 * np.ones(n), calib['pairs'], ['I1']).tocsr()
+        t, r = _timed(lambda: orc.ground_state_eigsh(H, k=2, seed=0))
+        out['c1'] = dict(value=t, unit='s per ground state', cores=1, kind='port',
+                         sample='whole C1: scipy eigsh(SA, k=2, tol=0) on the CSR H '
+                                '(equilibrium.py:76-86), assembly excluded', E0=r['E0'])
+    if 'c2' in which:
+        n, num = 25, 8
+        calib = W.from_positions(W.square_positions(n), W.c6_coupling(20.0), terms=('I1',))
+        Om, delta = np.ones(n), -0.8 * np.ones(n)
+        d = hv_ref.diag(n, delta, calib['pairs'])
+        z = next(orc.rademacher_probes(1 << n, 1, 0))
+        c, r = 435.47, 448.41          # the C2 bounds (any enclosure times the same)
+
+        def kpm_probe():
+            # kpm.py:139-149: t_{k+1} = 2·H~ t_k − t_{k−1}, one H·v per moment
+            tp, tc = z.copy(), (hv_ref.hv_full(n, Om, d, z) - c * z) / r
+            acc = [z @ z, tc @ z]
+            for _ in range(2, num + 1):
+                tn = 2.0 * (hv_ref.hv_full(n, Om, d, tc) - c * tc) / r - tp
+                tp, tc = tc, tn
+                acc.append(tc @ z)
+            return acc
+        t, _ = _timed(kpm_probe)
+        scale = 16 * 1024 / num
+        out['c2'] = dict(value=t * scale, unit='s per 16 probes x 1024 moments', cores=thr,
+                         kind='port', sample='one probe x %d moments of the reference KPM '
+                         'recurrence (kpm.py:139-149) at N=25 with the OpenMP matrix-free H·v, '
+                         'scaled by %d' % (num, scale), seconds_per_sample=t)
+    if 'c3' in which:
+        n, steps = 34, 200
+        calib = W.chain_calib(n, 30.0)
+        codes = hv_ref.fib_codes(n)
+        d = hv_ref.diag(n, -1.5 * np.ones(n), calib['pairs'], codes)
+        x = np.random.default_rng(0).random(len(codes))
+        hv_ref.hv_fib(n, np.ones(n), d, codes, x)
+        t = min(_timed(lambda: hv_ref.hv_fib(n, np.ones(n), d, codes, x))[0] for _ in range(3))
+        scale = steps * (hv_ref.fib_dim(40) / len(codes)) * (40 / n)
+        out['c3'] = dict(value=t * scale, unit='s per 200 FP64 Lanczos H·v', cores=thr,
+                         kind='port', sample='the chain-sector H·v (OpenMP, codes and binary-'
+                         'search ranks) at N=34 (D = %d), best of 3, scaled by 200 steps x '
+                         'D(40)/D(34) x 40/34; the reference caps sectors at 2^20 states'
+                         % len(codes), seconds_per_sample=t)
+    if 'c4' in which:
+        # the reference's per-instance ground state (equilibrium.py:35-96):
+        # dense eigh up to dim 2048, else eigsh(SA, k=2, tol=0).  eigsh at
+        # tol=0 takes ~40 s at N=16 and ~140 s at N=18 on one core, so one
+        # instance per N in 10..14 is timed and every larger instance is
+        # charged the N=14 time: a lower bound on the ensemble's CPU time
+        models = disorder_ensemble(count=512, seed=0)
+        per_n = {}
+        for m in models:
+            if m.n in per_n or m.n > 14:
+                continue
+            H = orc.build_H_sparse(m.n, m.Om, m.delta, m.pairs(), ['I1']).tocsr()
+            if (1 << m.n) <= 2048:
+                per_n[m.n] = _timed(lambda: orc.ground_state_dense(H))[0]
+            else:
+                per_n[m.n] = _timed(lambda: orc.ground_state_eigsh(H, k=2, seed=0))[0]
+        tot = sum(per_n[min(int(m.n), 14)] for m in models)
+        out['c4'] = dict(value=len(models) / tot, unit='solves/s (upper bound)', cores=1,
+                         kind='port', sample='one instance per N = 10..14 of the 512 (dense eigh '
+                         'to dim 2048, else scipy eigsh(SA, k=2, tol=0), as equilibrium.py:35-96; '
+                         'assembly excluded); instances with N > 14 charged the N = 14 time, so '
+                         'the CPU figure is an upper bound on its rate',
+                         seconds_per_n={int(k): v for k, v in sorted(per_n.items())})
+    if 'c5' in which:
+        n, ks, dt = 20, 1, 0.05
+        calib = W.chain_calib(n, 10.0)
+        rng = np.random.default_rng(0)
+        Om = rng.uniform(0.5, 1.5, 200)
+        De = rng.uniform(-1.0, 2.0, 200)
+        unit = dt / orc.CONV            # natural units, as the GPU C5
+        segs = [{'t0': k * unit, 'Om': Om[k] * np.ones(n), 'delta': -De[k] * np.ones(n)}
+                for k in range(ks)]
+        psi0 = np.zeros(1 << n, complex)
+        psi0[0] = 1.0
+        t, _ = _timed(lambda: orc.propagate_segments(n, calib['pairs'], ['I1'], segs, psi0,
+                                                    [0.0, ks * unit], method='expm_multiply'))
+        out['c5'] = dict(value=t * 200 / ks, unit='s per 200-slice forward', cores=1,
+                         kind='port', sample='%d slices of expm_multiply stepping '
+                         '(engine_ed.py:145) at N=20 in natural units, CSR assembly included, '
+                         'scaled by 200/%d; the reference has no gradient (a central-difference '
+                         'gradient of 400 parameters costs 800 forwards)' % (ks, ks),
+                         seconds_per_sample=t)
+    return out
 
 
 def _jsonable(o):
@@ -215,6 +352,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = _cpu_baseline(n, rows=1 << max(10, n - 6))
+        cpu['host_cpu'] = _cpu_model_name()
+        cpu['configs'] = _cpu_configs(['hv'] + [c for c in args.configs.split(',') if c])
+        print('[bench] cpu baselines done', file=sys.stderr, flush=True)
     out = {
         'metric': 'H·v/s at N=28 (full 2^28 basis, matrix-free, FP64)',
         'value': f64['hv_per_s'],

@@ -279,7 +279,7 @@ constexpr int EVO_WIDE_MAXN = EVO_WIDE_TB + EVO_WIDE_FAR;
 template <class V> struct alignas(2 * sizeof(V)) V2 { V v[2]; };
 
 template <class V>
-__global__ __launch_bounds__(1024) void k_evo_wide(FullModel M, EvoIO<V> io, Reduce rd) {
+__global__ __launch_bounds__(1024, 8) void k_evo_wide(FullModel M, EvoIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
   constexpr int TB = EVO_WIDE_TB;
   __shared__ V2<V> lds[1024];
