@@ -82,6 +82,28 @@ __device__ __forceinline__ void apply_scalars(const ApplyIO<V>& io, double& s, d
 #ifndef RYD_FAR_UNROLL
 #define RYD_FAR_UNROLL 1
 #endif
+#ifndef RYD_FAR_BATCH
+#define RYD_FAR_BATCH 1
+#endif
+// Far bits from the highest down (the partner tile furthest away first):
+// n = 28, 2.36 -> 2.34 ms FP64, 1.21 -> 1.19 ms FP32 per H·v; 512-thread
+// pass-A tiles (2^13 codes, two blocks per CU) lose (2.44 ms FP64).
+#ifndef RYD_FAR_REV
+#define RYD_FAR_REV 1
+#endif
+// Streaming accesses of the split H·v (n = 28, ms per H·v FP64 / FP32):
+// pass A's partial as 16-byte non-temporal stores, pass B's single reads of
+// x, the partial and z and its write of y non-temporal: 2.50 / 1.27 ->
+// 2.36 / 1.21.  Batching pass A's far-partner loads (2 or 3 bits per round
+// trip) lost (2.55), and so did a persistent pass A that prefetched the next
+// tile into registers (2.75: the prefetch sits ahead of the far-partner
+// loads in the in-order load queue, and the blocks of an XCD drift apart).
+#ifndef RYD_PASSA_VEC_NT
+#define RYD_PASSA_VEC_NT 1
+#endif
+#ifndef RYD_TOP_NT
+#define RYD_TOP_NT 1
+#endif
 // Pass A of a split H·v (n >= 23) runs blocks of 2^TT threads, so its tiles
 // are 2^(TT-8) times the single-pass tile and fewer bits are far: 1024
 // threads give 15-bit FP32, 14-bit FP64 and 13-bit complex128 tiles (128 KB
@@ -230,8 +252,34 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
     }
   }
   // bits TB .. TB+nfar-1: partner tile through the cache hierarchy
+#if RYD_FAR_BATCH > 1
+  // RYD_FAR_BATCH far bits' partner groups requested together
+  for (int k0 = 0; k0 < io.nfar; k0 += RYD_FAR_BATCH) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      V4<V> q[RYD_FAR_BATCH];
+#pragma unroll
+      for (int u = 0; u < RYD_FAR_BATCH; ++u) {
+        const int k = k0 + u < io.nfar ? k0 + u : k0;
+        q[u] = *reinterpret_cast<const V4<V>*>(io.x + (((tile ^ (1ull << k)) << TB) + r * GS + t * 4));
+      }
+#pragma unroll
+      for (int u = 0; u < RYD_FAR_BATCH; ++u) {
+        const int k = k0 + u;
+        const double on = k < io.nfar ? s : 0.0;
+        const int p = TB + (k < io.nfar ? k : k0);
+        const int bit = (int)((tile >> (k < io.nfar ? k : k0)) & 1ull);
+        const RT ar = (RT)(M.ar[p] * on);
+        const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * on);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q[u].v[c]);
+      }
+    }
+  }
+#else
 #pragma unroll RYD_FAR_UNROLL
-  for (int k = 0; k < io.nfar; ++k) {
+  for (int kk = 0; kk < io.nfar; ++kk) {
+    const int k = RYD_FAR_REV ? io.nfar - 1 - kk : kk;
     const int p = TB + k;
     const uint64_t pbase = (tile ^ (1ull << k)) << TB;
     const int bit = (int)((tile >> k) & 1ull);
@@ -244,12 +292,18 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
       for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
     }
   }
+#endif
 
   if (io.partial) {   // pass A: raw sum for the top-bit pass, streamed past L2
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int r = 0; r < R; ++r) {
+#if RYD_PASSA_VEC_NT
+      nt_store4(io.y + base + r * GS + t * 4, acc[r]);
+#else
 #pragma unroll
       for (int c = 0; c < 4; ++c) nt_store(io.y + base + r * GS + t * 4 + c, acc[r][c]);
+#endif
+    }
     return;
   }
   // combine, store, reduce
@@ -338,6 +392,16 @@ template <class V> constexpr int run_bits() {
   return sizeof(V) == 4 ? RYD_RUNB4 : (sizeof(V) == 16 ? RYD_RUNB16 : RYD_RUNB8);
 }
 
+// pass B reads x, the partial and z exactly once: streaming loads with
+// RYD_TOP_NT
+template <class V> __device__ __forceinline__ V top_load(const V* p) {
+#if RYD_TOP_NT
+  return nt_load(p);
+#else
+  return *p;
+#endif
+}
+
 template <class V>
 __global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
@@ -373,10 +437,10 @@ __global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, Appl
   {
     V xv[J];
 #pragma unroll
-    for (int j = 0; j < J; ++j) xv[j] = io.x[ebase + j * jstride];
+    for (int j = 0; j < J; ++j) xv[j] = top_load(io.x + ebase + j * jstride);
     if constexpr (PRE) {
 #pragma unroll
-      for (int j = 0; j < J; ++j) pa[j] = io.y[ebase + j * jstride];
+      for (int j = 0; j < J; ++j) pa[j] = top_load(io.y + ebase + j * jstride);
     }
 #pragma unroll
     for (int j = 0; j < J; ++j) lds[u + NU * j] = vscale(xv[j], sr);
@@ -407,8 +471,12 @@ __global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, Appl
     const V xj = lds[u + NU * j];
     V v = vscale(a, chr);
     vfma(v, cxr, xj);
-    if (io.z) vfma(v, czr, io.z[e]);
+    if (io.z) vfma(v, czr, top_load(io.z + e));
+#if RYD_TOP_NT
+    nt_store(io.y + e, v);
+#else
     io.y[e] = v;
+#endif
     if (io.reduce) {
       s0 += rdot(xj, v);
       s1 += nrm2(v);
