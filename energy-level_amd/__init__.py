@@ -10,7 +10,7 @@ from .model import RydModel, from_calib, active_terms, chain_positions, square_p
     from_positions, c6_coupling
 from .plan import Plan, CsrPlan, cast
 from .lanczos import hip_lanczos_ground_state, lanczos_run, ritz_vectors, rayleigh_residual, \
-    site_occupations, hamiltonian_linear_operator_hip
+    site_occupations, hamiltonian_linear_operator_hip, ground_is_unique, locked_lowest
 from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, sector_codes, \
     sector_rank, chain_edges, number_sector_basis, check_number_conserving, validate_edges, \
     graph_plan, independent_states_hip, independent_set_count_hip, blockade_edges, build_plan_hip
@@ -31,6 +31,7 @@ from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_
 __all__ = ['hamiltonian_linear_operator_hip', 'loop_observables', 'link_loop_observables', 'RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_positions',
            'from_positions', 'c6_coupling', 'Plan', 'CsrPlan', 'batched_ground_state_dense_hip', 'cast', 'kpm_moments_gpu_hip', 'hip_lanczos_ground_state',
            'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
+           'ground_is_unique', 'locked_lowest',
            'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',
            'dos_from_moments', 'probe_bits', 'fibonacci', 'chain_blockade_dim',
            'is_chain_blockade_basis', 'sector_codes', 'sector_rank', 'chain_edges', 'CONV',

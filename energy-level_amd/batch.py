@@ -11,6 +11,7 @@ tridiagonal of each instance (lowest genuine Ritz values).
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -442,7 +443,10 @@ def batched_ground_state_dense_hip(H_batch, backend='auto', vram_budget=1.2e9, d
     src/core/manybody/equilibrium.py:251-311) with its signature: H_batch is a
     (B, dim, dim) Hermitian stack with dim = 2^n; → dict(E0 (B,), gap (B,),
     n_atom (B, n), backend).  Each H is copied into a device CSR plan (its
-    non-zeros) and solved by the device Lanczos; <n_i> of the ground state
+    non-zeros) and solved by the device Lanczos, E1 counted with
+    multiplicity (the second level over the complement of the ground
+    vector, so a degenerate ground level gives gap 0 as eigh's w[1] - w[0]
+    does); <n_i> of the ground state
     comes from the full-basis occupation kernel of an n-site plan (site 0 is
     the most significant bit, as bit_matrix).  ``backend`` and
     ``vram_budget`` are accepted for the signature; one instance is resident
@@ -470,7 +474,15 @@ def batched_ground_state_dense_hip(H_batch, backend='auto', vram_budget=1.2e9, d
                 out = hip_lanczos_ground_state(None, plan=P, device=device, return_state=True)
             finally:
                 P.close()
-            E0[b], gap[b] = out['E0'], out['gap']
+            E0[b] = out['E0']
+            ev = np.asarray(out['evals'], float)
+            # eigh always gives w[1] - w[0]: the best estimate even when the
+            # residuals miss the tolerance (then with a warning)
+            gap[b] = out['gap'] if np.isfinite(out['gap']) else (
+                float(ev[1] - ev[0]) if len(ev) > 1 else 0.0)
+            if not out['converged']:
+                warnings.warn('instance %d: %s' % (b, out.get('warning', 'not converged')),
+                              RuntimeWarning)
             v = torch.as_tensor(np.asarray(out['vec']), device=occ_plan.device)
             v = v.to(torch.complex128 if torch.is_complex(v) else torch.float64)
             n_atom[b] = np.asarray(site_occupations(occ_plan, v))

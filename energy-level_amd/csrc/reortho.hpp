@@ -155,4 +155,18 @@ __global__ __launch_bounds__(256) void k_rl_scale(V* __restrict__ out, const V* 
     out[e] = vscale(in[e], s);
 }
 
+// x *= 1/sqrt(s2[1]) (s2[1] = |x|^2 from a k_dots reduction); left as is
+// when the norm is zero
+template <class V>
+__global__ __launch_bounds__(256) void k_scale_rsqrt(V* __restrict__ x, int64_t D,
+                                                     const double* __restrict__ s2) {
+  using RT = typename VT<V>::R;
+  const double q = s2[1];
+  if (!(q > 0.0)) return;
+  const RT f = (RT)(1.0 / sqrt(q));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride)
+    x[i] = vscale(x[i], f);
+}
+
 }  // namespace ryd

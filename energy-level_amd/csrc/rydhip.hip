@@ -361,6 +361,11 @@ unsigned chan_grid(const ryd_plan* P) {
 
 unsigned fib_grid(const ryd_plan* P) {
   const int64_t nh = P->fib_nh;
+  if (RYD_FIB_CHUNK > 0 && nh >= 8 * RYD_FIB_CHUNK) {
+    // whole runs of RYD_FIB_CHUNK per XCD (fib_block_hi); the tail blocks idle
+    const int64_t run = 8 * (int64_t)RYD_FIB_CHUNK;
+    return (unsigned)(run * ((nh + run - 1) / run));
+  }
   return (unsigned)(nh >= 64 ? 8 * ((nh + 7) / 8) : nh);
 }
 
@@ -674,9 +679,12 @@ int do_cheb(const ryd_plan* P, int dtype, const uint32_t* probe, void* t0, void*
 // Lanczos with two classical Gram-Schmidt passes against every stored row
 // (reortho.hpp); basis rows are unit vectors, row 0 is phi·phi_scale on exit.
 template <class V>
-int do_lanczos_reortho(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, void* w_v,
-                       int64_t k, double phi_scale, double* alpha, double* nrm, void* state,
-                       double tiny, int passes, double* gs_ws, void* ws, hipStream_t st) {
+int do_lanczos_reortho(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, int64_t nlock,
+                       void* w_v, int64_t k, double phi_scale, double* alpha, double* nrm,
+                       void* state, double tiny, int passes, double* gs_ws, void* ws,
+                       hipStream_t st) {
+  // rows [0, nlock): locked unit vectors (swept by every Gram-Schmidt pass,
+  // never part of the recurrence); row nlock: the start vector
   V* basis = static_cast<V*>(basis_v);
   V* w = static_cast<V*>(w_v);
   LzState* S = static_cast<LzState*>(state);
@@ -684,13 +692,37 @@ int do_lanczos_reortho(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, 
   const unsigned sg = stream_grid(D);
   const int64_t nchunk = (D + GS_CHUNK - 1) / GS_CHUNK;
   double* partials = gs_ws;
-  double* coef = gs_ws + (size_t)2 * k * nchunk;
+  double* coef = gs_ws + (size_t)2 * (k + nlock) * nchunk;
+  V* q0 = basis + nlock * ld;
   k_lz_init<<<1, 1, 0, st>>>(S, tiny);
   LAUNCH_CHECK();
-  k_axpby<V><<<sg, 256, 0, st>>>(nullptr, basis, D, 0.0, phi_scale);
+  k_axpby<V><<<sg, 256, 0, st>>>(nullptr, q0, D, 0.0, phi_scale);
   LAUNCH_CHECK();
+  if (nlock > 0) {
+    // the start vector orthogonal to the locked rows, then unit again
+    for (int pass = 0; pass < 2; ++pass) {
+      k_gs_dots<V><<<(unsigned)nchunk, 256, 0, st>>>(basis, ld, D, (int)nlock, q0, partials,
+                                                     &S->stop);
+      LAUNCH_CHECK();
+      k_gs_finish<<<(unsigned)nlock, 256, 0, st>>>(partials, nchunk, coef, &S->stop);
+      LAUNCH_CHECK();
+      for (int r0 = 0; r0 < nlock; r0 += GS_MAX_ROWS) {
+        const int cnt = (int)std::min<int64_t>(GS_MAX_ROWS, nlock - r0);
+        k_gs_axpy<V><<<sg, 256, 0, st>>>(basis + (int64_t)r0 * ld, ld, D, cnt, coef + 2 * r0, q0,
+                                         &S->stop);
+        LAUNCH_CHECK();
+      }
+    }
+    Reduce rn = make_reduce(ws, POST_STORE, coef);
+    k_dots<V><<<sg, 256, 0, st>>>(nullptr, q0, D, rn);
+    LAUNCH_CHECK();
+    k_reduce_finish<<<1, 256, 0, st>>>(rn, sg);
+    LAUNCH_CHECK();
+    k_scale_rsqrt<V><<<sg, 256, 0, st>>>(q0, D, coef);
+    LAUNCH_CHECK();
+  }
   for (int64_t j = 0; j < k; ++j) {
-    V* qj = basis + j * ld;
+    V* qj = q0 + j * ld;
     ApplyIO<V> io;
     io.x = qj;
     io.z = nullptr;
@@ -710,7 +742,7 @@ int do_lanczos_reortho(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, 
     k_rl_three_term<V><<<sg, 256, 0, st>>>(w, qj, j > 0 ? qj - ld : qj, D, alpha, nrm, (int)j,
                                            &S->stop);
     LAUNCH_CHECK();
-    const int rows = (int)(j + 1);
+    const int rows = (int)(nlock + j + 1);
     for (int pass = 0; pass < passes; ++pass) {
       k_gs_dots<V><<<(unsigned)nchunk, 256, 0, st>>>(basis, ld, D, rows, w, partials, &S->stop);
       LAUNCH_CHECK();
@@ -1538,8 +1570,29 @@ int ryd_lanczos_reortho(const ryd_plan* P, int dtype, void* basis, int64_t ld, v
     return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho: phi_scale must be finite and > 0");
   if (int rc = check_dtype(P, dtype)) return rc;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  DISPATCH(dtype, do_lanczos_reortho, P, dtype, basis, ld, w, k, phi_scale, alpha, nrm, state,
-           tiny, passes, static_cast<double*>(gs_ws), ws, st);
+  DISPATCH(dtype, do_lanczos_reortho, P, dtype, basis, ld, (int64_t)0, w, k, phi_scale, alpha, nrm,
+           state, tiny, passes, static_cast<double*>(gs_ws), ws, st);
+}
+
+int ryd_lanczos_reortho_locked(const ryd_plan* P, int dtype, void* basis, int64_t ld, int64_t nlock,
+                               void* w, int64_t k, double phi_scale, double* alpha, double* nrm,
+                               void* state, double tiny, int passes, void* gs_ws, void* ws,
+                               void* stream) {
+  if (!P || !basis || !w || !alpha || !nrm || !state || !gs_ws || !ws)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_lanczos_reortho_locked");
+  if (nlock < 0 || k < 1 || nlock + k > P->dim)
+    return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho_locked: need nlock >= 0, k >= 1, nlock + k <= dim");
+  if (nlock > 0 && passes < 1)
+    return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho_locked: locked rows need passes >= 1");
+  if (ld < P->dim) return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho_locked: row stride below dim");
+  if (passes < 0 || passes > 4)
+    return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho_locked: passes in 0..4");
+  if (!(phi_scale > 0.0) || !std::isfinite(phi_scale))
+    return fail(RYD_ERR_INVALID, "ryd_lanczos_reortho_locked: phi_scale must be finite and > 0");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_lanczos_reortho, P, dtype, basis, ld, nlock, w, k, phi_scale, alpha, nrm,
+           state, tiny, passes, static_cast<double*>(gs_ws), ws, st);
 }
 
 int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream) {

@@ -38,7 +38,7 @@ from .plan import Plan, cast
 
 __all__ = ['hip_lanczos_ground_state', 'lanczos_run', 'ritz_vectors', 'rayleigh_residual',
            'site_occupations', 'restricted_plan', 'lanczos_vector',
-           'hamiltonian_linear_operator_hip']
+           'hamiltonian_linear_operator_hip', 'ground_is_unique', 'locked_lowest']
 
 
 def _tridiag(a, b):
@@ -428,6 +428,95 @@ def _solve(P, start, m, cycles, tol, want, stats, key, dtype64):
     return vecs, R, evals, res
 
 
+def ground_is_unique(P):
+    """True when the lowest eigenvalue is simple by Perron-Frobenius: a real
+    model without pair flips and every Ω_i nonzero, on the full basis, the
+    chain sector or a blockade-graph basis (single flips connect every basis
+    state to the empty one, and a diagonal ±1 gauge makes every off-diagonal
+    element -|Ω_i|/2 <= 0)."""
+    m = getattr(P, 'model', None)
+    if m is None or getattr(P, 'basis', None) not in ('full', 'chain_fib', 'graph'):
+        return False
+    om = np.asarray(m.Om)
+    return bool(m.is_real and not m.has_pair_flips and om.size and np.all(om.real != 0))
+
+
+LOCK_STEPS = (48, 144, 432)     # lengths of the locked runs, longest last
+
+
+def locked_lowest(P, x0, tol, seed=1, budget_bytes=None):
+    """Lowest eigenpair of H on the orthogonal complement of x0: runs of
+    ``ryd_lanczos_reortho_locked`` (x0 locked, full reorthogonalisation) of
+    growing length until its lowest Ritz pair passes the bound test
+    ``tol``·max(1,|θ|), then that pair's FP64 Rayleigh quotient and explicit
+    residual.  With x0 the ground vector this is the second level counted
+    with multiplicity: E0 again when the ground state is degenerate, as the
+    reference's two-vector block Krylov (matrix_free.py:219-266) and eigh
+    (equilibrium.py:65-75, :272) report.  Returns (E1, residual, x1) or None
+    when the stored basis of the first run exceeds ``budget_bytes`` (default:
+    a quarter of the free device memory)."""
+    import ctypes
+    import torch
+    L = nat.lib()
+    dim = P.dim
+    if dim < 2:
+        return None
+    dt = x0.dtype
+    itemsize = torch.tensor([], dtype=dt).element_size()
+    if budget_bytes is None:
+        free, _ = torch.cuda.mem_get_info(P.device)
+        budget_bytes = free // 4
+    code = nat.dtype_code(dt)
+    gen = torch.Generator(device=P.device)
+    gen.manual_seed(int(seed))
+    phi = torch.randn(dim, dtype=torch.float64, device=P.device, generator=gen)
+    if dt == torch.complex128:
+        phi = phi.to(dt) + 1j * torch.randn(dim, dtype=torch.float64, device=P.device,
+                                            generator=gen).to(dt)
+    tiny = 1e-13
+    best = None
+    for k in LOCK_STEPS:
+        k = int(min(k, dim - 1))
+        if (k + 1) * dim * itemsize > budget_bytes:
+            break
+        basis = torch.empty((k + 1, dim), dtype=dt, device=P.device)
+        basis[0].copy_(x0 / torch.linalg.vector_norm(x0))
+        basis[1].copy_(phi)
+        w = torch.empty(dim, dtype=dt, device=P.device)
+        alpha = torch.zeros(k, dtype=torch.float64, device=P.device)
+        nrm = torch.zeros(k + 1, dtype=torch.float64, device=P.device)
+        state = torch.zeros(int(L.ryd_lanczos_state_bytes()), dtype=torch.uint8, device=P.device)
+        gsws = torch.empty(max(1, int(L.ryd_reortho_workspace_bytes(dim, k + 1))),
+                           dtype=torch.uint8, device=P.device)
+        nat.check(L.ryd_lanczos_reortho_locked(
+            P.handle, code, nat.ptr(basis), dim, 1, nat.ptr(w), k,
+            1.0 / float(torch.linalg.vector_norm(phi)), nat.ptr(alpha), nat.ptr(nrm),
+            nat.ptr(state), tiny, 2, nat.ptr(gsws), nat.ptr(P.workspace()), nat.stream_ptr()))
+        stopped, steps = ctypes.c_int(0), ctypes.c_int64(0)
+        nat.check(L.ryd_lanczos_status(nat.ptr(state), ctypes.byref(stopped), ctypes.byref(steps),
+                                       nat.stream_ptr()))
+        j = int(steps.value) if stopped.value else k
+        if j < 1:
+            return None
+        a = alpha[:j].cpu().numpy()
+        nr = nrm[:j + 1].cpu().numpy()
+        theta, Y = _tridiag(a, nr[1:j])
+        blast = 0.0 if stopped.value else float(nr[j])
+        bound = abs(blast * Y[-1, 0])
+        y = np.ascontiguousarray(Y[:, 0])
+        ones = torch.ones(j, dtype=torch.float64, device=P.device)
+        x1 = torch.empty(dim, dtype=dt, device=P.device)
+        nat.check(L.ryd_ritz_stored(code, nat.ptr(basis[1]), dim, dim, j, y.ctypes.data, None,
+                                    nat.ptr(ones), 0, 0, nat.ptr(x1), None,
+                                    nat.ptr(P.workspace()), nat.stream_ptr()))
+        E1, r1 = rayleigh_residual(P, x1, theta[0])
+        best = (E1, r1, x1)
+        del basis, gsws
+        if stopped.value or k >= dim - 1 or bound <= tol * max(1.0, abs(theta[0])):
+            break
+    return best
+
+
 def _start_vector(dim, seed, complex_kind, device):
     import torch
     gen = torch.Generator(device=device)
@@ -489,7 +578,7 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
                              basis_indices=None, m=48, seed=0, dtype='float64', device='cuda',
                              tol=None, max_restarts=8, return_state=False,
                              memory_budget_mb=None, refine=True, plan=None, basis='full',
-                             blockade_edges=None):
+                             blockade_edges=None, check_degeneracy=None):
     """Ground state and gap with residual certification (see module docstring).
 
     Keys as ``torch_lanczos_ground_state``: E0, gap, evals, residuals,
@@ -509,6 +598,15 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     take the chain sector kernels).  Given together with ``basis_indices``,
     the graph basis is used when it holds exactly those codes (a plan from
     ``build_plan``), else the codes themselves.
+
+    ``check_degeneracy``: a single Lanczos vector sees each eigenvalue once,
+    while the reference's two-vector block Krylov reports a degenerate
+    ground level twice (gap 0).  Unless the level is simple by
+    Perron-Frobenius (``ground_is_unique``), E1 is taken again over the
+    orthogonal complement of the ground vector (``locked_lowest``) and
+    replaces the first run's E1 when it lies lower; ``None`` (default) does
+    this when needed, ``False`` never.  ``multiplicity_check`` in the result
+    says which case applied.
     """
     import torch
     if str(device).startswith('cpu'):
@@ -556,9 +654,25 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
         vecs, R, evals, res = _solve(P, start, int(m), cycles, tolerance, want, stats,
                                      'steps_fp64', tdt64)
     steps_bulk, steps64, restarts = stats['steps_bulk'], stats['steps_fp64'], stats['restarts']
-    evals = np.asarray(evals)
-    res = np.asarray(res)
+    evals = np.array(evals, dtype=float)
+    res = np.array(res, dtype=float)
     count = len(evals)
+    unique = ground_is_unique(P)
+    mult = 'simple (Perron-Frobenius)' if unique else 'not checked'
+    if want == 2 and not unique and check_degeneracy is not False and len(vecs):
+        x0 = vecs[0] if vecs[0].dtype == tdt64 else cast(vecs[0], tdt64)
+        lk = locked_lowest(P, x0, tolerance, seed=int(seed) + 1)
+        if lk is None:
+            mult = 'not checked (the locked run does not fit device memory)'
+        else:
+            E1, r1, _ = lk
+            mult = 'checked (second level over the complement of the ground vector)'
+            if count < 2:
+                evals = np.append(evals, E1)
+                res = np.append(res, r1)
+                count = 2
+            elif E1 < evals[1] - tolerance * max(1.0, abs(evals[1])):
+                evals[1], res[1] = E1, r1
     flags = res <= tolerance * np.maximum(1.0, np.abs(evals))
     converged = bool(count == want and np.all(flags))
     if R.stopped and count == want:
@@ -568,6 +682,7 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
                evals=evals, residuals=res, converged=converged,
                backend='hip-matrixfree-lanczos', dim=dim, krylov_dim=int(steps_bulk + steps64),
                restarts=int(restarts), tolerance=tolerance,
+               multiplicity_check=mult,
                precision=dict(bulk='float32' if low else 'float64',
                               alpha_beta='float64', tridiagonal='float64',
                               refinement='float64' if (not low or refine) else None,

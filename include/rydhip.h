@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 7
+#define RYD_ABI_VERSION 8
 
 enum ryd_status {
   RYD_OK = 0,
@@ -169,6 +169,22 @@ size_t ryd_reortho_workspace_bytes(int64_t dim, int64_t k);
 int ryd_lanczos_reortho(const ryd_plan* plan, int dtype, void* basis, int64_t ld, void* w,
                         int64_t k, double phi_scale, double* alpha, double* nrm, void* state,
                         double tiny, int passes, void* gs_ws, void* ws, void* stream);
+
+/* ryd_lanczos_reortho with nlock locked rows in front of the start vector:
+ * rows [0, nlock) of `basis` hold orthonormal vectors that every
+ * Gram-Schmidt sweep includes but the recurrence never uses; row nlock holds
+ * phi (scaled by phi_scale, made orthogonal to the locked rows by two sweeps
+ * and normalised again), rows nlock+1 .. nlock+k-1 receive the Lanczos
+ * vectors.  The tridiagonal is then that of H restricted to the orthogonal
+ * complement of the locked rows: with the converged ground vector locked its
+ * lowest Ritz value is E1, or E0 again when the ground state is degenerate,
+ * the multiplicity the reference's two-vector block Krylov
+ * (matrix_free.py:219-266) and eigh (equilibrium.py:65-75, :272) report.
+ * gs_ws: ryd_reortho_workspace_bytes(dim, nlock + k). */
+int ryd_lanczos_reortho_locked(const ryd_plan* plan, int dtype, void* basis, int64_t ld,
+                               int64_t nlock, void* w, int64_t k, double phi_scale,
+                               double* alpha, double* nrm, void* state, double tiny, int passes,
+                               void* gs_ws, void* ws, void* stream);
 
 /* Reads the state written by ryd_lanczos (synchronises the stream). */
 int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream);
