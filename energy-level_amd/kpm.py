@@ -46,10 +46,12 @@ def probe_bits(D, nvec, seed):
 
 
 def spectral_bounds(plan, m=120, seed=0, tol=1e-8, margin=1e-3):
-    """(emin, emax, c, r) from one FP64 device Lanczos run (both ends converge)."""
+    """(emin, emax, c, r) from one FP64 device Lanczos run in which both the
+    lowest and the highest Ritz value pass the bound test (as the
+    reference's eigsh SA and LA runs, kpm.py:92-93)."""
     import torch
     v0 = _start_vector(plan.dim, seed, not plan.is_real, plan.device)
-    R = lanczos_run(plan, v0, m, 8, tol, want=1, store=False)    # no Ritz vectors needed
+    R = lanczos_run(plan, v0, m, 8, tol, want=1, store=False, top=True)   # no Ritz vectors needed
     # Ritz values lie inside the spectrum; widen each end by its residual
     # bound |beta_last · y[-1]| so that [emin, emax] covers the spectrum
     lo, hi = int(np.argmin(R.theta)), int(np.argmax(R.theta))
@@ -82,27 +84,45 @@ def kpm_moments_gpu_hip(H, num=300, nvec=48, seed=0, device='cuda', chunk=32,
     P = CsrPlan(H, device=device)
     try:
         D, nvec = P.dim, int(nvec)
+        words = (D + 31) // 32
         if memory_budget_mb is not None:
-            item = 8 if dtype == 'float32' else 16
-            need = P.nnz * 20 + (D + 1) * 8 + 3 * D * item
-            if need > float(memory_budget_mb) * 2 ** 20:
-                raise MemoryError('KPM operator and vectors need %.1f MB > budget %.1f MB'
-                                  % (need / 2 ** 20, float(memory_budget_mb)))
+            # the reference's rule and accounting (kpm.py:41-47 with
+            # resources.budget_bytes / require_workspace): refuse (ValueError)
+            # only when the operator and one probe do not fit, else shrink
+            # the probe block to the budget.  The block size also fixes the
+            # torch.randint stream, so the same budget draws the same probes.
+            # This path holds less than that estimate: the CSR operator
+            # (nnz·(4 + 8 or 16) + 8·(D + 1) bytes), three recurrence
+            # vectors, the packed probes (4 bytes per 32 entries), a block's
+            # int8 draw and 4·D + D/4 bytes while one column is packed.
+            budget = float(memory_budget_mb)
+            if not np.isfinite(budget) or budget <= 0:
+                raise ValueError('memory_budget_mb must be finite and positive')
+            budget = int(budget * 1024 ** 2)
+            itemsize = 8 if dtype == 'float32' else 16
+            base = (P.nnz + D) * (itemsize + 16) * 3
+            one = base + D * itemsize * 6
+            if one > budget:
+                raise ValueError('GPU KPM operator and one probe need at least %.2f MiB, over '
+                                 'the budget of %.2f MiB' % (one / 1024 ** 2, budget / 1024 ** 2))
+            chunk = min(int(chunk), max(1, (budget - base) // (D * itemsize * 6)))
         gen = torch.Generator(device=P.device)
         gen.manual_seed(int(seed))
         chunk = max(1, min(int(chunk), nvec))
-        words = (D + 31) // 32
-        weight = torch.pow(2, torch.arange(32, device=P.device, dtype=torch.int64))
+        # bit k of word i = entry 32·i + k (two's-complement int32 words)
+        shifts = torch.arange(32, device=P.device, dtype=torch.int32)
         probes, done = [], 0
         while done < nvec:
             b = min(chunk, nvec - done)
             z = torch.randint(0, 2, (D, b), device=P.device, generator=gen, dtype=torch.int8)
             for j in range(b):
-                bits = torch.zeros(words * 32, dtype=torch.int64, device=P.device)
+                bits = torch.zeros(words * 32, dtype=torch.int32, device=P.device)
                 bits[:D] = z[:, j]
-                w = (bits.view(words, 32) * weight).sum(dim=1)
-                w = torch.where(w >= 2 ** 31, w - 2 ** 32, w)      # same bits as uint32
+                # the int64 sum of the int32 terms (bit 31 weighs -2^31) is the
+                # word's signed value, exactly representable in int32
+                w = torch.bitwise_left_shift(bits.view(words, 32), shifts).sum(dim=1)
                 probes.append(w.to(torch.int32).contiguous())
+                del bits, w
             done += b
             del z
         _, _, c, r = spectral_bounds(P, margin=1e-3)
@@ -167,6 +187,13 @@ def kpm_moments_hip(calib=None, Om=None, delta=None, terms=None, *, plan=None, n
     else:
         for k in range(1, num + 1):
             mu[k] = S[k, 3]
+    # with every eigenvalue of H~ in [-1, 1], |z^T T_n(H~) z| <= |z|^2 = D for
+    # each ±1 probe, so |mu_n| <= 1; an eigenvalue outside makes T_n grow
+    # exponentially, which this catches before the moments are used
+    if not np.all(np.isfinite(mu)) or np.max(np.abs(mu)) > 1.0 + 1e-3:
+        raise RuntimeError('KPM moments exceed 1 in magnitude (max %.6g): the bounds '
+                           '[c - r, c + r] do not enclose the spectrum'
+                           % float(np.max(np.abs(np.nan_to_num(mu, nan=np.inf)))))
     if return_sums:
         return mu, c, r, S
     return mu, c, r

@@ -145,7 +145,7 @@ def _fits(plan, nbytes, dt, cached_only=False):
     return nbytes + 8 * plan.dim * itemsize + (4 << 30) <= free + cached
 
 
-def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
+def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', top=False):
     """Cycles of ``m`` device-side Lanczos steps from ``v0`` until the ``want``
     lowest genuine Ritz values pass the bound test or the budget is spent.
 
@@ -159,7 +159,8 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
     three vectors and only that tail is replayed.
 
     Convergence is tested every ``CHECK_STEPS`` steps (at most ``m``); the
-    step budget is ``m·max_cycles``."""
+    step budget is ``m·max_cycles``.  ``top``: the highest Ritz value must
+    pass the same bound test as the ``want`` lowest (spectral bounds)."""
     import ctypes
     import torch
     L = nat.lib()
@@ -282,6 +283,9 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto'):
                                 max(float(tol), 1e-12), merge_rel=merge_rel)
         est = [abs(blast * Y[-1, i]) for i in picks]
         ok = all(e <= tol * max(1.0, abs(theta[i])) for e, i in zip(est, picks))
+        if top:
+            hi = int(np.argmax(theta))
+            ok = ok and abs(blast * Y[-1, hi]) <= tol * max(1.0, abs(theta[hi]))
         if ok or stopped or j >= kmax:
             break
     R.j, R.theta, R.Y, R.picks, R.est, R.stopped, R.cycles = j, theta, Y, picks, est, stopped, cycles

@@ -98,3 +98,46 @@ def test_batched_ground_state_dense_signature(pkg, dev):
         assert np.allclose(out['n_atom'][b], bits @ np.abs(V[:, 0]) ** 2, atol=1e-7)
     with pytest.raises(ValueError):
         pkg.batched_ground_state_dense_hip(np.zeros((2, 6, 6)), device=dev)
+
+
+def test_kpm_gpu_mirror_budget_rule(pkg, dev):
+    """kpm.py:41-47: a budget that holds the operator and k probes shrinks
+    the probe block to k (and so draws the stream of chunk=k); one below a
+    single probe, or a non-positive budget, is a ValueError."""
+    rng = np.random.default_rng(7)
+    D, num, nvec, seed = 1500, 20, 6, 3
+    H = _hermitian(D, 0.006, rng, False)
+    itemsize = 16
+    base = (H.nnz + D) * (itemsize + 16) * 3
+    per = D * itemsize * 6
+    budget_mb = (base + 2 * per + per // 2) / 1024 ** 2        # room for 2 probes
+    mu_b, c, r = pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, seed=seed, device=dev, chunk=32,
+                                         memory_budget_mb=budget_mb)
+    mu_2, c2, r2 = pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, seed=seed, device=dev, chunk=2)
+    assert (c, r) == (c2, r2)
+    assert np.array_equal(mu_b, mu_2)
+    mu_32, _, _ = pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, seed=seed, device=dev, chunk=32)
+    assert not np.array_equal(mu_32, mu_2)        # the block size shapes the probe draw
+    with pytest.raises(ValueError):
+        pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, device=dev,
+                                memory_budget_mb=(base + per // 2) / 1024 ** 2)
+    with pytest.raises(ValueError):
+        pkg.kpm_moments_gpu_hip(H, num=num, nvec=nvec, device=dev, memory_budget_mb=0)
+
+
+def test_kpm_bounds_enclose_both_ends_and_moment_guard(pkg, dev):
+    """The device bounds converge the highest Ritz value as well as the
+    lowest (eigsh SA and LA, kpm.py:92-93): [emin, emax] covers eigvalsh.
+    Bounds that leave eigenvalues outside make |mu_n| exceed 1, which is
+    refused."""
+    rng = np.random.default_rng(9)
+    D = 2500
+    H = _hermitian(D, 0.004, rng, True)
+    w = np.linalg.eigvalsh(H.toarray())
+    P = pkg.CsrPlan(H, device=dev)
+    emin, emax, c, r = pkg.spectral_bounds(P, margin=1e-3)
+    assert emin <= w[0] + 1e-9 and emax >= w[-1] - 1e-9
+    assert c - r <= w[0] and c + r >= w[-1]
+    with pytest.raises(RuntimeError):
+        pkg.kpm_moments_hip(plan=P, num=30, nvec=2, seed=1, c=c, r=r / 3.0)
+    P.close()
