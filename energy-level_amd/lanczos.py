@@ -8,7 +8,7 @@ Algorithm (not the reference's block Krylov; only converged outputs are
 compared):
 
 * the three-term recurrence runs on the device with FP64 alpha/beta and no
-  host synchronisation between convergence checks (every 20 steps,
+  host synchronisation between convergence checks (every 10 steps,
   ``ryd_lanczos``); the recurrence is continued, not restarted, up to
   ``m·(max_restarts+1)`` steps;
 * after each cycle the FP64 tridiagonal matrix is diagonalised on the host;
@@ -125,7 +125,7 @@ CHUNK_BYTES = 32 << 30      # stored-basis block: one size for every element typ
                             # cache reuses the blocks of an FP32 run for the FP64 refinement
 
 
-CHECK_STEPS = 20            # steps between convergence checks (the budget stays m·cycles)
+CHECK_STEPS = 10            # steps between convergence checks (the budget stays m·cycles)
 
 
 def _fits(plan, nbytes, dt, cached_only=False):
@@ -145,7 +145,8 @@ def _fits(plan, nbytes, dt, cached_only=False):
     return nbytes + 8 * plan.dim * itemsize + (4 << 30) <= free + cached
 
 
-def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', top=False):
+def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', top=False,
+                check_steps=None):
     """Cycles of ``m`` device-side Lanczos steps from ``v0`` until the ``want``
     lowest genuine Ritz values pass the bound test or the budget is spent.
 
@@ -158,7 +159,8 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     rotating vectors.  When storage stops, the rest of the run continues with
     three vectors and only that tail is replayed.
 
-    Convergence is tested every ``CHECK_STEPS`` steps (at most ``m``); the
+    Convergence is tested every ``check_steps`` (default ``CHECK_STEPS``,
+    at most ``m``) steps; the
     step budget is ``m·max_cycles``.  ``top``: the highest Ritz value must
     pass the same bound test as the ``want`` lowest (spectral bounds)."""
     import ctypes
@@ -179,7 +181,7 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     R.split = None                # first step run without storage (None: all stored)
     storing = (dim >= STORE_MIN_DIM) if store == 'auto' else bool(store)
     cached_only = store == 'auto'
-    check = max(1, min(int(m), CHECK_STEPS))
+    check = max(1, min(int(m), int(check_steps or CHECK_STEPS)))
     code = nat.dtype_code(dt)
     ws = plan.workspace()
     st = nat.stream_ptr()
@@ -416,11 +418,12 @@ def _certify(P, vecs, R, dtype64):
     return np.asarray(evals), np.asarray(res)
 
 
-def _solve(P, start, m, cycles, tol, want, stats, key, dtype64):
+def _solve(P, start, m, cycles, tol, want, stats, key, dtype64, check_steps=None, attempts=None):
     """Lanczos runs with explicit restarts from the Ritz pair until the explicit
-    residuals of the ``want`` lowest Ritz pairs pass ``tol``·max(1,|E|)."""
-    for attempt in range(cycles):
-        R = lanczos_run(P, start, m, cycles, tol, want=want)
+    residuals of the ``want`` lowest Ritz pairs pass ``tol``·max(1,|E|) (at
+    most ``attempts`` runs, default ``cycles``)."""
+    for attempt in range(int(attempts or cycles)):
+        R = lanczos_run(P, start, m, cycles, tol, want=want, check_steps=check_steps)
         vecs = ritz_vectors(P, R)
         R.chunks, R.bufs = [], None          # the stored basis is not needed past here
         stats[key] += R.j
@@ -582,7 +585,8 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
                              basis_indices=None, m=48, seed=0, dtype='float64', device='cuda',
                              tol=None, max_restarts=8, return_state=False,
                              memory_budget_mb=None, refine=True, plan=None, basis='full',
-                             blockade_edges=None, check_degeneracy=None):
+                             blockade_edges=None, check_degeneracy=None, bulk_tol=None,
+                             refine_check_steps=None):
     """Ground state and gap with residual certification (see module docstring).
 
     Keys as ``torch_lanczos_ground_state``: E0, gap, evals, residuals,
@@ -611,6 +615,10 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     replaces the first run's E1 when it lies lower; ``None`` (default) does
     this when needed, ``False`` never.  ``multiplicity_check`` in the result
     says which case applied.
+
+    ``bulk_tol``: residual tolerance of the FP32 stage (default 1e-6 before
+    an FP64 refinement, else ``tol``); ``refine_check_steps``: steps between
+    the refinement's convergence checks (default ``CHECK_STEPS``).
     """
     import torch
     if str(device).startswith('cpu'):
@@ -648,15 +656,22 @@ def hip_lanczos_ground_state(calib, Om=None, delta=None, gamma=None, terms=None,
     work = tdt32 if low else tdt64
     v0 = _start_vector(dim, seed, complex_kind, P.device)
     cycles = int(max_restarts) + 1
-    bulk_tol = 2e-5 if low else tolerance
+    # the FP32 stage before a refinement runs to 1e-6 (C3: 160 FP32 + 70 FP64
+    # steps, 1.29 s, against 140 + 120 steps, 1.97 s, at the reference's
+    # FP32 tolerance 2e-5; plain FP64 200 steps, 2.51 s) and gets one run:
+    # a model whose FP32 residuals stall above it hands over to FP64 as is
+    if low:
+        bulk_tol = float(bulk_tol) if bulk_tol is not None else (1e-6 if refine else tolerance)
+    else:
+        bulk_tol = tolerance
     stats = dict(steps_bulk=0, steps_fp64=0, restarts=0)
     vecs, R, evals, res = _solve(P, v0.to(work), int(m), cycles, bulk_tol, want, stats,
-                                 'steps_bulk', tdt64)
+                                 'steps_bulk', tdt64, attempts=1 if (low and refine) else None)
     if low and refine:
         # FP64 refinement seeded by the FP32 Ritz pair
         start = _combine(vecs, tdt64)
         vecs, R, evals, res = _solve(P, start, int(m), cycles, tolerance, want, stats,
-                                     'steps_fp64', tdt64)
+                                     'steps_fp64', tdt64, check_steps=refine_check_steps)
     steps_bulk, steps64, restarts = stats['steps_bulk'], stats['steps_fp64'], stats['restarts']
     evals = np.array(evals, dtype=float)
     res = np.array(res, dtype=float)
