@@ -159,8 +159,8 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     rotating vectors.  When storage stops, the rest of the run continues with
     three vectors and only that tail is replayed.
 
-    Convergence is tested every ``check_steps`` (default ``CHECK_STEPS``,
-    at most ``m``) steps; the
+    Convergence is tested every ``check_steps`` (default ``CHECK_STEPS`` from
+    dim = STORE_MIN_DIM on, twice that below; at most ``m``) steps; the
     step budget is ``m·max_cycles``.  ``top``: the highest Ritz value must
     pass the same bound test as the ``want`` lowest (spectral bounds)."""
     import ctypes
@@ -181,7 +181,10 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     R.split = None                # first step run without storage (None: all stored)
     storing = (dim >= STORE_MIN_DIM) if store == 'auto' else bool(store)
     cached_only = store == 'auto'
-    check = max(1, min(int(m), int(check_steps or CHECK_STEPS)))
+    # small problems are bound by the host round trip of a check, large ones
+    # by their steps: check less often below STORE_MIN_DIM
+    default_check = CHECK_STEPS if dim >= STORE_MIN_DIM else 2 * CHECK_STEPS
+    check = max(1, min(int(m), int(check_steps or default_check)))
     code = nat.dtype_code(dt)
     ws = plan.workspace()
     st = nat.stream_ptr()
