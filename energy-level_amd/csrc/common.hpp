@@ -94,25 +94,16 @@ __device__ __forceinline__ c64 nt_load(const c64* p) {
   return {__builtin_nontemporal_load(&p->re), __builtin_nontemporal_load(&p->im)};
 }
 
-// Four consecutive elements with 16-byte non-temporal stores.
+// 16-byte vector types for non-temporal stores
 typedef float  nt_f4 __attribute__((ext_vector_type(4)));
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void nt_store4(float* p, const float (&v)[4]) {
-  __builtin_nontemporal_store(nt_f4{v[0], v[1], v[2], v[3]}, reinterpret_cast<nt_f4*>(p));
-}
-__device__ __forceinline__ void nt_store4(double* p, const double (&v)[4]) {
-  __builtin_nontemporal_store(nt_d2{v[0], v[1]}, reinterpret_cast<nt_d2*>(p));
-  __builtin_nontemporal_store(nt_d2{v[2], v[3]}, reinterpret_cast<nt_d2*>(p + 2));
-}
-__device__ __forceinline__ void nt_store4(c32* p, const c32 (&v)[4]) {
-  float* f = reinterpret_cast<float*>(p);
-  __builtin_nontemporal_store(nt_f4{v[0].re, v[0].im, v[1].re, v[1].im}, reinterpret_cast<nt_f4*>(f));
-  __builtin_nontemporal_store(nt_f4{v[2].re, v[2].im, v[3].re, v[3].im}, reinterpret_cast<nt_f4*>(f + 4));
-}
-__device__ __forceinline__ void nt_store4(c64* p, const c64 (&v)[4]) {
-  double* d = reinterpret_cast<double*>(p);
-  for (int c = 0; c < 4; ++c)
-    __builtin_nontemporal_store(nt_d2{v[c].re, v[c].im}, reinterpret_cast<nt_d2*>(d + 2 * c));
+
+// One 16-byte non-temporal store of a 16-byte-aligned group of elements.
+template <class G>
+__device__ __forceinline__ void nt_store16(void* p, const G& g) {
+  static_assert(sizeof(G) == 16, "16-byte groups");
+  const nt_f4 v = __builtin_bit_cast(nt_f4, g);
+  __builtin_nontemporal_store(v, reinterpret_cast<nt_f4*>(p));
 }
 
 // Raw buffer loads over [p, p + bytes): the hardware range check returns zero

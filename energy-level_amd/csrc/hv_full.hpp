@@ -82,9 +82,6 @@ __device__ __forceinline__ void apply_scalars(const ApplyIO<V>& io, double& s, d
 #ifndef RYD_FAR_UNROLL
 #define RYD_FAR_UNROLL 1
 #endif
-#ifndef RYD_FAR_BATCH
-#define RYD_FAR_BATCH 1
-#endif
 // Far bits from the highest down (the partner tile furthest away first):
 // n = 28, 2.36 -> 2.34 ms FP64, 1.21 -> 1.19 ms FP32 per H·v; 512-thread
 // pass-A tiles (2^13 codes, two blocks per CU) lose (2.44 ms FP64).
@@ -98,9 +95,6 @@ __device__ __forceinline__ void apply_scalars(const ApplyIO<V>& io, double& s, d
 // trip) lost (2.55), and so did a persistent pass A that prefetched the next
 // tile into registers (2.75: the prefetch sits ahead of the far-partner
 // loads in the in-order load queue, and the blocks of an XCD drift apart).
-#ifndef RYD_PASSA_VEC_NT
-#define RYD_PASSA_VEC_NT 1
-#endif
 #ifndef RYD_TOP_NT
 #define RYD_TOP_NT 1
 #endif
@@ -132,15 +126,28 @@ template <class V, int TT = 8> constexpr int tile_bits() {
   return 2 + TT + (tile_r<V>() >= 8 ? 3 : (tile_r<V>() >= 4 ? 2 : (tile_r<V>() >= 2 ? 1 : 0)));
 }
 
+// Register groups of 16 contiguous bytes: EPG = 16 / sizeof(V) codes per
+// thread and group (FP32 4, FP64 and complex64 2, complex128 1), so every
+// global load and store of a wavefront is one contiguous 1 KB run and every
+// LDS access a conflict-free 16-byte read.  (Four codes per thread made the
+// 8- and 16-byte types issue 16-byte accesses at a 32- or 64-byte stride:
+// a load-and-store-only pass A moved 4.4 TB/s in FP64 against 5.7 in FP32.)
+template <class V> struct alignas(16) G16 { V v[16 / sizeof(V)]; };
+template <class V> constexpr int epg() { return (int)(16 / sizeof(V)); }
+template <class V> constexpr int epg_bits() { return sizeof(V) == 4 ? 2 : (sizeof(V) == 8 ? 1 : 0); }
+
 template <class V, int TT = 8>
 __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> io, Reduce rd) {
   using RT = typename VT<V>::R;
-  constexpr int R = tile_r<V>();
   constexpr int TB = tile_bits<V, TT>();
-  constexpr int GS = 4 << TT;                  // codes per register group
-  constexpr int TILE = GS * R;
+  constexpr int EPG = epg<V>();                  // codes per thread and group
+  constexpr int EB = epg_bits<V>();
+  constexpr int GS = EPG << TT;                  // codes per register group
+  constexpr int R = (1 << TB) / GS;              // groups per thread
+  constexpr int RB = TB - EB - TT;               // bits carried by the group index
   static_assert(TT >= 8 && TT <= 10 && TB <= 15, "tiles: 256..1024 threads, at most 2^15 codes");
-  __shared__ V lds[TILE];
+  static_assert(R * GS == (1 << TB) && (1 << RB) == R, "tile: whole groups");
+  __shared__ G16<V> lds[R << TT];
   __shared__ double Fx[16];
   __shared__ double T0[256];
   __shared__ double T1[1 << (TB - 8)];
@@ -158,16 +165,16 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
   const uint64_t base = tile << TB;
 
   // stage the tile (scaled) in registers and LDS
-  V xs[R][4];
+  V xs[R][EPG];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + base + r * GS + t * 4);
+    G16<V> q = *reinterpret_cast<const G16<V>*>(io.x + base + r * GS + t * EPG);
 #pragma unroll
-    for (int c = 0; c < 4; ++c) xs[r][c] = vscale(q.v[c], sr);
-    V4<V> w;
+    for (int c = 0; c < EPG; ++c) xs[r][c] = vscale(q.v[c], sr);
+    G16<V> w;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) w.v[c] = xs[r][c];
-    *reinterpret_cast<V4<V>*>(lds + r * GS + t * 4) = w;
+    for (int c = 0; c < EPG; ++c) w.v[c] = xs[r][c];
+    lds[(r << TT) + t] = w;
   }
 
   // per-tile cross field of the high bits on the low bits
@@ -199,84 +206,58 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
   __syncthreads();
 
   // diagonal
-  V acc[R][4];
+  V acc[R][EPG];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int e0 = r * GS + t * 4;
-    V4<double> el = *reinterpret_cast<const V4<double>*>(M.elo + e0);
+    const int e0 = r * GS + t * EPG;
     const double t1 = T1[e0 >> 8];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const double d = eh + el.v[c] + T0[(e0 + c) & 255] + t1;
+    for (int c = 0; c < EPG; ++c) {
+      const double d = eh + M.elo[e0 + c] + T0[(e0 + c) & 255] + t1;
       acc[r][c] = vscale(xs[r][c], (RT)d);
     }
   }
 
-  // bits 0..1: inside the thread's 4-run
+  // bits 0..EB-1: inside the thread's group
 #pragma unroll
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < EB; ++p) {
     const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
 #pragma unroll
     for (int r = 0; r < R; ++r)
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 0; c < EPG; ++c) {
         const int bit = (c >> p) & 1;
         vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r][c ^ (1 << p)]);
       }
   }
-  // bits 2+TT..TB-1: across the thread's r groups
+  // bits EB+TT..TB-1: across the thread's register groups
 #pragma unroll
-  for (int k = 0; (1 << k) < R; ++k) {
-    const int p = 2 + TT + k;
+  for (int k = 0; k < RB; ++k) {
+    const int p = EB + TT + k;
     const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int bit = (r >> k) & 1;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r ^ (1 << k)][c]);
+      for (int c = 0; c < EPG; ++c) vcfma(acc[r][c], ar, bit ? ai : -ai, xs[r ^ (1 << k)][c]);
     }
   }
-  // bits 2..TT+1: partner thread through LDS
+  // bits EB..EB+TT-1: partner thread through LDS
 #pragma unroll
   for (int k = 0; k < TT; ++k) {
-    const int p = 2 + k;
+    const int p = EB + k;
     const RT ar = (RT)M.ar[p], ai = (RT)M.ai[p];
     const int bit = (t >> k) & 1;
     const RT aim = bit ? ai : -ai;
     const int tp = t ^ (1 << k);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      V4<V> q = *reinterpret_cast<const V4<V>*>(lds + r * GS + tp * 4);
+      const G16<V> q = lds[(r << TT) + tp];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
+      for (int c = 0; c < EPG; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
     }
   }
   // bits TB .. TB+nfar-1: partner tile through the cache hierarchy
-#if RYD_FAR_BATCH > 1
-  // RYD_FAR_BATCH far bits' partner groups requested together
-  for (int k0 = 0; k0 < io.nfar; k0 += RYD_FAR_BATCH) {
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      V4<V> q[RYD_FAR_BATCH];
-#pragma unroll
-      for (int u = 0; u < RYD_FAR_BATCH; ++u) {
-        const int k = k0 + u < io.nfar ? k0 + u : k0;
-        q[u] = *reinterpret_cast<const V4<V>*>(io.x + (((tile ^ (1ull << k)) << TB) + r * GS + t * 4));
-      }
-#pragma unroll
-      for (int u = 0; u < RYD_FAR_BATCH; ++u) {
-        const int k = k0 + u;
-        const double on = k < io.nfar ? s : 0.0;
-        const int p = TB + (k < io.nfar ? k : k0);
-        const int bit = (int)((tile >> (k < io.nfar ? k : k0)) & 1ull);
-        const RT ar = (RT)(M.ar[p] * on);
-        const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * on);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q[u].v[c]);
-      }
-    }
-  }
-#else
 #pragma unroll RYD_FAR_UNROLL
   for (int kk = 0; kk < io.nfar; ++kk) {
     const int k = RYD_FAR_REV ? io.nfar - 1 - kk : kk;
@@ -287,22 +268,19 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
     const RT aim = (RT)((bit ? M.ai[p] : -M.ai[p]) * s);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      V4<V> q = *reinterpret_cast<const V4<V>*>(io.x + pbase + r * GS + t * 4);
+      const G16<V> q = *reinterpret_cast<const G16<V>*>(io.x + pbase + r * GS + t * EPG);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
+      for (int c = 0; c < EPG; ++c) vcfma(acc[r][c], ar, aim, q.v[c]);
     }
   }
-#endif
 
   if (io.partial) {   // pass A: raw sum for the top-bit pass, streamed past L2
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-#if RYD_PASSA_VEC_NT
-      nt_store4(io.y + base + r * GS + t * 4, acc[r]);
-#else
+      G16<V> o;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) nt_store(io.y + base + r * GS + t * 4 + c, acc[r][c]);
-#endif
+      for (int c = 0; c < EPG; ++c) o.v[c] = acc[r][c];
+      nt_store16(io.y + base + r * GS + t * EPG, o);
     }
     return;
   }
@@ -311,14 +289,14 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
   const RT chr = (RT)ch, cxr = (RT)cx, czr = (RT)cz;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const uint64_t g0 = base + r * GS + t * 4;
-    V4<V> zq;
-    if (io.z) zq = *reinterpret_cast<const V4<V>*>(io.z + g0);
+    const uint64_t g0 = base + r * GS + t * EPG;
+    G16<V> zq;
+    if (io.z) zq = *reinterpret_cast<const G16<V>*>(io.z + g0);
     uint32_t pw = 0;
     if (io.probe) pw = io.probe[g0 >> 5] >> (g0 & 31);
-    V4<V> o;
+    G16<V> o;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < EPG; ++c) {
       V v = vscale(acc[r][c], chr);
       vfma(v, cxr, xs[r][c]);
       if (io.z) vfma(v, czr, zq.v[c]);
@@ -330,7 +308,7 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
         if (io.probe) s3 += zdot(((pw >> c) & 1u) ? 1.0 : -1.0, v);
       }
     }
-    *reinterpret_cast<V4<V>*>(io.y + g0) = o;
+    *reinterpret_cast<G16<V>*>(io.y + g0) = o;
   }
   if (io.reduce) block_reduce_store(s0, s1, s2, s3, rd.partials);
 }
