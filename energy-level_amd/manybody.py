@@ -189,6 +189,19 @@ def _pair_values(vals):
     return (v[:, 0] + 1j * v[:, 1] - 1j * v[:, 2] + v[:, 3]) / 4.0
 
 
+def _stage_progress(arr, stage, frac):
+    """The three-slot progress protocol of the reference's engines
+    (src/core/coupling.py:24-36, used by engine_ed.py:283-307): arr[0] the
+    fraction done within the stage, arr[1] the caller's cancel flag (read
+    only), arr[2] the stage (0 preparation, 1 integration, 2 finish); a
+    two-slot array gets the fraction only."""
+    if arr is None:
+        return
+    arr[0] = min(max(float(frac), 0.0), 1.0)
+    if len(arr) > 2:
+        arr[2] = float(stage)
+
+
 def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamma=None,
                         terms=None, segments=None, progress_arr=None, log=None, want_corr=True,
                         want_entropy=True, use_sectors='auto', dense_max=None, fullpop_max=None,
@@ -296,6 +309,7 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     def on_state(k, P, x):
         if progress_arr is not None and float(progress_arr[1]) != 0:
             raise RuntimeError('cancelled')
+        _stage_progress(progress_arr, 1, (k + 1) / n_t)
         if want_entropy:
             if projected and state['codes'] is None:
                 state['codes'] = sector_codes(P)
@@ -326,6 +340,7 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
         if k == n_t - 1 and want_corr:
             state['corr'] = corr_matrix_hip(P, x)[0]
 
+    _stage_progress(progress_arr, 1, 0.0)
     out = propagate_segments_hip(calib, segs, t, psi0=psi0v, terms=terms, interp=drive_interp,
                                  dtype=dtype, device=device, return_states=False,
                                  basis_probabilities_max_dim=fullpop_max, pauli_strings=specs,
@@ -394,4 +409,5 @@ def evolve_manybody_hip(calib, psi0=None, t_grid=None, Om=None, delta=None, gamm
     # engine_ed.py:153): the loss is a physical loss under gamma, else the
     # numerical norm drift
     from .results import describe_result
+    _stage_progress(progress_arr, 2, 1.0)
     return describe_result(result, calib, loss_kind='norm_drift' if hermitian else 'physical_loss')

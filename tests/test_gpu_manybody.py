@@ -221,3 +221,23 @@ def test_propagate_lossy_every_basis_kind(pkg, dev, basis):
     assert np.max(np.abs(psi.cpu().numpy() - ref)) < 1e-11
     assert np.linalg.norm(ref) < 0.95                    # the loss acted
     P.close()
+
+
+def test_progress_protocol_and_cancellation(pkg, dev):
+    """The reference engines' three-slot progress array (coupling.py:24-36,
+    engine_ed.py:283-307): arr[0] in-stage fraction, arr[1] cancel flag,
+    arr[2] stage; a set cancel flag stops the run at the next output time
+    (runtime.check_cancel raises RuntimeError)."""
+    n = 6
+    calib = pkg.from_positions(pkg.chain_positions(n), pkg.c6_coupling(10.0), terms=('I1',))
+    segs = [dict(t0=0.0, Om=np.ones(n), phase=np.zeros(n), delta=np.zeros(n))]
+    t = np.linspace(0.0, 40.0, 5)
+    arr = np.zeros(3)
+    pkg.evolve_manybody_hip(calib, t_grid=t, segments=segs, device=dev, progress_arr=arr)
+    assert arr[0] == 1.0 and arr[1] == 0.0 and arr[2] == 2.0
+    two = np.zeros(2)                    # a two-slot array gets the fraction only
+    pkg.evolve_manybody_hip(calib, t_grid=t, segments=segs, device=dev, progress_arr=two)
+    assert two[0] == 1.0 and two[1] == 0.0
+    stop = np.array([0.0, 1.0, 0.0])
+    with pytest.raises(RuntimeError):
+        pkg.evolve_manybody_hip(calib, t_grid=t, segments=segs, device=dev, progress_arr=stop)
