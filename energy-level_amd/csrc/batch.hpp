@@ -344,6 +344,42 @@ __global__ __launch_bounds__(256) void k_batch_zero(V* __restrict__ x, int64_t n
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = (V)0;
 }
 
+// Ritz vectors from a stored Krylov basis (row j = u_j of every instance,
+// rows total elements apart): ritz_b = Σ_j (ycoef[b][j] / nrm_b[j]) u_j, one
+// streaming pass over the rows instead of a replay of the recurrence.  Rows
+// whose coefficient is zero are skipped (the host leaves the rows past an
+// instance's breakdown at zero; those rows were never written and may hold
+// anything).  At most RYD_BATCH_COMBINE_MAX rows (one LDS table of FP64
+// coefficients).
+#define RYD_BATCH_COMBINE_MAX 2048
+template <class V>
+__global__ __launch_bounds__(256) void k_batch_ritz_stored(BatchLz L, const V* __restrict__ rows,
+                                                           int64_t total, int nrows,
+                                                           const double* __restrict__ ycoef,
+                                                           V* __restrict__ ritz) {
+  __shared__ double cf[RYD_BATCH_COMBINE_MAX];
+  const int gt = blockIdx.x;
+  const int b = L.tile_inst[gt];
+  const BatchInst& I = L.inst[b];
+  const int ltile = (int)(gt - I.tile0);
+  const int D = 1 << I.tb;
+  const int64_t base = I.off + ((int64_t)ltile << I.tb);
+  const double* nrm = L.nrm + (size_t)b * (L.ldk + 1);
+  for (int j = threadIdx.x; j < nrows; j += blockDim.x) {
+    const double y = ycoef[(size_t)b * L.ldk + j];
+    cf[j] = y != 0.0 ? y / nrm[j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < D; e += blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < nrows; ++j) {
+      const double c = cf[j];                 // block-uniform
+      if (c != 0.0) acc += c * (double)rows[(size_t)j * total + base + e];
+    }
+    ritz[base + e] = (V)acc;
+  }
+}
+
 // nrm[b][0] = |start_b|
 template <class V>
 __global__ __launch_bounds__(256) void k_batch_norm0(BatchLz L, const V* __restrict__ vall) {

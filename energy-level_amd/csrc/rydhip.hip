@@ -2432,8 +2432,14 @@ struct ryd_batch {
   int ntiles[3] = {0, 0, 0};
   double* d_tables = nullptr;        // all elo/ehi/vb tables
   int choice = 2;                    // chosen candidate (tb = 12 by default)
-  // owned Krylov workspace (fixed addresses for graph capture)
-  void* d_vec = nullptr;             // 3 * total elements
+  // owned Krylov workspace (fixed addresses for graph capture): three
+  // rotating rows of total elements
+  void* d_vec = nullptr;
+  // caller-lent memory for the whole basis (steps + 1 rows): the recurrence
+  // then writes every row there, and Ritz vectors combine the rows of the
+  // last run instead of replaying it (basis_valid: that run has happened)
+  void* basis = nullptr;
+  bool basis_valid = false;
   double* d_alpha = nullptr;         // [count][steps]
   double* d_nrm = nullptr;           // [count][steps+1]
   int* d_stop = nullptr;             // [count]
@@ -2468,27 +2474,33 @@ __global__ void k_batch_reset(int* stop, int count) {
   if (i < count) stop[i] = 0;
 }
 
+// Row 0 of the recurrence: where the start vectors go.
+void* batch_rows(const ryd_batch* B) { return B->basis ? B->basis : B->d_vec; }
+
 template <class V>
 int batch_sequence(ryd_batch* B, int k, hipStream_t st, V* ritz = nullptr,
                    const double* ycoef = nullptr) {
-  const size_t vs = sizeof(V);
-  V* base = static_cast<V*>(B->d_vec);
-  V* U[3] = {base, base + B->total, base + 2 * B->total};
+  V* base = static_cast<V*>(batch_rows(B));
+  // u_j: row j of a lent basis, else one of three rotating rows (u_{-1}
+  // is never read: its coefficient is zero at j = 0)
+  auto U = [&](int j) -> V* {
+    const int r = B->basis ? (j < 0 ? 0 : j) : ((j % 3) + 3) % 3;
+    return base + (size_t)r * B->total;
+  };
   const unsigned g = (unsigned)B->ntiles[k];
   k_batch_reset<<<(B->count + 255) / 256, 256, 0, st>>>(B->d_stop, B->count);
   LAUNCH_CHECK();
-  k_batch_norm0<V><<<g, 256, 0, st>>>(batch_lz(B, k, 0), U[0]);
+  k_batch_norm0<V><<<g, 256, 0, st>>>(batch_lz(B, k, 0), U(0));
   LAUNCH_CHECK();
   k_batch_finish<<<B->count, 256, 0, st>>>(batch_lz(B, k, -1), 1);
   LAUNCH_CHECK();
-  (void)vs;
   for (int j = 0; j < B->steps; ++j) {
     BatchLz L = batch_lz(B, k, j);
-    k_batch_apply<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 2) % 3], U[(j + 1) % 3]);
+    k_batch_apply<V><<<g, 256, 0, st>>>(L, U(j), U(j - 1), U(j + 1));
     LAUNCH_CHECK();
     k_batch_finish<<<B->count, 256, 0, st>>>(L, 0);
     LAUNCH_CHECK();
-    k_batch_orth<V><<<g, 256, 0, st>>>(L, U[j % 3], U[(j + 1) % 3], ritz, ycoef);
+    k_batch_orth<V><<<g, 256, 0, st>>>(L, U(j), U(j + 1), ritz, ycoef);
     LAUNCH_CHECK();
     k_batch_finish<<<B->count, 256, 0, st>>>(L, 1);
     LAUNCH_CHECK();
@@ -2653,8 +2665,11 @@ extern "C" int ryd_batch_run(ryd_batch* B, const void* v0, int use_graph, void* 
   // order after the caller's stream, copy the start vectors, run, order back
   HIP_TRY(hipEventRecord(B->ev, caller));
   HIP_TRY(hipStreamWaitEvent(B->stream, B->ev, 0));
-  HIP_TRY(hipMemcpyAsync(B->d_vec, v0, es * (size_t)B->total, hipMemcpyDeviceToDevice, B->stream));
+  HIP_TRY(hipMemcpyAsync(batch_rows(B), v0, es * (size_t)B->total, hipMemcpyDeviceToDevice,
+                         B->stream));
+  B->basis_valid = false;
   if (int rc = batch_launch(B, B->choice, use_graph)) return rc;
+  B->basis_valid = B->basis != nullptr;
   HIP_TRY(hipEventRecord(B->ev, B->stream));
   HIP_TRY(hipStreamWaitEvent(caller, B->ev, 0));
   return RYD_OK;
@@ -2672,7 +2687,8 @@ extern "C" int ryd_batch_autotune(ryd_batch* B, const void* v0, int32_t* chosen_
   for (int k = 0; k < 3; ++k) {
     float ms = 0.f;
     for (int rep = 0; rep < 2; ++rep) {   // first pass warms the kernels
-      HIP_TRY(hipMemcpyAsync(B->d_vec, v0, es * (size_t)B->total, hipMemcpyDeviceToDevice, B->stream));
+      HIP_TRY(hipMemcpyAsync(batch_rows(B), v0, es * (size_t)B->total, hipMemcpyDeviceToDevice,
+                             B->stream));
       HIP_TRY(hipEventRecord(a, B->stream));
       if (int rc = batch_launch(B, k, 0)) return rc;
       HIP_TRY(hipEventRecord(b, B->stream));
@@ -2726,6 +2742,30 @@ extern "C" int ryd_ritz_lowest(int32_t count, const int32_t* steps, const double
   return RYD_OK;
 }
 
+extern "C" int64_t ryd_batch_basis_bytes(const ryd_batch* B) {
+  if (!B) return -1;
+  const int64_t es = B->dtype == RYD_F32 ? 4 : 8;
+  return es * (int64_t)(B->steps + 1) * B->total;
+}
+
+extern "C" int ryd_batch_attach_basis(ryd_batch* B, void* basis, int64_t bytes) {
+  if (!B) return fail(RYD_ERR_INVALID, "null batch");
+  if (basis && bytes < ryd_batch_basis_bytes(B))
+    return fail(RYD_ERR_INVALID, "ryd_batch_attach_basis: the basis needs ryd_batch_basis_bytes");
+  if (basis && (reinterpret_cast<uintptr_t>(basis) & 15))
+    return fail(RYD_ERR_INVALID, "ryd_batch_attach_basis: the basis must be 16-byte aligned");
+  // the captured graph holds the old row addresses
+  if (B->exec) {
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+    B->exec_choice = -1;
+  }
+  B->basis = basis;
+  B->basis_valid = false;
+  return RYD_OK;
+}
+
 extern "C" int ryd_batch_set_tile_bits(ryd_batch* B, int32_t tile_bits) {
   if (!B || tile_bits < 10 || tile_bits > 12)
     return fail(RYD_ERR_INVALID, "ryd_batch_set_tile_bits: tile bits must be 10, 11 or 12");
@@ -2733,13 +2773,27 @@ extern "C" int ryd_batch_set_tile_bits(ryd_batch* B, int32_t tile_bits) {
   return RYD_OK;
 }
 
+// Ritz vectors by combining the lent basis rows of the last run (the
+// coefficient table of k_batch_ritz_stored holds 2048 rows) or by replay.
+bool batch_combines(const ryd_batch* B) {
+  return B->basis && B->basis_valid && B->steps <= RYD_BATCH_COMBINE_MAX;
+}
+
 template <class V>
 int batch_ritz(ryd_batch* B, V* ritz, double* occ) {
   hipStream_t st = B->stream;
   const int k = B->choice;
-  k_batch_zero<V><<<stream_grid(B->total), 256, 0, st>>>(ritz, B->total);
-  LAUNCH_CHECK();
-  if (int rc = batch_sequence<V>(B, k, st, ritz, B->d_ycoef)) return rc;
+  if (batch_combines(B)) {
+    // the rows of the last run are still there: one streaming combination
+    const BatchLz L = batch_lz(B, k, 0);
+    k_batch_ritz_stored<V><<<(unsigned)B->ntiles[k], 256, 0, st>>>(
+        L, static_cast<const V*>(B->basis), B->total, B->steps, B->d_ycoef, ritz);
+    LAUNCH_CHECK();
+  } else {
+    k_batch_zero<V><<<stream_grid(B->total), 256, 0, st>>>(ritz, B->total);
+    LAUNCH_CHECK();
+    if (int rc = batch_sequence<V>(B, k, st, ritz, B->d_ycoef)) return rc;
+  }
   const BatchLz L = batch_lz(B, k, 0);
   k_batch_occ<V><<<(unsigned)B->ntiles[k], 256, 0, st>>>(L, ritz, B->d_occp);
   LAUNCH_CHECK();
@@ -2759,7 +2813,9 @@ extern "C" int ryd_batch_ritz(ryd_batch* B, const void* v0, const double* ycoef,
   HIP_TRY(hipStreamWaitEvent(B->stream, B->ev, 0));
   HIP_TRY(hipMemcpyAsync(B->d_ycoef, ycoef, sizeof(double) * (size_t)B->count * B->steps,
                          hipMemcpyHostToDevice, B->stream));
-  HIP_TRY(hipMemcpyAsync(B->d_vec, v0, es * (size_t)B->total, hipMemcpyDeviceToDevice, B->stream));
+  if (!batch_combines(B))   // a replay starts again from v0
+    HIP_TRY(hipMemcpyAsync(batch_rows(B), v0, es * (size_t)B->total, hipMemcpyDeviceToDevice,
+                           B->stream));
   const int rc = B->dtype == RYD_F32 ? batch_ritz<float>(B, static_cast<float*>(ritz), occ)
                                      : batch_ritz<double>(B, static_cast<double*>(ritz), occ);
   if (rc) return rc;

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 8
+#define RYD_ABI_VERSION 9
 
 enum ryd_status {
   RYD_OK = 0,
@@ -389,6 +389,18 @@ int ryd_batch_autotune(ryd_batch* batch, const void* v0, int32_t* chosen_tile_bi
 int ryd_ritz_lowest(int32_t count, const int32_t* steps, const double* alpha, int64_t lda,
                     const double* offd, int64_t ldo, const double* beta_last, double conv_tol,
                     double merge_rel, double* E, double* bound, double* y, int64_t ldy);
+/* Bytes of a whole Krylov basis of the batch: (steps + 1) rows of all
+ * instances in the batch's element type.  No reference counterpart. */
+int64_t ryd_batch_basis_bytes(const ryd_batch* batch);
+/* Lends the batch device memory (16-byte aligned, at least
+ * ryd_batch_basis_bytes) for its whole Krylov basis: the recurrence then
+ * keeps every row, and ryd_batch_ritz combines the rows of the last run
+ * (steps <= 2048) instead of replaying it.  basis = NULL returns to the
+ * batch's own three rotating rows.  The memory stays the caller's: it must
+ * outlive its use (detach or destroy the batch first); a caching allocator's
+ * block avoids a driver allocation and release of tens of GB per batch.
+ * Drops the captured graph.  No reference counterpart. */
+int ryd_batch_attach_basis(ryd_batch* batch, void* basis, int64_t bytes);
 /* Uses tiles of 2^tile_bits codes (10, 11 or 12) without timing, e.g. the
  * autotuner's choice for another batch of the same instances (an FP64
  * refinement of an FP32 bulk run).  No reference counterpart. */
@@ -398,8 +410,10 @@ int ryd_batch_set_tile_bits(ryd_batch* batch, int32_t tile_bits);
 int ryd_batch_results(const ryd_batch* batch, double* alpha, double* nrm, int32_t* stop);
 /* Ritz vectors and site occupations of every instance (the n_atom of
  * batched_ground_state, equilibrium.py:251-311, there from dense
- * eigenvectors).  Replays the recurrence from the same device start vectors
- * v0 with the chosen tile size, accumulating x_b = Σ_j ycoef[b][j] q_j
+ * eigenvectors).  With a lent basis, combines the rows of the last run
+ * (v0 must be that run's start vectors; it is not read), else replays the
+ * recurrence from the device start vectors v0 with the chosen tile size;
+ * either way x_b = Σ_j ycoef[b][j] q_j
  * (ycoef: host [count][steps], the tridiagonal eigenvector, zero past the
  * instance's steps) into ritz (device, batch element type, concatenated like
  * v0), then occ (device, [count][25]): occ[b][i] = Σ_s |x_b(s)|^2 n_i(s) for

@@ -89,3 +89,55 @@ def test_batch_occupations_vs_dense(pkg, dev, dtype, tol):
         assert np.max(np.abs(out['n_atom'][b] - ref)) < tol, (n, out['n_atom'][b], ref)
         checked += 1
     assert checked >= 6
+
+
+@pytest.mark.parametrize('dtype,rtol', [('float64', 1e-12), ('float32', 1e-5)])
+def test_batch_stored_basis_matches_replay(pkg, dev, dtype, rtol):
+    """A lent basis (every Lanczos row kept) gives the replay's recurrence bit
+    for bit and its Ritz vectors and occupations to rounding, including an
+    instance with a zero start vector (stopped before its first step, as a
+    finished instance of a restarted batch is): its rows are never written,
+    so the lent memory is filled with NaN first, and its zero coefficients
+    must skip them."""
+    import ctypes
+    import torch
+    ns = [4, 7, 10, 11, 12, 9]
+    models, _ = _models(pkg, ns, seed=6)
+    Bs = pkg.BatchLanczos(models, steps=90, dtype=dtype, device=dev, store_basis=True)
+    Br = pkg.BatchLanczos(models, steps=90, dtype=dtype, device=dev, store_basis=False)
+    assert Bs.stores_basis and not Br.stores_basis
+    Bs._basis.view(Bs.tdt).fill_(float('nan'))
+    v0 = Bs.start_vectors(2)
+    v0[int(Bs.offsets[1]):int(Bs.offsets[2])] = 0.0
+    for graph in (False, True):
+        Bs.run(v0, use_graph=graph)
+        Br.run(v0, use_graph=graph)
+        a_s, n_s, st_s = Bs.coefficients()
+        a_r, n_r, st_r = Br.coefficients()
+        live = np.arange(len(ns)) != 1      # instance 1's rows past its stop are never set
+        assert np.array_equal(a_s[live], a_r[live]) and np.array_equal(n_s[live], n_r[live])
+        assert np.array_equal(st_s, st_r) and n_s[1, 0] == 0.0
+    assert st_s[1] == 1
+    Bs.ground_states()
+    Br.ground_states()
+    which = np.zeros(len(ns), np.int64)
+    which[1] = -1
+    x_s, occ_s = Bs.ritz(v0, Bs.restart_coefficients(which))
+    x_r, occ_r = Br.ritz(v0, Br.restart_coefficients(which))
+    assert torch.isfinite(x_s).all()
+    assert not x_s[int(Bs.offsets[1]):int(Bs.offsets[2])].any()
+    scale = torch.linalg.vector_norm(x_r.double()).item()
+    assert torch.max(torch.abs(x_s.double() - x_r.double())).item() <= rtol * scale
+    assert np.allclose(occ_s.cpu().numpy(), occ_r.cpu().numpy(), rtol=10 * rtol, atol=0)
+    # too small a buffer is refused; detaching returns to the rotating rows
+    L = pkg._native.lib()
+    need = L.ryd_batch_basis_bytes(Bs._h)
+    assert need == (90 + 1) * Bs.total * Bs._basis.element_size() * (8 if dtype == 'float64' else 4)
+    assert L.ryd_batch_attach_basis(Bs._h, ctypes.c_void_p(Bs._basis.data_ptr()),
+                                    need - 1) == pkg._native.RYD_ERR_INVALID
+    pkg._native.check(L.ryd_batch_attach_basis(Bs._h, None, 0))
+    Bs.run(v0, use_graph=True)
+    x_d, _ = Bs.ritz(v0, Bs.restart_coefficients(which))
+    assert torch.max(torch.abs(x_d.double() - x_r.double())).item() <= rtol * scale
+    Bs.close()
+    Br.close()
