@@ -9,7 +9,8 @@ Prints one JSON line (value = H·v/s in FP64; the FP32 form is reported in
 ``fp32``; the five configurations follow in ``configs`` with ``g1``, the
 blockade-graph basis of a 6x7 lattice enumerated on the device, and the
 batched Lanczos solves/s and the adjoint grad/s of the metric at the top
-level).  ``roofline`` is for the dominant kernel (k_full_tiled) with
+level; ``step_rates`` times one Chebyshev step and one Lanczos step at the
+same size against 3·D·b).  ``roofline`` is for the dominant kernel (k_full_tiled) with
 algorithmic bytes 2·D·sizeof(element) (x read once, y written once) over
 its mean duration measured with HIP events on the launch stream.
 ``cpu_baseline`` times the oracle's OpenMP matrix-free H·v on the host
@@ -345,6 +346,15 @@ def main():
         print(json.dumps({k: v for k, v in res[args.only].items()}))
         return
     f64 = res['f64']
+    from energy_level_amd import workloads as W
+    step_rates = {}
+    for name, dt in (('f64', torch.float64), ('f32', torch.float32)):
+        step_rates[name] = W.time_steps(plan, dt, steps=10)
+        torch.cuda.empty_cache()
+        print('[bench] %s steps: Chebyshev %.3f ms, Lanczos %.3f ms (3·D·b at %.0f / %.0f GB/s)'
+              % (name, step_rates[name]['cheb_step_ms'], step_rates[name]['lanczos_step_ms'],
+                 step_rates[name]['cheb_step_gbps_3Db'], step_rates[name]['lanczos_step_gbps_3Db']),
+              file=sys.stderr, flush=True)
     host = _time_hv_host(plan, 3)
     print('[bench] host-resident vectors: %.1f ms/H·v' % host['ms_per_step'], file=sys.stderr,
           flush=True)
@@ -386,6 +396,7 @@ def main():
                  'frac': res['f32']['achieved_gbps'] / HBM_PEAK_GBPS},
         'cpu_baseline': cpu,
         'pcie_inclusive': host,
+        'step_rates': step_rates,
         'configs': cfg,
     }
     c4, c5 = cfg.get('c4') or {}, cfg.get('c5') or {}

@@ -333,6 +333,11 @@ __global__ __launch_bounds__(1 << TT) void k_full_tiled(FullModel M, ApplyIO<V> 
 #ifndef RYD_TOP_PREFETCH
 #define RYD_TOP_PREFETCH 1
 #endif
+// z of an FP32 step in the same burst: n = 28 Chebyshev step 1.64 -> 1.33 ms,
+// n = 25 0.204 -> 0.171 ms; FP64 loses (2.82 -> 2.92 ms: the registers)
+#ifndef RYD_TOP_ZPRE
+#define RYD_TOP_ZPRE 1
+#endif
 #ifndef RYD_TOPB4
 #define RYD_TOPB4 8
 #endif
@@ -411,7 +416,9 @@ __global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, Appl
   // read of the block is in flight before the barrier (n = 28: FP64
   // 2.65 -> 2.53 ms, FP32 1.33 -> 1.28 ms per H·v)
   constexpr bool PRE = RYD_TOP_PREFETCH && J * sizeof(V) <= 128 && (!VT<V>::cplx || J <= 8);
+  constexpr bool PREZ = PRE && RYD_TOP_ZPRE && sizeof(V) == 4;
   V pa[PRE ? J : 1];
+  V zv[PREZ ? J : 1];
   {
     V xv[J];
 #pragma unroll
@@ -419,6 +426,12 @@ __global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, Appl
     if constexpr (PRE) {
 #pragma unroll
       for (int j = 0; j < J; ++j) pa[j] = top_load(io.y + ebase + j * jstride);
+    }
+    if constexpr (PREZ) {
+      if (io.z) {
+#pragma unroll
+        for (int j = 0; j < J; ++j) zv[j] = top_load(io.z + ebase + j * jstride);
+      }
     }
 #pragma unroll
     for (int j = 0; j < J; ++j) lds[u + NU * j] = vscale(xv[j], sr);
@@ -449,7 +462,10 @@ __global__ __launch_bounds__(1 << top_ub<V>()) void k_full_top(FullModel M, Appl
     const V xj = lds[u + NU * j];
     V v = vscale(a, chr);
     vfma(v, cxr, xj);
-    if (io.z) vfma(v, czr, top_load(io.z + e));
+    if (io.z) {
+      if constexpr (PREZ) vfma(v, czr, zv[j]);
+      else vfma(v, czr, top_load(io.z + e));
+    }
 #if RYD_TOP_NT
     nt_store(io.y + e, v);
 #else

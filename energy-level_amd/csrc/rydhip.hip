@@ -181,15 +181,25 @@ void build_tables(int n, int tb, const std::vector<double>& dl, const std::vecto
 // Split threshold: from this n on, the top bits run in a second pass
 // (k_full_top) so that pass A's partner tiles stay within the caches.  The
 // split costs a write and a read of the partial: it pays from n = 23 for
-// 8- and 16-byte elements, from n = 25 for an FP32 H·v, and only beyond
-// n = 30 for an FP32 step that also streams z (measured on MI355X).
+// 8- and 16-byte elements and for an FP32 H·v, from n = 25 for an FP32 step
+// that also streams z.  Measured on MI355X, FP32 ms per H·v / per Chebyshev
+// step (H·v with z): n = 23 split 0.035 / 0.050, one pass 0.040 / 0.045;
+// n = 24 0.076 / 0.107 against 0.085 / 0.093; n = 26 0.27 / 0.40 against
+// 0.27 / 0.44; n = 28 1.18 / 1.64 against 1.18 (split) / 2.12.
 #ifndef RYD_SPLIT_MIN_N
 #define RYD_SPLIT_MIN_N 23
+#endif
+// FP32 thresholds above SPLIT_MIN_N: an H·v, and a step that streams z
+#ifndef RYD_SPLIT4_DN
+#define RYD_SPLIT4_DN 0
+#endif
+#ifndef RYD_SPLIT4Z_DN
+#define RYD_SPLIT4Z_DN 2
 #endif
 constexpr int SPLIT_MIN_N = RYD_SPLIT_MIN_N;
 constexpr int split_min_n(size_t elem, bool with_z) {
   if (elem >= 8) return SPLIT_MIN_N;
-  return with_z ? SPLIT_MIN_N + 8 : SPLIT_MIN_N + 2;
+  return with_z ? SPLIT_MIN_N + RYD_SPLIT4Z_DN : SPLIT_MIN_N + RYD_SPLIT4_DN;
 }
 
 // Chain sector: split the top FIB_K bits into a second pass from this n on.

@@ -12,6 +12,7 @@ import time
 
 import numpy as np
 
+from . import _native as nat
 from .model import from_positions, chain_positions, square_positions, c6_coupling, from_calib
 
 __all__ = ['chain_calib', 'lattice_calib', 'warm_device_memory', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5',
@@ -58,6 +59,63 @@ def time_hv(plan, dtype, steps=10, warmup=2, seed=0):
     b.record()
     torch.cuda.synchronize()
     return a.elapsed_time(b) / steps
+
+
+def time_steps(plan, dtype, steps=10, warmup=2, seed=0):
+    """Mean time (ms) of one Chebyshev step and of one Lanczos step on
+    ``plan``, HIP events on the current stream, vectors resident.
+
+    Chebyshev: y = 2·H̃x − z as one fused launch (``ryd_apply`` with z, the
+    step of kpm.py:146-148).  Lanczos: ``steps`` device steps of
+    ``ryd_lanczos`` (the apply with −β·u_{j−1} and the α/β sums, the
+    orthogonalisation u_{j+1} = (w − α u_j)/β, the scalar finishes), as
+    torch_lanczos_ground_state's Krylov loop (matrix_free.py:231-251) runs
+    them.  Both compare with 3·D·b, the bytes of a step that reads two
+    vectors and writes one."""
+    import ctypes
+    import torch
+    g = torch.Generator(device=plan.device)
+    g.manual_seed(seed)
+    x = torch.randn(plan.dim, dtype=torch.float64, device=plan.device, generator=g).to(dtype)
+    z = torch.randn(plan.dim, dtype=torch.float64, device=plan.device, generator=g).to(dtype)
+    y = torch.empty_like(x)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    for _ in range(warmup):
+        plan.apply(x, y, s=0.01, ch=2.0, cz=-1.0, z=z)
+    a, b = ev(), ev()
+    a.record()
+    for _ in range(steps):
+        plan.apply(x, y, s=0.01, ch=2.0, cz=-1.0, z=z)
+    b.record()
+    torch.cuda.synchronize()
+    cheb = a.elapsed_time(b) / steps
+    L = nat.lib()
+    code = nat.dtype_code(dtype)
+    alpha = torch.zeros(warmup + steps + 1, dtype=torch.float64, device=plan.device)
+    nrm = torch.zeros(warmup + steps + 2, dtype=torch.float64, device=plan.device)
+    state = torch.zeros(int(L.ryd_lanczos_state_bytes()), dtype=torch.uint8, device=plan.device)
+    tiny = 1e-6 if dtype == torch.float32 else 1e-13
+    st = nat.stream_ptr()
+
+    def run(j0, k):
+        nat.check(L.ryd_lanczos(plan.handle, code, nat.ptr(x), nat.ptr(y), nat.ptr(z), j0, k,
+                                nat.ptr(alpha), nat.ptr(nrm), nat.ptr(state), float(tiny), None,
+                                None, None, None, 0, nat.ptr(plan.workspace()), st))
+    run(0, warmup)
+    a, b = ev(), ev()
+    a.record()
+    run(warmup, steps)
+    b.record()
+    torch.cuda.synchronize()
+    flag, nvalid = ctypes.c_int(0), ctypes.c_int64(0)
+    nat.check(L.ryd_lanczos_status(nat.ptr(state), ctypes.byref(flag), ctypes.byref(nvalid), st))
+    if flag.value:
+        raise RuntimeError('Lanczos timing run broke down at step %d' % nvalid.value)
+    lz = a.elapsed_time(b) / steps
+    bts = 3.0 * plan.dim * x.element_size()
+    return dict(cheb_step_ms=cheb, cheb_step_gbps_3Db=bts / (cheb * 1e-3) / 1e9,
+                lanczos_step_ms=lz, lanczos_step_gbps_3Db=bts / (lz * 1e-3) / 1e9,
+                bytes_3Db=bts)
 
 
 def run_c1(device='cuda', steps=200, seed=0):
