@@ -32,9 +32,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes of one FP64 H.v at N=28 (both launches), measured with rocprofv3
-# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 3.04 + 2.22 GB,
-# pass B 4.30 + 2.15 GB
-MEASURED_TRAFFIC_F64_N28 = 11.71e9
+# FETCH_SIZE/WRITE_SIZE scaled by a plain 2 GiB copy: pass A 3.37 + 2.15 GB,
+# pass B 4.29 + 2.15 GB
+MEASURED_TRAFFIC_F64_N28 = 11.96e9
 N_SITES = 28
 
 

@@ -72,7 +72,9 @@ __device__ __forceinline__ void apply_scalars(const ApplyIO<V>& io, double& s, d
 // Measured at n = 28: FP32 13-bit tiles 1.47 ms per H·v against 1.52 ms at
 // 12 bits (and C2's KPM steps 3 % faster); FP64 11-bit tiles or an unrolled
 // far-bit loop were slower (with the 1024-thread pass A, unrolled by 2 or 4:
-// 2.58 against 2.52 ms per FP64 H·v).
+// 2.58 against 2.52 ms per FP64 H·v).  Halving them (RYD_TILE_R8 = 2,
+// RYD_TILE_R4 = 4: pass-A tiles of 2^13 FP64 / 2^14 FP32 codes, two blocks
+// per CU, one more far bit) loses: 2.25 -> 2.35 ms FP64, 1.19 -> 1.26 ms FP32.
 #ifndef RYD_TILE_R4
 #define RYD_TILE_R4 8
 #endif
