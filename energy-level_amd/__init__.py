@@ -17,8 +17,8 @@ from .basis import fibonacci, chain_blockade_dim, is_chain_blockade_basis, secto
 from .sector_scan import scan_ground_sectors_hip
 from .evolve import CONV, cheb_coefficients, terms_for, PulseProblem, PulseEvolution, z2_index, \
     slice_bounds, segment_intervals, segment_drive, propagate_segments_hip
-from .batch import BatchLanczos, disorder_ensemble, batched_ground_state_hip, \
-    batched_ground_state_dense_hip
+from .batch import BatchLanczos, BatchCsrLanczos, CsrInstance, disorder_ensemble, \
+    batched_ground_state_hip, batched_ground_state_dense_hip, batched_ground_state_csr_hip
 from .dispatch import EQ_BACKENDS_HIP, normalize_backend, solve_equilibrium_hip, \
     spectrum_lanczos_hip, quadrature, lorentz_spectrum, source_spectrum_hip, resolve_source
 from .results import describe_result
@@ -29,7 +29,7 @@ from .kpm import kpm_moments_hip, spectral_bounds, jackson_kernel, thermal_from_
     dos_from_moments, probe_bits, kpm_moments_gpu_hip
 
 __all__ = ['hamiltonian_linear_operator_hip', 'loop_observables', 'link_loop_observables', 'RydModel', 'from_calib', 'active_terms', 'chain_positions', 'square_positions',
-           'from_positions', 'c6_coupling', 'Plan', 'CsrPlan', 'batched_ground_state_dense_hip', 'cast', 'kpm_moments_gpu_hip', 'hip_lanczos_ground_state',
+           'from_positions', 'c6_coupling', 'Plan', 'CsrPlan', 'batched_ground_state_dense_hip', 'batched_ground_state_csr_hip', 'BatchCsrLanczos', 'CsrInstance', 'cast', 'kpm_moments_gpu_hip', 'hip_lanczos_ground_state',
            'lanczos_run', 'ritz_vectors', 'rayleigh_residual', 'site_occupations',
            'ground_is_unique', 'locked_lowest',
            'kpm_moments_hip', 'spectral_bounds', 'jackson_kernel', 'thermal_from_moments',

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, 'librydhip.so')
 
 RYD_OK, RYD_ERR_INVALID, RYD_ERR_MEMORY, RYD_ERR_DEVICE, RYD_ERR_UNSUPPORTED = range(5)
 RYD_F32, RYD_F64, RYD_C64, RYD_C128 = range(4)
-ABI_VERSION = 9
+ABI_VERSION = 10
 RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH, RYD_BASIS_CSR = range(5)
 
 _vp = ctypes.c_void_p
@@ -92,6 +92,9 @@ SIGNATURES = {
                                                  ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     'ryd_batch_create': (ctypes.c_int, [ctypes.POINTER(Model), ctypes.c_int32, ctypes.c_int,
                                         ctypes.c_int32, ctypes.POINTER(_vp)]),
+    'ryd_batch_create_csr': (ctypes.c_int, [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                            ctypes.c_int32, ctypes.POINTER(_vp)]),
+    'ryd_batch_set_lock': (ctypes.c_int, [_vp, _vp]),
     'ryd_batch_destroy': (ctypes.c_int, [_vp]),
     'ryd_batch_offset': (_i64, [_vp, ctypes.c_int32]),
     'ryd_batch_run': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),

@@ -18,8 +18,8 @@ import numpy as np
 from . import _native as nat
 from .model import RydModel, from_positions, chain_positions, c6_coupling, from_calib
 
-__all__ = ['BatchLanczos', 'disorder_ensemble', 'batched_ground_state_hip',
-           'certified_ground_states']
+__all__ = ['BatchLanczos', 'BatchCsrLanczos', 'CsrInstance', 'disorder_ensemble',
+           'batched_ground_state_hip', 'batched_ground_state_csr_hip', 'certified_ground_states']
 
 BMAXN = 24            # largest instance n (occupation rows hold BMAXN sites + the norm)
 
@@ -97,19 +97,27 @@ class BatchLanczos:
         self.total = int(self.offsets[-1])
         self.tile_bits = 12
         self.autotune_ms = None
+        self._lend_basis(store_basis)
+
+    def _lend_basis(self, store_basis):
+        """Lend a torch buffer for the whole Krylov basis (``store_basis`` as
+        in the class docstring)."""
+        import torch
         self._basis = None
         if store_basis not in (True, False, 'auto'):
             raise ValueError("store_basis must be True, False or 'auto'")
-        if store_basis:
-            need = int(L.ryd_batch_basis_bytes(h))
-            if _basis_fits(self.device, need, cached_only=store_basis == 'auto'):
-                try:
-                    buf = torch.empty(need, dtype=torch.uint8, device=self.device)
-                except torch.OutOfMemoryError:
-                    buf = None
-                if buf is not None:
-                    nat.check(L.ryd_batch_attach_basis(h, nat.ptr(buf), need))
-                    self._basis = buf
+        if not store_basis:
+            return
+        L = nat.lib()
+        need = int(L.ryd_batch_basis_bytes(self._h))
+        if _basis_fits(self.device, need, cached_only=store_basis == 'auto'):
+            try:
+                buf = torch.empty(need, dtype=torch.uint8, device=self.device)
+            except torch.OutOfMemoryError:
+                buf = None
+            if buf is not None:
+                nat.check(L.ryd_batch_attach_basis(self._h, nat.ptr(buf), need))
+                self._basis = buf
 
     @property
     def stores_basis(self):
@@ -263,6 +271,137 @@ class BatchLanczos:
         return np.ascontiguousarray(out + np.where(two, y[:, 1], 0.0))
 
 
+class CsrInstance:
+    """One Hermitian instance for :class:`BatchCsrLanczos`: any sparse or
+    dense square matrix (rows and columns in the order of its vectors),
+    kept as a sorted CSR without explicit zeros.  ``n`` is its qubit count
+    when dim = 2^n (n <= 24: occupations, site i = bit n-1-i of the row
+    index), else 0."""
+
+    def __init__(self, H):
+        import scipy.sparse as sp
+        A = sp.csr_matrix(H)
+        if A.ndim != 2 or A.shape[0] != A.shape[1]:
+            raise ValueError('instances must be square, got %r' % (A.shape,))
+        A.eliminate_zeros()
+        A.sum_duplicates()
+        A.sort_indices()
+        self.H = A
+        self.dim = int(A.shape[0])
+        self.complex = bool(np.iscomplexobj(A.data) and np.any(A.data.imag != 0))
+        bits = max(self.dim - 1, 0).bit_length()
+        self.n = bits if (1 << bits) == self.dim and bits <= BMAXN else 0
+
+
+class BatchCsrLanczos(BatchLanczos):
+    """``ryd_batch_create_csr`` owner: the batched Lanczos of
+    :class:`BatchLanczos` over :class:`CsrInstance` items (any Hermitian
+    matrices, any dims).  ``dtype`` 'float32' / 'float64' is the precision;
+    the vectors are complex when any instance is.  The tile size is fixed
+    (``autotune`` and ``set_tile_bits`` do nothing); ``set_lock`` runs the
+    recurrence in the complement of per-instance vectors.  ``force_complex``
+    keeps complex vectors for real instances too (a batch shrunk out of a
+    mixed one continues from complex restart vectors)."""
+
+    def __init__(self, instances, steps=150, dtype='float64', device='cuda', store_basis='auto',
+                 force_complex=False):
+        import torch
+        nat.require_gpu()
+        if dtype not in ('float32', 'float64'):
+            raise ValueError('dtype must be float32/float64')
+        self.models = [m if isinstance(m, CsrInstance) else CsrInstance(m) for m in instances]
+        self.count = len(self.models)
+        if self.count < 1:
+            raise ValueError('empty batch')
+        self.steps = int(steps)
+        self.dtype = dtype
+        cplx = bool(force_complex) or any(m.complex for m in self.models)
+        self.tdt = {(False, 'float32'): torch.float32, (False, 'float64'): torch.float64,
+                    (True, 'float32'): torch.complex64, (True, 'float64'): torch.complex128}[
+            (cplx, dtype)]
+        dv = torch.device(device)
+        self.device = torch.device('cuda', torch.cuda.current_device() if dv.index is None
+                                   else dv.index)
+        dims = np.array([m.dim for m in self.models], np.int64)
+        nnz = np.array([m.H.nnz for m in self.models], np.int64)
+        base = np.concatenate([[0], np.cumsum(nnz)[:-1]])
+        indptr = np.ascontiguousarray(np.concatenate(
+            [[0]] + [m.H.indptr[1:].astype(np.int64) + base[k] for k, m in enumerate(self.models)]),
+            np.int64)
+        indices = np.ascontiguousarray(np.concatenate(
+            [m.H.indices.astype(np.int64) for m in self.models]), np.int64)
+        data = np.concatenate([m.H.data for m in self.models])
+        val_re = np.ascontiguousarray(data.real, np.float64)
+        val_im = np.ascontiguousarray(data.imag, np.float64) if cplx else None
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            nat.check(nat.lib().ryd_batch_create_csr(
+                self.count, dims.ctypes.data, indptr.ctypes.data,
+                indices.ctypes.data if indices.size else None,
+                val_re.ctypes.data if val_re.size else None,
+                val_im.ctypes.data if val_im is not None and val_im.size else None,
+                nat.dtype_code(self.tdt), self.steps, ctypes.byref(h)))
+        self._h = h
+        L = nat.lib()
+        self.offsets = np.array([L.ryd_batch_offset(h, i) for i in range(self.count + 1)], np.int64)
+        self.total = int(self.offsets[-1])
+        self.tile_bits = None
+        self.autotune_ms = None
+        self._lock = None
+        self._lend_basis(store_basis)
+
+    def close(self):
+        super().close()
+        self._lock = None
+
+    def start_vectors(self, seed=0):
+        """Concatenated N(0,1) start vectors (complex: both parts) from a
+        seeded device generator."""
+        import torch
+        g = torch.Generator(device=self.device)
+        g.manual_seed(int(seed))
+        dt = torch.complex128 if self.tdt.is_complex else torch.float64
+        return torch.randn(self.total, dtype=dt, device=self.device, generator=g).to(self.tdt)
+
+    def set_tile_bits(self, bits):
+        pass
+
+    def autotune(self, v0):
+        return None
+
+    def set_lock(self, g):
+        """Lock vectors (concatenated unit vectors in the batch's layout, a
+        device tensor kept alive here) or None."""
+        if g is not None:
+            if g.numel() != self.total or g.dtype != self.tdt or g.device != self.device:
+                raise ValueError('lock vectors must be a %s tensor of length %d on %s'
+                                 % (self.tdt, self.total, self.device))
+            g = g.contiguous()
+        nat.check(nat.lib().ryd_batch_set_lock(self._h, nat.ptr(g)))
+        self._lock = g
+
+    def project_off(self, v, g):
+        """v with each instance's component along its g removed (device op)."""
+        import torch
+        inst = self.instance_index().long()
+        c = torch.zeros(self.count, dtype=v.dtype, device=v.device)
+        c.index_add_(0, inst, torch.conj(g) * v)
+        return v - c[inst] * g
+
+
+def _make_batch(items, steps, dtype, device, store_basis, force_complex=False):
+    """The batch kind for these instances: full-basis Rydberg models, or CSR."""
+    if isinstance(items[0], RydModel):
+        return BatchLanczos(items, steps=steps, dtype=dtype, device=device,
+                            store_basis=store_basis)
+    return BatchCsrLanczos(items, steps=steps, dtype=dtype, device=device,
+                           store_basis=store_basis, force_complex=force_complex)
+
+
+def _any_complex(items):
+    return any(getattr(m, 'complex', False) for m in items)
+
+
 def disorder_ensemble(count=512, nmin=10, nmax=20, sigma=0.1, deltas=None, c6=10.0, omega=1.0,
                       seed=0):
     """Chain instances with seeded N ~ U{nmin..nmax}, Gaussian jitter σ on all
@@ -293,47 +432,103 @@ def _segments(B, x, local):
     return torch.cat([x[int(B.offsets[i]):int(B.offsets[i + 1])] for i in local])
 
 
+def _reset_records(out, count):
+    """Per-pair certification records of a stage (E0 and E1 separately, each
+    from the first cycle whose bound passes)."""
+    out['_rec'] = dict(c0=np.zeros(count, bool), c1=np.zeros(count, bool),
+                       E0=np.zeros(count), E1=np.zeros(count),
+                       b0=np.full(count, np.inf), b1=np.full(count, np.inf))
+
+
 def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph, tile_bits,
-           device, occ_rows=None, seeds=None, finish_last=True, store_basis='auto'):
+           device, occ_rows=None, seeds=None, finish_last=True, store_basis='auto', vecs=None,
+           want=2, lock=None):
     """Batched Lanczos cycles over the instances ``idx`` (indices into
     ``models``) from their concatenated start vectors ``v``, until each
-    instance's two lowest genuine Ritz pairs have bounds <= tol·max(1,|E|)
-    or ``cycles`` are spent.
+    instance's lowest ``want`` genuine Ritz pairs have bounds <=
+    tol·max(1,|E|) or ``cycles`` are spent.  Each pair is certified by the
+    first cycle whose bound passes (``out['_rec']``, kept across the stages
+    of one precision): once both pairs are nearly converged, a restart from
+    their sum starts a recurrence whose next norm is tiny, and the rounding
+    it amplifies keeps the two bounds from passing in the same cycle.
 
     A finished instance records E0, gap, bounds and flags in ``out`` (at its
-    index) and then its lowest Ritz vector's site weights (``occ_rows``) or,
-    with ``seeds`` (a dict), the sum of its two lowest Ritz vectors as the
-    seed of a next stage.  The others restart from that sum: while they are
-    at least half of the batch they stay in it (a finished instance gets a
-    zero start vector, which the batch kernels skip: its norm is zero, so its
-    stop flag is set before the first step); below that the cycle continues
-    on a new batch of those instances only, so that the launches shrink with
-    the work.  With ``finish_last`` False the instances still unconverged
-    after the last cycle are not recorded.  Returns (cycles run, indices
-    still unconverged, their restart vectors concatenated, or None)."""
+    index) and then its lowest Ritz vector's site weights (``occ_rows``) and,
+    with ``vecs`` (a dict), that vector itself, or, with ``seeds`` (a dict),
+    the sum of its two lowest Ritz vectors as the seed of a next stage.  The
+    others restart from that sum: while they are at least half of the batch
+    they stay in it (a finished instance gets a zero start vector, which the
+    batch kernels skip: its norm is zero, so its stop flag is set before the
+    first step); below that the cycle continues on a new batch of those
+    instances only, so that the launches shrink with the work.  With
+    ``lock`` (a dict of vectors, CSR batches) every batch runs in the
+    complement of each instance's vector (``BatchCsrLanczos.set_lock``; the
+    start and restart vectors are projected off it), so its lowest pair is
+    the next level.  With ``finish_last`` False the instances still
+    unconverged after the last cycle are not recorded.  Returns (cycles run,
+    indices still unconverged, their restart vectors concatenated, or None)."""
+    import torch
+    fc = _any_complex(models)
     idx = np.asarray(idx, np.int64)
-    B = BatchLanczos([models[i] for i in idx], steps=steps, dtype=dtype, device=device,
-                     store_basis=store_basis)
-    try:
+    R = out['_rec']
+
+    def new_batch(sel):
+        B = _make_batch([models[i] for i in sel], steps, dtype, device, store_basis, fc)
         if tile_bits is not None:
             B.set_tile_bits(tile_bits)
+        G = None
+        if lock is not None:
+            G = torch.cat([lock[int(i)] for i in sel]).to(B.tdt)
+            inst = B.instance_index().long()
+            nrm = torch.zeros(B.count, dtype=torch.float64, device=B.device)
+            nrm.index_add_(0, inst, (torch.abs(G) ** 2).to(torch.float64))
+            G = G / torch.sqrt(nrm)[inst].to(G.dtype)
+            B.set_lock(G)
+        return B, G
+
+    B, G = new_batch(idx)
+    if G is not None:
+        v = B.project_off(v.to(B.tdt), G)
+    try:
         act = np.ones(len(idx), bool)
         done = 0
         for cyc in range(int(cycles)):
             B.run(v, use_graph=use_graph)
             done += 1
             gs = B.ground_states(tol, act, merge_rel)
+            E1 = gs['E0'] + gs['gap']
+            ok0 = act & gs['converged_E0']
+            with np.errstate(invalid='ignore'):
+                ok1 = act & (gs['ritz_bound_gap'] <= tol * np.maximum(1.0, np.abs(E1)))
+            n0 = ok0 & ~R['c0'][idx]
+            R['c0'][idx[n0]] = True
+            R['E0'][idx[n0]] = gs['E0'][n0]
+            R['b0'][idx[n0]] = gs['ritz_bound'][n0]
+            n1 = ok1 & ~R['c1'][idx]
+            R['c1'][idx[n1]] = True
+            R['E1'][idx[n1]] = E1[n1]
+            R['b1'][idx[n1]] = gs['ritz_bound_gap'][n1]
+            cert = R['c0'][idx] & (R['c1'][idx] if want == 2 else True)
             last = cyc == cycles - 1
-            fin = act & (gs['converged'] | (last and finish_last))
+            fin = act & (cert | (last and finish_last))
             g = idx[fin]
-            for key in ('E0', 'gap', 'ritz_bound', 'ritz_bound_gap', 'converged', 'converged_E0'):
-                out[key][g] = gs[key][fin]
+            c0f, c1f = R['c0'][g], R['c1'][g]
+            e0 = np.where(c0f, R['E0'][g], gs['E0'][fin])
+            out['E0'][g] = e0
+            out['gap'][g] = np.where(c1f, R['E1'][g], E1[fin]) - e0
+            out['ritz_bound'][g] = np.where(c0f, R['b0'][g], gs['ritz_bound'][fin])
+            out['ritz_bound_gap'][g] = np.where(c1f, R['b1'][g], gs['ritz_bound_gap'][fin])
+            out['converged_E0'][g] = c0f
+            out['converged'][g] = (c0f & c1f) if want == 2 else c0f
             out['steps_' + dtype][idx[act]] += gs['steps'][act]
             rest = act & ~fin
             which = np.where(rest, 1, np.where(fin, 1 if seeds is not None else 0, -1))
-            if not (np.any(rest) or seeds is not None or occ_rows is not None):
+            if not (np.any(rest) or seeds is not None or occ_rows is not None
+                    or vecs is not None):
                 return done, idx[:0], None
             x, occ = B.ritz(v, B.restart_coefficients(which))
+            if G is not None:
+                x = B.project_off(x, G)
             if occ_rows is not None and np.any(fin):
                 occ_np = occ.cpu().numpy()
                 for l in np.nonzero(fin)[0]:
@@ -341,6 +536,9 @@ def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph,
             if seeds is not None:
                 for l in np.nonzero(fin)[0]:
                     seeds[int(idx[l])] = x[int(B.offsets[l]):int(B.offsets[l + 1])].clone()
+            elif vecs is not None:
+                for l in np.nonzero(fin)[0]:
+                    vecs[int(idx[l])] = x[int(B.offsets[l]):int(B.offsets[l + 1])].clone()
             if not np.any(rest):
                 return done, idx[:0], None
             if last:
@@ -351,10 +549,9 @@ def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph,
                 v = _segments(B, x, loc)
                 idx = idx[loc]
                 B.close()
-                B = BatchLanczos([models[i] for i in idx], steps=steps, dtype=dtype, device=device,
-                                 store_basis=store_basis)
-                if tile_bits is not None:
-                    B.set_tile_bits(tile_bits)
+                B, G = new_batch(idx)
+                if G is not None:
+                    v = B.project_off(v.to(B.tdt), G)
                 act = np.ones(len(idx), bool)
             else:
                 v = _masked(B, x, rest)
@@ -367,7 +564,7 @@ def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph,
 def certified_ground_states(models, steps=150, dtype='float32', refine=True, bulk_tol=1e-6,
                             tol=1e-9, bulk_cycles=2, refine_schedule=((60, 1), (300, 1), (1000, 3)),
                             seed=0, device='cuda', autotune=True, use_graph=True, observables=True,
-                            tile_bits=None, seed_noise=1e-3, store_basis='auto'):
+                            tile_bits=None, seed_noise=1e-3, store_basis='auto', vectors=None):
     """Batched ground states with a convergence test per instance.
 
     Bulk: ``steps``-step batched Lanczos cycles in ``dtype`` (one captured
@@ -398,7 +595,11 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
     vector).  ``tile_bits`` skips the autotuner.  ``store_basis`` is passed
     to every :class:`BatchLanczos` of the stages (True lets the first call
     take the basis memory into torch's cache; a repeated call with 'auto'
-    then reuses it: 3.72 s instead of 3.88 s on the ensemble above)."""
+    then reuses it: 3.72 s instead of 3.88 s on the ensemble above).
+    ``models`` are :class:`RydModel` instances (the full-basis batch) or
+    :class:`CsrInstance` items (any Hermitian matrices, ``BatchCsrLanczos``);
+    ``vectors`` (a dict) receives each instance's final lowest Ritz vector
+    (device, the batch's layout and element type)."""
     import torch
     count = len(models)
     two_stage = bool(refine) and dtype == 'float32'
@@ -409,7 +610,7 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
     occ_rows = [None] * count if observables else None
     merge = {'float32': 1e-6, 'float64': 1e-10}
     schedule = [(int(st), int(cy)) for st, cy in refine_schedule if int(cy) > 0]
-    B = BatchLanczos(models, steps=steps, dtype=dtype, device=device, store_basis=False)
+    B = _make_batch(models, steps, dtype, device, False, _any_complex(models))
     try:
         v0 = B.start_vectors(seed)
         if tile_bits is not None:
@@ -424,9 +625,11 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
     # one stage (an FP64 bulk, or no refinement) takes the whole cycle budget
     budget = bulk_cycles if two_stage else bulk_cycles + sum(cy for _, cy in schedule)
     seeds = {} if two_stage else None
+    _reset_records(out, count)
     done, _, _ = _stage(models, all_idx, v0, dtype, steps, bulk_tol if two_stage else tol,
                         merge[dtype], budget, out, use_graph, tile_bits, device,
-                        None if two_stage else occ_rows, seeds, store_basis=store_basis)
+                        None if two_stage else occ_rows, seeds, store_basis=store_basis,
+                        vecs=None if two_stage else vectors)
     del v0
     out['cycles_' + dtype] = done
     out['bulk_converged'] = out['converged'].copy()
@@ -437,20 +640,22 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
         gen.manual_seed(int(seed) + 1)
         parts = []
         for b in range(count):
-            x = seeds.pop(b).to(torch.float64)
+            x = seeds.pop(b)
+            x = x.to(torch.complex128 if x.is_complex() else torch.float64)
             d = x.numel()
-            noise = torch.randn(d, dtype=torch.float64, device=x.device, generator=gen)
+            noise = torch.randn(d, dtype=x.dtype, device=x.device, generator=gen)
             parts.append(x / torch.linalg.vector_norm(x) + (seed_noise / np.sqrt(d)) * noise)
         vrest = torch.cat(parts)
         del parts
         done64, rest = 0, all_idx
+        _reset_records(out, count)
         for k, (st, cy) in enumerate(schedule):
             if not len(rest):
                 break
             d, rest, vrest = _stage(models, rest, vrest, 'float64', st, tol, merge['float64'], cy,
                                     out, use_graph, tile_bits, device, occ_rows,
                                     finish_last=(k == len(schedule) - 1),
-                                    store_basis=store_basis)
+                                    store_basis=store_basis, vecs=vectors)
             done64 += d
         out['cycles_float64'] = done64
     if observables:
@@ -491,23 +696,85 @@ def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device=
     return out
 
 
+def _locked_second_level(items, vecs, tol, steps=150, seed=1, device='cuda', store_basis='auto',
+                         schedule=((150, 3), (300, 2), (1000, 3))):
+    """The second level of every CSR instance counted with multiplicity:
+    batched FP64 Lanczos cycles in the complement of the instance's ground
+    vector (``_stage`` with ``lock``; seeded start vectors projected off it)
+    until the lowest Ritz pair there has a bound <= tol·max(1,|E|), as eigh's
+    w[1] (equilibrium.py:272; a degenerate ground level gives E1 = E0).
+    ``schedule`` = ((steps, cycles), ...) as the refinement of
+    :func:`certified_ground_states`: short cycles first, long ones for the
+    instances whose next level has a close neighbour.  Returns (E1, bound,
+    converged), NaN / inf where an instance has no complement (dim 1)."""
+    import torch
+    count = len(items)
+    out = dict(E0=np.full(count, np.nan), gap=np.full(count, np.nan),
+               ritz_bound=np.full(count, np.inf), ritz_bound_gap=np.full(count, np.inf),
+               converged=np.zeros(count, bool), converged_E0=np.zeros(count, bool),
+               steps_float64=np.zeros(count, np.int64))
+    _reset_records(out, count)
+    rest = np.array([i for i in range(count) if items[i].dim > 1], np.int64)
+    if not rest.size:
+        return out['E0'], out['ritz_bound'], out['converged_E0']
+    dt = torch.complex128 if _any_complex(items) else torch.float64
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    v = torch.randn(int(sum(items[i].dim for i in rest)), dtype=dt, device=dev, generator=gen)
+    sched = [(int(st), int(cy)) for st, cy in schedule if int(cy) > 0]
+    for k, (st, cy) in enumerate(sched):
+        if not len(rest):
+            break
+        _, rest, v = _stage(items, rest, v, 'float64', st, tol, 1e-10, cy, out, True, None, device,
+                            finish_last=(k == len(sched) - 1), store_basis=store_basis, want=1,
+                            lock=vecs)
+    return out['E0'], out['ritz_bound'], out['converged_E0']
+
+
+def batched_ground_state_csr_hip(mats, steps=150, dtype='float64', tol=1e-9, seed=0,
+                                 device='cuda', observables=True, multiplicity=True,
+                                 store_basis='auto'):
+    """Ground states of many Hermitian instances of any kind and size (dense
+    or sparse matrices, real or complex, uneven dims) in batched launches:
+    ``certified_ground_states`` over :class:`CsrInstance` items (E0 and its
+    Ritz vector certified to ``tol``·max(1,|E|), as torch_lanczos_ground_state's
+    rule, matrix_free.py:258-266), then, with ``multiplicity``, one locked
+    batched run per instance set for E1 counted with multiplicity
+    (:func:`_locked_second_level`), so gap = E1 - E0 is eigh's w[1] - w[0]
+    (equilibrium.py:272) also for a degenerate ground level.  ``dtype``
+    'float32' runs an FP32 (complex64) bulk before the FP64 refinement.
+    Returns E0, gap, converged (E0 and, with ``multiplicity``, E1), the
+    bounds and, with ``observables``, n_atom (instances with dim 2^n)."""
+    items = [m if isinstance(m, CsrInstance) else CsrInstance(m) for m in mats]
+    vecs = {}
+    out = certified_ground_states(items, steps=steps, dtype=dtype, tol=tol, seed=seed,
+                                  device=device, autotune=False, observables=observables,
+                                  store_basis=store_basis, vectors=vecs)
+    if multiplicity:
+        E1, b1, c1 = _locked_second_level(items, vecs, tol, steps=steps, seed=seed + 7,
+                                          device=device, store_basis=store_basis)
+        out['gap_distinct'] = out['gap']
+        out['gap'] = E1 - out['E0']
+        out['ritz_bound_gap'] = b1
+        out['converged_E1'] = c1
+        out['converged'] = out['converged_E0'] & (c1 | np.array([m.dim < 2 for m in items]))
+    out['backend'] = 'hip-batched-csr-lanczos'
+    return out
+
+
 def batched_ground_state_dense_hip(H_batch, backend='auto', vram_budget=1.2e9, device='cuda'):
     """batched_ground_state (tomisnot/energy-level
     src/core/manybody/equilibrium.py:251-311) with its signature: H_batch is a
     (B, dim, dim) Hermitian stack with dim = 2^n; → dict(E0 (B,), gap (B,),
-    n_atom (B, n), backend).  Each H is copied into a device CSR plan (its
-    non-zeros) and solved by the device Lanczos, E1 counted with
-    multiplicity (the second level over the complement of the ground
-    vector, so a degenerate ground level gives gap 0 as eigh's w[1] - w[0]
-    does); <n_i> of the ground state
-    comes from the full-basis occupation kernel of an n-site plan (site 0 is
-    the most significant bit, as bit_matrix).  ``backend`` and
-    ``vram_budget`` are accepted for the signature; one instance is resident
-    at a time."""
-    from .lanczos import hip_lanczos_ground_state, site_occupations
-    from .model import from_calib
-    from .plan import CsrPlan, Plan
-    import torch
+    n_atom (B, n), backend).  The stack goes to one CSR batch
+    (:func:`batched_ground_state_csr_hip`: every instance in the same
+    launches, E0 certified to 1e-9·max(1,|E0|), E1 counted with multiplicity
+    so that a degenerate ground level gives gap 0 as eigh's w[1] - w[0]
+    does); <n_i> of the ground state with site 0 the most significant bit,
+    as bit_matrix.  ``backend`` and ``vram_budget`` are accepted for the
+    signature.  An instance whose E0 or E1 misses the tolerance within the
+    cycle budget keeps its best estimate, with a warning."""
     Hb = np.asarray(H_batch)
     if Hb.ndim != 3 or Hb.shape[1] != Hb.shape[2]:
         raise ValueError('H_batch must be shaped (B, dim, dim), got %r' % (Hb.shape,))
@@ -515,31 +782,14 @@ def batched_ground_state_dense_hip(H_batch, backend='auto', vram_budget=1.2e9, d
     n = int(round(np.log2(dim))) if dim > 0 else 0
     if dim < 1 or (1 << n) != dim:
         raise ValueError('dim=%d is not 2^N' % dim)
-    occ_plan = Plan(from_calib({'atoms': [{}] * n, 'pairs': []}, np.zeros(n), np.zeros(n)),
-                    device=device)
-    E0 = np.zeros(B)
-    gap = np.zeros(B)
-    n_atom = np.zeros((B, n))
-    try:
-        for b in range(B):
-            P = CsrPlan(Hb[b], device=device)
-            try:
-                out = hip_lanczos_ground_state(None, plan=P, device=device, return_state=True)
-            finally:
-                P.close()
-            E0[b] = out['E0']
-            ev = np.asarray(out['evals'], float)
-            # eigh always gives w[1] - w[0]: the best estimate even when the
-            # residuals miss the tolerance (then with a warning)
-            gap[b] = out['gap'] if np.isfinite(out['gap']) else (
-                float(ev[1] - ev[0]) if len(ev) > 1 else 0.0)
-            if not out['converged']:
-                warnings.warn('instance %d: %s' % (b, out.get('warning', 'not converged')),
-                              RuntimeWarning)
-            v = torch.as_tensor(np.asarray(out['vec']), device=occ_plan.device)
-            v = v.to(torch.complex128 if torch.is_complex(v) else torch.float64)
-            n_atom[b] = np.asarray(site_occupations(occ_plan, v))
-    finally:
-        occ_plan.close()
-    return dict(E0=E0, gap=gap, n_atom=n_atom, backend='hip-csr-lanczos')
-
+    if B == 0:
+        return dict(E0=np.zeros(0), gap=np.zeros(0), n_atom=np.zeros((0, n)),
+                    backend='hip-batched-csr-lanczos')
+    out = batched_ground_state_csr_hip([Hb[b] for b in range(B)], device=device)
+    bad = np.nonzero(~out['converged'])[0]
+    if bad.size:
+        warnings.warn('%d of %d instances did not converge to 1e-9 within the cycle budget; '
+                      'indices %s' % (bad.size, B, bad[:16].tolist()), RuntimeWarning)
+    n_atom = np.array([np.asarray(a, float) for a in out['n_atom']]).reshape(B, n)
+    return dict(E0=np.asarray(out['E0']), gap=np.asarray(out['gap']), n_atom=n_atom,
+                backend=out['backend'])

@@ -26,6 +26,7 @@ struct BatchInst {
   const double* ehi;      // [2^(n-tb)]
   const double* vb;       // [n*n]
   double ar[BMAXN];       // Ω/2 by bit position (real drive)
+  int64_t dim;            // codes (rows) of the instance
 };
 
 struct BatchLz {
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(64) void k_batch_occ_finish(BatchLz L, const double
 template <class V>
 __global__ __launch_bounds__(256) void k_batch_zero(V* __restrict__ x, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = (V)0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) x[i] = vzero(V{});
 }
 
 // Ritz vectors from a stored Krylov basis (row j = u_j of every instance,

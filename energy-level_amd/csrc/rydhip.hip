@@ -23,6 +23,7 @@
 #include "sector.hpp"
 #include "evolve.hpp"
 #include "batch.hpp"
+#include "batch_csr.hpp"
 #include "tridiag.hpp"
 #include "reortho.hpp"
 
@@ -2461,9 +2462,49 @@ struct ryd_batch {
   hipGraphExec_t exec = nullptr;
   int exec_choice = -1;
   hipEvent_t ev = nullptr;
+  // CSR instances (ryd_batch_create_csr): tiles of csr_tr rows, candidate 0
+  bool csr = false;
+  bool csr_wave = false;             // one row per wavefront (long rows)
+  int csr_tr = 0;
+  int64_t* d_csr_ptr = nullptr;      // [total + 1]
+  int32_t* d_csr_col = nullptr;      // [nnz], column inside the instance
+  double* d_csr_re = nullptr;        // [nnz]
+  double* d_csr_im = nullptr;        // [nnz] or null
+  double* d_gam = nullptr;           // [count][4]: γ, <g, u_j>
+  const void* lock = nullptr;        // caller's lock vectors (concatenated) or null
 };
 
 namespace {
+
+size_t batch_es(const ryd_batch* B) {
+  switch (B->dtype) {
+    case RYD_F32: return 4;
+    case RYD_F64: return 8;
+    case RYD_C64: return 8;
+    default: return 16;
+  }
+}
+
+// breakdown threshold of |u_{j+1}|: single or double precision vectors
+double batch_tiny(const ryd_batch* B) {
+  return (B->dtype == RYD_F32 || B->dtype == RYD_C64) ? 1e-6 : 1e-13;
+}
+
+BcLz bc_lz(const ryd_batch* B, int j) {
+  BcLz L;
+  L.inst = B->d_inst[0];
+  L.tile_inst = B->d_tile_inst[0];
+  L.alpha = B->d_alpha;
+  L.nrm = B->d_nrm;
+  L.stop = B->d_stop;
+  L.partial = B->d_partial;
+  L.gam = B->d_gam;
+  L.ldk = B->steps;
+  L.j = j;
+  L.tr = B->csr_tr;
+  L.tiny = batch_tiny(B);
+  return L;
+}
 
 BatchLz batch_lz(const ryd_batch* B, int k, int j) {
   BatchLz L;
@@ -2475,7 +2516,7 @@ BatchLz batch_lz(const ryd_batch* B, int k, int j) {
   L.partial = B->d_partial;
   L.ldk = B->steps;
   L.j = j;
-  L.tiny = B->dtype == RYD_F32 ? 1e-6 : 1e-13;
+  L.tiny = batch_tiny(B);
   return L;
 }
 
@@ -2518,16 +2559,62 @@ int batch_sequence(ryd_batch* B, int k, hipStream_t st, V* ritz = nullptr,
   return RYD_OK;
 }
 
+// The recurrence over CSR instances (one tile shape), optionally locked
+// against the caller's vectors and accumulating Ritz vectors (replay).
+template <class V>
+int bc_sequence(ryd_batch* B, hipStream_t st, V* ritz = nullptr, const double* ycoef = nullptr) {
+  V* base = static_cast<V*>(batch_rows(B));
+  auto U = [&](int j) -> V* {
+    const int r = B->basis ? (j < 0 ? 0 : j) : ((j % 3) + 3) % 3;
+    return base + (size_t)r * B->total;
+  };
+  const V* lock = static_cast<const V*>(B->lock);
+  const BcMat M{B->d_csr_ptr, B->d_csr_col, B->d_csr_re, B->d_csr_im};
+  const unsigned g = (unsigned)B->ntiles[0];
+  k_batch_reset<<<(B->count + 255) / 256, 256, 0, st>>>(B->d_stop, B->count);
+  LAUNCH_CHECK();
+  k_bc_norm0<V><<<g, 256, 0, st>>>(bc_lz(B, 0), U(0), lock);
+  LAUNCH_CHECK();
+  k_bc_finish<<<B->count, 256, 0, st>>>(bc_lz(B, -1), 1, lock != nullptr);
+  LAUNCH_CHECK();
+  for (int j = 0; j < B->steps; ++j) {
+    const BcLz L = bc_lz(B, j);
+    if (B->csr_wave)
+      k_bc_apply<V, true><<<g, 256, 0, st>>>(L, M, U(j), U(j - 1), U(j + 1), lock);
+    else
+      k_bc_apply<V, false><<<g, 256, 0, st>>>(L, M, U(j), U(j - 1), U(j + 1), lock);
+    LAUNCH_CHECK();
+    k_bc_finish<<<B->count, 256, 0, st>>>(L, 0, lock != nullptr);
+    LAUNCH_CHECK();
+    k_bc_orth<V><<<g, 256, 0, st>>>(L, U(j), U(j + 1), ritz, ycoef, lock);
+    LAUNCH_CHECK();
+    k_bc_finish<<<B->count, 256, 0, st>>>(L, 1, lock != nullptr);
+    LAUNCH_CHECK();
+  }
+  return RYD_OK;
+}
+
+// one recurrence of the batch in its element type
+int batch_sequence_any(ryd_batch* B, int k, hipStream_t st) {
+  if (B->csr) {
+    switch (B->dtype) {
+      case RYD_F32: return bc_sequence<float>(B, st);
+      case RYD_F64: return bc_sequence<double>(B, st);
+      case RYD_C64: return bc_sequence<c32>(B, st);
+      default: return bc_sequence<c64>(B, st);
+    }
+  }
+  return B->dtype == RYD_F32 ? batch_sequence<float>(B, k, st) : batch_sequence<double>(B, k, st);
+}
+
 int batch_launch(ryd_batch* B, int k, int use_graph) {
   hipStream_t st = B->stream;
-  if (!use_graph) {
-    return B->dtype == RYD_F32 ? batch_sequence<float>(B, k, st) : batch_sequence<double>(B, k, st);
-  }
+  if (!use_graph) return batch_sequence_any(B, k, st);
   if (B->exec == nullptr || B->exec_choice != k) {
     if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = B->dtype == RYD_F32 ? batch_sequence<float>(B, k, st) : batch_sequence<double>(B, k, st);
+    int rc = batch_sequence_any(B, k, st);
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc) return rc;
     if (e != hipSuccess) return fail(RYD_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
@@ -2645,6 +2732,130 @@ extern "C" int ryd_batch_create(const ryd_model* models, int32_t count, int dtyp
   return RYD_OK;
 }
 
+extern "C" int ryd_batch_create_csr(int32_t count, const int64_t* dims, const int64_t* indptr,
+                                    const int64_t* indices, const double* val_re,
+                                    const double* val_im, int dtype, int32_t steps,
+                                    ryd_batch** out) {
+  if (!dims || !indptr || !out || count < 1 || steps < 1)
+    return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_create_csr");
+  if (dtype < RYD_F32 || dtype > RYD_C128)
+    return fail(RYD_ERR_INVALID, "ryd_batch_create_csr: unknown dtype");
+  *out = nullptr;
+  // instances, tiles and the row pointers, all checked on the host
+  int64_t total = 0;
+  for (int32_t b = 0; b < count; ++b) {
+    if (dims[b] < 1 || dims[b] > (int64_t)INT32_MAX)
+      return fail(RYD_ERR_INVALID, "csr batch: every dim must be 1..2^31-1");
+    total += dims[b];
+  }
+  if (indptr[0] != 0) return fail(RYD_ERR_INVALID, "csr batch: indptr[0] must be 0");
+  for (int64_t r = 0; r < total; ++r)
+    if (indptr[r + 1] < indptr[r]) return fail(RYD_ERR_INVALID, "csr batch: indptr must not decrease");
+  const int64_t nnz = indptr[total];
+  if (nnz > 0 && (!indices || !val_re)) return fail(RYD_ERR_INVALID, "null argument");
+  const bool cvec = dtype == RYD_C64 || dtype == RYD_C128;
+  std::vector<int32_t> col((size_t)(nnz > 0 ? nnz : 1), 0);
+  bool cvals = false;
+  {
+    int64_t r = 0;
+    for (int32_t b = 0; b < count; ++b) {
+      for (int64_t k = 0; k < dims[b]; ++k, ++r) {
+        for (int64_t q = indptr[r]; q < indptr[r + 1]; ++q) {
+          if (indices[q] < 0 || indices[q] >= dims[b])
+            return fail(RYD_ERR_INVALID, "csr batch: column index outside its instance");
+          if (!std::isfinite(val_re[q]) || (val_im && !std::isfinite(val_im[q])))
+            return fail(RYD_ERR_INVALID, "csr batch: non-finite value");
+          col[(size_t)q] = (int32_t)indices[q];
+          if (val_im && val_im[q] != 0.0) cvals = true;
+        }
+      }
+    }
+  }
+  if (cvals && !cvec)
+    return fail(RYD_ERR_INVALID, "csr batch: complex values need a complex dtype");
+  ryd_batch* B = new ryd_batch();
+  B->csr = true;
+  B->count = count;
+  B->dtype = dtype;
+  B->steps = steps;
+  B->choice = 0;
+  B->total = total;
+  (void)hipGetDevice(&B->device);
+  auto bad = [&](int code, const std::string& msg) {
+    ryd_batch_destroy(B);
+    return fail(code, msg);
+  };
+  // long rows (a dense or nearly dense matrix) run one row per wavefront
+  B->csr_wave = total > 0 && nnz >= 16 * total;
+  B->csr_tr = B->csr_wave ? 64 : 1024;
+  std::vector<BatchInst> hinst((size_t)count);
+  std::vector<int> tmap;
+  int64_t off = 0;
+  for (int32_t b = 0; b < count; ++b) {
+    BatchInst I{};
+    const int64_t d = dims[b];
+    int bits = 0;
+    while (bits < 62 && (1ll << bits) < d) ++bits;
+    I.n = (1ll << bits) == d && bits <= BMAXN ? bits : 0;
+    I.tb = 0;
+    I.ntiles = (int)((d + B->csr_tr - 1) / B->csr_tr);
+    I.off = off;
+    I.tile0 = (int64_t)tmap.size();
+    I.dim = d;
+    hinst[(size_t)b] = I;
+    for (int t = 0; t < I.ntiles; ++t) tmap.push_back(b);
+    B->n.push_back(I.n);
+    B->off.push_back(off);
+    off += d;
+  }
+  if (tmap.size() > (size_t)INT32_MAX) return bad(RYD_ERR_INVALID, "csr batch: too many tiles");
+  B->ntiles[0] = (int)tmap.size();
+  B->maxtiles = tmap.size();
+  hipError_t e = hipMalloc(&B->d_inst[0], sizeof(BatchInst) * count);
+  if (e == hipSuccess) e = hipMemcpy(B->d_inst[0], hinst.data(), sizeof(BatchInst) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&B->d_tile_inst[0], sizeof(int) * tmap.size());
+  if (e == hipSuccess) e = hipMemcpy(B->d_tile_inst[0], tmap.data(), sizeof(int) * tmap.size(), hipMemcpyHostToDevice);
+  const size_t nb = (size_t)(nnz > 0 ? nnz : 1);
+  if (e == hipSuccess) e = hipMalloc(&B->d_csr_ptr, sizeof(int64_t) * (size_t)(total + 1));
+  if (e == hipSuccess) e = hipMemcpy(B->d_csr_ptr, indptr, sizeof(int64_t) * (size_t)(total + 1), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&B->d_csr_col, sizeof(int32_t) * nb);
+  if (e == hipSuccess) e = hipMemcpy(B->d_csr_col, col.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&B->d_csr_re, sizeof(double) * nb);
+  if (e == hipSuccess && nnz > 0)
+    e = hipMemcpy(B->d_csr_re, val_re, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
+  if (e == hipSuccess && cvals) e = hipMalloc(&B->d_csr_im, sizeof(double) * nb);
+  if (e == hipSuccess && cvals)
+    e = hipMemcpy(B->d_csr_im, val_im, sizeof(double) * (size_t)nnz, hipMemcpyHostToDevice);
+  const size_t es = batch_es(B);
+  if (e == hipSuccess) e = hipMalloc(&B->d_vec, es * 3 * (size_t)total);
+  if (e == hipSuccess) e = hipMalloc(&B->d_alpha, sizeof(double) * (size_t)count * steps);
+  if (e == hipSuccess) e = hipMalloc(&B->d_nrm, sizeof(double) * (size_t)count * (steps + 1));
+  if (e == hipSuccess) e = hipMalloc(&B->d_stop, sizeof(int) * count);
+  if (e == hipSuccess) e = hipMalloc(&B->d_partial, sizeof(double) * 3 * B->maxtiles);
+  if (e == hipSuccess) e = hipMalloc(&B->d_gam, sizeof(double) * 4 * (size_t)count);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&B->ev, hipEventDisableTiming);
+  if (e != hipSuccess)
+    return bad(e == hipErrorOutOfMemory ? RYD_ERR_MEMORY : RYD_ERR_DEVICE,
+               std::string("csr batch allocation: ") + hipGetErrorString(e));
+  *out = B;
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_set_lock(ryd_batch* B, const void* lock) {
+  if (!B) return fail(RYD_ERR_INVALID, "null batch");
+  if (!B->csr) return fail(RYD_ERR_UNSUPPORTED, "lock vectors need a CSR batch");
+  // the captured graph holds the lock pointer
+  if (B->exec) {
+    HIP_TRY(hipStreamSynchronize(B->stream));
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+    B->exec_choice = -1;
+  }
+  B->lock = lock;
+  return RYD_OK;
+}
+
 extern "C" int ryd_batch_destroy(ryd_batch* B) {
   if (!B) return RYD_OK;
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
@@ -2654,7 +2865,8 @@ extern "C" int ryd_batch_destroy(ryd_batch* B) {
     if (B->d_inst[k]) (void)hipFree(B->d_inst[k]);
     if (B->d_tile_inst[k]) (void)hipFree(B->d_tile_inst[k]);
   }
-  void* ptrs[] = {B->d_tables, B->d_vec, B->d_alpha, B->d_nrm, B->d_stop, B->d_partial};
+  void* ptrs[] = {B->d_tables, B->d_vec, B->d_alpha, B->d_nrm, B->d_stop, B->d_partial,
+                  B->d_csr_ptr, B->d_csr_col, B->d_csr_re, B->d_csr_im, B->d_gam};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (B->ev) (void)hipEventDestroy(B->ev);
@@ -2671,7 +2883,7 @@ extern "C" int64_t ryd_batch_offset(const ryd_batch* B, int32_t index) {
 extern "C" int ryd_batch_run(ryd_batch* B, const void* v0, int use_graph, void* stream) {
   if (!B || !v0) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_run");
   hipStream_t caller = static_cast<hipStream_t>(stream);
-  const size_t es = B->dtype == RYD_F32 ? 4 : 8;
+  const size_t es = batch_es(B);
   // order after the caller's stream, copy the start vectors, run, order back
   HIP_TRY(hipEventRecord(B->ev, caller));
   HIP_TRY(hipStreamWaitEvent(B->stream, B->ev, 0));
@@ -2688,7 +2900,8 @@ extern "C" int ryd_batch_run(ryd_batch* B, const void* v0, int use_graph, void* 
 extern "C" int ryd_batch_autotune(ryd_batch* B, const void* v0, int32_t* chosen_tile_bits,
                                   double* ms_out /* [3] or NULL */) {
   if (!B || !v0) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_autotune");
-  const size_t es = B->dtype == RYD_F32 ? 4 : 8;
+  if (B->csr) return fail(RYD_ERR_UNSUPPORTED, "a CSR batch has one tile shape");
+  const size_t es = batch_es(B);
   hipEvent_t a, b;
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
@@ -2754,8 +2967,7 @@ extern "C" int ryd_ritz_lowest(int32_t count, const int32_t* steps, const double
 
 extern "C" int64_t ryd_batch_basis_bytes(const ryd_batch* B) {
   if (!B) return -1;
-  const int64_t es = B->dtype == RYD_F32 ? 4 : 8;
-  return es * (int64_t)(B->steps + 1) * B->total;
+  return (int64_t)batch_es(B) * (int64_t)(B->steps + 1) * B->total;
 }
 
 extern "C" int ryd_batch_attach_basis(ryd_batch* B, void* basis, int64_t bytes) {
@@ -2779,6 +2991,7 @@ extern "C" int ryd_batch_attach_basis(ryd_batch* B, void* basis, int64_t bytes) 
 extern "C" int ryd_batch_set_tile_bits(ryd_batch* B, int32_t tile_bits) {
   if (!B || tile_bits < 10 || tile_bits > 12)
     return fail(RYD_ERR_INVALID, "ryd_batch_set_tile_bits: tile bits must be 10, 11 or 12");
+  if (B->csr) return fail(RYD_ERR_UNSUPPORTED, "a CSR batch has one tile shape");
   B->choice = tile_bits - 10;
   return RYD_OK;
 }
@@ -2790,7 +3003,29 @@ bool batch_combines(const ryd_batch* B) {
 }
 
 template <class V>
+int bc_ritz(ryd_batch* B, V* ritz, double* occ) {
+  hipStream_t st = B->stream;
+  const BcLz L = bc_lz(B, 0);
+  const unsigned g = (unsigned)B->ntiles[0];
+  if (batch_combines(B)) {
+    k_bc_ritz_stored<V><<<g, 256, 0, st>>>(L, static_cast<const V*>(B->basis), B->total, B->steps,
+                                           B->d_ycoef, ritz);
+    LAUNCH_CHECK();
+  } else {
+    k_batch_zero<V><<<stream_grid(B->total), 256, 0, st>>>(ritz, B->total);
+    LAUNCH_CHECK();
+    if (int rc = bc_sequence<V>(B, st, ritz, B->d_ycoef)) return rc;
+  }
+  k_bc_occ<V><<<g, 256, 0, st>>>(L, ritz, B->d_occp);
+  LAUNCH_CHECK();
+  k_batch_occ_finish<<<B->count, 64, 0, st>>>(batch_lz(B, 0, 0), B->d_occp, occ);
+  LAUNCH_CHECK();
+  return RYD_OK;
+}
+
+template <class V>
 int batch_ritz(ryd_batch* B, V* ritz, double* occ) {
+  if (B->csr) return bc_ritz<V>(B, ritz, occ);
   hipStream_t st = B->stream;
   const int k = B->choice;
   if (batch_combines(B)) {
@@ -2816,7 +3051,7 @@ extern "C" int ryd_batch_ritz(ryd_batch* B, const void* v0, const double* ycoef,
                               double* occ, void* stream) {
   if (!B || !v0 || !ycoef || !ritz || !occ) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_ritz");
   hipStream_t caller = static_cast<hipStream_t>(stream);
-  const size_t es = B->dtype == RYD_F32 ? 4 : 8;
+  const size_t es = batch_es(B);
   if (!B->d_ycoef) HIP_TRY(hipMalloc(&B->d_ycoef, sizeof(double) * (size_t)B->count * B->steps));
   if (!B->d_occp) HIP_TRY(hipMalloc(&B->d_occp, sizeof(double) * B->maxtiles * (BMAXN + 1)));
   HIP_TRY(hipEventRecord(B->ev, caller));
@@ -2826,8 +3061,13 @@ extern "C" int ryd_batch_ritz(ryd_batch* B, const void* v0, const double* ycoef,
   if (!batch_combines(B))   // a replay starts again from v0
     HIP_TRY(hipMemcpyAsync(batch_rows(B), v0, es * (size_t)B->total, hipMemcpyDeviceToDevice,
                            B->stream));
-  const int rc = B->dtype == RYD_F32 ? batch_ritz<float>(B, static_cast<float*>(ritz), occ)
-                                     : batch_ritz<double>(B, static_cast<double*>(ritz), occ);
+  int rc;
+  switch (B->dtype) {
+    case RYD_F32: rc = batch_ritz<float>(B, static_cast<float*>(ritz), occ); break;
+    case RYD_F64: rc = batch_ritz<double>(B, static_cast<double*>(ritz), occ); break;
+    case RYD_C64: rc = bc_ritz<c32>(B, static_cast<c32*>(ritz), occ); break;
+    default: rc = bc_ritz<c64>(B, static_cast<c64*>(ritz), occ); break;
+  }
   if (rc) return rc;
   HIP_TRY(hipEventRecord(B->ev, B->stream));
   HIP_TRY(hipStreamWaitEvent(caller, B->ev, 0));

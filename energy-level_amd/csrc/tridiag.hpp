@@ -206,11 +206,25 @@ inline void lowest_pairs(const double* a, const double* b, int k, double beta_la
     return vec[have[j]];
   };
   auto bound_of = [&](int j) { return std::fabs(beta_last * vector_of(j)[k - 1]); };
+  // A kept value and its ghost copies (within tol) are one eigenvalue of H;
+  // the copy with the smallest bound certifies it and gives its vector (the
+  // newest copy of a converged value carries a large last component, so the
+  // lowest copy alone can hold a bound far above the converged one).
   int kept[2] = {-1, -1};
+  int best[2] = {-1, -1};
+  double bb[2] = {INFINITY, INFINITY};
   int nk = 0;
-  for (int j = 0; j < k && nk < 2; ++j) {
+  for (int j = 0; j < k; ++j) {
     const double t = value(j);
-    if (nk > 0 && std::fabs(t - th[kept[nk - 1]]) <= tol) continue;   // ghost copy
+    if (nk > 0 && std::fabs(t - th[kept[nk - 1]]) <= tol) {   // ghost copy
+      const double bj = bound_of(j);
+      if (bj < bb[nk - 1]) {
+        bb[nk - 1] = bj;
+        best[nk - 1] = j;
+      }
+      continue;
+    }
+    if (nk == 2) break;
     if (nk > 0) {
       // simple: no other eigenvalue of T within tol (the values are sorted,
       // so the neighbours decide)
@@ -222,14 +236,17 @@ inline void lowest_pairs(const double* a, const double* b, int k, double beta_la
         if (above > below && !(bound_of(j) <= conv_tol * std::fmax(1.0, std::fabs(t)))) continue;
       }
     }
-    kept[nk++] = j;
+    kept[nk] = j;
+    best[nk] = j;
+    bb[nk] = bound_of(j);
+    ++nk;
   }
   for (int q = 0; q < nk; ++q) {
     E[q] = th[kept[q]];
-    bound[q] = bound_of(kept[q]);
+    bound[q] = bb[q];
     double* out = q == 0 ? y0 : y1;
     if (out) {
-      const std::vector<double>& v = vector_of(kept[q]);
+      const std::vector<double>& v = vector_of(best[q]);
       for (int i = 0; i < k; ++i) out[i] = v[i];
     }
   }

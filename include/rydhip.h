@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 9
+#define RYD_ABI_VERSION 10
 
 enum ryd_status {
   RYD_OK = 0,
@@ -363,6 +363,31 @@ int ryd_evolve_adjoint_stored(const ryd_plan* plan, int dtype, void* g, int64_t 
 typedef struct ryd_batch ryd_batch;
 int ryd_batch_create(const ryd_model* models, int32_t count, int dtype, int32_t steps,
                      ryd_batch** out);
+/* A batch over caller-supplied sparse Hermitian instances of any size
+ * (e.g. the dense stack of batched_ground_state, equilibrium.py:251-311,
+ * passed without its zeros): CSR host arrays of the concatenated rows,
+ * indptr[total+1] from 0 (total = Σ dims), indices[nnz] the column inside
+ * the row's own instance (0 .. dims[b]-1), val_re/val_im[nnz] (val_im NULL
+ * for real values; complex values need dtype C64 or C128).  Vectors in
+ * dtype (F32, F64, C64, C128) with FP64 alpha/beta; tiles of rows (1024
+ * rows, one per thread, or 64 rows, one per wavefront, when rows average 16
+ * or more entries).  Every other ryd_batch_* entry point applies except the
+ * tile-size ones (autotune, set_tile_bits: RYD_ERR_UNSUPPORTED); occupations
+ * are reported for instances whose dim is 2^n (n <= 24), bit n-1-i of the
+ * row index being site i.  Replaces the reference's dense batched eigh for
+ * instances of any Hamiltonian.  dims[b] < 2^31. */
+int ryd_batch_create_csr(int32_t count, const int64_t* dims, const int64_t* indptr,
+                         const int64_t* indices, const double* val_re, const double* val_im,
+                         int dtype, int32_t steps, ryd_batch** out);
+/* CSR batches: per-instance lock vectors g_b (device, the batch element type,
+ * concatenated like the start vectors, unit norm), or NULL.  Each step then
+ * removes the component <g_b, w> g_b, so the recurrence runs in the
+ * complement of g_b: with g_b a converged ground vector the lowest Ritz value
+ * is the second level counted with multiplicity, as eigh's w[1]
+ * (equilibrium.py:272).  Start vectors must be orthogonal to g_b.  The
+ * memory stays the caller's; drops the captured graph.  No reference
+ * counterpart (the reference diagonalises densely). */
+int ryd_batch_set_lock(ryd_batch* batch, const void* lock);
 int ryd_batch_destroy(ryd_batch* batch);
 int64_t ryd_batch_offset(const ryd_batch* batch, int32_t index);
 /* Runs the recurrence from the device start vectors v0, ordered after and
@@ -380,7 +405,8 @@ int ryd_batch_autotune(ryd_batch* batch, const void* v0, int32_t* chosen_tile_bi
  * offd[b*ldo + i] (i < steps[b]-1), beta_last[b] = the next Lanczos norm
  * (0 after a breakdown).  Eigenvalues by Sturm bisection; a value within
  * merge_rel·|T| of the previous kept one merges (a ghost copy: 1e-10 for a
- * recurrence in FP64 vectors, 1e-6 in FP32 ones); a simple value
+ * recurrence in FP64 vectors, 1e-6 in FP32 ones), and the kept value takes
+ * the bound and eigenvector of the copy with the smallest bound; a simple value
  * that T without its first row and column also has is spurious
  * (Cullum-Willoughby) unless its bound |beta_last · y[k-1]| is at most
  * conv_tol·max(1,|E|); the lowest value is always kept.  Outputs (host):

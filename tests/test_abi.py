@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(pkg):
 def test_abi_version_and_error_path(pkg):
     from energy_level_amd import _native as nat
     L = nat.lib()
-    assert L.ryd_abi_version() == 9
+    assert L.ryd_abi_version() == 10
     # invalid model is rejected before any device work
     handle = ctypes.c_void_p()
     m = nat.Model()
@@ -67,3 +67,39 @@ def test_argument_checks_before_device_work(pkg):
     assert L.ryd_plan_create_csr(0, ptr, col, val, None, ctypes.byref(h)) == nat.RYD_ERR_INVALID
     assert L.ryd_evolve_forward(None, nat.RYD_C64, None, 1, None, None, None, None, 4, None,
                                 None, None) == nat.RYD_ERR_INVALID
+
+
+def test_csr_batch_argument_checks_before_device_work(pkg):
+    """ryd_batch_create_csr validates the concatenated CSR on the host: a
+    column outside its own instance, complex values with a real dtype, a
+    decreasing row pointer; the lock and tile-size calls refuse a null batch."""
+    import numpy as np
+    from energy_level_amd import _native as nat
+    L = nat.lib()
+    h = ctypes.c_void_p()
+    dims = np.array([2, 1], np.int64)
+    ip = np.array([0, 1, 2, 3], np.int64)
+    re = np.array([1.0, 1.0, 2.0])
+    im = np.array([0.5, -0.5, 0.0])
+    # row 1 of instance 0 points at column 2 (outside dim 2): refused
+    bad_cols = np.array([1, 2, 0], np.int64)
+    assert L.ryd_batch_create_csr(2, dims.ctypes.data, ip.ctypes.data, bad_cols.ctypes.data,
+                                  re.ctypes.data, None, nat.RYD_F64, 8,
+                                  ctypes.byref(h)) == nat.RYD_ERR_INVALID
+    assert 'outside its instance' in nat.last_error()
+    cols = np.array([1, 0, 0], np.int64)
+    assert L.ryd_batch_create_csr(2, dims.ctypes.data, ip.ctypes.data, cols.ctypes.data,
+                                  re.ctypes.data, im.ctypes.data, nat.RYD_F64, 8,
+                                  ctypes.byref(h)) == nat.RYD_ERR_INVALID
+    assert 'complex dtype' in nat.last_error()
+    ip_bad = np.array([0, 2, 1, 3], np.int64)
+    assert L.ryd_batch_create_csr(2, dims.ctypes.data, ip_bad.ctypes.data, cols.ctypes.data,
+                                  re.ctypes.data, None, nat.RYD_F64, 8,
+                                  ctypes.byref(h)) == nat.RYD_ERR_INVALID
+    zero = np.array([0, 1], np.int64)
+    assert L.ryd_batch_create_csr(2, zero.ctypes.data, ip.ctypes.data, cols.ctypes.data,
+                                  re.ctypes.data, None, nat.RYD_F64, 8,
+                                  ctypes.byref(h)) == nat.RYD_ERR_INVALID
+    assert L.ryd_batch_set_lock(None, None) == nat.RYD_ERR_INVALID
+    assert L.ryd_batch_basis_bytes(None) == -1
+    assert L.ryd_batch_attach_basis(None, None, 0) == nat.RYD_ERR_INVALID

@@ -1,0 +1,126 @@
+"""Batched Lanczos over arbitrary Hermitian instances (ryd_batch_create_csr):
+uneven dims, real and complex, sparse and dense rows, against numpy's dense
+eigensolver; the locked second run against eigh's w[1] with multiplicity;
+graph replays against the eager sequence; the dense-stack mirror of
+batched_ground_state (equilibrium.py:251-311) against eigh."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand_sparse_herm(d, rng, cplx, density=0.05):
+    A = sp.random(d, d, density=density, random_state=rng, format='csr')
+    if cplx:
+        A = A + 1j * sp.random(d, d, density=density, random_state=rng, format='csr')
+    H = (A + A.conj().T) * 0.5 + sp.diags(rng.normal(size=d))
+    return sp.csr_matrix(H)
+
+
+def _rand_dense_herm(d, rng, cplx):
+    A = rng.normal(size=(d, d))
+    if cplx:
+        A = A + 1j * rng.normal(size=(d, d))
+    return (A + A.conj().T) / 2
+
+
+def _check(out, mats, tol_e=1e-9, tol_gap=1e-8):
+    for b, H in enumerate(mats):
+        Hd = H.toarray() if sp.issparse(H) else np.asarray(H)
+        w = np.linalg.eigvalsh(Hd)
+        assert abs(out['E0'][b] - w[0]) <= tol_e * max(1.0, abs(w[0])), (b, out['E0'][b], w[0])
+        assert abs(out['gap'][b] - (w[1] - w[0])) <= tol_gap * max(1.0, abs(w[1])), \
+            (b, out['gap'][b], w[1] - w[0])
+
+
+@pytest.mark.parametrize('cplx', [False, True])
+def test_uneven_sparse_instances_vs_eigvalsh(pkg, dev, cplx):
+    rng = np.random.default_rng(3 + cplx)
+    dims = [17, 50, 128, 300, 1000, 64, 2]
+    mats = [_rand_sparse_herm(d, rng, cplx) for d in dims]
+    out = pkg.batched_ground_state_csr_hip(mats, device=dev)
+    assert out['converged'].all()
+    _check(out, mats)
+
+
+def test_dense_rows_run_one_row_per_wavefront(pkg, dev):
+    """Dense instances (rows of 256 entries) take the wavefront-per-row apply."""
+    rng = np.random.default_rng(11)
+    mats = [_rand_dense_herm(256, rng, True) for _ in range(3)] + [_rand_dense_herm(96, rng, False)]
+    B = pkg.BatchCsrLanczos(mats, steps=20, dtype='float64', device=dev)
+    import torch
+    assert B.tdt == torch.complex128
+    B.close()
+    out = pkg.batched_ground_state_csr_hip(mats, device=dev)
+    _check(out, mats)
+
+
+def test_fp32_bulk_then_fp64_refinement(pkg, dev):
+    rng = np.random.default_rng(5)
+    mats = [_rand_sparse_herm(d, rng, True, density=0.02) for d in (400, 700, 256)]
+    out = pkg.batched_ground_state_csr_hip(mats, dtype='float32', device=dev)
+    assert out['converged'].all()
+    _check(out, mats)
+
+
+def test_degenerate_levels_counted_with_multiplicity(pkg, dev):
+    """H ⊗ I_2 doubles every level and a diagonal whose minimum repeats has a
+    double ground level: gap 0 there, as eigh gives, and the true gap for the
+    simple instances of the same batch."""
+    rng = np.random.default_rng(21)
+    D = 64
+    d = rng.uniform(0.0, 5.0, D)
+    d[5] = d[40] = -1.0
+    Hdiag = np.diag(d)
+    rest = [i for i in range(D) if i not in (5, 40)]
+    for i, j in zip(rest[:-1], rest[1:]):
+        Hdiag[i, j] = Hdiag[j, i] = 0.05
+    mats = [sp.kron(_rand_sparse_herm(60, rng, True, 0.1), sp.eye(2), format='csr'),
+            sp.csr_matrix(Hdiag),
+            _rand_sparse_herm(120, rng, True, 0.1),
+            sp.kron(sp.csr_matrix(_rand_dense_herm(32, rng, False)), sp.eye(2), format='csr')]
+    out = pkg.batched_ground_state_csr_hip(mats, device=dev)
+    _check(out, mats)
+    assert abs(out['gap'][0]) <= 1e-8 and abs(out['gap'][1]) <= 1e-8 and abs(out['gap'][3]) <= 1e-8
+    assert out['gap'][2] > 1e-6
+
+
+def test_occupations_and_graph_replays(pkg, dev):
+    """n_atom of 2^n instances against the dense ground vector (site 0 the
+    most significant bit); captured replays reproduce the eager run."""
+    import torch
+    rng = np.random.default_rng(9)
+    mats = [_rand_sparse_herm(1 << n, rng, False, 0.1) for n in (6, 8, 7)]
+    out = pkg.batched_ground_state_csr_hip(mats, device=dev)
+    for b, H in enumerate(mats):
+        w, v = np.linalg.eigh(H.toarray())
+        if w[1] - w[0] < 1e-3:
+            continue
+        n = int(np.log2(H.shape[0]))
+        p = np.abs(v[:, 0]) ** 2
+        idx = np.arange(H.shape[0])
+        ref = np.array([p[((idx >> (n - 1 - i)) & 1) == 1].sum() for i in range(n)])
+        assert np.max(np.abs(out['n_atom'][b] - ref)) < 1e-7
+    B = pkg.BatchCsrLanczos(mats, steps=60, dtype='float64', device=dev, store_basis=False)
+    v0 = B.start_vectors(4)
+    B.run(v0, use_graph=False)
+    a0, n0, _ = B.coefficients()
+    for rep in range(3):
+        B.run(v0, use_graph=True)
+        a, nr, _ = B.coefficients()
+        assert np.array_equal(a, a0) and np.array_equal(nr, n0), rep
+        torch.randn(1000, device=dev).sum().item()
+    B.close()
+
+
+def test_dense_stack_mirror_matches_eigh(pkg, dev):
+    rng = np.random.default_rng(2)
+    Hs = np.stack([_rand_dense_herm(64, rng, True) for _ in range(5)])
+    out = pkg.batched_ground_state_dense_hip(Hs, device=dev)
+    assert out['backend'] == 'hip-batched-csr-lanczos'
+    assert out['n_atom'].shape == (5, 6)
+    for b in range(5):
+        w = np.linalg.eigvalsh(Hs[b])
+        assert abs(out['E0'][b] - w[0]) <= 1e-9 * max(1.0, abs(w[0]))
+        assert abs(out['gap'][b] - (w[1] - w[0])) <= 1e-8

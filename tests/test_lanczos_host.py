@@ -139,3 +139,32 @@ def test_native_selection_with_batch_layout(pkg):
                                         E.ctypes.data, Bd.ctypes.data, None, 0))
     for b in range(count):
         assert np.allclose(E[b], refs[b], rtol=0, atol=1e-12), (b, E[b], refs[b])
+
+
+def test_native_bound_is_the_best_copy_of_a_ghost_cluster(pkg):
+    """Run past convergence, the lowest eigenvalue has a cluster of ghost
+    copies in T whose bounds |beta·y[k-1]| differ by orders of magnitude (the
+    newest copy is still forming).  The native selection reports the copy
+    with the smallest bound: the value is certified (bound <= 1e-9) and the
+    returned vector is a unit eigenvector of T for it, with that bound."""
+    from energy_level_amd.lanczos import _tridiag
+    for seed in range(6):
+        rng = np.random.default_rng(100 + seed)
+        lam = np.sort(rng.uniform(-1.0, 1.0, 800))
+        lam[0] = -1.6
+        a, B = _recurrence(lam, rng.standard_normal(lam.size), 300)
+        th, Y = _tridiag(a, B[:-1])
+        scale = max(1.0, np.max(np.abs(a)) + 2 * np.max(np.abs(B[:-1])))
+        cl = np.nonzero(np.abs(th - th.min()) <= 1e-10 * scale)[0]
+        assert cl.size > 1                            # ghost copies are present
+        bounds = np.abs(B[-1] * Y[-1, cl])
+        assert bounds.max() > 1e3 * bounds.min()      # and their bounds differ
+        E, Bd, Yn = _native_lowest([a], [B[:-1]], [B[-1]], 1e-9)
+        k = len(a)
+        y = Yn[0, 0, :k]
+        T = np.diag(a) + np.diag(B[:-1], 1) + np.diag(B[:-1], -1)
+        assert abs(E[0, 0] - (-1.6)) <= 1e-10
+        assert Bd[0, 0] <= 1e-9
+        assert abs(np.linalg.norm(y) - 1.0) <= 1e-10
+        assert np.linalg.norm(T @ y - E[0, 0] * y) <= 1e-9 * scale
+        assert abs(Bd[0, 0] - abs(B[-1] * y[-1])) <= 1e-14 * max(1.0, Bd[0, 0])
