@@ -231,6 +231,25 @@ def _cpu_configs(which):
                          'assembly excluded); instances with N > 14 charged the N = 14 time, so '
                          'the CPU figure is an upper bound on its rate',
                          seconds_per_n={int(k): v for k, v in sorted(per_n.items())})
+    if 'd1' in which:
+        # batched_ground_state's dense eigh of the whole stack
+        # (equilibrium.py:272, torch.linalg.eigh): numpy's LAPACK eigh on 4
+        # of the 64 dense N=10 instances, scaled by 16
+        models = disorder_ensemble(count=64, nmin=10, nmax=10, seed=0)
+        Hs = np.stack([orc.build_H_sparse(m.n, m.Om, m.delta, m.pairs(), ['I1']).toarray()
+                       .astype(np.complex128) for m in models[:4]])
+        np.linalg.eigh(Hs[:1])
+        t, _ = _timed(lambda: np.linalg.eigh(Hs))
+        try:
+            from threadpoolctl import threadpool_info
+            blas = max((p.get('num_threads', 1) for p in threadpool_info()
+                        if p.get('user_api') == 'blas'), default=None)
+        except ImportError:
+            blas = None
+        out['d1'] = dict(value=64 / (t * 16), unit='solves/s', cores=blas, kind='port',
+            sample='numpy (LAPACK) eigh of 4 of the 64 dense complex128 N=10 instances, as '
+                   'equilibrium.py:272 diagonalises the stack, scaled by 16',
+            seconds_per_sample=t)
     if 'c5' in which:
         n, ks, dt = 20, 1, 0.05
         calib = W.chain_calib(n, 10.0)
@@ -285,10 +304,11 @@ def main():
                     help='time one element type only (profiling runs)')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run only a plain device copy of one FP64 N-site vector (counter scale)')
-    ap.add_argument('--configs', default='c1,c2,c3,c4,c5,c5r,g1,k1',
+    ap.add_argument('--configs', default='c1,c2,c3,c4,c5,c5r,g1,k1,d1',
                     help='comma list of the configurations to run after the headline: the '
                          'five workloads, c5r (C5 timed in the reference units), g1 (blockade-graph '
-                         'lattice) and k1 (kagome XY, J channels) ("" for none)')
+                         'lattice), k1 (kagome XY, J channels) and d1 (dense-stack batched ground '
+                         'states) ("" for none)')
     ap.add_argument('--graph-lattice', default=None, metavar='WxH[d]',
                     help='time only the blockade-graph H·v on a w x h square lattice '
                          '("6x7"; suffix d adds diagonal edges) and print its summary')

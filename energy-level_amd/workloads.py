@@ -15,7 +15,7 @@ import numpy as np
 from . import _native as nat
 from .model import from_positions, chain_positions, square_positions, c6_coupling, from_calib
 
-__all__ = ['chain_calib', 'lattice_calib', 'warm_device_memory', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5',
+__all__ = ['chain_calib', 'lattice_calib', 'warm_device_memory', 'run_d1', 'run_c1', 'run_c2', 'run_c3', 'run_c4', 'run_c5',
            'run_g1', 'run_k1', 'time_hv']
 
 
@@ -116,6 +116,79 @@ def time_steps(plan, dtype, steps=10, warmup=2, seed=0):
     return dict(cheb_step_ms=cheb, cheb_step_gbps_3Db=bts / (cheb * 1e-3) / 1e9,
                 lanczos_step_ms=lz, lanczos_step_gbps_3Db=bts / (lz * 1e-3) / 1e9,
                 bytes_3Db=bts)
+
+
+def _dense_h(m):
+    """Dense complex128 H of a full-basis model with real drives and no J/A
+    pairs (the input of the dense-stack workload): site i is bit n-1-i of the
+    row index, diagonal Σ delta_i n_i + Σ V_ij n_i n_j, flips Ω_i/2."""
+    n = int(m.n)
+    om = np.asarray(m.Om, complex)
+    if np.any(om.imag != 0) or m.has_pair_flips:
+        raise ValueError('the dense-stack builder takes real drives without J/A pairs')
+    D = 1 << n
+    codes = np.arange(D)
+    occ = ((codes[:, None] >> (n - 1 - np.arange(n))[None, :]) & 1).astype(float)
+    diag = occ @ np.asarray(m.delta, float)
+    for i, j, v in zip(m.pair_i, m.pair_j, m.pair_v):
+        diag += v * occ[:, i] * occ[:, j]
+    H = np.diag(diag).astype(np.complex128)
+    for i in range(n):
+        H[codes ^ (1 << (n - 1 - i)), codes] += 0.5 * om.real[i]
+    return H
+
+
+def run_d1(device='cuda', count=64, n=10, seed=0, sequential=16):
+    """Dense-stack batched_ground_state (equilibrium.py:251-311 takes a
+    (B, dim, dim) stack): ``count`` disordered chains at N=``n`` (jitter
+    0.1a, Δ over 32 values in [−2, 4], Ω=1, C6=10) as dense complex128
+    matrices of dim 2^n, solved by ``batched_ground_state_dense_hip`` (one
+    CSR batch: E0 certified to 1e-9·max(1,|E0|), E1 counted with
+    multiplicity by a locked batched run), timed after one warm-up call.
+    Beside it the device Lanczos one instance at a time (the mirror's
+    previous form) over the first ``sequential`` instances.  The dense
+    builder is checked against the device plan's H·v, and E0 and the gap of
+    four instances against numpy's eigvalsh."""
+    import torch
+    from .batch import batched_ground_state_dense_hip, disorder_ensemble
+    from .lanczos import hip_lanczos_ground_state
+    from .plan import CsrPlan, Plan
+    models = disorder_ensemble(count=count, nmin=n, nmax=n, seed=seed)
+    Hs = np.stack([_dense_h(m) for m in models])
+    P = Plan(models[0], device=device)
+    g = torch.Generator(device=P.device)
+    g.manual_seed(seed)
+    x = torch.randn(P.dim, dtype=torch.complex128, device=P.device, generator=g)
+    y = P.hv(x).cpu().numpy()
+    P.close()
+    xh = x.cpu().numpy()
+    builder_err = float(np.linalg.norm(Hs[0] @ xh - y) / np.linalg.norm(y))
+    if builder_err > 1e-12:
+        raise RuntimeError('dense builder disagrees with the device H·v (%.2e)' % builder_err)
+    batched_ground_state_dense_hip(Hs, device=device)
+    _sync()
+    t0 = time.perf_counter()
+    r = batched_ground_state_dense_hip(Hs, device=device)
+    _sync()
+    t_batch = time.perf_counter() - t0
+    _sync()
+    t0 = time.perf_counter()
+    for b in range(min(sequential, count)):
+        Pc = CsrPlan(Hs[b], device=device)
+        hip_lanczos_ground_state(None, plan=Pc, device=device)
+        Pc.close()
+    _sync()
+    t_seq = (time.perf_counter() - t0) / min(sequential, count)
+    w = np.linalg.eigvalsh(Hs[:4])
+    return dict(config='D1 dense-stack batched_ground_state, %d x N=%d (dim %d, complex128)'
+                       % (count, n, 1 << n),
+                count=count, dim=1 << n, seconds_batched=t_batch,
+                solves_per_s_batched=count / t_batch,
+                seconds_per_solve_sequential=t_seq, solves_per_s_sequential=1.0 / t_seq,
+                speedup_vs_sequential=t_seq * count / t_batch,
+                E0_err_vs_eigvalsh=float(np.max(np.abs(r['E0'][:4] - w[:, 0]))),
+                gap_err_vs_eigvalsh=float(np.max(np.abs(r['gap'][:4] - (w[:, 1] - w[:, 0])))),
+                builder_rel_err=builder_err)
 
 
 def run_c1(device='cuda', steps=200, seed=0):

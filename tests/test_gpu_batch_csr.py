@@ -124,3 +124,13 @@ def test_dense_stack_mirror_matches_eigh(pkg, dev):
         w = np.linalg.eigvalsh(Hs[b])
         assert abs(out['E0'][b] - w[0]) <= 1e-9 * max(1.0, abs(w[0]))
         assert abs(out['gap'][b] - (w[1] - w[0])) <= 1e-8
+
+
+def test_dense_stack_workload_small(pkg, dev):
+    """The dense-stack workload at a small size: its builder agrees with the
+    device H·v and the batched results with eigvalsh."""
+    from energy_level_amd import workloads as W
+    r = W.run_d1(dev, count=8, n=8, sequential=2)
+    assert r['builder_rel_err'] <= 1e-12
+    assert r['E0_err_vs_eigvalsh'] <= 1e-9 * 10
+    assert r['gap_err_vs_eigvalsh'] <= 1e-8
