@@ -562,7 +562,7 @@ def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph,
 
 
 def certified_ground_states(models, steps=150, dtype='float32', refine=True, bulk_tol=1e-6,
-                            tol=1e-9, bulk_cycles=2, refine_schedule=((60, 1), (300, 1), (1000, 3)),
+                            tol=1e-9, bulk_cycles=3, refine_schedule=((300, 1), (700, 1), (1500, 2)),
                             seed=0, device='cuda', autotune=True, use_graph=True, observables=True,
                             tile_bits=None, seed_noise=1e-3, store_basis='auto', vectors=None):
     """Batched ground states with a convergence test per instance.
@@ -583,10 +583,16 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
     1e-10·|T| in FP64 ones (their rounding).  An FP64 bulk (or no
     refinement) runs ``bulk_cycles`` cycles and then the schedule's cycle
     counts as further cycles of ``steps`` steps.  Defaults measured on the
-    512-instance ensemble (N 10-20, 150-step cycles): FP32 to 1e-6 and the
-    schedule ((60, 1), (300, 1), (1000, 3)) certify all 512 instances in
-    3.95 s; FP32 to 1e-5 with ((50, 1), (200, 2), (1000, 2)) left 4 of them
-    unconverged in 3.79 s, and 300-step cycles took 4.39 s.
+    512-instance ensemble (N 10-20, 150-step cycles, each pair certified
+    independently, lent basis): E1, not E0, sets the FP64 time (the slow
+    instances have a close E2), so no instance finishes in a 60-step first
+    cycle; three FP32 cycles to 1e-6 and ((300, 1), (700, 1), (1500, 2))
+    certify all 512 in 2.34 s, against 2.90 s for two FP32 cycles and
+    ((60, 1), (300, 1), (1000, 3)), 2.53 s for two FP32 cycles and the new
+    schedule, 2.62-3.05 s for ((300, 1), (700, 1), (2000, 1)),
+    ((300, 1), (800, 1), (1800, 2)) and ((350, 1), (900, 1), (2000, 1)).
+    Earlier, without per-pair certification: FP32 to 1e-5 with ((50, 1),
+    (200, 2), (1000, 2)) left 4 instances unconverged.
 
     Returns E0, gap, ritz_bound (E0), ritz_bound_gap (E1), converged (both
     pairs, the final stage), converged_E0, bulk_converged,
@@ -667,8 +673,8 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
 
 def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device='cuda',
                              autotune=True, use_graph=True, observables=True, refine=True,
-                             tol=1e-9, bulk_tol=1e-6, bulk_cycles=2,
-                             refine_schedule=((60, 1), (300, 1), (1000, 3))):
+                             tol=1e-9, bulk_tol=1e-6, bulk_cycles=3,
+                             refine_schedule=((300, 1), (700, 1), (1500, 2))):
     """Dict like batched_ground_state (tomisnot/energy-level
     src/core/manybody/equilibrium.py:251-311): E0 (B,), gap (B,), n_atom (a
     list, one array of <n_i> per instance since N may differ) and backend,

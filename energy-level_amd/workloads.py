@@ -389,7 +389,7 @@ def run_c4(device='cuda', count=512, steps=150, seed=0, plain=True, plain_count=
         for b in idx:
             g1 = certified_ground_states([models[b]], steps=2 * steps, dtype='float64', tol=tol,
                                          seed=seed, device=device, observables=False,
-                                         tile_bits=tb, bulk_cycles=8)
+                                         tile_bits=tb, bulk_cycles=9)
             Eg.append(float(g1['E0'][0]))
             convg.append(bool(g1['converged'][0]))
         _sync()
