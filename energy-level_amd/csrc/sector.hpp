@@ -444,18 +444,13 @@ __global__ __launch_bounds__(NW * 64, NW >= 16 ? 4 : RYD_FIB_FL_WAVES(V)) void k
   const int cnt = (int)M.cnt_lo[hL];
   const int64_t R = H.R;
   const int ng = (int)(hL ? M.W[FIB_NOUT - 1] : M.W[FIB_NOUT]);   // F(11) or F(12)
-  {
-    V v[SEPT];
+  // the block's codes are requested first; the group tables below are
+  // computed while the loads are in flight, and only then staged in LDS
+  V xstage[SEPT];
 #pragma unroll
-    for (int j = 0; j < SEPT; ++j) {
-      const int k = j * NT + t;
-      v[j] = k < cnt ? io.x[R + k] : vzero(v[0]);
-    }
-#pragma unroll
-    for (int j = 0; j < SEPT; ++j) {
-      const int k = j * NT + t;
-      if (k < cnt) xt[k] = vscale(v[j], sr);
-    }
+  for (int j = 0; j < SEPT; ++j) {
+    const int k = j * NT + t;
+    xstage[j] = k < cnt ? io.x[R + k] : vzero(xstage[0]);
   }
   if (t < 64) xt[FIB_ZB + t] = vzero(xt[0]);
   for (int g = t; g < ng; g += NT) {
@@ -481,6 +476,11 @@ __global__ __launch_bounds__(NW * 64, NW >= 16 ? 4 : RYD_FIB_FL_WAVES(V)) void k
     }
     row[FIB_NOUT] = base;
     row[FIB_NOUT + 1] = (int)o;
+  }
+#pragma unroll
+  for (int j = 0; j < SEPT; ++j) {
+    const int k = j * NT + t;
+    if (k < cnt) xt[k] = vscale(xstage[j], sr);
   }
   fib_cross_tables(M, h, Fx, T0, T1, T2);     // its barriers publish xt, the stripe and gpb
   // per-lane inner code, flip offsets and coefficients (upper neighbour of
