@@ -185,6 +185,8 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     # by their steps: check less often below STORE_MIN_DIM
     default_check = CHECK_STEPS if dim >= STORE_MIN_DIM else 2 * CHECK_STEPS
     check = max(1, min(int(m), int(check_steps or default_check)))
+    if not (float(tol) > 0.0):
+        check = max(1, int(m))   # nothing can pass a zero tolerance: test once per cycle
     code = nat.dtype_code(dt)
     ws = plan.workspace()
     st = nat.stream_ptr()

@@ -204,17 +204,26 @@ def run_c1(device='cuda', steps=200, seed=0):
     model = from_calib(calib, np.ones(n), -1.2 * np.ones(n))
     P = Plan(model, device=device)
     v0 = _start_vector(P.dim, seed, False, P.device)
+
+    def solve():
+        R = lanczos_run(P, v0, steps, 1, 0.0, want=2)
+        x = ritz_vectors(P, R)
+        E0, res = rayleigh_residual(P, x[0], R.theta[R.picks[0]])
+        return R, E0, res, site_occupations(P, x[0])
+
+    # the first call in a process also loads these kernels' code
     _sync()
     t0 = time.perf_counter()
-    R = lanczos_run(P, v0, steps, 1, 0.0, want=2)
-    x = ritz_vectors(P, R)
-    E0, res = rayleigh_residual(P, x[0], R.theta[R.picks[0]])
-    n_atom = site_occupations(P, x[0])
+    solve()
+    _sync()
+    t_cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    R, E0, res, n_atom = solve()
     _sync()
     t = time.perf_counter() - t0
     return dict(config='C1 chain N=12 FP64 Lanczos', steps=int(R.j), E0=E0, residual=res,
                 gap=float(R.theta[R.picks[1]] - R.theta[R.picks[0]]), n_atom=n_atom.tolist(),
-                seconds=t, calib=calib)
+                seconds=t, seconds_first_call=t_cold, calib=calib)
 
 
 def run_c2(device='cuda', num=1024, nvec=16, seed=0, margin=0.01, dtypes=('float32', 'float64')):
