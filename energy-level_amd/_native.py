@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, 'librydhip.so')
 
 RYD_OK, RYD_ERR_INVALID, RYD_ERR_MEMORY, RYD_ERR_DEVICE, RYD_ERR_UNSUPPORTED = range(5)
 RYD_F32, RYD_F64, RYD_C64, RYD_C128 = range(4)
-ABI_VERSION = 10
+ABI_VERSION = 11
 RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH, RYD_BASIS_CSR = range(5)
 
 _vp = ctypes.c_void_p
@@ -61,6 +61,9 @@ SIGNATURES = {
     'ryd_lanczos_reortho_locked': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _i64, _i64, _vp, _i64,
                                                   _dbl, _vp, _vp, _vp, _dbl, ctypes.c_int, _vp,
                                                   _vp, _vp]),
+    'ryd_lanczos_locked': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _i64, _vp,
+                                          _vp, _vp, _vp, _dbl, _vp, _vp, _vp, _vp, ctypes.c_int,
+                                          _vp, _vp]),
     'ryd_lanczos_status': (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(_i64), _vp]),
     'ryd_lanczos_stored': (ctypes.c_int, [_vp, ctypes.c_int, _vp, _i64, _i64, _i64, _i64, _vp, _vp,

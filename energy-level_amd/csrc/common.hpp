@@ -173,6 +173,8 @@ enum PostOp : int {
   POST_ACCUM = 2,     // out[k] += sum_k
   POST_LZ_ALPHA = 3,  // Lanczos: alpha[j] = sum_0
   POST_LZ_NORM = 4,   // Lanczos: nrm[j+1] = sqrt(sum_1); breakdown check
+  POST_LZ_LOCK = 5,   // locked Lanczos: γ = (sum_0, sum_2) to out[0..1],
+                      // nrm[j+1] = sqrt(sum_1 − |γ|^2); breakdown check
 };
 
 // Device-side Lanczos state (caller-owned device memory).
@@ -223,7 +225,7 @@ __device__ inline void block_reduce_store(double v0, double v1, double v2, doubl
 __global__ __launch_bounds__(256) void k_reduce_finish(Reduce rd, unsigned nblocks) {
   __shared__ double red_lds[4 * NRED];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (rd.st && rd.post != POST_LZ_NORM && rd.st->stop) return;
+  if (rd.st && rd.post != POST_LZ_NORM && rd.post != POST_LZ_LOCK && rd.st->stop) return;
   double s[NRED] = {0.0, 0.0, 0.0, 0.0};
   for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
@@ -255,6 +257,19 @@ __global__ __launch_bounds__(256) void k_reduce_finish(Reduce rd, unsigned nbloc
       const double nv = sqrt(t[1] > 0.0 ? t[1] : 0.0);
       rd.nrm[rd.j + 1] = nv;
       if (!(nv > rd.st->tiny)) {   // also catches NaN
+        rd.st->stop = 1;
+        rd.st->steps = rd.j + 1;
+      }
+      break;
+    }
+    case POST_LZ_LOCK: {
+      if (rd.st->stop) break;
+      rd.out[0] = t[0];
+      rd.out[1] = t[2];
+      const double n2 = t[1] - (t[0] * t[0] + t[2] * t[2]);
+      const double nv = sqrt(n2 > 0.0 ? n2 : 0.0);
+      rd.nrm[rd.j + 1] = nv;
+      if (!(nv > rd.st->tiny)) {
         rd.st->stop = 1;
         rd.st->steps = rd.j + 1;
       }

@@ -169,4 +169,52 @@ __global__ __launch_bounds__(256) void k_scale_rsqrt(V* __restrict__ x, int64_t 
     x[i] = vscale(x[i], f);
 }
 
+// Locked orth step: u_pre = w − (alpha_j / nrm_j) u_j in place of w, with the
+// sums {Re<g, u_pre>, |u_pre|^2, Im<g, u_pre>} (POST_LZ_LOCK then forms
+// γ = <g, u_pre> and the norm of u_pre − γ g); the replay skips the sums and
+// accumulates the Ritz vectors from u_j as k_lz_orth does.
+template <class V>
+__global__ __launch_bounds__(256) void k_lz_orth_lock(const V* __restrict__ uj, V* __restrict__ w,
+                                                      const V* __restrict__ g, int64_t D,
+                                                      const double* alpha, const double* nrm,
+                                                      int j, V* ritz0, V* ritz1,
+                                                      const double* yc0, const double* yc1,
+                                                      const int* stop, int reduce, Reduce rd) {
+  using RT = typename VT<V>::R;
+  if (stop && load_coherent(stop)) return;
+  const double inv = 1.0 / nrm[j];
+  const RT coef = (RT)(-alpha[j] * inv);
+  const RT r0c = ritz0 ? (RT)(yc0[j] * inv) : (RT)0;
+  const RT r1c = ritz1 ? (RT)(yc1[j] * inv) : (RT)0;
+  double nn = 0.0, gre = 0.0, gim = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    const V u = uj[i];
+    V v = w[i];
+    vfma(v, coef, u);
+    w[i] = v;
+    if (reduce) {
+      nn += nrm2(v);
+      cdot_acc(g[i], v, gre, gim);
+    }
+    if (ritz0) { V a = ritz0[i]; vfma(a, r0c, u); ritz0[i] = a; }
+    if (ritz1) { V a = ritz1[i]; vfma(a, r1c, u); ritz1[i] = a; }
+  }
+  if (reduce) block_reduce_store(gre, nn, gim, 0.0, rd.partials);
+}
+
+// u_{j+1} = u_pre − γ g, γ = gam[0] + i·gam[1] (real types use gam[0])
+template <class V>
+__global__ __launch_bounds__(256) void k_lz_unlock(V* __restrict__ w, const V* __restrict__ g,
+                                                   int64_t D, const double* gam, const int* stop) {
+  if (stop && load_coherent(stop)) return;
+  const double gr = gam[0], gi = gam[1];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < D; i += stride) {
+    V v = w[i];
+    csub(v, gr, gi, g[i]);
+    w[i] = v;
+  }
+}
+
 }  // namespace ryd

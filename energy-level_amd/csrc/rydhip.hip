@@ -1058,6 +1058,70 @@ int build_graph(ryd_plan* P, const ryd_model* m) {
 
 }  // namespace
 
+// Three-term recurrence in the complement of a unit vector g (the lock):
+// steps j0 .. j0+steps-1 as do_lanczos, except that the orth step leaves
+// u_pre = w − (alpha_j / nrm_j) u_j with the sums of <g, u_pre>, a finish
+// stores γ_j = <g, u_pre> to gam[2j..2j+1] and nrm_{j+1} = |u_pre − γ_j g|,
+// and k_lz_unlock removes γ_j g.  The start vector must be orthogonal to g.
+// The replay reads the stored alpha, nrm and gam.
+template <class V>
+int do_lanczos_locked(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2,
+                      const void* lock_v, int64_t j0, int64_t steps, double* alpha, double* nrm,
+                      double* gam, void* state, double tiny, void* ritz0, void* ritz1,
+                      const double* y0, const double* y1, int replay, void* ws, hipStream_t st) {
+  V* B[3] = {static_cast<V*>(b0), static_cast<V*>(b1), static_cast<V*>(b2)};
+  const V* g = static_cast<const V*>(lock_v);
+  LzState* S = static_cast<LzState*>(state);
+  const int64_t D = P->dim;
+  const unsigned sg = stream_grid(D);
+  if (j0 == 0 && !replay) {
+    k_lz_init<<<1, 1, 0, st>>>(S, tiny);
+    LAUNCH_CHECK();
+    Reduce rd = make_reduce(ws, POST_LZ_NORM, nullptr);
+    rd.nrm = nrm;
+    rd.st = S;
+    rd.j = -1;
+    k_dots<V><<<sg, 256, 0, st>>>(nullptr, B[0], D, rd);
+    LAUNCH_CHECK();
+    k_reduce_finish<<<1, 256, 0, st>>>(rd, sg);
+    LAUNCH_CHECK();
+  }
+  const int* stop = replay ? nullptr : &S->stop;
+  for (int64_t j = j0; j < j0 + steps; ++j) {
+    V* uj = B[j % 3];
+    V* ujm = B[(j + 2) % 3];
+    V* w = B[(j + 1) % 3];
+    ApplyIO<V> io;
+    io.x = uj;
+    io.z = j > 0 ? ujm : nullptr;
+    io.y = w;
+    io.probe = nullptr;
+    io.smode = SM_LANCZOS;
+    io.s = io.ch = io.cx = io.cz = 0.0;
+    io.nrm = nrm;
+    io.j = (int)j;
+    io.stop = stop;
+    io.reduce = !replay;
+    Reduce ra = make_reduce(ws, POST_LZ_ALPHA, nullptr);
+    ra.alpha = alpha;
+    ra.st = S;
+    ra.j = (int)j;
+    if (int rc = launch_apply<V>(P, dtype, io, ra, st)) return rc;
+    Reduce rb = make_reduce(ws, POST_LZ_LOCK, gam + 2 * j);
+    rb.nrm = nrm;
+    rb.st = S;
+    rb.j = (int)j;
+    k_lz_orth_lock<V><<<sg, 256, 0, st>>>(uj, w, g, D, alpha, nrm, (int)j, static_cast<V*>(ritz0),
+                                          static_cast<V*>(ritz1), y0, y1, stop, !replay, rb);
+    LAUNCH_CHECK();
+    if (!replay) k_reduce_finish<<<1, 256, 0, st>>>(rb, sg);
+    LAUNCH_CHECK();
+    k_lz_unlock<V><<<sg, 256, 0, st>>>(w, g, D, gam + 2 * j, stop);
+    LAUNCH_CHECK();
+  }
+  return RYD_OK;
+}
+
 #define DISPATCH(dtype, FN, ...)                                 \
   switch (dtype) {                                               \
     case RYD_F32: return FN<float>(__VA_ARGS__);                 \
@@ -1561,6 +1625,20 @@ int ryd_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int6
   hipStream_t st = static_cast<hipStream_t>(stream);
   DISPATCH(dtype, do_lanczos, P, dtype, b0, b1, b2, j0, steps, alpha, nrm, state, tiny, ritz0,
            ritz1, y0, y1, replay, ws, st);
+}
+
+int ryd_lanczos_locked(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2,
+                       const void* lock, int64_t j0, int64_t steps, double* alpha, double* nrm,
+                       double* gam, void* state, double tiny, void* ritz0, void* ritz1,
+                       const double* y0, const double* y1, int replay, void* ws, void* stream) {
+  if (!P || !b0 || !b1 || !b2 || !lock || !alpha || !nrm || !gam || !state || !ws)
+    return fail(RYD_ERR_INVALID, "null argument to ryd_lanczos_locked");
+  if (j0 < 0 || steps < 0) return fail(RYD_ERR_INVALID, "negative step range");
+  if ((ritz0 && !y0) || (ritz1 && !y1)) return fail(RYD_ERR_INVALID, "Ritz vector without coefficients");
+  if (int rc = check_dtype(P, dtype)) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DISPATCH(dtype, do_lanczos_locked, P, dtype, b0, b1, b2, lock, j0, steps, alpha, nrm, gam,
+           state, tiny, ritz0, ritz1, y0, y1, replay, ws, st);
 }
 
 size_t ryd_reortho_workspace_bytes(int64_t dim, int64_t k) {

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 10
+#define RYD_ABI_VERSION 11
 
 enum ryd_status {
   RYD_OK = 0,
@@ -145,7 +145,8 @@ int ryd_apply(const ryd_plan* plan, int dtype, const void* x, const void* z, voi
  *   state      : device buffer of ryd_lanczos_state_bytes(); stop flag and
  *                step count after a breakdown (|u_{j+1}| <= tiny).
  *   replay = 1 : recompute the same vectors from stored alpha/nrm and add
- *                Σ_j yk[j]/nrm[j]·u_j to ritz0/ritz1 (either may be NULL);
+ *                Σ_j yk[j]/nrm[j]·u_j to ritz0/ritz1 (either may be NULL;
+ *                y0/y1 device FP64);
  *                b0 must again hold the start vector at j0 = 0. */
 int ryd_lanczos(const ryd_plan* plan, int dtype, void* b0, void* b1, void* b2, int64_t j0,
                 int64_t steps, double* alpha, double* nrm, void* state, double tiny,
@@ -187,6 +188,24 @@ int ryd_lanczos_reortho_locked(const ryd_plan* plan, int dtype, void* basis, int
                                void* gs_ws, void* ws, void* stream);
 
 /* Reads the state written by ryd_lanczos (synchronises the stream). */
+/* The recurrence of ryd_lanczos in the orthogonal complement of a unit
+ * vector `lock` (device, plan dtype; e.g. a converged ground vector): step j
+ * leaves u_pre = w − (alpha_j/nrm_j) u_j, stores γ_j = <lock, u_pre> to
+ * gam[2j], gam[2j+1] (device FP64, >= 2·(j0+steps) entries; imaginary part
+ * 0 for real types) and continues with u_{j+1} = u_pre − γ_j lock, nrm[j+1]
+ * = |u_{j+1}|.  Removing the lock component of the finished residual keeps
+ * <lock, u_j> at rounding level (removing only <lock, w> lets alpha_j·
+ * <lock, q_j> grow every step).  The start vector in b0 must be orthogonal
+ * to lock.  replay = 1 reads the stored alpha, nrm and gam and accumulates
+ * Ritz vectors as ryd_lanczos does (y0, y1: device FP64 coefficients).  The lowest genuine Ritz value is then
+ * the next level counted with multiplicity, as eigh's w[1]
+ * (src/core/manybody/equilibrium.py:65-75, :272) and the reference's
+ * two-vector block Krylov (matrix_free.py:219-266) report it; O(steps)
+ * instead of the O(steps²·dim) of ryd_lanczos_reortho_locked. */
+int ryd_lanczos_locked(const ryd_plan* plan, int dtype, void* b0, void* b1, void* b2,
+                       const void* lock, int64_t j0, int64_t steps, double* alpha, double* nrm,
+                       double* gam, void* state, double tiny, void* ritz0, void* ritz1,
+                       const double* y0, const double* y1, int replay, void* ws, void* stream);
 int ryd_lanczos_status(const void* state, int* stopped, int64_t* steps, void* stream);
 
 /* The same recurrence keeping every Lanczos vector (no replay for Ritz
