@@ -101,3 +101,30 @@ def test_real_chain_is_simple_by_perron_frobenius(pkg, dev):
     w = np.linalg.eigvalsh(H.toarray())
     assert abs(r2['E0'] - w[0]) <= 1e-9 * max(1.0, abs(w[0]))
     assert abs(r2['gap'] - (w[1] - w[0])) <= 1e-8
+
+
+def test_locked_recurrence_stays_in_the_complement(pkg, dev):
+    """ryd_lanczos_locked over 300 steps on a 64-state diagonal with a double
+    minimum and a weakly coupled rest (|alpha| up to 5): locked against one
+    ground vector, the lowest level of the complement is the other ground
+    vector (E1 = E0 = -1, as eigh), the Ritz vector stays orthogonal to the
+    lock, and no Ritz value falls below the spectrum (removing only <g, w>
+    each step lets alpha·<g, q_j> grow and gave values near -1.8 here)."""
+    import torch
+    rng = np.random.default_rng(21)
+    D = 64
+    d = rng.uniform(0.0, 5.0, D)
+    d[5] = d[40] = -1.0
+    H = np.diag(d)
+    rest = [i for i in range(D) if i not in (5, 40)]
+    for i, j in zip(rest[:-1], rest[1:]):
+        H[i, j] = H[j, i] = 0.05
+    w, V = np.linalg.eigh(H)
+    P = pkg.CsrPlan(sp.csr_matrix(H), device=dev)
+    g = torch.as_tensor(0.6 * V[:, 0] + 0.8 * V[:, 1], device=dev)
+    E1, r1, x1 = pkg.locked_lowest(P, g, 0.0, max_steps=300)
+    P.close()
+    assert abs(E1 - w[1]) <= 1e-10
+    assert r1 <= 1e-8
+    x = x1 / torch.linalg.vector_norm(x1)
+    assert abs(float(torch.vdot(g, x))) <= 1e-10
