@@ -702,7 +702,7 @@ def batched_ground_state_hip(models, steps=150, dtype='float32', seed=0, device=
     return out
 
 
-def _locked_second_level(items, vecs, tol, steps=150, seed=1, device='cuda', store_basis='auto',
+def _locked_second_level(items, vecs, tol, seed=1, device='cuda', store_basis='auto',
                          schedule=((150, 3), (300, 2), (1000, 3))):
     """The second level of every CSR instance counted with multiplicity:
     batched FP64 Lanczos cycles in the complement of the instance's ground
@@ -758,8 +758,8 @@ def batched_ground_state_csr_hip(mats, steps=150, dtype='float64', tol=1e-9, see
                                   device=device, autotune=False, observables=observables,
                                   store_basis=store_basis, vectors=vecs)
     if multiplicity:
-        E1, b1, c1 = _locked_second_level(items, vecs, tol, steps=steps, seed=seed + 7,
-                                          device=device, store_basis=store_basis)
+        E1, b1, c1 = _locked_second_level(items, vecs, tol, seed=seed + 7, device=device,
+                                          store_basis=store_basis)
         out['gap_distinct'] = out['gap']
         out['gap'] = E1 - out['E0']
         out['ritz_bound_gap'] = b1
