@@ -18,7 +18,7 @@ import numpy as np
 from . import _native as nat
 from .model import RydModel, from_positions, chain_positions, c6_coupling, from_calib
 
-__all__ = ['BatchLanczos', 'BatchCsrLanczos', 'CsrInstance', 'disorder_ensemble',
+__all__ = ['BatchLanczos', 'BatchCsrLanczos', 'CsrInstance', 'csr_instances', 'disorder_ensemble',
            'batched_ground_state_hip', 'batched_ground_state_csr_hip', 'certified_ground_states']
 
 BMAXN = 24            # largest instance n (occupation rows hold BMAXN sites + the norm)
@@ -293,6 +293,27 @@ class CsrInstance:
         self.n = bits if (1 << bits) == self.dim and bits <= BMAXN else 0
 
 
+def csr_instances(mats):
+    """:class:`CsrInstance` of every matrix (instances pass through).  The
+    conversions run on up to 16 host threads: a dense matrix is scanned for
+    its non-zeros, which for a stack of dense matrices (64 × 16 MB at
+    dim 1024, complex128) is memory-bound host work that numpy and scipy
+    do with the GIL released."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    todo = [k for k, m in enumerate(mats) if not isinstance(m, CsrInstance)]
+    out = list(mats)
+    workers = min(16, os.cpu_count() or 1, len(todo))
+    if workers <= 1:
+        for k in todo:
+            out[k] = CsrInstance(mats[k])
+        return out
+    with ThreadPoolExecutor(workers) as ex:
+        for k, inst in zip(todo, ex.map(lambda k: CsrInstance(mats[k]), todo)):
+            out[k] = inst
+    return out
+
+
 class BatchCsrLanczos(BatchLanczos):
     """``ryd_batch_create_csr`` owner: the batched Lanczos of
     :class:`BatchLanczos` over :class:`CsrInstance` items (any Hermitian
@@ -309,7 +330,7 @@ class BatchCsrLanczos(BatchLanczos):
         nat.require_gpu()
         if dtype not in ('float32', 'float64'):
             raise ValueError('dtype must be float32/float64')
-        self.models = [m if isinstance(m, CsrInstance) else CsrInstance(m) for m in instances]
+        self.models = csr_instances(instances)
         self.count = len(self.models)
         if self.count < 1:
             raise ValueError('empty batch')
@@ -754,7 +775,7 @@ def batched_ground_state_csr_hip(mats, steps=150, dtype='float64', tol=1e-9, see
     'float32' runs an FP32 (complex64) bulk before the FP64 refinement.
     Returns E0, gap, converged (E0 and, with ``multiplicity``, E1), the
     bounds and, with ``observables``, n_atom (instances with dim 2^n)."""
-    items = [m if isinstance(m, CsrInstance) else CsrInstance(m) for m in mats]
+    items = csr_instances(mats)
     vecs = {}
     out = certified_ground_states(items, steps=steps, dtype=dtype, tol=tol, seed=seed,
                                   device=device, autotune=False, observables=observables,

@@ -168,3 +168,31 @@ def test_native_bound_is_the_best_copy_of_a_ghost_cluster(pkg):
         assert abs(np.linalg.norm(y) - 1.0) <= 1e-10
         assert np.linalg.norm(T @ y - E[0, 0] * y) <= 1e-9 * scale
         assert abs(Bd[0, 0] - abs(B[-1] * y[-1])) <= 1e-14 * max(1.0, Bd[0, 0])
+
+
+def test_csr_instances_threaded_conversion_matches_one_at_a_time(pkg):
+    """The threaded conversion of a mixed list (dense complex and real
+    arrays, a scipy matrix, an existing instance) gives the same CSR as
+    converting each matrix alone, in the list's order."""
+    import scipy.sparse as sp
+    batch = pkg.batch
+    rng = np.random.default_rng(5)
+    mats = []
+    for k in range(20):
+        dim = 16 << (k % 3)
+        A = np.zeros((dim, dim), complex if k % 2 else float)
+        r, c = rng.integers(0, dim, 3 * dim), rng.integers(0, dim, 3 * dim)
+        A[r, c] = rng.standard_normal(r.size) + (1j * rng.standard_normal(r.size) if k % 2 else 0)
+        A = A + A.conj().T
+        if k == 7:
+            A = sp.csr_matrix(A)
+        mats.append(A)
+    mats.append(batch.CsrInstance(mats[0]))
+    got = batch.csr_instances(mats)
+    assert len(got) == len(mats) and got[-1] is mats[-1]
+    for m, g in zip(mats, got):
+        ref = m if isinstance(m, batch.CsrInstance) else batch.CsrInstance(m)
+        assert (g.dim, g.n, g.complex) == (ref.dim, ref.n, ref.complex)
+        assert np.array_equal(g.H.indptr, ref.H.indptr)
+        assert np.array_equal(g.H.indices, ref.H.indices)
+        assert np.array_equal(g.H.data, ref.H.data)
