@@ -16,6 +16,7 @@
 #include "common.hpp"
 #include "hv_full.hpp"
 #include "stream_ops.hpp"
+#include "lz_small.hpp"
 #include "channels.hpp"
 #include "graph_apply.hpp"
 #include "measure.hpp"
@@ -558,6 +559,18 @@ __global__ void k_lz_init(LzState* s, double tiny) {
   s->tiny = tiny;
 }
 
+// One-launch Lanczos (k_lz_small) where launch_apply would run the standard
+// full-basis kernels and the vector fits one workgroup's LDS.  Measured at
+// n = 12 FP64 (C1, 200 steps and the replay): see lz_small.hpp.
+#ifndef RYD_LZ_SMALL
+#define RYD_LZ_SMALL 1
+#endif
+template <class V>
+bool lz_small(const ryd_plan* P) {
+  return RYD_LZ_SMALL && P->basis == RYD_BASIS_FULL && !P->generic && P->n >= 1 &&
+         P->n <= lzs_maxn<V>();
+}
+
 template <class V>
 int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64_t j0,
                int64_t steps, double* alpha, double* nrm, void* state, double tiny, void* ritz0,
@@ -566,6 +579,14 @@ int do_lanczos(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2, int64
   V* B[3] = {static_cast<V*>(b0), static_cast<V*>(b1), static_cast<V*>(b2)};
   LzState* S = static_cast<LzState*>(state);
   const int64_t D = P->dim;
+  if (lz_small<V>(P)) {
+    const FullModel M = full_model(P, tile_bits<V>());
+    k_lz_small<V><<<1, LZS_THREADS, 0, st>>>(M, B[0], B[1], B[2], j0, steps, alpha, nrm, S, tiny,
+                                             static_cast<V*>(ritz0), static_cast<V*>(ritz1), y0,
+                                             y1, replay, nullptr, 0, 0);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   const unsigned sg = stream_grid(D);
   if (j0 == 0 && !replay) {
     k_lz_init<<<1, 1, 0, st>>>(S, tiny);
@@ -625,6 +646,14 @@ int do_lanczos_stored(const ryd_plan* P, int dtype, void* basis_v, int64_t ld, i
   LzState* S = static_cast<LzState*>(state);
   const int64_t D = P->dim;
   const unsigned sg = stream_grid(D);
+  if (lz_small<V>(P)) {
+    const FullModel M = full_model(P, tile_bits<V>());
+    k_lz_small<V><<<1, LZS_THREADS, 0, st>>>(M, nullptr, nullptr, nullptr, j0, steps, alpha, nrm,
+                                             S, tiny, nullptr, nullptr, nullptr, nullptr, 0,
+                                             basis, ld, jbase);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   if (j0 == 0) {
     k_lz_init<<<1, 1, 0, st>>>(S, tiny);
     LAUNCH_CHECK();
@@ -1131,6 +1160,24 @@ int do_lanczos_locked(const ryd_plan* P, int dtype, void* b0, void* b1, void* b2
     default: return fail(RYD_ERR_INVALID, "unknown dtype");     \
   }
 
+// k_ritz_stored_sliced below RITZ_SLICED_MAX_DIM, else one thread per element
+template <class V>
+void launch_ritz_stored(const void* basis, int64_t ld, int64_t dim, int count, const double* yd,
+                        int two, const double* nrm, int64_t j0, int accumulate, void* out0,
+                        void* out1, hipStream_t st) {
+  const V* b = static_cast<const V*>(basis);
+  V* o0 = static_cast<V*>(out0);
+  V* o1 = static_cast<V*>(out1);
+  if (dim <= RITZ_SLICED_MAX_DIM) {
+    const unsigned g = (unsigned)((dim + RITZ_E - 1) / RITZ_E);
+    k_ritz_stored_sliced<V><<<g, 256, 0, st>>>(b, ld, dim, count, yd, two, nrm, j0, accumulate,
+                                               o0, o1);
+  } else {
+    k_ritz_stored<V><<<stream_grid(dim), 256, 0, st>>>(b, ld, dim, count, yd, two, nrm, j0,
+                                                       accumulate, o0, o1);
+  }
+}
+
 extern "C" {
 
 int ryd_abi_version(void) { return RYD_ABI_VERSION; }
@@ -1586,23 +1633,18 @@ int ryd_ritz_stored(int dtype, const void* basis, int64_t ld, int64_t dim, int64
   HIP_TRY(hipMemcpyAsync(yd, y0, sizeof(double) * count, hipMemcpyHostToDevice, st));
   if (y1) HIP_TRY(hipMemcpyAsync(yd + RITZ_MAX, y1, sizeof(double) * count, hipMemcpyHostToDevice, st));
   const int two = y1 ? 1 : 0;
-  const unsigned g = stream_grid(dim);
   switch (dtype) {
     case RYD_F32:
-      k_ritz_stored<float><<<g, 256, 0, st>>>(static_cast<const float*>(basis), ld, dim, (int)count,
-          yd, two, nrm, j0, accumulate, static_cast<float*>(out0), static_cast<float*>(out1));
+      launch_ritz_stored<float>(basis, ld, dim, (int)count, yd, two, nrm, j0, accumulate, out0, out1, st);
       break;
     case RYD_F64:
-      k_ritz_stored<double><<<g, 256, 0, st>>>(static_cast<const double*>(basis), ld, dim, (int)count,
-          yd, two, nrm, j0, accumulate, static_cast<double*>(out0), static_cast<double*>(out1));
+      launch_ritz_stored<double>(basis, ld, dim, (int)count, yd, two, nrm, j0, accumulate, out0, out1, st);
       break;
     case RYD_C64:
-      k_ritz_stored<c32><<<g, 256, 0, st>>>(static_cast<const c32*>(basis), ld, dim, (int)count,
-          yd, two, nrm, j0, accumulate, static_cast<c32*>(out0), static_cast<c32*>(out1));
+      launch_ritz_stored<c32>(basis, ld, dim, (int)count, yd, two, nrm, j0, accumulate, out0, out1, st);
       break;
     case RYD_C128:
-      k_ritz_stored<c64><<<g, 256, 0, st>>>(static_cast<const c64*>(basis), ld, dim, (int)count,
-          yd, two, nrm, j0, accumulate, static_cast<c64*>(out0), static_cast<c64*>(out1));
+      launch_ritz_stored<c64>(basis, ld, dim, (int)count, yd, two, nrm, j0, accumulate, out0, out1, st);
       break;
     default:
       return fail(RYD_ERR_INVALID, "unknown dtype");

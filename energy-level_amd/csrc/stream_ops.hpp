@@ -91,6 +91,55 @@ __global__ __launch_bounds__(256) void k_ritz_stored(const V* __restrict__ basis
   }
 }
 
+// The same combination for small dims (a few thousand elements, where one
+// thread per element walking every row is bound by load latency): a block is
+// RITZ_E consecutive elements × RITZ_S row slices (slice s takes rows
+// s, s + RITZ_S, ...), the slices' sums added in LDS in slice order.
+constexpr int RITZ_E = 16, RITZ_S = 16;
+constexpr int64_t RITZ_SLICED_MAX_DIM = 1 << 20;
+template <class V>
+__global__ __launch_bounds__(256) void k_ritz_stored_sliced(const V* __restrict__ basis,
+                                                            int64_t ld, int64_t dim, int count,
+                                                            const double* y, int two,
+                                                            const double* nrm, int64_t j0,
+                                                            int accumulate, V* __restrict__ out0,
+                                                            V* __restrict__ out1) {
+  using RT = typename VT<V>::R;
+  static_assert(RITZ_E * RITZ_S == 256, "one thread per element and slice");
+  __shared__ RT c0[RITZ_MAX], c1[RITZ_MAX];
+  __shared__ V p0[RITZ_S][RITZ_E], p1[RITZ_S][RITZ_E];
+  for (int j = threadIdx.x; j < count; j += blockDim.x) {
+    const double inv = 1.0 / nrm[j0 + j];
+    c0[j] = (RT)(y[j] * inv);
+    c1[j] = two ? (RT)(y[RITZ_MAX + j] * inv) : (RT)0;
+  }
+  __syncthreads();
+  const int e = threadIdx.x % RITZ_E, sl = threadIdx.x / RITZ_E;
+  const int64_t i = (int64_t)blockIdx.x * RITZ_E + e;
+  V a0 = vzero(V{}), a1 = vzero(V{});
+  if (i < dim) {
+#pragma unroll 4
+    for (int j = sl; j < count; j += RITZ_S) {
+      const V u = basis[(int64_t)j * ld + i];
+      vfma(a0, c0[j], u);
+      if (two) vfma(a1, c1[j], u);
+    }
+  }
+  p0[sl][e] = a0;
+  p1[sl][e] = a1;
+  __syncthreads();
+  if (sl == 0 && i < dim) {
+    V r0 = accumulate ? out0[i] : vzero(V{});
+    V r1 = (two && accumulate) ? out1[i] : vzero(V{});
+    for (int k = 0; k < RITZ_S; ++k) {
+      vfma(r0, (RT)1, p0[k][e]);
+      if (two) vfma(r1, (RT)1, p1[k][e]);
+    }
+    out0[i] = r0;
+    if (two) out1[i] = r1;
+  }
+}
+
 // sums: <a, b>, |b|^2, |a|^2 ; used for norms and Rayleigh quotients
 template <class V>
 __global__ __launch_bounds__(256) void k_dots(const V* __restrict__ a, const V* __restrict__ b,

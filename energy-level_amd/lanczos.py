@@ -155,9 +155,11 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     uses) so that Ritz vectors are one streaming combination instead of a
     replay of the recurrence.  True: allocate as memory allows; 'auto'
     (dim >= 2^22): only into blocks torch's allocator already holds unused
-    (a repeated solve reuses the blocks of the previous one); False: three
-    rotating vectors.  When storage stops, the rest of the run continues with
-    three vectors and only that tail is replayed.
+    (a repeated solve reuses the blocks of the previous one), and always for
+    the one-launch runs of small full-basis plans (``Plan.lanczos_one_launch``:
+    a few MB, and a replay would repeat every step); False: three rotating
+    vectors.  When storage stops, the rest of the run continues with three
+    vectors and only that tail is replayed.
 
     Convergence is tested every ``check_steps`` (default ``CHECK_STEPS`` from
     dim = STORE_MIN_DIM on, twice that below; at most ``m``) steps; the
@@ -179,8 +181,11 @@ def lanczos_run(plan, v0, m, max_cycles, tol, want=2, tiny=None, store='auto', t
     R.chunks = []                 # (jbase, rows tensor) of a stored basis
     R.bufs = None
     R.split = None                # first step run without storage (None: all stored)
-    storing = (dim >= STORE_MIN_DIM) if store == 'auto' else bool(store)
-    cached_only = store == 'auto'
+    # one-launch runs (small full-basis plans) store their few MB: a Ritz
+    # vector is then one combination instead of a replay of every step
+    one = bool(getattr(plan, 'lanczos_one_launch', None) and plan.lanczos_one_launch(dt))
+    storing = (dim >= STORE_MIN_DIM or one) if store == 'auto' else bool(store)
+    cached_only = store == 'auto' and not one
     # small problems are bound by the host round trip of a check, large ones
     # by their steps: check less often below STORE_MIN_DIM
     default_check = CHECK_STEPS if dim >= STORE_MIN_DIM else 2 * CHECK_STEPS

@@ -109,6 +109,13 @@ class Plan:
         m = int(nat.lib().ryd_plan_rank_mode(self._h))
         return {0: 'full', 1: 'number', 2: 'search', 3: 'graph', 4: 'chain'}.get(m, 'unknown')
 
+    def lanczos_one_launch(self, dtype):
+        """True when ryd_lanczos runs every step of a call in one launch:
+        full basis without J/A channels, D <= 8192 (complex128 4096)."""
+        import torch
+        lim = 4096 if dtype == torch.complex128 else 8192
+        return self.basis == 'full' and not self.model.has_pair_flips and self.dim <= lim
+
     def workspace(self):
         """Zero-filled workspace of ``ryd_workspace_bytes`` (reused by this plan)."""
         import torch

@@ -139,7 +139,11 @@ int ryd_apply(const ryd_plan* plan, int dtype, const void* x, const void* z, voi
  * of torch_lanczos_ground_state (matrix_free.py:231-251) and the recurrence of
  * spectrum.lanczos (spectrum.py:76-96).
  *   b0, b1, b2 : three vectors of dim elements; at j0 = 0, b0 holds the start
- *                vector (any norm).  Vector u_j lives in b[j % 3].
+ *                vector (any norm).  Vector u_j lives in b[j % 3]; on return
+ *                the last two (u_{j0+steps-1}, u_{j0+steps}, or the residual
+ *                at a breakdown) are there, the third buffer is scratch
+ *                (full-basis plans up to n = 13, complex128 12, run every
+ *                step of a call in one launch).
  *   alpha, nrm : device FP64 arrays (>= j0+steps and >= j0+steps+1 entries);
  *                alpha_j = alpha[j], beta_j = nrm[j+1], nrm[0] = |start|.
  *   state      : device buffer of ryd_lanczos_state_bytes(); stop flag and
