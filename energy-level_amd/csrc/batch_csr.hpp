@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "batch.hpp"
 #include "reortho.hpp"   // csub, and cdot_acc: conj(a)·b accumulated in FP64
+#include "lz_small.hpp"  // LZS_THREADS, LZS_WAVES
 
 namespace ryd {
 
@@ -314,6 +315,200 @@ __global__ __launch_bounds__(256) void k_bc_occ(BcLz L, const V* __restrict__ xa
   if (t <= BMAXN) {
     const bool used = t == BMAXN || t < I.n;
     partial[(size_t)gt * (BMAXN + 1) + t] = used ? red[t][0] + red[t][1] + red[t][2] + red[t][3] : 0.0;
+  }
+}
+
+// ---- one launch per run for small instances -----------------------------------
+// Instances of at most LZS_THREADS·bcc_maxe<V>() rows with short rows: one
+// 1024-thread workgroup per instance runs the whole sequence of a run
+// (stop reset, |u_0| and <g, u_0>, then per step the apply with alpha and
+// <g, w>, the finishing formulas, the orth with γ g and the Ritz
+// accumulation, the norm and breakdown test), u_j in LDS for the column
+// gathers, each thread's rows of u_{j-1} and w in registers, the sums formed
+// in the same order on every thread.  alpha, nrm, stop and gam receive what
+// the multi-launch form writes; the stored form writes every row, the
+// rotating form the two live vectors on exit (as lz_small.hpp).
+template <class V> constexpr int bcc_maxe() { return sizeof(V) == 16 ? 4 : 8; }
+
+// three block sums, the same on every thread; callers alternate two slots
+__device__ __forceinline__ void bcc_sum3(double& a, double& b, double& c,
+                                         double (*red)[LZS_WAVES]) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  c = wave_sum(c);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+    red[2][w] = c;
+  }
+  __syncthreads();
+  double x = 0.0, y = 0.0, z = 0.0;
+#pragma unroll
+  for (int k = 0; k < LZS_WAVES; ++k) {
+    x += red[0][k];
+    y += red[1][k];
+    z += red[2][k];
+  }
+  a = x;
+  b = y;
+  c = z;
+}
+
+template <class V>
+__global__ __launch_bounds__(LZS_THREADS) void k_bc_cycle(BcLz L, BcMat M, V* base, int64_t total,
+                                                          int stored, const V* __restrict__ lock,
+                                                          V* __restrict__ ritz,
+                                                          const double* __restrict__ ycoef,
+                                                          int steps) {
+  using RT = typename VT<V>::R;
+  constexpr int E = bcc_maxe<V>();
+  __shared__ V xs[LZS_THREADS * E];
+  __shared__ double redA[3][LZS_WAVES], redB[3][LZS_WAVES];
+  const int b = blockIdx.x;
+  const BatchInst& I = L.inst[b];
+  const int dim = (int)I.dim;
+  const int tid = threadIdx.x;
+  auto U = [&](int j) -> V* {
+    const int r = stored ? j : ((j % 3) + 3) % 3;
+    return base + (size_t)r * (size_t)total + I.off;
+  };
+  const V* g = lock ? lock + I.off : nullptr;
+  const int64_t* ptr = M.ptr + I.off;
+  V* xr = ritz ? ritz + I.off : nullptr;
+  double* alpha = L.alpha + (size_t)b * L.ldk;
+  double* nrm = L.nrm + (size_t)b * (L.ldk + 1);
+  double* gm = L.gam + 4 * (size_t)b;
+  const double* yc = ritz ? ycoef + (size_t)b * L.ldk : nullptr;
+
+  // |u_0|, <g, u_0>
+  double nn = 0.0, g2 = 0.0, g3 = 0.0;
+  {
+    const V* u0 = U(0);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int r = tid + k * LZS_THREADS;
+      if (r >= dim) continue;
+      const V u = u0[r];
+      xs[r] = u;
+      nn += nrm2(u);
+      if (g) cdot_acc(g[r], u, g2, g3);
+    }
+  }
+  bcc_sum3(nn, g2, g3, redB);                      // also publishes xs
+  double nj = sqrt(nn > 0.0 ? nn : 0.0);
+  const bool brk0 = !(nj > L.tiny);
+  if (tid == 0) {
+    L.stop[b] = brk0 ? 1 : 0;
+    nrm[0] = nj;
+    if (g) {
+      gm[2] = g2;
+      gm[3] = g3;
+    }
+  }
+  if (brk0) return;
+
+  V um[E], w[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    um[k] = vzero(V{});
+    w[k] = vzero(V{});
+  }
+  double njm = 1.0;
+  for (int j = 0; j < steps; ++j) {
+    const double s = 1.0 / nj;
+    const RT sr = (RT)s;
+    const RT czr = (RT)(j > 0 ? -nj / njm : 0.0);
+    // apply: w = H (u_j / nrm_j) - (nrm_j / nrm_{j-1}) u_{j-1}
+    double sa = 0.0, gre = 0.0, gim = 0.0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int r = tid + k * LZS_THREADS;
+      if (r >= dim) continue;
+      V acc = vzero(V{});
+      const int64_t qa = ptr[r], qb = ptr[r + 1];
+      // four entries' loads in flight at once (they come from L2)
+#pragma unroll 4
+      for (int64_t q = qa; q < qb; ++q)
+        vcfma(acc, (RT)(M.vre[q] * s), (RT)(M.vim ? M.vim[q] * s : 0.0), xs[M.col[q]]);
+      if (j > 0) vfma(acc, czr, um[k]);
+      w[k] = acc;
+      sa += rdot(vscale(xs[r], sr), acc);
+      if (g) cdot_acc(g[r], acc, gre, gim);
+    }
+    bcc_sum3(sa, gre, gim, redA);                  // also: every gather of xs is done
+    const double a = sa;
+    double gm0 = 0.0, gm1 = 0.0;
+    if (g) {
+      const double c = a / nj;
+      gm0 = gre - c * g2;
+      gm1 = gim - c * g3;
+    }
+    if (tid == 0) {
+      alpha[j] = a;
+      if (g) {
+        gm[0] = gm0;
+        gm[1] = gm1;
+      }
+    }
+    // orth: u_{j+1} = w - (alpha_j / nrm_j) u_j [- γ g]
+    const RT coef = (RT)(-a / nj);
+    const RT rc = xr ? (RT)(yc[j] / nj) : (RT)0;
+    V* wrow = stored ? U(j + 1) : nullptr;
+    double acc2 = 0.0, h0 = 0.0, h1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int r = tid + k * LZS_THREADS;
+      if (r >= dim) continue;
+      const V ue = xs[r];
+      V v = w[k];
+      vfma(v, coef, ue);
+      if (g) {
+        const V ge = g[r];
+        csub(v, gm0, gm1, ge);
+        cdot_acc(ge, v, h0, h1);
+      }
+      acc2 += nrm2(v);
+      if (xr) {
+        V q = xr[r];
+        vfma(q, rc, ue);
+        xr[r] = q;
+      }
+      if (wrow) wrow[r] = v;
+      w[k] = v;
+      um[k] = ue;
+      xs[r] = v;                                   // only this thread reads xs[r] now
+    }
+    bcc_sum3(acc2, h0, h1, redB);                  // also publishes xs
+    const double nv = sqrt(acc2 > 0.0 ? acc2 : 0.0);
+    const bool brk = !(nv > L.tiny);               // also catches NaN
+    if (tid == 0) {
+      nrm[j + 1] = nv;
+      if (brk) L.stop[b] = 1;
+      if (g) {
+        gm[2] = h0;
+        gm[3] = h1;
+      }
+    }
+    g2 = h0;
+    g3 = h1;
+    if (brk || j + 1 == steps) {
+      if (!stored) {
+        V* pu = U(j);
+        V* pw = U(j + 1);
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const int r = tid + k * LZS_THREADS;
+          if (r < dim) {
+            pu[r] = um[k];
+            pw[r] = w[k];
+          }
+        }
+      }
+      return;
+    }
+    njm = nj;
+    nj = nv;
   }
 }
 

@@ -2586,6 +2586,7 @@ struct ryd_batch {
   bool csr = false;
   bool csr_wave = false;             // one row per wavefront (long rows)
   int csr_tr = 0;
+  int64_t csr_maxdim = 0;            // rows of the largest instance
   int64_t* d_csr_ptr = nullptr;      // [total + 1]
   int32_t* d_csr_col = nullptr;      // [nnz], column inside the instance
   double* d_csr_re = nullptr;        // [nnz]
@@ -2681,6 +2682,17 @@ int batch_sequence(ryd_batch* B, int k, hipStream_t st, V* ritz = nullptr,
 
 // The recurrence over CSR instances (one tile shape), optionally locked
 // against the caller's vectors and accumulating Ritz vectors (replay).
+// One launch per run (k_bc_cycle) when every instance fits one workgroup's
+// LDS and rows are short; measured on the dense-stack workload (64 × 1024).
+#ifndef RYD_BC_FUSED
+#define RYD_BC_FUSED 1
+#endif
+template <class V>
+bool bc_fused(const ryd_batch* B) {
+  return RYD_BC_FUSED && B->csr && !B->csr_wave && B->csr_maxdim >= 1 &&
+         B->csr_maxdim <= (int64_t)LZS_THREADS * bcc_maxe<V>();
+}
+
 template <class V>
 int bc_sequence(ryd_batch* B, hipStream_t st, V* ritz = nullptr, const double* ycoef = nullptr) {
   V* base = static_cast<V*>(batch_rows(B));
@@ -2690,6 +2702,13 @@ int bc_sequence(ryd_batch* B, hipStream_t st, V* ritz = nullptr, const double* y
   };
   const V* lock = static_cast<const V*>(B->lock);
   const BcMat M{B->d_csr_ptr, B->d_csr_col, B->d_csr_re, B->d_csr_im};
+  if (bc_fused<V>(B)) {
+    k_bc_cycle<V><<<B->count, LZS_THREADS, 0, st>>>(bc_lz(B, 0), M, base, B->total,
+                                                    B->basis ? 1 : 0, lock, ritz, ycoef,
+                                                    B->steps);
+    LAUNCH_CHECK();
+    return RYD_OK;
+  }
   const unsigned g = (unsigned)B->ntiles[0];
   k_batch_reset<<<(B->count + 255) / 256, 256, 0, st>>>(B->d_stop, B->count);
   LAUNCH_CHECK();
@@ -2923,6 +2942,7 @@ extern "C" int ryd_batch_create_csr(int32_t count, const int64_t* dims, const in
     I.tile0 = (int64_t)tmap.size();
     I.dim = d;
     hinst[(size_t)b] = I;
+    if (d > B->csr_maxdim) B->csr_maxdim = d;
     for (int t = 0; t < I.ntiles; ++t) tmap.push_back(b);
     B->n.push_back(I.n);
     B->off.push_back(off);

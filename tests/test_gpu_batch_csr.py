@@ -134,3 +134,38 @@ def test_dense_stack_workload_small(pkg, dev):
     assert r['builder_rel_err'] <= 1e-12
     assert r['E0_err_vs_eigvalsh'] <= 1e-9 * 10
     assert r['gap_err_vs_eigvalsh'] <= 1e-8
+
+
+@pytest.mark.parametrize('cplx,dims', [
+    (False, [2, 5, 100, 1023, 1024, 1025, 3000, 8192]),   # one launch per run (<= 8192 rows)
+    (False, [300, 9000]),                                 # 9000 rows: the multi-launch form
+    (True, [3, 64, 1000, 4096]),                          # complex128: one launch up to 4096
+])
+@pytest.mark.parametrize('store', [True, False])
+def test_batch_coefficients_match_the_oracle_recurrence(pkg, dev, cplx, dims, store):
+    """Each instance's alpha/beta of one batched run against the oracle's
+    Lanczos from the same start vector (first min(20, dim-1) steps, where the
+    plain recurrence stays within rounding of the reorthogonalised one)."""
+    from oracle import rydberg_oracle as orc
+    rng = np.random.default_rng(11 + len(dims) + cplx)
+    mats = [_rand_sparse_herm(d, rng, cplx, density=min(1.0, 6.0 / d)) for d in dims]
+    B = pkg.BatchCsrLanczos(mats, steps=20, dtype='float64', store_basis=store)
+    try:
+        v0 = B.start_vectors(seed=4)
+        B.run(v0, use_graph=False)
+        alpha, nrm, stop = B.coefficients()
+        v = v0.cpu().numpy()
+        for b, H in enumerate(mats):
+            d = dims[b]
+            K = 2 if d == 2 else min(20, d - 1)
+            x0 = v[B.offsets[b]:B.offsets[b + 1]].astype(complex)
+            ra, rb, n0sq = orc.lanczos(lambda q: H @ q, x0, K)
+            hn = abs(H).sum(axis=1).max()
+            assert abs(nrm[b, 0] ** 2 - n0sq) <= 1e-12 * n0sq
+            k = len(ra)
+            assert np.max(np.abs(alpha[b, :k] - ra)) <= 1e-9 * hn, (b, d)
+            assert np.max(np.abs(nrm[b, 1:1 + len(rb)] - rb), initial=0.0) <= 1e-9 * hn, (b, d)
+            if d == 2:
+                assert stop[b] == 1
+    finally:
+        B.close()
