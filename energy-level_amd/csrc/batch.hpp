@@ -76,6 +76,15 @@ __global__ __launch_bounds__(256) void k_batch_finish(BatchLz L, int post) {
   }
 }
 
+// Blocks go round-robin to the eight XCDs (block b runs on XCD b % 8).  XCD x
+// takes the x-th eighth of the global tiles in order, so the partner tiles of
+// an instance (bits >= tb) are fetched into one XCD's L2 instead of all
+// eight; the last G % 8 tiles keep their own block.
+__device__ inline int batch_tile_of(int b, int G) {
+  const int per = G >> 3;
+  return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+}
+
 // apply half-step (real element types).  Instances with tb >= 10 use the
 // full-basis tile layout (element e = r·1024 + t·4 + c, R = 2^(tb-10) groups):
 // bits 0..1 and 10..tb-1 flip in registers, bits 2..9 through LDS, bits >= tb
@@ -88,7 +97,7 @@ __global__ __launch_bounds__(256) void k_batch_apply(BatchLz L, const V* __restr
   __shared__ double Fx[16];
   __shared__ double T0[256];
   __shared__ double T1[16];
-  const int gt = blockIdx.x;
+  const int gt = batch_tile_of(blockIdx.x, gridDim.x);
   const int b = L.tile_inst[gt];
   const BatchInst& I = L.inst[b];
   if (load_coherent(&L.stop[b])) return;
