@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, 'librydhip.so')
 
 RYD_OK, RYD_ERR_INVALID, RYD_ERR_MEMORY, RYD_ERR_DEVICE, RYD_ERR_UNSUPPORTED = range(5)
 RYD_F32, RYD_F64, RYD_C64, RYD_C128 = range(4)
-ABI_VERSION = 11
+ABI_VERSION = 12
 RYD_BASIS_FULL, RYD_BASIS_CODES, RYD_BASIS_CHAIN_FIB, RYD_BASIS_GRAPH, RYD_BASIS_CSR = range(5)
 
 _vp = ctypes.c_void_p
@@ -101,6 +101,8 @@ SIGNATURES = {
     'ryd_batch_destroy': (ctypes.c_int, [_vp]),
     'ryd_batch_offset': (_i64, [_vp, ctypes.c_int32]),
     'ryd_batch_run': (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    'ryd_batch_run_steps': (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp]),
+    'ryd_batch_stop': (ctypes.c_int, [_vp, _vp]),
     'ryd_batch_autotune': (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int32), _vp]),
     'ryd_ritz_lowest': (ctypes.c_int, [ctypes.c_int32, _vp, _vp, _i64, _vp, _i64, _vp, _dbl, _dbl,
                                        _vp, _vp, _vp, _i64]),

@@ -163,6 +163,7 @@ class BatchLanczos:
         if v0.numel() != self.total or v0.dtype != self.tdt or v0.device != self.device:
             raise ValueError('start vectors must be a %s tensor of length %d on %s'
                              % (self.tdt, self.total, self.device))
+        self._hstop = None
         nat.check(nat.lib().ryd_batch_run(self._h, nat.ptr(v0.contiguous()), int(bool(use_graph)),
                                           nat.stream_ptr()))
 
@@ -207,14 +208,55 @@ class BatchLanczos:
         occ = occ.cpu().numpy()
         return [occ[b, :int(m.n)] / occ[b, BMAXN] for b, m in enumerate(self.models)]
 
+    def _host_stopped(self):
+        hs = getattr(self, '_hstop', None)
+        return np.zeros(self.count, bool) if hs is None else hs < self.steps
+
     def _steps_of(self, nrm, stop):
-        """Valid steps per instance: all, or up to the breakdown."""
-        k = np.full(self.count, self.steps, np.int32)
+        """Valid steps per instance: all, up to the breakdown, or up to the
+        step where :meth:`run_parts` stopped it."""
+        hs = getattr(self, '_hstop', None)
+        k = np.full(self.count, self.steps, np.int32) if hs is None else hs.astype(np.int32)
         tiny = 1e-6 if self.dtype == 'float32' else 1e-13
         for b in np.nonzero(stop)[0]:
-            bad = np.nonzero(~(nrm[b, 1:] > tiny))[0]
-            k[b] = int(bad[0]) + 1 if bad.size else self.steps
+            bad = np.nonzero(~(nrm[b, 1:int(k[b]) + 1] > tiny))[0]
+            if bad.size:
+                k[b] = int(bad[0]) + 1
         return k
+
+    def run_parts(self, v0, tol, merge_rel, need0, need1, part=150):
+        """The recurrence from ``v0`` in parts of ``part`` steps (no graph).
+        After each part, an instance whose still-needed Ritz pairs (``need0``,
+        ``need1``: bool per instance) have bounds <= tol·max(1,|E|) stops at
+        that step, so a long cycle costs only the steps its instances use;
+        :meth:`ground_states` then reads each at its own step count."""
+        if v0.numel() != self.total or v0.dtype != self.tdt or v0.device != self.device:
+            raise ValueError('start vectors must be a %s tensor of length %d on %s'
+                             % (self.tdt, self.total, self.device))
+        v0 = v0.contiguous()
+        self._hstop = np.full(self.count, self.steps, np.int32)
+        need0, need1 = np.asarray(need0, bool), np.asarray(need1, bool)
+        j = 0
+        while j < self.steps:
+            j1 = min(self.steps, j + int(part))
+            nat.check(nat.lib().ryd_batch_run_steps(self._h, nat.ptr(v0) if j == 0 else None,
+                                                    j, j1, nat.stream_ptr()))
+            j = j1
+            if j >= self.steps:
+                break
+            _, _, stop = self.coefficients()
+            live = (need0 | need1) & (stop == 0)
+            if not np.any(live):
+                break
+            self._hstop[live] = j
+            gs = self.ground_states(tol, live, merge_rel)
+            E1 = gs['E0'] + gs['gap']
+            with np.errstate(invalid='ignore'):
+                ok1 = gs['ritz_bound_gap'] <= tol * np.maximum(1.0, np.abs(E1))
+            done = live & (gs['converged_E0'] | ~need0) & (ok1 | ~need1)
+            self._hstop[live & ~done] = self.steps
+            if np.any(done):
+                nat.check(nat.lib().ryd_batch_stop(self._h, np.ascontiguousarray(done, np.int32).ctypes.data))
 
     def ground_states(self, tol=None, active=None, merge_rel=None):
         """E0, gap (lowest two genuine Ritz values, ``ryd_ritz_lowest``), their
@@ -235,7 +277,8 @@ class BatchLanczos:
             a = np.ascontiguousarray(alpha[idx])
             o = np.ascontiguousarray(nrm[idx, 1:])
             kk = np.ascontiguousarray(k[idx])
-            bl = np.where(stop[idx] != 0, 0.0, nrm[idx, kk])
+            dstop = (stop != 0) & ~self._host_stopped()
+            bl = np.where(dstop[idx], 0.0, nrm[idx, kk])
             bl = np.ascontiguousarray(bl, np.float64)
             Es = np.zeros((idx.size, 2))
             Bs = np.zeros((idx.size, 2))
@@ -249,9 +292,9 @@ class BatchLanczos:
             self._ycoef[idx] = Ys
         out = dict(E0=np.where(np.isnan(E[:, 0]), 0.0, E[:, 0]), gap=E[:, 1] - E[:, 0],
                    ritz_bound=np.where(np.isinf(bnd[:, 0]), 0.0, bnd[:, 0]),
-                   ritz_bound_gap=np.where(stop != 0, 0.0, bnd[:, 1]),
+                   ritz_bound_gap=np.where((stop != 0) & ~self._host_stopped(), 0.0, bnd[:, 1]),
                    steps=np.where(np.isin(np.arange(self.count), idx), k, 0),
-                   stopped=stop.astype(bool))
+                   stopped=(stop != 0) & ~self._host_stopped())
         if tol is not None:
             E0, E1 = out['E0'], out['E0'] + np.nan_to_num(out['gap'])
             c0 = out['ritz_bound'] <= tol * np.maximum(1.0, np.abs(E0))
@@ -463,7 +506,7 @@ def _reset_records(out, count):
 
 def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph, tile_bits,
            device, occ_rows=None, seeds=None, finish_last=True, store_basis='auto', vecs=None,
-           want=2, lock=None):
+           want=2, lock=None, parts=None):
     """Batched Lanczos cycles over the instances ``idx`` (indices into
     ``models``) from their concatenated start vectors ``v``, until each
     instance's lowest ``want`` genuine Ritz pairs have bounds <=
@@ -514,7 +557,11 @@ def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph,
         act = np.ones(len(idx), bool)
         done = 0
         for cyc in range(int(cycles)):
-            B.run(v, use_graph=use_graph)
+            if parts and type(B) is BatchLanczos and B.stores_basis and steps >= 2 * parts:
+                B.run_parts(v, tol, merge_rel, act & ~R['c0'][idx],
+                            act & (~R['c1'][idx] if want == 2 else False), parts)
+            else:
+                B.run(v, use_graph=use_graph)
             done += 1
             gs = B.ground_states(tol, act, merge_rel)
             E1 = gs['E0'] + gs['gap']
@@ -585,7 +632,8 @@ def _stage(models, idx, v, dtype, steps, tol, merge_rel, cycles, out, use_graph,
 def certified_ground_states(models, steps=150, dtype='float32', refine=True, bulk_tol=1e-6,
                             tol=1e-9, bulk_cycles=4, refine_schedule=((300, 1), (700, 1), (1500, 2)),
                             seed=0, device='cuda', autotune=True, use_graph=True, observables=True,
-                            tile_bits=None, seed_noise=1e-3, store_basis='auto', vectors=None):
+                            tile_bits=None, seed_noise=1e-3, store_basis='auto', vectors=None,
+                            refine_parts=150):
     """Batched ground states with a convergence test per instance.
 
     Bulk: ``steps``-step batched Lanczos cycles in ``dtype`` (one captured
@@ -625,6 +673,11 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
     to every :class:`BatchLanczos` of the stages (True lets the first call
     take the basis memory into torch's cache; a repeated call with 'auto'
     then reuses it: 3.72 s instead of 3.88 s on the ensemble above).
+    ``refine_parts``: FP64 cycles of at least twice that many steps run in
+    parts of that many steps (:meth:`BatchLanczos.run_parts`, full-basis
+    batches with a stored basis), and an instance whose needed pairs pass
+    stops at the end of a part instead of running to the cycle's end (None:
+    whole cycles).
     ``models`` are :class:`RydModel` instances (the full-basis batch) or
     :class:`CsrInstance` items (any Hermitian matrices, ``BatchCsrLanczos``);
     ``vectors`` (a dict) receives each instance's final lowest Ritz vector
@@ -684,7 +737,7 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
             d, rest, vrest = _stage(models, rest, vrest, 'float64', st, tol, merge['float64'], cy,
                                     out, use_graph, tile_bits, device, occ_rows,
                                     finish_last=(k == len(schedule) - 1),
-                                    store_basis=store_basis, vecs=vectors)
+                                    store_basis=store_basis, vecs=vectors, parts=refine_parts)
             done64 += d
         out['cycles_float64'] = done64
     if observables:

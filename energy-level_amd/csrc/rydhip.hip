@@ -2577,6 +2577,7 @@ struct ryd_batch {
   double* d_partial = nullptr;       // [max tiles]
   size_t maxtiles = 0;
   double* d_ycoef = nullptr;         // [count][steps] Ritz coefficients (replay)
+  int next_step = -1;                // step where the last ryd_batch_run_steps ended
   double* d_occp = nullptr;          // [max tiles][BMAXN+1] occupation partials
   hipStream_t stream = nullptr;      // capture / replay stream (default priority)
   hipGraphExec_t exec = nullptr;
@@ -2651,7 +2652,8 @@ void* batch_rows(const ryd_batch* B) { return B->basis ? B->basis : B->d_vec; }
 
 template <class V>
 int batch_sequence(ryd_batch* B, int k, hipStream_t st, V* ritz = nullptr,
-                   const double* ycoef = nullptr) {
+                   const double* ycoef = nullptr, int j0 = 0, int j1 = -1) {
+  if (j1 < 0) j1 = B->steps;
   V* base = static_cast<V*>(batch_rows(B));
   // u_j: row j of a lent basis, else one of three rotating rows (u_{-1}
   // is never read: its coefficient is zero at j = 0)
@@ -2660,13 +2662,15 @@ int batch_sequence(ryd_batch* B, int k, hipStream_t st, V* ritz = nullptr,
     return base + (size_t)r * B->total;
   };
   const unsigned g = (unsigned)B->ntiles[k];
-  k_batch_reset<<<(B->count + 255) / 256, 256, 0, st>>>(B->d_stop, B->count);
-  LAUNCH_CHECK();
-  k_batch_norm0<V><<<g, 256, 0, st>>>(batch_lz(B, k, 0), U(0));
-  LAUNCH_CHECK();
-  k_batch_finish<<<B->count, 256, 0, st>>>(batch_lz(B, k, -1), 1);
-  LAUNCH_CHECK();
-  for (int j = 0; j < B->steps; ++j) {
+  if (j0 == 0) {
+    k_batch_reset<<<(B->count + 255) / 256, 256, 0, st>>>(B->d_stop, B->count);
+    LAUNCH_CHECK();
+    k_batch_norm0<V><<<g, 256, 0, st>>>(batch_lz(B, k, 0), U(0));
+    LAUNCH_CHECK();
+    k_batch_finish<<<B->count, 256, 0, st>>>(batch_lz(B, k, -1), 1);
+    LAUNCH_CHECK();
+  }
+  for (int j = j0; j < j1; ++j) {
     BatchLz L = batch_lz(B, k, j);
     k_batch_apply<V><<<g, 256, 0, st>>>(L, U(j), U(j - 1), U(j + 1));
     LAUNCH_CHECK();
@@ -3030,10 +3034,58 @@ extern "C" int ryd_batch_run(ryd_batch* B, const void* v0, int use_graph, void* 
   HIP_TRY(hipMemcpyAsync(batch_rows(B), v0, es * (size_t)B->total, hipMemcpyDeviceToDevice,
                          B->stream));
   B->basis_valid = false;
+  B->next_step = -1;
   if (int rc = batch_launch(B, B->choice, use_graph)) return rc;
   B->basis_valid = B->basis != nullptr;
   HIP_TRY(hipEventRecord(B->ev, B->stream));
   HIP_TRY(hipStreamWaitEvent(caller, B->ev, 0));
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_run_steps(ryd_batch* B, const void* v0, int32_t j0, int32_t j1,
+                                   void* stream) {
+  if (!B || (j0 == 0 && !v0) || j0 < 0 || j1 <= j0 || j1 > B->steps)
+    return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_run_steps");
+  if (B->csr) return fail(RYD_ERR_UNSUPPORTED, "steps in parts run on full-basis batches");
+  if (j0 > 0 && B->next_step != j0)
+    return fail(RYD_ERR_INVALID, "ryd_batch_run_steps continues at the step where the last call ended");
+  hipStream_t caller = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipEventRecord(B->ev, caller));
+  HIP_TRY(hipStreamWaitEvent(B->stream, B->ev, 0));
+  if (j0 == 0) {
+    HIP_TRY(hipMemcpyAsync(batch_rows(B), v0, batch_es(B) * (size_t)B->total,
+                           hipMemcpyDeviceToDevice, B->stream));
+    B->basis_valid = false;
+  }
+  B->next_step = -1;
+  const int rc = B->dtype == RYD_F32
+                     ? batch_sequence<float>(B, B->choice, B->stream, nullptr, nullptr, j0, j1)
+                     : batch_sequence<double>(B, B->choice, B->stream, nullptr, nullptr, j0, j1);
+  if (rc) return rc;
+  B->next_step = j1;
+  B->basis_valid = j1 == B->steps && B->basis != nullptr;
+  HIP_TRY(hipEventRecord(B->ev, B->stream));
+  HIP_TRY(hipStreamWaitEvent(caller, B->ev, 0));
+  return RYD_OK;
+}
+
+extern "C" int ryd_batch_stop(ryd_batch* B, const int32_t* flags) {
+  if (!B || !flags) return fail(RYD_ERR_INVALID, "bad argument to ryd_batch_stop");
+  std::vector<int> st((size_t)B->count);
+  HIP_TRY(hipMemcpyAsync(st.data(), B->d_stop, sizeof(int) * st.size(), hipMemcpyDeviceToHost,
+                         B->stream));
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  bool all = true;
+  for (int b = 0; b < B->count; ++b) {
+    if (flags[b]) st[b] = 1;
+    all = all && st[b] != 0;
+  }
+  HIP_TRY(hipMemcpyAsync(B->d_stop, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice,
+                         B->stream));
+  HIP_TRY(hipStreamSynchronize(B->stream));
+  // every instance stopped: the rows each one ran are final (Ritz vectors
+  // combine them; coefficients past an instance's stop are zero, unread)
+  if (all && B->next_step > 0) B->basis_valid = B->basis != nullptr;
   return RYD_OK;
 }
 

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RYD_ABI_VERSION 11
+#define RYD_ABI_VERSION 12
 
 enum ryd_status {
   RYD_OK = 0,
@@ -416,6 +416,16 @@ int64_t ryd_batch_offset(const ryd_batch* batch, int32_t index);
 /* Runs the recurrence from the device start vectors v0, ordered after and
  * before `stream`. */
 int ryd_batch_run(ryd_batch* batch, const void* v0, int use_graph, void* stream);
+/* Steps [j0, j1) of the same recurrence without a graph (full-basis batches):
+ * j0 = 0 starts it from v0, a later call continues where the last one ended,
+ * so the caller can stop converged instances between parts
+ * (ryd_batch_stop).  Replaces the fixed-length run of the reference's dense
+ * batch (equilibrium.py:251-311) for certified solves: an instance leaves
+ * its cycle at the step its Ritz pairs pass, not at the cycle's end. */
+int ryd_batch_run_steps(ryd_batch* batch, const void* v0, int32_t j0, int32_t j1, void* stream);
+/* Sets the stop flag of every instance b with flags[b] != 0 (host array of
+ * `count`): its recurrence stays at the step reached. */
+int ryd_batch_stop(ryd_batch* batch, const int32_t* flags);
 /* Times the recurrence for tiles of 2^10, 2^11, 2^12 codes and keeps the
  * fastest; ms_out (host, 3 entries) may be NULL. */
 int ryd_batch_autotune(ryd_batch* batch, const void* v0, int32_t* chosen_tile_bits,

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(pkg):
 def test_abi_version_and_error_path(pkg):
     from energy_level_amd import _native as nat
     L = nat.lib()
-    assert L.ryd_abi_version() == 11
+    assert L.ryd_abi_version() == 12
     # invalid model is rejected before any device work
     handle = ctypes.c_void_p()
     m = nat.Model()

@@ -141,3 +141,55 @@ def test_batch_stored_basis_matches_replay(pkg, dev, dtype, rtol):
     assert torch.max(torch.abs(x_d.double() - x_r.double())).item() <= rtol * scale
     Bs.close()
     Br.close()
+
+
+def test_certified_fp64_cycles_in_parts_match_whole_cycles(pkg, dev):
+    """FP64 refinement cycles run in parts, an instance stopping at the part
+    where its pairs pass, give the whole-cycle results (both certified to
+    1e-9) in no more steps; E0 and gap checked against the oracle's eigsh."""
+    ns = [10, 12, 13, 14, 15, 16, 11, 16, 14, 13, 15, 12]
+    models, calibs = _models(pkg, ns, seed=5)
+    kw = dict(steps=150, dtype='float32', tol=1e-9, bulk_cycles=2,
+              refine_schedule=((300, 1), (700, 2)), seed=2, device=dev, autotune=False,
+              tile_bits=12, observables=False, store_basis=True)
+    whole = pkg.batch.certified_ground_states(models, refine_parts=None, **kw)
+    parts = pkg.batch.certified_ground_states(models, refine_parts=100, **kw)
+    assert np.all(whole['converged']) and np.all(parts['converged'])
+    assert int(parts['steps_float64'].sum()) <= int(whole['steps_float64'].sum())
+    for b, (calib, De) in enumerate(calibs):
+        n = ns[b]
+        s = max(1.0, abs(whole['E0'][b]))
+        assert abs(parts['E0'][b] - whole['E0'][b]) <= 1e-9 * s
+        assert abs(parts['gap'][b] - whole['gap'][b]) <= 2e-9 * s
+        H = orc.build_H_sparse(n, np.ones(n), -De * np.ones(n), calib['pairs'], ['I1'])
+        e0 = orc.ground_state_eigsh(H)['E0']
+        assert abs(parts['E0'][b] - e0) <= 1e-9 * s, (n, parts['E0'][b], e0)
+
+
+def test_run_parts_stops_at_the_part_and_reads_its_steps(pkg, dev):
+    """An instance stopped by run_parts keeps the step count where it passed;
+    its Ritz values equal those of a whole run cut at that step."""
+    import torch
+    ns = [10, 12, 14]
+    models, _ = _models(pkg, ns, seed=7)
+    B = pkg.BatchLanczos(models, steps=400, dtype='float64', device=dev, store_basis=True)
+    v0 = B.start_vectors(4)
+    need = np.ones(3, bool)
+    B.run_parts(v0, 1e-9, 1e-10, need, need, part=50)
+    gs = B.ground_states(1e-9, None, 1e-10)
+    k = gs['steps']
+    assert np.all(k % 50 == 0) and np.all(k >= 50) and np.all(k <= 400)
+    W = pkg.BatchLanczos(models, steps=400, dtype='float64', device=dev, store_basis=True)
+    W.run(v0, use_graph=False)
+    a1, n1, _ = B.coefficients()
+    a2, n2, _ = W.coefficients()
+    for b in range(3):
+        kb = int(k[b])
+        assert np.array_equal(a1[b, :kb], a2[b, :kb])
+        assert np.array_equal(n1[b, :kb + 1], n2[b, :kb + 1])
+        if kb < 400:
+            assert gs['converged'][b]
+    x, _ = B.ritz(v0, B.restart_coefficients(np.zeros(3, np.int64)))
+    assert bool(torch.isfinite(x).all())
+    B.close()
+    W.close()
