@@ -662,6 +662,9 @@ def certified_ground_states(models, steps=150, dtype='float32', refine=True, bul
     ((60, 1), (300, 1), (1000, 3)), 2.53 s for two FP32 cycles and the new
     schedule, 2.62-3.05 s for ((300, 1), (700, 1), (2000, 1)),
     ((300, 1), (800, 1), (1800, 2)) and ((350, 1), (900, 1), (2000, 1)).
+    With the FP64 cycles in parts of 150 steps (``refine_parts``), instances
+    leave a cycle at the part where they pass: 1.53 s (201,600 FP64
+    instance-steps instead of 311,000).
     Earlier, without per-pair certification: FP32 to 1e-5 with ((50, 1),
     (200, 2), (1000, 2)) left 4 instances unconverged.
 
